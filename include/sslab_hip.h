@@ -1,0 +1,92 @@
+/*
+ * sslab_hip.h -- C ABI of libsslab_hip.so, the MI355X (gfx950) slab direct solver.
+ *
+ * Plain pointers and sizes only.  All matrix pointers are DEVICE pointers to
+ * row-major (C-order) FP64 data, exactly the memory order of the numpy float64
+ * C-order arrays the reference passes around.  `stream` is a hipStream_t (NULL
+ * = the default stream); every call is asynchronous on it unless stated.
+ * Every function returns 0 on success or a non-zero status; the message of the
+ * last failure in the calling thread is returned by sslab_last_error().
+ *
+ * Reference citations are to SimonDirckx/SslabLU.
+ */
+#ifndef SSLAB_HIP_H
+#define SSLAB_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSLAB_OK 0
+#define SSLAB_BAD_ARGUMENT 1
+#define SSLAB_HIP_ERROR 2
+#define SSLAB_WORKSPACE_TOO_SMALL 3
+#define SSLAB_SINGULAR 4
+#define SSLAB_NOT_SUPPORTED 5
+
+/* ABI version (bumped on any signature change). */
+int sslab_abi_version(void);
+/* Message of the last failed call in this thread ("" if none). */
+const char* sslab_last_error(void);
+
+/* ---- dense primitives ------------------------------------------------------
+ * C_b = alpha * op(A_b) op(B_b) + beta * C_b, b < batch, X_b = X + b*strideX.
+ * Replaces the numpy `@` products on block lists in
+ * direct_solve/omsdirectsolve.py:89-96,125-139 and multislab/omsdirectsolve.py:55-63.
+ */
+int sslab_dgemm_strided_batched(int transA, int transB, int M, int N, int K, double alpha,
+                                const double* A, int lda, long long strideA, const double* B,
+                                int ldb, long long strideB, double beta, double* C, int ldc,
+                                long long strideC, int batch, void* stream);
+
+/* In-place LU with partial pivoting (scipy.linalg.lu_factor semantics: ipiv is
+ * 0-based, info = i+1 for the first zero pivot).  Replaces lu_factor at
+ * direct_solve/omsdirectsolve.py:84,96,210,333. */
+int sslab_dgetrf_batched(int n, double* A, int lda, long long strideA, int* ipiv,
+                         long long strideP, int* info, int batch, void* stream);
+
+/* X = A^{-1} B from getrf factors; B and X distinct; perm_work: batch*n ints.
+ * Replaces lu_solve at direct_solve/omsdirectsolve.py:89-91,132-139,264-277. */
+int sslab_dgetrs_batched(int n, int nrhs, const double* LU, int lda, long long strideA,
+                         const int* ipiv, long long strideP, const double* B, int ldb,
+                         long long strideB, double* X, int ldx, long long strideX,
+                         int* perm_work, int batch, void* stream);
+
+/* ---- red-black block-tridiagonal solver ------------------------------------
+ * System (non-cyclic): T_i x_i + SiM_i x_{i-1} + SiP_i x_{i+1} = v_i, i < nb,
+ * with T, SiM, SiP each nb contiguous m x m blocks (SiM_0, SiP_{nb-1} unused).
+ * Any nb >= 1 (no power-of-two restriction).
+ * Replaces RedBlackSolver.factorize / .solve / .build_block_RB_solver_level
+ * (direct_solve/omsdirectsolve.py:180-219, 221-279, 283-333).
+ */
+int sslab_rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
+                   long long* work_ints);
+/* Number of ints written to `info` by sslab_rb_factor (one per eliminated block). */
+int sslab_rb_info_count(int nb);
+int sslab_rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
+                    double* factor, double* work, int* iwork, int* info, void* stream);
+/* V (nb*m x nrhs, leading dim ldv) holds the right-hand sides and is overwritten;
+ * X (nb*m x nrhs, ldx) receives the solution. */
+int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
+                   double* X, int ldx, void* stream);
+
+/* ---- Thomas block-tridiagonal solver ---------------------------------------
+ * Same system; sequential block elimination with explicit block inverses.
+ * Replaces BlockTridiagonalSolver.factorize_helper / .solve_helper
+ * (direct_solve/omsdirectsolve.py:50-100, 102-141) and the identity-diagonal
+ * build_block_tridiagonal_solver / block_tridiagonal_solve
+ * (multislab/omsdirectsolve.py:17-99, with T_i = I).
+ * factor layout: A (nb-1 blocks) | B^{-1} (nb) | LU(B) (nb) | C (nb-1);
+ * iwork starts with the nb*m pivots of LU(B). */
+int sslab_thomas_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
+                       long long* work_ints);
+int sslab_thomas_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
+                        double* factor, double* work, int* iwork, int* info, void* stream);
+int sslab_thomas_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
+                       double* X, int ldx, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSLAB_HIP_H */

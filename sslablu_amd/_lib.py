@@ -1,0 +1,107 @@
+"""ctypes binding of libsslab_hip.so (C ABI declared in include/sslab_hip.h).
+
+torch is imported first on purpose: the library's DT_NEEDED libamdhip64.so.7 then
+resolves to the HIP runtime torch already loaded, so device pointers and stream
+handles from torch are valid inside the library (one runtime per process).
+
+There is no CPU fallback: if the library is missing or no GPU is visible, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede loading the library, see module docstring)
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libsslab_hip.so"
+
+_c_int = ctypes.c_int
+_c_ll = ctypes.c_longlong
+_c_d = ctypes.c_double
+_vp = ctypes.c_void_p
+
+# name -> argtypes (all return int status)
+_SIGS = {
+    "sslab_abi_version": [],
+    "sslab_dgemm_strided_batched": [_c_int, _c_int, _c_int, _c_int, _c_int, _c_d,
+                                    _vp, _c_int, _c_ll, _vp, _c_int, _c_ll, _c_d,
+                                    _vp, _c_int, _c_ll, _c_int, _vp],
+    "sslab_dgetrf_batched": [_c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int, _vp],
+    "sslab_dgetrs_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int,
+                             _c_ll, _vp, _c_int, _c_ll, _vp, _c_int, _vp],
+    "sslab_rb_sizes": [_c_int, _c_int, _vp, _vp, _vp],
+    "sslab_rb_info_count": [_c_int],
+    "sslab_rb_factor": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "sslab_rb_solve": [_c_int, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _vp],
+    "sslab_thomas_sizes": [_c_int, _c_int, _vp, _vp, _vp],
+    "sslab_thomas_factor": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "sslab_thomas_solve": [_c_int, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _vp],
+}
+
+EXPORTED = tuple(_SIGS) + ("sslab_last_error",)
+
+_lib = None
+
+
+class SslabError(RuntimeError):
+    """A libsslab_hip.so call returned a non-zero status."""
+
+
+def load(path: str | os.PathLike | None = None):
+    """Load (once) and return the ctypes library object.  Raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise SslabError(
+            f"{p} not found: build it with `python -m sslablu_amd.build` "
+            "(there is no CPU fallback for the HIP path)")
+    lib = ctypes.CDLL(str(p))
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _c_int
+    lib.sslab_last_error.argtypes = []
+    lib.sslab_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args) -> int:
+    """Call entry point `name`; raise SslabError on a non-zero status."""
+    lib = load()
+    st = getattr(lib, name)(*args)
+    if st != 0:
+        msg = lib.sslab_last_error().decode(errors="replace")
+        raise SslabError(f"{name} failed (status {st}): {msg}")
+    return st
+
+
+def require_gpu():
+    """Raise unless a ROCm GPU is visible to torch (no CPU fallback exists)."""
+    if not torch.cuda.is_available():
+        raise SslabError("no ROCm GPU visible: the slab solver runs only on the GPU")
+
+
+def stream_handle(stream=None) -> int:
+    """hipStream_t of `stream` (or torch's current stream) as an int."""
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def ptr(t: torch.Tensor) -> int:
+    return int(t.data_ptr())
+
+
+def sizes(kind: str, nb: int, m: int):
+    """(factor_doubles, work_doubles, work_ints) for kind in {'rb', 'thomas'}."""
+    f = ctypes.c_longlong()
+    w = ctypes.c_longlong()
+    i = ctypes.c_longlong()
+    call(f"sslab_{kind}_sizes", nb, m, ctypes.addressof(f), ctypes.addressof(w),
+         ctypes.addressof(i))
+    return f.value, w.value, i.value

@@ -1,0 +1,162 @@
+"""Device-resident block-tridiagonal S-system factorization (FP64, gfx950).
+
+    T_i x_i + SiM_i x_{i-1} + SiP_i x_{i+1} = b_i,   i = 0..nb-1
+
+`BlockTriFactor` owns the factor tensors (torch, on the GPU) and calls
+libsslab_hip.so for the factorization and the multi-RHS solves:
+
+* method "rb": red-black cyclic reduction (any nb; log2(nb) levels, every level one
+  batched MFMA GEMM launch per sweep direction);
+* method "thomas": sequential block elimination (direct_solve/omsdirectsolve.py:48-141).
+
+Periodic (cyclic) systems add the corner blocks E (row 0, column nb-1) and F (row
+nb-1, column 0) through a rank-2m Woodbury correction as in
+direct_solve/omsdirectsolve.py:143-175, but stored as Z = T^{-1}U (N x 2m) and
+K = (I + V^T Z)^{-1} (2m x 2m) instead of the dense N x N `smw_block`:
+x = y - Z K [y_{nb-1}; y_0].
+"""
+from __future__ import annotations
+
+import warnings
+
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_handle
+
+F64 = torch.float64
+
+
+def _dev_blocks(x, nb, m, device):
+    """Stack a list of m x m blocks (numpy / torch) into a contiguous (nb, m, m)
+    float64 device tensor."""
+    if isinstance(x, torch.Tensor) and x.dim() == 3:
+        t = x.to(device=device, dtype=F64)
+    else:
+        t = torch.stack([torch.as_tensor(b, dtype=F64) for b in x]).to(device)
+    t = t.contiguous()
+    if tuple(t.shape) != (nb, m, m):
+        raise ValueError(f"expected {nb} blocks of {m}x{m}, got {tuple(t.shape)}")
+    return t
+
+
+class BlockTriFactor:
+    """Factorization of one block-tridiagonal system on the GPU."""
+
+    def __init__(self, method: str = "rb", device=None):
+        if method not in ("rb", "thomas"):
+            raise ValueError(f"unknown method {method!r}")
+        _lib.require_gpu()
+        _lib.load()
+        self.method = method
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.nb = self.m = 0
+        self.factor = None
+        self.info = None
+        self.cyclic = False
+        self.Z = None
+        self.K = None
+
+    # -- factorization ---------------------------------------------------------
+    def factorize(self, T, SiM, SiP, E=None, F=None, stream=None):
+        """T, SiM, SiP: (nb, m, m) blocks (SiM[0] and SiP[nb-1] are ignored).
+        E, F: optional corner blocks for a periodic system."""
+        nb = len(T)
+        m = T[0].shape[0]
+        dev = self.device
+        with torch.cuda.device(dev):
+            Td = _dev_blocks(T, nb, m, dev)
+            Md = _dev_blocks(SiM, nb, m, dev)
+            Pd = _dev_blocks(SiP, nb, m, dev)
+            nfac, nwork, niwork = _lib.sizes(self.method, nb, m)
+            self.factor = torch.empty(nfac, dtype=F64, device=dev)
+            work = torch.empty(max(nwork, 1), dtype=F64, device=dev)
+            iwork = torch.empty(max(niwork, 1), dtype=torch.int32, device=dev)
+            ninfo = _lib.load().sslab_rb_info_count(nb) if self.method == "rb" else nb
+            info = torch.zeros(ninfo, dtype=torch.int32, device=dev)
+            s = stream_handle(stream)
+            call(f"sslab_{self.method}_factor", nb, m, ptr(Td), ptr(Md), ptr(Pd),
+                 ptr(self.factor), ptr(work), ptr(iwork), ptr(info), s)
+            self.nb, self.m = nb, m
+            if self.method == "thomas":
+                # keep the pivots of LU(B_i) for the reference's `B` attribute
+                self.ipiv = iwork[: nb * m].view(nb, m).clone()
+            self.info = info
+            bad = int((info != 0).sum().item())
+            if bad:
+                warnings.warn(f"{bad} exactly singular pivot block(s) in the block-tridiagonal "
+                              "factorization; results will contain inf/nan", RuntimeWarning)
+            self.cyclic = E is not None or F is not None
+            if self.cyclic:
+                self._factor_woodbury(E, F, stream)
+        return self
+
+    def _factor_woodbury(self, E, F, stream):
+        nb, m, dev = self.nb, self.m, self.device
+        if nb < 2:
+            raise ValueError("a cyclic block system needs at least two blocks")
+        Ed = torch.as_tensor(E, dtype=F64).to(dev) if E is not None else torch.zeros(m, m, dtype=F64, device=dev)
+        Fd = torch.as_tensor(F, dtype=F64).to(dev) if F is not None else torch.zeros(m, m, dtype=F64, device=dev)
+        U = torch.zeros(nb * m, 2 * m, dtype=F64, device=dev)
+        U[0:m, 0:m] = Ed
+        U[(nb - 1) * m:, m:] = Fd
+        self.cyclic = False  # plain solve for Z
+        Z = self.solve(U, stream=stream)
+        self.cyclic = True
+        # G = I + V^T Z with V^T Z = [Z_{nb-1}; Z_0]
+        G = torch.eye(2 * m, dtype=F64, device=dev)
+        G[0:m] += Z[(nb - 1) * m:]
+        G[m:] += Z[0:m]
+        K = torch.empty_like(G)
+        ipiv = torch.empty(2 * m, dtype=torch.int32, device=dev)
+        perm = torch.empty(2 * m, dtype=torch.int32, device=dev)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        eye = torch.eye(2 * m, dtype=F64, device=dev)
+        s = stream_handle(stream)
+        call("sslab_dgetrf_batched", 2 * m, ptr(G), 2 * m, 0, ptr(ipiv), 0, ptr(info), 1, s)
+        call("sslab_dgetrs_batched", 2 * m, 2 * m, ptr(G), 2 * m, 0, ptr(ipiv), 0, ptr(eye),
+             2 * m, 0, ptr(K), 2 * m, 0, ptr(perm), 1, s)
+        self.Z, self.K = Z, K
+
+    # -- solves -----------------------------------------------------------------
+    def solve_(self, V: torch.Tensor, X: torch.Tensor, stream=None):
+        """Solve in place of preallocated device buffers: V (N, r) holds the
+        right-hand sides and is overwritten, X (N, r) receives the solution.
+        Both float64, row-major with unit column stride."""
+        nb, m = self.nb, self.m
+        N = nb * m
+        if V.shape[0] != N or X.shape[0] != N or V.dim() != 2 or X.shape[1] != V.shape[1]:
+            raise ValueError("solve_: V and X must be (nb*m, r)")
+        if V.stride(1) != 1 or X.stride(1) != 1 or V.dtype != F64 or X.dtype != F64:
+            raise ValueError("solve_: float64 buffers with unit column stride required")
+        r = V.shape[1]
+        s = stream_handle(stream)
+        call(f"sslab_{self.method}_solve", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
+             ptr(X), X.stride(0), s)
+        if self.cyclic:
+            # t = K [x_{nb-1}; x_0];  x -= Z t
+            t = torch.empty(2 * m, r, dtype=F64, device=X.device)
+            K = self.K
+            call("sslab_dgemm_strided_batched", 0, 0, 2 * m, r, m, 1.0, ptr(K), 2 * m, 0,
+                 ptr(X[(nb - 1) * m:]), X.stride(0), 0, 0.0, ptr(t), r, 0, 1, s)
+            call("sslab_dgemm_strided_batched", 0, 0, 2 * m, r, m, 1.0, ptr(K[:, m:]), 2 * m, 0,
+                 ptr(X), X.stride(0), 0, 1.0, ptr(t), r, 0, 1, s)
+            call("sslab_dgemm_strided_batched", 0, 0, N, r, 2 * m, -1.0, ptr(self.Z), 2 * m, 0,
+                 ptr(t), r, 0, 1.0, ptr(X), X.stride(0), 0, 1, s)
+        return X
+
+    def solve(self, B: torch.Tensor, stream=None) -> torch.Tensor:
+        """Return X with X = S^{-1} B for a device tensor B of shape (N,) or (N, r)."""
+        one_d = B.dim() == 1
+        V = B.reshape(B.shape[0], -1).to(device=self.device, dtype=F64).clone(memory_format=torch.contiguous_format)
+        X = torch.empty_like(V)
+        self.solve_(V, X, stream)
+        return X[:, 0] if one_d else X
+
+    # -- introspection ----------------------------------------------------------
+    @property
+    def nbytes(self) -> int:
+        n = self.factor.numel() * 8 if self.factor is not None else 0
+        if self.Z is not None:
+            n += (self.Z.numel() + self.K.numel()) * 8
+        return n

@@ -1,0 +1,378 @@
+// Block-tridiagonal S-system factor / solve on gfx950 (FP64).
+//
+//   T_i x_i + SiM_i x_{i-1} + SiP_i x_{i+1} = v_i ,  i = 0..nb-1   (non-cyclic)
+//
+// Red-black (cyclic) reduction, restated for a GPU:  at every level the odd
+// nodes are eliminated and the even nodes form the next, half-size level, as in
+// direct_solve/omsdirectsolve.py:283-333 (RedBlackSolver.build_block_RB_solver_level),
+// generalised to any block count (an even node at the right end simply has no
+// right neighbour) so no identity padding is needed.  Each level keeps only what
+// the solve needs, with explicit inverses so every solve step is a GEMM:
+//
+//   odd  j = 2c+1 :  TG_c = [ Tinv_j | Tinv_j SiM_j | Tinv_j SiP_j ]      (m x 3m)
+//   even i = 2b   :  WW_b = [ SiM_i Tinv_{i-1} | SiP_i Tinv_{i+1} ]        (m x 2m)
+//   next level    :  T'_b   = T_i - W^L_b SiP_{i-1} - W^R_b SiM_{i+1}
+//                    SiM'_b = -W^L_b SiM_{i-1},   SiP'_b = -W^R_b SiP_{i+1}
+//
+// Solve, per level, forward (one launch):  v_i -= W^L_b v_{i-1} + W^R_b v_{i+1}
+// for even i and y_j = Tinv_j v_j for odd j;  backward (one launch):
+// x_j = y_j - (Tinv SiM)_j x_{j-1} - (Tinv SiP)_j x_{j+1}.  Level-l node k lives
+// in global block slot k << l, so every level works in place on V and X.
+//
+// Thomas (direct_solve/omsdirectsolve.py:48-141) is kept as the faithful
+// sequential variant: A_i = SiM_i B_{i-1}^{-1}, B_i = T_i - A_i SiP_{i-1}.
+#include "blocktri_f64.hpp"
+
+#include <vector>
+
+#include "gemm_f64.hpp"
+#include "lu_f64.hpp"
+
+namespace sslab {
+
+namespace {
+
+struct RbLevel {
+  int n, nE, nO;
+  long long tg_off;  // factor offset (doubles) of TG blocks (nO x m x 3m)
+  long long ww_off;  // factor offset of WW blocks (nE x m x 2m)
+};
+
+struct RbPlan {
+  int nb, m, L;
+  std::vector<RbLevel> lv;
+  long long top_off;       // Tinv of the coarsest block (m x m)
+  long long factor_doubles;
+  // workspace (doubles): two ping-pong level sets (T', SiM', SiP'), LU blocks,
+  // the stacked [I | SiM | SiP] right-hand sides.
+  long long set_blocks;    // blocks per array in one ping-pong set
+  long long work_doubles;
+  long long work_ints;     // ipiv + perm + info
+  int max_odd;
+};
+
+RbPlan rb_plan(int nb, int m) {
+  RbPlan p{};
+  p.nb = nb;
+  p.m = m;
+  const long long m2 = (long long)m * m;
+  long long off = 0;
+  int n = nb;
+  p.max_odd = 1;
+  while (n > 1) {
+    RbLevel l{};
+    l.n = n;
+    l.nE = (n + 1) / 2;
+    l.nO = n / 2;
+    l.tg_off = off;
+    off += (long long)l.nO * 3 * m2;
+    l.ww_off = off;
+    off += (long long)l.nE * 2 * m2;
+    if (l.nO > p.max_odd) p.max_odd = l.nO;
+    p.lv.push_back(l);
+    n = l.nE;
+  }
+  p.L = (int)p.lv.size();
+  p.top_off = off;
+  off += m2;
+  p.factor_doubles = off;
+  p.set_blocks = (nb + 1) / 2;
+  p.work_doubles = 2 * 3 * p.set_blocks * m2 + (long long)p.max_odd * m2 +
+                   (long long)p.max_odd * 3 * m2;
+  p.work_ints = (long long)p.max_odd * m * 2 + p.max_odd;
+  return p;
+}
+
+}  // namespace
+
+int rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
+             long long* work_ints) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1, "rb_sizes: nb and m must be positive");
+  RbPlan p = rb_plan(nb, m);
+  if (factor_doubles) *factor_doubles = p.factor_doubles;
+  if (work_doubles) *work_doubles = p.work_doubles;
+  if (work_ints) *work_ints = p.work_ints;
+  return kOk;
+}
+
+int rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
+              double* factor, double* work, int* iwork, int* info, hipStream_t st) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1, "rb_factor: nb and m must be positive");
+  SSLAB_REQUIRE(T && factor && work && iwork, "rb_factor: null pointer");
+  SSLAB_REQUIRE(nb == 1 || (SiM && SiP), "rb_factor: null off-diagonal blocks");
+  const RbPlan p = rb_plan(nb, m);
+  const long long m2 = (long long)m * m;
+  const long long setsz = 3 * p.set_blocks * m2;
+  double* sets[2] = {work, work + setsz};
+  double* LUw = work + 2 * setsz;
+  double* Rs = LUw + (long long)p.max_odd * m2;
+  int* ipiv = iwork;
+  int* perm = iwork + (long long)p.max_odd * m;
+  int* linfo = perm + (long long)p.max_odd * m;
+  // info layout: one int per odd node per level, then one for the top block.
+
+  const double* cT = T;
+  const double* cM = SiM;
+  const double* cP = SiP;
+  int info_base = 0;
+  for (int l = 0; l < p.L; ++l) {
+    const RbLevel& v = p.lv[l];
+    const int n = v.n, nE = v.nE, nO = v.nO;
+    double* TG = factor + v.tg_off;
+    double* WW = factor + v.ww_off;
+    double* nT = sets[(l + 1) & 1];
+    double* nM = nT + p.set_blocks * m2;
+    double* nP = nM + p.set_blocks * m2;
+
+    // 1-2. LU of the odd diagonal blocks.
+    SSLAB_TRY(copy_block_batched(m, m, cT + m2, m, 2 * m2, LUw, m, m2, nO, st));
+    SSLAB_TRY(getrf_batched(m, LUw, m, m2, ipiv, m, linfo, nO, st));
+    if (info) {
+      SSLAB_HIP_CHECK(hipMemcpyAsync(info + info_base, linfo, sizeof(int) * nO,
+                                     hipMemcpyDeviceToDevice, st));
+    }
+    info_base += nO;
+    // 3. Right-hand sides [I | SiM_j | SiP_j] for odd j (SiP of a last odd node
+    //    without a right neighbour is zero).
+    SSLAB_TRY(set_identity_batched(m, Rs, 3 * m, 3 * m2, nO, st));
+    SSLAB_TRY(copy_block_batched(m, m, cM + m2, m, 2 * m2, Rs + m, 3 * m, 3 * m2, nO, st));
+    const int nOr = (n % 2 == 0) ? nO - 1 : nO;  // odd nodes with a right neighbour
+    SSLAB_TRY(copy_block_batched(m, m, cP + m2, m, 2 * m2, Rs + 2 * m, 3 * m, 3 * m2, nOr, st));
+    if (nOr < nO)
+      SSLAB_TRY(zero_block_batched(m, m, Rs + (long long)nOr * 3 * m2 + 2 * m, 3 * m, 3 * m2, 1,
+                                   st));
+    // 4. TG = T_odd^{-1} [I | SiM | SiP].
+    SSLAB_TRY(getrs_batched(m, 3 * m, LUw, m, m2, ipiv, m, Rs, 3 * m, 3 * m2, TG, 3 * m, 3 * m2,
+                            perm, nO, st));
+    // 5. W^L_b = SiM_{2b} Tinv_{b-1} (b >= 1),  W^R_b = SiP_{2b} Tinv_b (b < nO).
+    {
+      GemmLaunch G{};
+      G.nd = 2;
+      G.d[0] = gemm_desc(m, m, nE - 1, WW + 2 * m2, 2 * m, 2 * m2, nullptr, 0, 0, 0.0);
+      gemm_add_term(G.d[0], gemm_term(cM + 2 * m2, m, 2 * m2, 0, TG, 3 * m, 3 * m2, 0, m, 1.0));
+      G.d[1] = gemm_desc(m, m, nO, WW + m, 2 * m, 2 * m2, nullptr, 0, 0, 0.0);
+      gemm_add_term(G.d[1], gemm_term(cP, m, 2 * m2, 0, TG, 3 * m, 3 * m2, 0, m, 1.0));
+      SSLAB_TRY(gemm_launch(G, st));
+    }
+    // 6. Next level: T', SiM', SiP'.
+    {
+      GemmLaunch G{};
+      int k = 0;
+      const int both = (nE < nO ? nE : nO) - 1;  // b in [1, min(nE,nO))
+      if (both > 0) {
+        GemmDesc d = gemm_desc(m, m, both, nT + m2, m, m2, cT + 2 * m2, m, 2 * m2, 1.0);
+        gemm_add_term(d, gemm_term(WW + 2 * m2, 2 * m, 2 * m2, 0, cP + m2, m, 2 * m2, 0, m, -1.0));
+        gemm_add_term(d, gemm_term(WW + 2 * m2 + m, 2 * m, 2 * m2, 0, cM + 3 * m2, m, 2 * m2, 0, m,
+                                   -1.0));
+        G.d[k++] = d;
+      }
+      {  // b = 0: right neighbour only (n >= 2 so node 1 exists)
+        GemmDesc d = gemm_desc(m, m, 1, nT, m, m2, cT, m, m2, 1.0);
+        gemm_add_term(d, gemm_term(WW + m, 2 * m, 2 * m2, 0, cM + m2, m, m2, 0, m, -1.0));
+        G.d[k++] = d;
+      }
+      if (nE > nO && nE - 1 >= 1) {  // n odd: last even node has a left neighbour only
+        const long long b = nE - 1;
+        GemmDesc d = gemm_desc(m, m, 1, nT + b * m2, m, m2, cT + 2 * b * m2, m, m2, 1.0);
+        gemm_add_term(d, gemm_term(WW + b * 2 * m2, 2 * m, 2 * m2, 0, cP + (2 * b - 1) * m2, m, m2,
+                                   0, m, -1.0));
+        G.d[k++] = d;
+      }
+      if (nE - 1 > 0) {  // SiM'_b = -W^L_b SiM_{2b-1}, b in [1, nE)
+        GemmDesc d = gemm_desc(m, m, nE - 1, nM + m2, m, m2, nullptr, 0, 0, 0.0);
+        gemm_add_term(d, gemm_term(WW + 2 * m2, 2 * m, 2 * m2, 0, cM + m2, m, 2 * m2, 0, m, -1.0));
+        G.d[k++] = d;
+      }
+      if (nE - 1 > 0) {  // SiP'_b = -W^R_b SiP_{2b+1}, b in [0, nE-1)
+        GemmDesc d = gemm_desc(m, m, nE - 1, nP, m, m2, nullptr, 0, 0, 0.0);
+        gemm_add_term(d, gemm_term(WW + m, 2 * m, 2 * m2, 0, cP + m2, m, 2 * m2, 0, m, -1.0));
+        G.d[k++] = d;
+      }
+      G.nd = k;
+      SSLAB_TRY(gemm_launch(G, st));
+    }
+    cT = nT;
+    cM = nM;
+    cP = nP;
+  }
+  // Coarsest block: Tinv_top.
+  SSLAB_TRY(copy_block_batched(m, m, cT, m, m2, LUw, m, m2, 1, st));
+  SSLAB_TRY(getrf_batched(m, LUw, m, m2, ipiv, m, linfo, 1, st));
+  if (info) {
+    SSLAB_HIP_CHECK(
+        hipMemcpyAsync(info + info_base, linfo, sizeof(int), hipMemcpyDeviceToDevice, st));
+  }
+  SSLAB_TRY(set_identity_batched(m, Rs, m, m2, 1, st));
+  SSLAB_TRY(getrs_batched(m, m, LUw, m, m2, ipiv, m, Rs, m, m2, factor + p.top_off, m, m2, perm, 1,
+                          st));
+  return kOk;
+}
+
+int rb_info_count(int nb) {
+  int n = nb, c = 0;
+  while (n > 1) {
+    c += n / 2;
+    n = (n + 1) / 2;
+  }
+  return c + 1;
+}
+
+int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, double* X, int ldx,
+             hipStream_t st) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1 && r >= 0, "rb_solve: bad sizes");
+  SSLAB_REQUIRE(ldv >= r && ldx >= r, "rb_solve: leading dimension < nrhs");
+  SSLAB_REQUIRE(factor && V && X && V != X, "rb_solve: null or aliased buffers");
+  if (r == 0) return kOk;
+  const RbPlan p = rb_plan(nb, m);
+  const long long m2 = (long long)m * m;
+  const long long bv = (long long)m * ldv;  // one block row of V
+  const long long bx = (long long)m * ldx;
+  for (int l = 0; l < p.L; ++l) {
+    const RbLevel& v = p.lv[l];
+    const int n = v.n, nE = v.nE, nO = v.nO;
+    const double* TG = factor + v.tg_off;
+    const double* WW = factor + v.ww_off;
+    const long long sv = (2LL << l) * bv, sx = (2LL << l) * bx;  // stride between same-parity nodes
+    const long long ov = (1LL << l) * bv, ox = (1LL << l) * bx;  // offset of the first odd node
+    GemmLaunch G{};
+    int k = 0;
+    const int both = (nE < nO ? nE : nO) - 1;
+    if (both > 0) {  // even b in [1, min(nE,nO))
+      GemmDesc d = gemm_desc(m, r, both, V + sv, ldv, sv, V + sv, ldv, sv, 1.0);
+      gemm_add_term(d, gemm_term(WW + 2 * m2, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
+      gemm_add_term(d, gemm_term(WW + 2 * m2 + m, 2 * m, 2 * m2, 0, V + sv + ov, ldv, sv, 0, m,
+                                 -1.0));
+      G.d[k++] = d;
+    }
+    {  // b = 0
+      GemmDesc d = gemm_desc(m, r, 1, V, ldv, sv, V, ldv, sv, 1.0);
+      gemm_add_term(d, gemm_term(WW + m, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
+      G.d[k++] = d;
+    }
+    if (nE > nO && nE - 1 >= 1) {
+      const long long b = nE - 1;
+      GemmDesc d = gemm_desc(m, r, 1, V + b * sv, ldv, sv, V + b * sv, ldv, sv, 1.0);
+      gemm_add_term(d, gemm_term(WW + b * 2 * m2, 2 * m, 2 * m2, 0, V + b * sv - ov, ldv, sv, 0, m,
+                                 -1.0));
+      G.d[k++] = d;
+    }
+    {  // odd: y_j = Tinv_j v_j into X
+      GemmDesc d = gemm_desc(m, r, nO, X + ox, ldx, sx, nullptr, 0, 0, 0.0);
+      gemm_add_term(d, gemm_term(TG, 3 * m, 3 * m2, 0, V + ov, ldv, sv, 0, m, 1.0));
+      G.d[k++] = d;
+    }
+    G.nd = k;
+    SSLAB_TRY(gemm_launch(G, st));
+  }
+  SSLAB_TRY(gemm_strided(0, 0, m, r, m, 1.0, factor + p.top_off, m, m2, V, ldv, 0, 0.0, X, ldx, 0,
+                         1, st));
+  for (int l = p.L - 1; l >= 0; --l) {
+    const RbLevel& v = p.lv[l];
+    const int n = v.n, nE = v.nE, nO = v.nO;
+    const double* TG = factor + v.tg_off;
+    const long long sx = (2LL << l) * bx;
+    const long long ox = (1LL << l) * bx;
+    GemmLaunch G{};
+    int k = 0;
+    const int nboth = (n % 2 == 0) ? nO - 1 : nO;  // odd nodes with both neighbours
+    (void)nE;
+    if (nboth > 0) {
+      GemmDesc d = gemm_desc(m, r, nboth, X + ox, ldx, sx, X + ox, ldx, sx, 1.0);
+      gemm_add_term(d, gemm_term(TG + m, 3 * m, 3 * m2, 0, X, ldx, sx, 0, m, -1.0));
+      gemm_add_term(d, gemm_term(TG + 2 * m, 3 * m, 3 * m2, 0, X + sx, ldx, sx, 0, m, -1.0));
+      G.d[k++] = d;
+    }
+    if (nboth < nO) {  // last odd node: left neighbour only
+      const long long c = nO - 1;
+      GemmDesc d = gemm_desc(m, r, 1, X + ox + c * sx, ldx, sx, X + ox + c * sx, ldx, sx, 1.0);
+      gemm_add_term(d, gemm_term(TG + c * 3 * m2 + m, 3 * m, 3 * m2, 0, X + c * sx, ldx, sx, 0, m,
+                                 -1.0));
+      G.d[k++] = d;
+    }
+    G.nd = k;
+    SSLAB_TRY(gemm_launch(G, st));
+  }
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Thomas (sequential) variant.
+// factor layout: A (nb-1 blocks) | Binv (nb) | LU (nb) | C (nb-1);  ipiv: nb*m ints.
+
+int thomas_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
+                 long long* work_ints) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1, "thomas_sizes: nb and m must be positive");
+  const long long m2 = (long long)m * m;
+  if (factor_doubles) *factor_doubles = (4LL * nb - 2) * m2;
+  if (work_doubles) *work_doubles = m2;
+  if (work_ints) *work_ints = (long long)nb * m + m;
+  return kOk;
+}
+
+int thomas_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
+                  double* factor, double* work, int* iwork, int* info, hipStream_t st) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1, "thomas_factor: nb and m must be positive");
+  SSLAB_REQUIRE(T && factor && work && iwork, "thomas_factor: null pointer");
+  const long long m2 = (long long)m * m;
+  double* A = factor;
+  double* Binv = A + (long long)(nb - 1) * m2;
+  double* LU = Binv + (long long)nb * m2;
+  double* C = LU + (long long)nb * m2;
+  int* ipiv = iwork;
+  int* perm = iwork + (long long)nb * m;
+  if (nb > 1) SSLAB_TRY(copy_block_batched(m, m, SiP, m, m2, C, m, m2, nb - 1, st));
+  SSLAB_TRY(copy_block_batched(m, m, T, m, m2, LU, m, m2, 1, st));
+  for (int i = 0; i < nb; ++i) {
+    double* LUi = LU + (long long)i * m2;
+    if (i > 0) {
+      // B_i = T_i - A_i C_{i-1}, with A_i = SiM_i Binv_{i-1}.
+      double* Ai = A + (long long)(i - 1) * m2;
+      SSLAB_TRY(gemm_strided(0, 0, m, m, m, 1.0, SiM + (long long)i * m2, m, 0,
+                             Binv + (long long)(i - 1) * m2, m, 0, 0.0, Ai, m, 0, 1, st));
+      GemmLaunch G{};
+      G.nd = 1;
+      G.d[0] = gemm_desc(m, m, 1, LUi, m, 0, T + (long long)i * m2, m, 0, 1.0);
+      gemm_add_term(G.d[0], gemm_term(Ai, m, 0, 0, C + (long long)(i - 1) * m2, m, 0, 0, m, -1.0));
+      SSLAB_TRY(gemm_launch(G, st));
+    }
+    SSLAB_TRY(getrf_batched(m, LUi, m, m2, ipiv + (long long)i * m, m, info ? info + i : nullptr,
+                            1, st));
+    SSLAB_TRY(set_identity_batched(m, work, m, m2, 1, st));
+    SSLAB_TRY(getrs_batched(m, m, LUi, m, m2, ipiv + (long long)i * m, m, work, m, m2,
+                            Binv + (long long)i * m2, m, m2, perm, 1, st));
+  }
+  return kOk;
+}
+
+int thomas_solve(int nb, int m, int r, const double* factor, double* V, int ldv, double* X,
+                 int ldx, hipStream_t st) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1 && r >= 0, "thomas_solve: bad sizes");
+  SSLAB_REQUIRE(ldv >= r && ldx >= r, "thomas_solve: leading dimension < nrhs");
+  SSLAB_REQUIRE(factor && V && X && V != X, "thomas_solve: null or aliased buffers");
+  if (r == 0) return kOk;
+  const long long m2 = (long long)m * m;
+  const double* A = factor;
+  const double* Binv = A + (long long)(nb - 1) * m2;
+  const double* C = Binv + 2LL * nb * m2;
+  const long long bv = (long long)m * ldv, bx = (long long)m * ldx;
+  // forward: d_i -= A_i d_{i-1}
+  for (int i = 1; i < nb; ++i)
+    SSLAB_TRY(gemm_strided(0, 0, m, r, m, -1.0, A + (long long)(i - 1) * m2, m, 0,
+                           V + (long long)(i - 1) * bv, ldv, 0, 1.0, V + (long long)i * bv, ldv, 0,
+                           1, st));
+  // backward: x_{n-1} = Binv d_{n-1};  x_i = Binv_i (d_i - C_i x_{i+1})
+  SSLAB_TRY(gemm_strided(0, 0, m, r, m, 1.0, Binv + (long long)(nb - 1) * m2, m, 0,
+                         V + (long long)(nb - 1) * bv, ldv, 0, 0.0, X + (long long)(nb - 1) * bx,
+                         ldx, 0, 1, st));
+  for (int i = nb - 2; i >= 0; --i) {
+    SSLAB_TRY(gemm_strided(0, 0, m, r, m, -1.0, C + (long long)i * m2, m, 0,
+                           X + (long long)(i + 1) * bx, ldx, 0, 1.0, V + (long long)i * bv, ldv, 0,
+                           1, st));
+    SSLAB_TRY(gemm_strided(0, 0, m, r, m, 1.0, Binv + (long long)i * m2, m, 0,
+                           V + (long long)i * bv, ldv, 0, 0.0, X + (long long)i * bx, ldx, 0, 1,
+                           st));
+  }
+  return kOk;
+}
+
+}  // namespace sslab

@@ -1,0 +1,100 @@
+// extern "C" entry points of libsslab_hip.so (declared in include/sslab_hip.h).
+#include <cstdarg>
+
+#include "../../include/sslab_hip.h"
+#include "blocktri_f64.hpp"
+#include "gemm_f64.hpp"
+#include "lu_f64.hpp"
+
+namespace sslab {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+const char* get_error() { return g_err; }
+
+}  // namespace sslab
+
+using sslab::as_stream;
+
+extern "C" {
+
+int sslab_abi_version(void) { return 1; }
+
+const char* sslab_last_error(void) { return sslab::get_error(); }
+
+int sslab_dgemm_strided_batched(int transA, int transB, int M, int N, int K, double alpha,
+                                const double* A, int lda, long long strideA, const double* B,
+                                int ldb, long long strideB, double beta, double* C, int ldc,
+                                long long strideC, int batch, void* stream) {
+  SSLAB_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "dgemm: negative size");
+  SSLAB_REQUIRE(C != nullptr, "dgemm: null C");
+  if (M == 0 || N == 0 || batch == 0) return 0;
+  SSLAB_REQUIRE(K == 0 || (A != nullptr && B != nullptr), "dgemm: null operand");
+  SSLAB_REQUIRE(ldc >= N, "dgemm: ldc < N");
+  SSLAB_REQUIRE(lda >= (transA ? M : K) && ldb >= (transB ? K : N), "dgemm: bad lda/ldb");
+  if (K == 0) {
+    // C = beta*C: express as a product with K = 0 and the input term only.
+    sslab::GemmLaunch L{};
+    L.nd = 1;
+    L.d[0] = sslab::gemm_desc(M, N, batch, C, ldc, strideC, beta != 0.0 ? C : nullptr, ldc,
+                              strideC, beta);
+    sslab::gemm_add_term(L.d[0], sslab::gemm_term(C, ldc, 0, 0, C, ldc, 0, 0, 0, 0.0));
+    return sslab::gemm_launch(L, as_stream(stream));
+  }
+  return sslab::gemm_strided(transA, transB, M, N, K, alpha, A, lda, strideA, B, ldb, strideB,
+                             beta, C, ldc, strideC, batch, as_stream(stream));
+}
+
+int sslab_dgetrf_batched(int n, double* A, int lda, long long strideA, int* ipiv,
+                         long long strideP, int* info, int batch, void* stream) {
+  return sslab::getrf_batched(n, A, lda, strideA, ipiv, strideP, info, batch, as_stream(stream));
+}
+
+int sslab_dgetrs_batched(int n, int nrhs, const double* LU, int lda, long long strideA,
+                         const int* ipiv, long long strideP, const double* B, int ldb,
+                         long long strideB, double* X, int ldx, long long strideX,
+                         int* perm_work, int batch, void* stream) {
+  return sslab::getrs_batched(n, nrhs, LU, lda, strideA, ipiv, strideP, B, ldb, strideB, X, ldx,
+                              strideX, perm_work, batch, as_stream(stream));
+}
+
+int sslab_rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
+                   long long* work_ints) {
+  return sslab::rb_sizes(nb, m, factor_doubles, work_doubles, work_ints);
+}
+
+int sslab_rb_info_count(int nb) { return nb >= 1 ? sslab::rb_info_count(nb) : 0; }
+
+int sslab_rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
+                    double* factor, double* work, int* iwork, int* info, void* stream) {
+  return sslab::rb_factor(nb, m, T, SiM, SiP, factor, work, iwork, info, as_stream(stream));
+}
+
+int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv, double* X,
+                   int ldx, void* stream) {
+  return sslab::rb_solve(nb, m, nrhs, factor, V, ldv, X, ldx, as_stream(stream));
+}
+
+int sslab_thomas_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
+                       long long* work_ints) {
+  return sslab::thomas_sizes(nb, m, factor_doubles, work_doubles, work_ints);
+}
+
+int sslab_thomas_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
+                        double* factor, double* work, int* iwork, int* info, void* stream) {
+  return sslab::thomas_factor(nb, m, T, SiM, SiP, factor, work, iwork, info, as_stream(stream));
+}
+
+int sslab_thomas_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
+                       double* X, int ldx, void* stream) {
+  return sslab::thomas_solve(nb, m, nrhs, factor, V, ldv, X, ldx, as_stream(stream));
+}
+
+}  // extern "C"

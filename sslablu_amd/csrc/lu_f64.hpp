@@ -1,0 +1,36 @@
+// Batched FP64 LU with partial pivoting and the solves built on it (gfx950).
+//
+// Conventions follow LAPACK getrf/getrs as exposed by scipy.linalg.lu_factor:
+// row-major storage here, the packed factor holds unit-lower L below the diagonal
+// and U on and above it, ipiv[i] (0-based) is the row swapped with row i at step i,
+// and info = i+1 for the first exactly-zero pivot U[i][i] (0 when none).
+#pragma once
+
+#include "sslab_common.hpp"
+
+namespace sslab {
+
+// In-place blocked right-looking LU of `batch` n x n matrices A_b = A + b*sA.
+// ipiv: batch x n ints (stride sP); info: batch ints (may be null).
+int getrf_batched(int n, double* A, int lda, long long sA, int* ipiv, long long sP,
+                  int* info, int batch, hipStream_t stream);
+
+// Solve A_b X_b = B_b with the factors from getrf_batched.  B (n x nrhs, row-major,
+// leading dimension ldb, stride sB) is read; X (same shape, ldx, sX) receives the
+// solution.  X may equal B.  perm: batch x n int scratch (stride n).
+int getrs_batched(int n, int nrhs, const double* LU, int lda, long long sA, const int* ipiv,
+                  long long sP, const double* B, int ldb, long long sB, double* X, int ldx,
+                  long long sX, int* perm, int batch, hipStream_t stream);
+
+// X_b = I (n x n, leading dimension ldx) for every batch entry.
+int set_identity_batched(int n, double* X, int ldx, long long sX, int batch, hipStream_t stream);
+
+// Y_b[r, c] = X_b[r, c] for an m x n block (row-major, leading dims), batched.
+int copy_block_batched(int m, int n, const double* X, int ldx, long long sX, double* Y, int ldy,
+                       long long sY, int batch, hipStream_t stream);
+
+// Fill an m x n block with zeros, batched (a kernel, never hipMemset).
+int zero_block_batched(int m, int n, double* Y, int ldy, long long sY, int batch,
+                       hipStream_t stream);
+
+}  // namespace sslab

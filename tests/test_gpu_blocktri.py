@@ -1,0 +1,186 @@
+"""GPU parity: dense primitives and block-tridiagonal solvers vs the oracle.
+
+Tolerance (plain FP64 GPU path): relative error <= 1e-10 against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import blocktri as ob
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+@pytest.fixture(scope="module")
+def torch_dev(gpu):
+    import torch
+    from sslablu_amd import _lib
+    _lib.load()
+    return torch, gpu
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [
+    (64, 64, 64, 0, 0), (100, 37, 53, 0, 0), (512, 64, 512, 0, 0), (513, 129, 300, 0, 0),
+    (77, 91, 45, 1, 0), (77, 91, 45, 0, 1), (130, 33, 17, 1, 1), (1, 1, 1, 0, 0),
+    (256, 256, 256, 0, 0), (1024, 1024, 64, 0, 0)])
+def test_dgemm(torch_dev, M, N, K, ta, tb):
+    torch, dev = torch_dev
+    from sslablu_amd.ops import dgemm
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = rng.standard_normal((K, M) if ta else (M, K))
+    B = rng.standard_normal((N, K) if tb else (K, N))
+    C0 = rng.standard_normal((M, N))
+    At = torch.tensor(A, device=dev)
+    Bt = torch.tensor(B, device=dev)
+    Ct = torch.tensor(C0, device=dev)
+    dgemm(At.t() if ta else At, Bt.t() if tb else Bt, Ct, alpha=-0.5, beta=2.0)
+    ref = -0.5 * ((A.T if ta else A) @ (B.T if tb else B)) + 2.0 * C0
+    assert rel(Ct.cpu().numpy(), ref) < 1e-13
+
+
+def test_dgemm_strided_batched(torch_dev):
+    torch, dev = torch_dev
+    from sslablu_amd._lib import call, ptr, stream_handle
+    rng = np.random.default_rng(5)
+    b, M, N, K = 7, 96, 48, 80
+    A = rng.standard_normal((b, M, K))
+    B = rng.standard_normal((b, K, N))
+    At, Bt = torch.tensor(A, device=dev), torch.tensor(B, device=dev)
+    Ct = torch.zeros(b, M, N, dtype=torch.float64, device=dev)
+    call("sslab_dgemm_strided_batched", 0, 0, M, N, K, 1.0, ptr(At), K, M * K, ptr(Bt), N, K * N,
+         0.0, ptr(Ct), N, M * N, b, stream_handle())
+    assert rel(Ct.cpu().numpy(), A @ B) < 1e-13
+
+
+@pytest.mark.parametrize("n,batch", [(1, 1), (5, 3), (33, 2), (64, 4), (200, 3), (512, 2),
+                                     (1000, 1)])
+def test_getrf_getrs(torch_dev, n, batch):
+    torch, dev = torch_dev
+    from sslablu_amd._lib import call, ptr, stream_handle
+    rng = np.random.default_rng(n + batch)
+    A = rng.standard_normal((batch, n, n))
+    B = rng.standard_normal((batch, n, 7))
+    At = torch.tensor(A, device=dev)
+    piv = torch.empty(batch, n, dtype=torch.int32, device=dev)
+    info = torch.zeros(batch, dtype=torch.int32, device=dev)
+    s = stream_handle()
+    call("sslab_dgetrf_batched", n, ptr(At), n, n * n, ptr(piv), n, ptr(info), batch, s)
+    Bt = torch.tensor(B, device=dev)
+    Xt = torch.empty_like(Bt)
+    perm = torch.empty(batch, n, dtype=torch.int32, device=dev)
+    call("sslab_dgetrs_batched", n, 7, ptr(At), n, n * n, ptr(piv), n, ptr(Bt), 7, n * 7,
+         ptr(Xt), 7, n * 7, ptr(perm), batch, s)
+    X = Xt.cpu().numpy()
+    assert int(info.abs().sum()) == 0
+    for k in range(batch):
+        lu, p = sla.lu_factor(A[k])
+        x_ref = np.linalg.solve(A[k], B[k])
+        assert rel(X[k], x_ref) < 1e-10
+        # same packed factor as LAPACK (pivot choices are well separated here)
+        assert np.array_equal(piv[k].cpu().numpy(), p)
+        assert rel(At[k].cpu().numpy(), lu) < 1e-12
+
+
+def test_getrf_reports_singular(torch_dev):
+    torch, dev = torch_dev
+    from sslablu_amd._lib import call, ptr, stream_handle
+    A = np.eye(6)
+    A[3, 3] = 0.0
+    At = torch.tensor(A, device=dev)
+    piv = torch.empty(6, dtype=torch.int32, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    call("sslab_dgetrf_batched", 6, ptr(At), 6, 0, ptr(piv), 0, ptr(info), 1, stream_handle())
+    assert int(info[0]) == 4  # LAPACK convention: 1-based index of the zero pivot
+
+
+@pytest.mark.parametrize("nSlabs,m", [(8, 3), (2, 1), (2, 5), (4, 3), (8, 1), (8, 5), (8, 10),
+                                      (16, 3), (32, 3)])
+@pytest.mark.parametrize("cls", ["RedBlackSolver", "BlockTridiagonalSolver"])
+def test_solvers_kat(torch_dev, nSlabs, m, cls):
+    """The reference's test_RB known-answer cases (direct_solve/test_RB.py:70-81)."""
+    from sslablu_amd.direct_solve import omsdirectsolve as ods
+    S, T, dense, rhs = ob.make_block_tridiagonal(nSlabs, m, seed=42)
+    solver = getattr(ods, cls)(m, cyclic=False)
+    solver.factorize(S, T)
+    x = solver.solve(rhs)
+    assert isinstance(x, np.ndarray) and x.shape == rhs.shape
+    x_ref = np.linalg.solve(dense, rhs)
+    assert rel(x, x_ref) < 1e-8
+    assert rel(x, x_ref) < TOL
+
+
+@pytest.mark.parametrize("nb,m,r", [(3, 7, 2), (5, 16, 3), (7, 160, 1), (12, 33, 5),
+                                    (31, 128, 8), (31, 512, 64)])
+@pytest.mark.parametrize("method", ["rb", "thomas"])
+def test_blocktri_vs_oracle(torch_dev, nb, m, r, method):
+    torch, dev = torch_dev
+    from sslablu_amd.blocktri import BlockTriFactor
+    rng = np.random.default_rng(nb * 1000 + m)
+    T = [np.eye(m) + 0.3 * rng.standard_normal((m, m)) / np.sqrt(m) for _ in range(nb)]
+    S = [(0.5 * rng.standard_normal((m, m)) / np.sqrt(m) if i > 0 else np.zeros((m, m)),
+          0.5 * rng.standard_normal((m, m)) / np.sqrt(m) if i < nb - 1 else np.zeros((m, m)))
+         for i in range(nb)]
+    rhs = rng.standard_normal((nb * m, r))
+    fac = BlockTriFactor(method).factorize(np.stack(T), np.stack([s[0] for s in S]),
+                                           np.stack([s[1] for s in S]))
+    x = fac.solve(torch.tensor(rhs, device=dev)).cpu().numpy()
+    x_ref = ob.thomas_direct(S, T, rhs)
+    assert rel(x, x_ref) < TOL
+    # residual against the assembled operator
+    if nb * m <= 4096:
+        A = ob.assemble_dense(S, T)
+        assert np.linalg.norm(A @ x - rhs) / np.linalg.norm(rhs) < 1e-12
+
+
+@pytest.mark.parametrize("nb", [2, 4, 5, 8])
+@pytest.mark.parametrize("cls", ["RedBlackSolver", "BlockTridiagonalSolver"])
+def test_cyclic(torch_dev, nb, cls):
+    from sslablu_amd.direct_solve import omsdirectsolve as ods
+    m = 6
+    rng = np.random.default_rng(nb)
+    T = [np.eye(m) * 3 + 0.2 * rng.standard_normal((m, m)) for _ in range(nb)]
+    S = [(0.3 * rng.standard_normal((m, m)), 0.3 * rng.standard_normal((m, m))) for _ in range(nb)]
+    rhs = rng.standard_normal((nb * m, 2))
+    x_ref = np.linalg.solve(ob.assemble_dense(S, T, cyclic=True), rhs)
+    solver = getattr(ods, cls)(m, cyclic=True)
+    solver.factorize(S, T)
+    assert rel(solver.solve(rhs), x_ref) < TOL
+
+
+def test_thomas_attributes_and_glob_dofs(torch_dev):
+    torch, dev = torch_dev
+    from sslablu_amd.direct_solve import omsdirectsolve as ods
+    S, T, dense, rhs = ob.make_block_tridiagonal(4, 5, seed=42)
+    solver = ods.BlockTridiagonalSolver(5)
+    solver.factorize(S, T)
+    A, B, C = ob.thomas_factor(S, T)
+    assert len(solver.A) == 3 and len(solver.B) == 4 and len(solver.C) == 3
+    for k in range(3):
+        assert rel(solver.A[k].cpu().numpy(), A[k]) < 1e-12
+        assert rel(solver.C[k].cpu().numpy(), C[k]) < 1e-15
+    for k in range(4):
+        Bk = ods.reconstruct_from_lu_factor((solver.B[k][0].cpu().numpy(),
+                                             solver.B[k][1].cpu().numpy()))
+        assert rel(Bk, ods.reconstruct_from_lu_factor(B[k])) < 1e-12
+    dofs = [range(k * 5, (k + 1) * 5) for k in range(4)]
+    assert rel(solver.solve(rhs, glob_target_dofs=dofs), np.linalg.solve(dense, rhs)) < TOL
+
+
+def test_rb_level_api(torch_dev):
+    """build_block_RB_solver_level on explicit lists matches the oracle level."""
+    torch, dev = torch_dev
+    from sslablu_amd.direct_solve import omsdirectsolve as ods
+    S, T, dense, rhs = ob.make_block_tridiagonal(4, 3, seed=42)
+    RB0 = ob.redblack_factor(S, T)[0]
+    ref = ob.redblack_level(3, 4, RB0)
+    got = ods.RedBlackSolver(3).build_block_RB_solver_level(3, 4, RB0)
+    for k in range(2):
+        assert rel(got[1][k].cpu().numpy(), ref[1][k]) < 1e-12
+        assert rel(got[0][k].cpu().numpy(), ref[0][k]) < 1e-12 or np.linalg.norm(ref[0][k]) == 0
+        assert rel(got[3][k].cpu().numpy(), ref[3][k]) < 1e-12 or np.linalg.norm(ref[3][k]) == 0
