@@ -1,0 +1,235 @@
+"""Benchmark: RHS solves/sec (+ p50/p99 latency, factor time) of the slab direct solver.
+
+Workload (the headline configuration): 2-D variable-coefficient Helmholtz
+  -Lap u - kappa^2 (1 - 0.5 exp(-|x - (1/2,1/2)|^2 / 0.05)) u = 0,  kappa = 20,
+on [0,1]^2 with 32 slabs (31 interfaces), Chebyshev HPS leaves with p = 16 edge nodes
+(q = 18 points per side), 32 leaves per interface (m = 512), 64 right-hand sides of
+seeded N(0,1) Dirichlet data.
+
+* factor (timed once, reported as factor_time_s): HPS leaf DtN maps -> slab merges ->
+  double-slab S maps -> red-black factorization, all on the GPU;
+* step (timed K times): boundary data -> S-system right-hand sides (oms rhs maps) ->
+  red-black solve, for the whole 64-column batch, inputs resident in HBM.
+
+Prints one JSON line (rank 0, N = 1 only).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+KAPPA = 20.0
+N_SLABS = 32
+NX, NY, Q = 1, 32, 18
+NRHS = 64
+SEED = 0
+
+
+class PDO2d:
+    """Coefficient container with the reference's field names (pdo/pdo.py:19-39)."""
+
+    def __init__(self, c11, c22, c1=None, c2=None, c=None):
+        self.c11, self.c22, self.c1, self.c2, self.c = c11, c22, c1, c2, c
+        self.c12 = None
+
+
+def helmholtz_pdo(kappa=KAPPA):
+    def one(P):
+        return np.ones(P.shape[0])
+
+    def c(P):
+        r2 = (P[:, 0] - 0.5) ** 2 + (P[:, 1] - 0.5) ** 2
+        return -kappa ** 2 * (1.0 - 0.5 * np.exp(-r2 / 0.05))
+
+    return PDO2d(one, one, c=c)
+
+
+def algorithmic_work(nb, m, g_sum, r, w=8):
+    """Per-batch algorithmic flops and bytes of one solve (method-independent Thomas
+    minimum): F = 2 r [(3nb-2) m^2 + m sum g],  B = w [(3nb-2) m^2 + m sum g] +
+    8 r (sum g + 2 nb m)."""
+    ent = (3 * nb - 2) * m * m + m * g_sum
+    return 2.0 * r * ent, w * ent + 8.0 * r * (g_sum + 2 * nb * m)
+
+
+def cpu_baseline(op, F_host, budget_s=20.0):
+    """The oracle's restatement of the reference CPU path (scipy lu_factor/lu_solve
+    Thomas, direct_solve/omsdirectsolve.py:48-141, plus the rhs maps of
+    multislab/oms.py:180-186), timed on the host cores on the same S-system."""
+    from oracle import blocktri as ob
+    import torch
+    g = op.g
+    m, nb = g.m, g.nb
+    SiM = op.SiM.cpu().numpy()
+    SiP = op.SiP.cpu().numpy()
+    Y = op.Y.cpu().numpy()
+    S = [(np.zeros((m, m)) if i == 0 else SiM[i], np.zeros((m, m)) if i == nb - 1 else SiP[i])
+         for i in range(nb)]
+    T = [np.eye(m)] * nb
+    nxp = g.nx * g.p
+    N = g.N
+
+    def rhs_maps(F):
+        out = np.empty((nb * m, F.shape[1]))
+        bot, top = F[:N * nxp], F[N * nxp:2 * N * nxp]
+        left, right = F[2 * N * nxp:2 * N * nxp + m], F[2 * N * nxp + m:]
+        for i in range(nb):
+            v = Y[i, :, 2 * m:2 * m + 2 * nxp] @ bot[i * nxp:(i + 2) * nxp]
+            v += Y[i, :, 2 * m + 2 * nxp:] @ top[i * nxp:(i + 2) * nxp]
+            if i == 0:
+                v += Y[i, :, :m] @ left
+            if i == nb - 1:
+                v += Y[i, :, m:2 * m] @ right
+            out[i * m:(i + 1) * m] = -v
+        return out
+
+    t0 = time.perf_counter()
+    fac = ob.thomas_factor(S, T)
+    t_fac = time.perf_counter() - t0
+    times = []
+    x = None
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        x = ob.thomas_solve(*fac, rhs_maps(F_host))
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > budget_s or len(times) >= 50:
+            break
+    t_med = float(np.median(times[1:] if len(times) > 2 else times))
+    return dict(value=F_host.shape[1] / t_med, t_solve=t_med, t_factor=t_fac, reps=len(times),
+                x=x)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: every box here has a single GPU and the "
+              "path is benchmarked at N=1 (multi-GPU sharding is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from sslablu_amd import _lib
+    from sslablu_amd.blocktri import BlockTriFactor
+    from sslablu_amd.hps import HPSSlabOperator, SlabGrid2D
+
+    _lib.require_gpu()
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+
+    # ---- factor ---------------------------------------------------------------
+    def factor():
+        op = HPSSlabOperator(helmholtz_pdo(), SlabGrid2D(N_SLABS, NX, NY, Q), dev).build()
+        eye = torch.eye(op.g.m, dtype=torch.float64, device=dev).expand(op.g.nb, op.g.m, op.g.m)
+        fac = BlockTriFactor("rb", dev).factorize(eye, op.SiM, op.SiP)
+        return op, fac
+
+    factor()  # warm-up (library load, allocator, code objects)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op, fac = factor()
+    torch.cuda.synchronize()
+    t_factor = time.perf_counter() - t0
+
+    g = op.g
+    m, nb = g.m, g.nb
+    ngb = g.gb_points().shape[0]
+    rng = np.random.default_rng(SEED)
+    F_host = rng.standard_normal((ngb, NRHS))
+    F = torch.tensor(F_host, device=dev)
+    V = torch.empty(nb * m, NRHS, dtype=torch.float64, device=dev)
+    X = torch.empty_like(V)
+
+    def step():
+        op.rhs_from_boundary(F, out=V)
+        fac.solve_(V, X)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in ev:
+        e0.record(stream)
+        step()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat_ms = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
+    ms_per_step = wall * 1e3 / args.steps
+
+    # ---- accuracy: independent FP64 Thomas path and the S-system residual --------
+    fac_th = BlockTriFactor("thomas", dev).factorize(
+        torch.eye(m, dtype=torch.float64, device=dev).expand(nb, m, m), op.SiM, op.SiP)
+    rhs = op.rhs_from_boundary(F)
+    x_th = fac_th.solve(rhs)
+    rel_err = (torch.linalg.norm(X - x_th) / torch.linalg.norm(x_th)).item()
+    Xb = X.view(nb, m, -1)
+    res = Xb.clone()
+    res[1:] += torch.bmm(op.SiM[1:], Xb[:-1])
+    res[:-1] += torch.bmm(op.SiP[:-1], Xb[1:])
+    resid = (torch.linalg.norm(res.reshape(nb * m, -1) - rhs) / torch.linalg.norm(rhs)).item()
+
+    # ---- roofline: the step is 12 launches of the FP64 MFMA GEMM kernel ----------------
+    g_sum = nb * 4 * g.nx * g.p + 2 * m
+    flops, nbytes = algorithmic_work(nb, m, g_sum, NRHS)
+    step_s = float(np.mean(lat_ms)) / 1e3
+    achieved_tf = flops / step_s / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": 78.6,
+                "unit": "TFLOP/s", "frac": round(achieved_tf / 78.6, 4), "traffic": None,
+                "kernel": "k_gemm_f64 (12 launches per step: 1 rhs-map + 2*5 red-black "
+                          "levels + coarsest block)",
+                "algorithmic_flops_per_step": flops, "algorithmic_bytes_per_step": nbytes}
+
+    out = {
+        "metric": "RHS solves/sec + p50/p99 latency at <=1e-8 rel err vs FP64; factor time",
+        "value": round(NRHS * args.steps / wall, 1),
+        "unit": "RHS solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded N(0,1) Dirichlet data, seed 0)",
+        "config": {"workload": "2D variable-coefficient Helmholtz kappa=20, 32 slabs "
+                               "(31 interfaces), HPS leaves p=16 (q=18), 32 leaves per "
+                               "interface (m=512), 64 RHS, red-black FP64",
+                   "slabs": N_SLABS, "m": m, "nrhs": NRHS, "method": "rb", "precision": "fp64"},
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
+        "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
+        "factor_time_s": round(t_factor, 4),
+        "rel_err_vs_fp64_thomas": rel_err,
+        "residual": resid,
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(op, F_host, args.cpu_budget)
+        out["cpu_baseline"] = {
+            "value": round(cb["value"], 1), "unit": "RHS solves/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "kind": "port",
+            "sample": f"oracle Thomas (scipy lu_factor/lu_solve) + rhs maps on the same "
+                      f"S-system, {NRHS} RHS per solve, median of {cb['reps']} solves; "
+                      f"CPU factor {cb['t_factor']:.2f} s (S assembly not included)",
+            "factor_s": round(cb["t_factor"], 3),
+        }
+        out["cpu_rel_diff"] = float(np.linalg.norm(X.cpu().numpy() - cb["x"]) /
+                                    np.linalg.norm(cb["x"]))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -85,6 +85,61 @@ int sslab_thomas_factor(int nb, int m, const double* T, const double* SiM, const
 int sslab_thomas_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
                        double* X, int ldx, void* stream);
 
+/* ---- HPS slab assembly (2-D Chebyshev leaves, q x q points, p = q - 2) -----
+ * Replaces the per-slab discretization + S-map assembly of
+ * multislab/oms.py:160-205 (solverWrapper.construct, solver/solver.py:79-163, and
+ * compute_stmaps + the dense assembler, oms.py:99-129, matAssembler.py:93-98),
+ * whose HPS leaf solver (hpsalt, solver/solver.py:121-144) is not in the reference.
+ *
+ * Leaf DtN maps T_l = N_b - N_i A_ii^{-1} A_ib (4p x 4p; edges left, right, bottom,
+ * top) for nleaves leaves.  Dx/Dxx/Dy/Dyy: q x q derivative matrices of the leaf
+ * grid; Nb (4p x 4p), Ni (4p x p^2): outward-normal derivative rows (shared).
+ * c11..c0: nleaves*p^2 coefficient values at the interior nodes (row r =
+ * (a-1)*p + (b-1), a along x), any may be NULL (= 0).  X (optional, nleaves x
+ * p^2 x 4p) receives the leaf solution operators A_ii^{-1} A_ib.  info: nleaves
+ * ints (LAPACK getrf info of A_ii). */
+int sslab_hps_leaf_sizes(int nleaves, int q, long long* work_doubles, long long* work_ints);
+int sslab_hps_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx,
+                       const double* Dy, const double* Dyy, const double* Nb,
+                       const double* Ni, const double* c11, const double* c22,
+                       const double* c1, const double* c2, const double* c0, double* T,
+                       double* X, double* work, int* iwork, int* info, void* stream);
+
+/* Merge nmerge box pairs (alpha_b = Ta + b*sTa, beta_b = Tb + b*sTb) that share c
+ * edge nodes (positions ca in alpha, cb in beta) into boxes with n_new boundary
+ * nodes; na[i] / nb_idx[i] give the position of merged node i in alpha / beta
+ * (-1 when absent).  All index arrays are device int arrays. */
+int sslab_hps_merge_sizes(int nmerge, int n_new, int c, long long* work_doubles,
+                          long long* work_ints);
+int sslab_hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta,
+                    long long sTa, const double* Tb, long long sTb, const int* ca,
+                    const int* cb, const int* na, const int* nb_idx, double* Tnew,
+                    long long sTnew, double* work, int* iwork, int* info, void* stream);
+
+/* Double-slab S maps: for pairs (slab i, slab i+1) of slab DtN maps Ts (n_s x n_s,
+ * stride sTs), Y_i = (T^i_cc + T^{i+1}_cc)^{-1} [T^i(ca, cola) | T^{i+1}(cb, colb)]
+ * (m x ncols, m = c), column j taken from slab i when cola[j] >= 0, else from
+ * slab i+1.  Workspace from sslab_hps_merge_sizes(npair, ncols, c). */
+int sslab_hps_pair_solve(int npair, int n_s, int c, int ncols, const double* Ts,
+                         long long sTs, const int* ca, const int* cb, const int* cola,
+                         const int* colb, double* Y, double* work, int* iwork, int* info,
+                         void* stream);
+
+/* S-system right-hand sides for r columns of global Dirichlet data (multislab/oms.py:
+ * 180-186, rhs = -(A_ii^{-1} A_ib[:, Igb] f)[Ic], for every double slab at once).
+ * Y: nb x m x ncols pair-solve output (ncols = 2m + 4 nxp, nxp = boundary nodes per
+ * slab along x); F: (2 N nxp + 2m) x r boundary data ordered [bottom | top | left |
+ * right] with leading dimension ldF; out: nb*m x r (ldo).  N = nb + 1. */
+int sslab_oms_rhs(int nb, int m, int nxp, int ncols, const double* Y, const double* F,
+                  int ldF, int r, int N, double* out, int ldo, void* stream);
+
+/* out[b][i][j] = sum over s in {0,1} of src_s[b][rid_s[i]][cid_s[j]] where both
+ * indices are >= 0 (index arrays may be NULL = absent). */
+int sslab_gather2(int batch, int nrows, int ncols, const double* s0, int ld0, long long st0,
+                  const double* s1, int ld1, long long st1, const int* rid0, const int* cid0,
+                  const int* rid1, const int* cid1, double* out, int ldo, long long sto,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,17 @@ _SIGS = {
     "sslab_thomas_sizes": [_c_int, _c_int, _vp, _vp, _vp],
     "sslab_thomas_factor": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "sslab_thomas_solve": [_c_int, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _vp],
+    "sslab_hps_leaf_sizes": [_c_int, _c_int, _vp, _vp],
+    "sslab_hps_leaf_dtn": [_c_int, _c_int] + [_vp] * 16 + [_vp],
+    "sslab_hps_merge_sizes": [_c_int, _c_int, _c_int, _vp, _vp],
+    "sslab_hps_merge": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll,
+                        _vp, _vp, _vp, _vp, _vp, _c_ll, _vp, _vp, _vp, _vp],
+    "sslab_hps_pair_solve": [_c_int, _c_int, _c_int, _c_int, _vp, _c_ll, _vp, _vp, _vp, _vp,
+                             _vp, _vp, _vp, _vp, _vp],
+    "sslab_oms_rhs": [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp,
+                      _c_int, _vp],
+    "sslab_gather2": [_c_int, _c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_int, _c_ll,
+                      _vp, _vp, _vp, _vp, _vp, _c_int, _c_ll, _vp],
 }
 
 EXPORTED = tuple(_SIGS) + ("sslab_last_error",)
@@ -105,3 +116,11 @@ def sizes(kind: str, nb: int, m: int):
     call(f"sslab_{kind}_sizes", nb, m, ctypes.addressof(f), ctypes.addressof(w),
          ctypes.addressof(i))
     return f.value, w.value, i.value
+
+
+def sizes2(name: str, *args):
+    """(work_doubles, work_ints) from a two-output size query `name`(*args, &w, &i)."""
+    w = ctypes.c_longlong()
+    i = ctypes.c_longlong()
+    call(name, *args, ctypes.addressof(w), ctypes.addressof(i))
+    return w.value, i.value

@@ -4,7 +4,9 @@
 #include "../../include/sslab_hip.h"
 #include "blocktri_f64.hpp"
 #include "gemm_f64.hpp"
+#include "hps_f64.hpp"
 #include "lu_f64.hpp"
+#include "oms_f64.hpp"
 
 namespace sslab {
 
@@ -95,6 +97,51 @@ int sslab_thomas_factor(int nb, int m, const double* T, const double* SiM, const
 int sslab_thomas_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
                        double* X, int ldx, void* stream) {
   return sslab::thomas_solve(nb, m, nrhs, factor, V, ldv, X, ldx, as_stream(stream));
+}
+
+int sslab_hps_leaf_sizes(int nleaves, int q, long long* work_doubles, long long* work_ints) {
+  return sslab::hps_leaf_sizes(nleaves, q, work_doubles, work_ints);
+}
+
+int sslab_hps_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx, const double* Dy,
+                       const double* Dyy, const double* Nb, const double* Ni, const double* c11,
+                       const double* c22, const double* c1, const double* c2, const double* c0,
+                       double* T, double* X, double* work, int* iwork, int* info, void* stream) {
+  return sslab::hps_leaf_dtn(nleaves, q, Dx, Dxx, Dy, Dyy, Nb, Ni, c11, c22, c1, c2, c0, T, X,
+                             work, iwork, info, as_stream(stream));
+}
+
+int sslab_hps_merge_sizes(int nmerge, int n_new, int c, long long* work_doubles,
+                          long long* work_ints) {
+  return sslab::hps_merge_sizes(nmerge, n_new, c, work_doubles, work_ints);
+}
+
+int sslab_hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta,
+                    long long sTa, const double* Tb, long long sTb, const int* ca, const int* cb,
+                    const int* na, const int* nb_idx, double* Tnew, long long sTnew, double* work,
+                    int* iwork, int* info, void* stream) {
+  return sslab::hps_merge(nmerge, n_a, n_b, n_new, c, Ta, sTa, Tb, sTb, ca, cb, na, nb_idx, Tnew,
+                          sTnew, work, iwork, info, as_stream(stream));
+}
+
+int sslab_hps_pair_solve(int npair, int n_s, int c, int ncols, const double* Ts, long long sTs,
+                         const int* ca, const int* cb, const int* cola, const int* colb, double* Y,
+                         double* work, int* iwork, int* info, void* stream) {
+  return sslab::hps_pair_solve(npair, n_s, c, ncols, Ts, sTs, ca, cb, cola, colb, Y, work, iwork,
+                               info, as_stream(stream));
+}
+
+int sslab_oms_rhs(int nb, int m, int nxp, int ncols, const double* Y, const double* F, int ldF,
+                  int r, int N, double* out, int ldo, void* stream) {
+  return sslab::oms_rhs(nb, m, nxp, ncols, Y, F, ldF, r, N, out, ldo, as_stream(stream));
+}
+
+int sslab_gather2(int batch, int nrows, int ncols, const double* s0, int ld0, long long st0,
+                  const double* s1, int ld1, long long st1, const int* rid0, const int* cid0,
+                  const int* rid1, const int* cid1, double* out, int ldo, long long sto,
+                  void* stream) {
+  return sslab::gather2(batch, nrows, ncols, s0, ld0, st0, s1, ld1, st1, rid0, cid0, rid1, cid1,
+                        out, ldo, sto, as_stream(stream));
 }
 
 }  // extern "C"

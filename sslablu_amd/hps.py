@@ -1,0 +1,346 @@
+"""2-D Chebyshev HPS slab assembly on the GPU: leaves -> slab DtN maps -> S-system.
+
+Geometry: [0,1]^2 cut into N single slabs of width H = 1/N (the double slabs of
+geometry/geom_2D/square.py:33-47 are neighbouring pairs of them); every single slab is
+tiled by nx x ny leaves of q x q Chebyshev points (p = q - 2 edge nodes per leaf side,
+corners excluded).  The interface between single slabs i and i+1 carries m = ny*p
+nodes -- the centre line Ic of double slab i (multislab/oms.py:38-52).
+
+The reference discretizes each double slab separately and builds its S maps by sparse
+solves (multislab/oms.py:99-129,160-205).  Here every single slab is discretized once:
+
+1. leaf DtN maps for all leaves in one batch (sslab_hps_leaf_dtn);
+2. binary merge trees, first across (nx) then up (ny), batched over all slabs
+   (sslab_hps_merge) -> one DtN map T_s per single slab, boundary ordered
+   [left, right, bottom, top];
+3. one batched pair solve per double slab (sslab_hps_pair_solve):
+   Y_i = (T^i_RR + T^{i+1}_LL)^{-1} [T^i_RL | T^{i+1}_LR | T^i_R,bot | T^{i+1}_L,bot |
+   T^i_R,top | T^{i+1}_L,top], which holds S_l = Y[:, :m], S_r = Y[:, m:2m] and the
+   boundary-data maps (the reference's A_ii^{-1} A_ib[:, J] restricted to Ic).
+
+Global boundary data are ordered [bottom (N*nx*p, by x) | top (N*nx*p) | left (m) |
+right (m)], so each double slab reads contiguous row ranges.
+
+All arithmetic runs in libsslab_hip.so; this module only prepares constants, index
+maps and coefficient samples (user callables evaluated on the host).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr, sizes2, stream_handle
+
+F64 = torch.float64
+I32 = torch.int32
+
+
+def cheb_unit(q: int):
+    """q ascending Chebyshev points on [0, 1] and the first-derivative matrix on them
+    (the Chebyshev collocation matrix of Trefethen, Spectral Methods in MATLAB, 2000)."""
+    n = q - 1
+    x = np.cos(np.pi * np.arange(q) / n)
+    c = np.ones(q)
+    c[0] = c[-1] = 2.0
+    c *= (-1.0) ** np.arange(q)
+    D = np.outer(c, 1.0 / c) / (x[:, None] - x[None, :] + np.eye(q))
+    D -= np.diag(D.sum(axis=1))
+    return (1.0 - x) / 2.0, -2.0 * D
+
+
+def leaf_constants(q: int, hx: float, hy: float):
+    """Derivative matrices of one leaf and its outward-normal flux rows
+    (Nb: 4p x 4p on the edge nodes, Ni: 4p x p^2 on the interior nodes)."""
+    t, D = cheb_unit(q)
+    p = q - 2
+    Dx, Dy = D / hx, D / hy
+    Dxx, Dyy = Dx @ Dx, Dy @ Dy
+    Nb = np.zeros((4 * p, 4 * p))
+    Ni = np.zeros((4 * p, p * p))
+    for k in range(p):
+        b = k + 1
+        for e, a0, sgn in ((k, 0, -1.0), (p + k, q - 1, 1.0)):      # left, right edges
+            Nb[e, k] += sgn * Dx[a0, 0]
+            Nb[e, p + k] += sgn * Dx[a0, q - 1]
+            for a2 in range(1, q - 1):
+                Ni[e, (a2 - 1) * p + (b - 1)] += sgn * Dx[a0, a2]
+        a = k + 1
+        for e, b0, sgn in ((2 * p + k, 0, -1.0), (3 * p + k, q - 1, 1.0)):  # bottom, top
+            Nb[e, 2 * p + k] += sgn * Dy[b0, 0]
+            Nb[e, 3 * p + k] += sgn * Dy[b0, q - 1]
+            for b2 in range(1, q - 1):
+                Ni[e, (a - 1) * p + (b2 - 1)] += sgn * Dy[b0, b2]
+    return dict(t=t, Dx=Dx, Dxx=Dxx, Dy=Dy, Dyy=Dyy, Nb=Nb, Ni=Ni)
+
+
+def _box_layout(nxb, nyb, p):
+    """Offsets of the [left, right, bottom, top] boundary segments of a box of
+    nxb x nyb leaves."""
+    L = nyb * p
+    B = nxb * p
+    return dict(left=0, right=L, bottom=2 * L, top=2 * L + B, n=2 * (L + B), L=L, B=B)
+
+
+def merge_maps(kind, nxb, nyb, p):
+    """Index maps of merging two nxb x nyb boxes across (kind 'h': alpha left of beta)
+    or up (kind 'v': alpha below beta).  Returns (ca, cb, na, nb, n_new, (nxb', nyb'))."""
+    s = _box_layout(nxb, nyb, p)
+    ar = np.arange
+    none = lambda k: -np.ones(k, dtype=np.int64)
+    if kind == "h":
+        ca = s["right"] + ar(s["L"])
+        cb = s["left"] + ar(s["L"])
+        na = np.concatenate([s["left"] + ar(s["L"]), none(s["L"]),
+                             s["bottom"] + ar(s["B"]), none(s["B"]),
+                             s["top"] + ar(s["B"]), none(s["B"])])
+        nbm = np.concatenate([none(s["L"]), s["right"] + ar(s["L"]),
+                              none(s["B"]), s["bottom"] + ar(s["B"]),
+                              none(s["B"]), s["top"] + ar(s["B"])])
+        new = (2 * nxb, nyb)
+    else:
+        ca = s["top"] + ar(s["B"])
+        cb = s["bottom"] + ar(s["B"])
+        na = np.concatenate([s["left"] + ar(s["L"]), none(s["L"]),
+                             s["right"] + ar(s["L"]), none(s["L"]),
+                             s["bottom"] + ar(s["B"]), none(s["B"])])
+        nbm = np.concatenate([none(s["L"]), s["left"] + ar(s["L"]),
+                              none(s["L"]), s["right"] + ar(s["L"]),
+                              none(s["B"]), s["top"] + ar(s["B"])])
+        new = (nxb, 2 * nyb)
+    return ca, cb, na, nbm, len(na), new
+
+
+def pair_maps(nx, ny, p):
+    """Index maps of the double-slab pair solve: shared edge and the column layout
+    [alpha.left | beta.right | alpha.bottom | beta.bottom | alpha.top | beta.top]."""
+    s = _box_layout(nx, ny, p)
+    ar = np.arange
+    none = lambda k: -np.ones(k, dtype=np.int64)
+    m, B = s["L"], s["B"]
+    ca = s["right"] + ar(m)
+    cb = s["left"] + ar(m)
+    cola = np.concatenate([s["left"] + ar(m), none(m), s["bottom"] + ar(B), none(B),
+                           s["top"] + ar(B), none(B)])
+    colb = np.concatenate([none(m), s["right"] + ar(m), none(B), s["bottom"] + ar(B),
+                           none(B), s["top"] + ar(B)])
+    return ca, cb, cola, colb
+
+
+def _is_pow2(k):
+    return k >= 1 and (k & (k - 1)) == 0
+
+
+class SlabGrid2D:
+    """Leaf tiling of [0,1]^2 by N single slabs of nx x ny leaves with q points."""
+
+    def __init__(self, N: int, nx: int, ny: int, q: int):
+        if N < 2:
+            raise ValueError("need at least two single slabs (one interface)")
+        if not (_is_pow2(nx) and _is_pow2(ny)):
+            raise NotImplementedError("leaves per slab across / up must be powers of two")
+        if q < 3:
+            raise ValueError("q >= 3 Chebyshev points per leaf side")
+        self.N, self.nx, self.ny, self.q = N, nx, ny, q
+        self.p = q - 2
+        self.H = 1.0 / N
+        self.hx = self.H / nx
+        self.hy = 1.0 / ny
+        self.m = ny * self.p
+        self.nb = N - 1
+        self.nleaves = N * nx * ny
+        self.t = cheb_unit(q)[0]
+
+    # leaf L = (s*ny + j)*nx + ix
+    def leaf_origins(self):
+        s, j, ix = np.meshgrid(np.arange(self.N), np.arange(self.ny), np.arange(self.nx),
+                               indexing="ij")
+        x0 = s.ravel() * self.H + ix.ravel() * self.hx
+        y0 = j.ravel() * self.hy
+        return x0, y0
+
+    def interior_points(self):
+        """(nleaves*p^2, 2) interior collocation nodes, row r = (a-1)*p + (b-1)."""
+        x0, y0 = self.leaf_origins()
+        ti = self.t[1:-1]
+        ga, gb = np.meshgrid(ti, ti, indexing="ij")
+        X = x0[:, None] + ga.ravel()[None, :] * self.hx
+        Y = y0[:, None] + gb.ravel()[None, :] * self.hy
+        return np.stack([X.ravel(), Y.ravel()], axis=1)
+
+    def edge_coords(self, n_leaves, h):
+        ti = self.t[1:-1]
+        return (np.arange(n_leaves)[:, None] + ti[None, :]).ravel() * h
+
+    def gb_points(self):
+        """Global boundary nodes in the order [bottom | top | left | right]."""
+        xs = self.edge_coords(self.N * self.nx, self.hx)
+        ys = self.edge_coords(self.ny, self.hy)
+        bottom = np.stack([xs, np.zeros_like(xs)], 1)
+        top = np.stack([xs, np.ones_like(xs)], 1)
+        left = np.stack([np.zeros_like(ys), ys], 1)
+        right = np.stack([np.ones_like(ys), ys], 1)
+        return np.concatenate([bottom, top, left, right])
+
+    def interface_points(self):
+        """(nb*m, 2) interface nodes, interface i at x = (i+1) H, bottom to top."""
+        ys = self.edge_coords(self.ny, self.hy)
+        pts = [np.stack([np.full_like(ys, (i + 1) * self.H), ys], 1) for i in range(self.nb)]
+        return np.concatenate(pts)
+
+
+def _coef(fn, P):
+    if fn is None:
+        return None
+    v = fn(P)
+    if isinstance(v, torch.Tensor):
+        v = v.detach().cpu().numpy()
+    v = np.asarray(v, dtype=np.float64)
+    if v.ndim == 0:
+        v = np.full(P.shape[0], float(v))
+    return v.reshape(-1)
+
+
+class HPSSlabOperator:
+    """Device-resident S-system of the oms method for one PDE on one SlabGrid2D.
+
+    After build(): Y (nb, m, 2m + 4*nx*p) float64 on the GPU (S maps and boundary-data
+    maps of every double slab), SiM / SiP (nb, m, m) contiguous copies of the coupling
+    blocks, and rhs_from_boundary() to map boundary data to S-system right-hand sides.
+    """
+
+    def __init__(self, pdo, grid: SlabGrid2D, device=None, keep_leaf_ops=False):
+        _lib.require_gpu()
+        _lib.load()
+        self.pdo = pdo
+        self.g = grid
+        self.device = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        self.keep_leaf_ops = keep_leaf_ops
+        self.info_bad = 0
+
+    # -------------------------------------------------------------------------
+    def _dev(self, a, dtype=F64):
+        return torch.as_tensor(np.ascontiguousarray(a)).to(device=self.device, dtype=dtype)
+
+    def leaf_dtn(self, stream=None):
+        g = self.g
+        q, p = g.q, g.p
+        c = leaf_constants(q, g.hx, g.hy)
+        P = g.interior_points()
+        pdo = self.pdo
+        coefs = []
+        for name in ("c11", "c22", "c1", "c2", "c"):
+            v = _coef(getattr(pdo, name, None), P)
+            coefs.append(self._dev(v) if v is not None else None)
+        consts = {k: self._dev(c[k]) for k in ("Dx", "Dxx", "Dy", "Dyy", "Nb", "Ni")}
+        nl = g.nleaves
+        T = torch.empty(nl, 4 * p, 4 * p, dtype=F64, device=self.device)
+        X = torch.empty(nl, p * p, 4 * p, dtype=F64, device=self.device) if self.keep_leaf_ops else None
+        wd, wi = sizes2("sslab_hps_leaf_sizes", nl, q)
+        work = torch.empty(wd, dtype=F64, device=self.device)
+        iwork = torch.empty(wi, dtype=I32, device=self.device)
+        info = torch.zeros(nl, dtype=I32, device=self.device)
+        cp = [ptr(x) if x is not None else None for x in coefs]
+        call("sslab_hps_leaf_dtn", nl, q, ptr(consts["Dx"]), ptr(consts["Dxx"]),
+             ptr(consts["Dy"]), ptr(consts["Dyy"]), ptr(consts["Nb"]), ptr(consts["Ni"]),
+             cp[0], cp[1], cp[2], cp[3], cp[4], ptr(T), ptr(X) if X is not None else None,
+             ptr(work), ptr(iwork), ptr(info), stream_handle(stream))
+        self._infos = [info]
+        self.leaf_X = X
+        return T
+
+    def merge_tree(self, T, stream=None):
+        g = self.g
+        p = g.p
+        nxb, nyb = 1, 1
+        boxes = T
+        while nxb < g.nx or nyb < g.ny:
+            kind = "h" if nxb < g.nx else "v"
+            ca, cb, na, nbm, n_new, new = merge_maps(kind, nxb, nyb, p)
+            n_old = boxes.shape[1]
+            nmerge = boxes.shape[0] // 2
+            c = len(ca)
+            maps = [self._dev(x, I32) for x in (ca, cb, na, nbm)]
+            out = torch.empty(nmerge, n_new, n_new, dtype=F64, device=self.device)
+            wd, wi = sizes2("sslab_hps_merge_sizes", nmerge, n_new, c)
+            work = torch.empty(wd, dtype=F64, device=self.device)
+            iwork = torch.empty(wi, dtype=I32, device=self.device)
+            info = torch.zeros(nmerge, dtype=I32, device=self.device)
+            s_old = n_old * n_old
+            call("sslab_hps_merge", nmerge, n_old, n_old, n_new, c, ptr(boxes), 2 * s_old,
+                 ptr(boxes) + 8 * s_old, 2 * s_old, *[ptr(x) for x in maps], ptr(out),
+                 n_new * n_new, ptr(work), ptr(iwork), ptr(info), stream_handle(stream))
+            self._infos.append(info)
+            boxes = out
+            nxb, nyb = new
+        return boxes
+
+    def pair_solve(self, Ts, stream=None):
+        g = self.g
+        p, m, nb = g.p, g.m, g.nb
+        ca, cb, cola, colb = pair_maps(g.nx, g.ny, p)
+        ncols = len(cola)
+        n_s = Ts.shape[1]
+        maps = [self._dev(x, I32) for x in (ca, cb, cola, colb)]
+        Y = torch.empty(nb, m, ncols, dtype=F64, device=self.device)
+        wd, wi = sizes2("sslab_hps_merge_sizes", nb, ncols, m)
+        work = torch.empty(wd, dtype=F64, device=self.device)
+        iwork = torch.empty(wi, dtype=I32, device=self.device)
+        info = torch.zeros(nb, dtype=I32, device=self.device)
+        call("sslab_hps_pair_solve", nb, n_s, m, ncols, ptr(Ts), n_s * n_s,
+             *[ptr(x) for x in maps], ptr(Y), ptr(work), ptr(iwork), ptr(info),
+             stream_handle(stream))
+        self._infos.append(info)
+        return Y
+
+    def build(self, stream=None):
+        """Run the whole assembly; returns self."""
+        with torch.cuda.device(self.device):
+            T = self.leaf_dtn(stream)
+            Ts = self.merge_tree(T, stream)
+            del T
+            self.Ts = Ts
+            self.Y = self.pair_solve(Ts, stream)
+            m = self.g.m
+            self.SiM = self.Y[:, :, 0:m].contiguous()
+            self.SiP = self.Y[:, :, m:2 * m].contiguous()
+            self.info_bad = int(sum(int((i != 0).sum()) for i in self._infos))
+        if self.info_bad:
+            import warnings
+            warnings.warn(f"{self.info_bad} singular local solve(s) in the HPS assembly "
+                          "(a leaf or slab Dirichlet problem at resonance?)", RuntimeWarning)
+        return self
+
+    # -------------------------------------------------------------------------
+    def S_rk_list(self):
+        """[[S_r], [S_l, S_r], ..., [S_l]] device views (multislab/oms.py:239-244)."""
+        nb = self.g.nb
+        out = []
+        for i in range(nb):
+            if i == 0:
+                out.append([self.SiP[0]])
+            elif i == nb - 1:
+                out.append([self.SiM[i]])
+            else:
+                out.append([self.SiM[i], self.SiP[i]])
+        return out
+
+    def rhs_from_boundary(self, F: torch.Tensor, out: torch.Tensor | None = None, stream=None):
+        """S-system right-hand sides rhs_i = -(A_ii^{-1} A_ib[:, Igb] f)[Ic] for r columns
+        of global boundary data F (N_gb x r, order of SlabGrid2D.gb_points())."""
+        g = self.g
+        m, nb, nxp = g.m, g.nb, g.nx * g.p
+        if F.dim() == 1:
+            F = F[:, None]
+        F = F.to(device=self.device, dtype=F64).contiguous()
+        r = F.shape[1]
+        ngb = 2 * g.N * nxp + 2 * m
+        if F.shape[0] != ngb:
+            raise ValueError(f"boundary data needs {ngb} rows, got {F.shape[0]}")
+        if out is None:
+            out = torch.empty(nb * m, r, dtype=F64, device=self.device)
+        ncols = self.Y.shape[2]
+        call("sslab_oms_rhs", nb, m, nxp, ncols, ptr(self.Y), ptr(F), F.stride(0), r,
+             g.N, ptr(out), out.stride(0), stream_handle(stream))
+        return out
