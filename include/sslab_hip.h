@@ -52,6 +52,26 @@ int sslab_dgetrs_batched(int n, int nrhs, const double* LU, int lda, long long s
                          long long strideB, double* X, int ldx, long long strideX,
                          int* perm_work, int batch, void* stream);
 
+/* Householder QR, optionally column-pivoted (m, n <= 1024), one matrix per
+ * workgroup: A_b (m x n) is overwritten by R (zeros below the diagonal); Q (m x m,
+ * optional) receives the complete orthogonal factor; perm (n ints, optional)
+ * switches on column pivoting and receives the order with A_in[:, perm] = Q R.
+ * Replaces numpy/torch qr in matAssembly/HBS/HBSnew.py:26 and
+ * ULVsparse.py:18-25,28-61,79 (the HBS basis / ULV factor steps). */
+int sslab_dgeqrf_batched(int m, int n, double* A, int lda, long long strideA, double* Q,
+                         int ldq, long long strideQ, int* perm, long long stridePerm,
+                         int batch, void* stream);
+
+/* In place X_b <- U_b^{-1} X_b for upper-triangular U_b (n x n).  Replaces the
+ * triangular np.linalg.solve of ULVsparse.block_solve (ULVsparse.py:213-226). */
+int sslab_dtrsm_upper_batched(int n, int nrhs, const double* U, int ldu, long long strideU,
+                              double* X, int ldx, long long strideX, int batch, void* stream);
+
+/* Y_b[diag block blk] += X[b*nblk + blk] (bs x bs blocks), b < nparent: the
+ * block_diag_add of ULVsparse.py:158-183. */
+int sslab_add_diag_blocks(int bs, int nblk, int nparent, const double* X, double* Y, int ldy,
+                          long long strideY, void* stream);
+
 /* ---- red-black block-tridiagonal solver ------------------------------------
  * System (non-cyclic): T_i x_i + SiM_i x_{i-1} + SiP_i x_{i+1} = v_i, i < nb,
  * with T, SiM, SiP each nb contiguous m x m blocks (SiM_0, SiP_{nb-1} unused).

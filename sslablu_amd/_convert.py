@@ -32,12 +32,12 @@ def from_device(t: torch.Tensor, was_numpy: bool, one_d: bool):
 
 
 def blocks_to_device(blocks, device):
-    """list of m x m arrays/tensors -> (n, m, m) contiguous float64 device tensor."""
+    """list of m x m blocks (numpy, torch, DeviceMatrix or HBS operators) -> (n, m, m)
+    contiguous float64 device tensor."""
     if isinstance(blocks, torch.Tensor):
         return blocks.to(device=device, dtype=F64).contiguous()
-    return torch.stack([torch.as_tensor(np.asarray(b, dtype=np.float64))
-                        if not isinstance(b, torch.Tensor) else b.to(dtype=F64).cpu()
-                        for b in blocks]).to(device).contiguous()
+    from .linop import as_tensor_block
+    return torch.stack([as_tensor_block(b, device) for b in blocks]).contiguous()
 
 
 def index_list(glob_target_dofs, nb, m):

@@ -32,6 +32,11 @@ _SIGS = {
     "sslab_dgetrf_batched": [_c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int, _vp],
     "sslab_dgetrs_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int,
                              _c_ll, _vp, _c_int, _c_ll, _vp, _c_int, _vp],
+    "sslab_dgeqrf_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_int, _c_ll, _vp,
+                             _c_ll, _c_int, _vp],
+    "sslab_dtrsm_upper_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_int, _c_ll,
+                                  _c_int, _vp],
+    "sslab_add_diag_blocks": [_c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_ll, _vp],
     "sslab_rb_sizes": [_c_int, _c_int, _vp, _vp, _vp],
     "sslab_rb_info_count": [_c_int],
     "sslab_rb_factor": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -73,7 +78,9 @@ def load(path: str | os.PathLike | None = None):
             "(there is no CPU fallback for the HIP path)")
     lib = ctypes.CDLL(str(p))
     for name, args in _SIGS.items():
-        fn = getattr(lib, name)
+        fn = getattr(lib, name, None)
+        if fn is None:  # stale build: calling this entry point raises (see call())
+            continue
         fn.argtypes = args
         fn.restype = _c_int
     lib.sslab_last_error.argtypes = []
@@ -85,7 +92,11 @@ def load(path: str | os.PathLike | None = None):
 def call(name: str, *args) -> int:
     """Call entry point `name`; raise SslabError on a non-zero status."""
     lib = load()
-    st = getattr(lib, name)(*args)
+    fn = getattr(lib, name, None)
+    if fn is None:
+        raise SslabError(f"{name} is not exported by {LIB_PATH}: rebuild with "
+                         "`python -m sslablu_amd.build`")
+    st = fn(*args)
     if st != 0:
         msg = lib.sslab_last_error().decode(errors="replace")
         raise SslabError(f"{name} failed (status {st}): {msg}")

@@ -7,6 +7,7 @@
 #include "hps_f64.hpp"
 #include "lu_f64.hpp"
 #include "oms_f64.hpp"
+#include "qr_f64.hpp"
 
 namespace sslab {
 
@@ -65,6 +66,24 @@ int sslab_dgetrs_batched(int n, int nrhs, const double* LU, int lda, long long s
                          int* perm_work, int batch, void* stream) {
   return sslab::getrs_batched(n, nrhs, LU, lda, strideA, ipiv, strideP, B, ldb, strideB, X, ldx,
                               strideX, perm_work, batch, as_stream(stream));
+}
+
+int sslab_dgeqrf_batched(int m, int n, double* A, int lda, long long strideA, double* Q, int ldq,
+                         long long strideQ, int* perm, long long stridePerm, int batch,
+                         void* stream) {
+  return sslab::geqrf_batched(m, n, A, lda, strideA, Q, ldq, strideQ, perm, stridePerm, batch,
+                              as_stream(stream));
+}
+
+int sslab_dtrsm_upper_batched(int n, int nrhs, const double* U, int ldu, long long strideU,
+                              double* X, int ldx, long long strideX, int batch, void* stream) {
+  return sslab::trsm_upper_batched(n, nrhs, U, ldu, strideU, X, ldx, strideX, batch,
+                                   as_stream(stream));
+}
+
+int sslab_add_diag_blocks(int bs, int nblk, int nparent, const double* X, double* Y, int ldy,
+                          long long strideY, void* stream) {
+  return sslab::add_diag_blocks(bs, nblk, nparent, X, Y, ldy, strideY, as_stream(stream));
 }
 
 int sslab_rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,

@@ -364,6 +364,29 @@ int getrs_batched(int n, int nrhs, const double* LU, int lda, long long sA, cons
   return kOk;
 }
 
+int trsm_upper_batched(int n, int nrhs, const double* U, int ldu, long long sU, double* X, int ldx,
+                       long long sX, int batch, hipStream_t stream) {
+  SSLAB_REQUIRE(n >= 0 && nrhs >= 0 && batch >= 0, "trsm: negative size");
+  SSLAB_REQUIRE(ldu >= n && ldx >= nrhs, "trsm: bad leading dimension");
+  if (n == 0 || nrhs == 0 || batch == 0) return kOk;
+  SSLAB_REQUIRE(batch <= 65535, "trsm: batch too large");
+  constexpr int NB = kTrsmNB;
+  const int nblk = ceil_div(n, NB);
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    const int kb = bi * NB;
+    const int nr = (n - kb) < NB ? (n - kb) : NB;
+    hipLaunchKernelGGL((k_trsm_cols<NB, false>), dim3(ceil_div(nrhs, 256), batch), dim3(256), 0,
+                       stream, nr, nrhs, U + (long long)kb * ldu + kb, ldu, sU,
+                       X + (long long)kb * ldx, ldx, sX);
+    SSLAB_HIP_CHECK(hipGetLastError());
+    if (kb > 0) {
+      SSLAB_TRY(gemm_strided(0, 0, kb, nrhs, nr, -1.0, U + kb, ldu, sU, X + (long long)kb * ldx,
+                             ldx, sX, 1.0, X, ldx, sX, batch, stream));
+    }
+  }
+  return kOk;
+}
+
 int set_identity_batched(int n, double* X, int ldx, long long sX, int batch, hipStream_t stream) {
   if (n <= 0 || batch <= 0) return kOk;
   SSLAB_REQUIRE(batch <= 65535 && n <= 65535, "set_identity: size too large");

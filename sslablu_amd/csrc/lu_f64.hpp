@@ -22,6 +22,10 @@ int getrs_batched(int n, int nrhs, const double* LU, int lda, long long sA, cons
                   long long sP, const double* B, int ldb, long long sB, double* X, int ldx,
                   long long sX, int* perm, int batch, hipStream_t stream);
 
+// In place X_b <- U_b^{-1} X_b for upper-triangular U_b (n x n, non-unit diagonal).
+int trsm_upper_batched(int n, int nrhs, const double* U, int ldu, long long sU, double* X, int ldx,
+                       long long sX, int batch, hipStream_t stream);
+
 // X_b = I (n x n, leading dimension ldx) for every batch entry.
 int set_identity_batched(int n, double* X, int ldx, long long sX, int batch, hipStream_t stream);
 

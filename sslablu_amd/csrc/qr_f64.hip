@@ -1,0 +1,175 @@
+// Batched FP64 Householder QR (optionally column-pivoted) for gfx950.
+//
+// One workgroup (4 wavefronts) factors one small matrix held in global memory (the
+// HBS boxes are at most a few hundred rows, so a matrix stays L2-resident): per
+// column j, an optional pivot search over the remaining column norms, the
+// reflector H_j = I - tau v v^T (LAPACK dlarfg convention, v_0 = 1) built in LDS,
+// its application to the trailing columns (one wavefront per column, wave
+// reductions for the dot products) and, when requested, to the explicit complete
+// Q (m x m) from the right, Q <- Q H_j, one wavefront per row.
+// On exit A holds R (zeros below the diagonal), Q the orthogonal factor with
+// A_in[:, perm] = Q R.
+#include "qr_f64.hpp"
+
+namespace sslab {
+
+namespace {
+
+constexpr int kQrThreads = 256;
+constexpr int kQrWaves = kQrThreads / 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__global__ void __launch_bounds__(kQrThreads)
+k_geqrf(int m, int n, double* __restrict__ A, int lda, long long sA, double* __restrict__ Q,
+        int ldq, long long sQ, int* __restrict__ perm, long long sPerm) {
+  __shared__ double v[kQrMaxRows];
+  __shared__ double cn[kQrMaxCols];
+  __shared__ double s_tau, s_beta, s_scale;
+  __shared__ int s_piv;
+  const int b = blockIdx.x;
+  double* Ab = A + (long long)b * sA;
+  double* Qb = Q ? Q + (long long)b * sQ : nullptr;
+  int* pb = perm ? perm + (long long)b * sPerm : nullptr;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  if (Qb)
+    for (long long e = tid; e < (long long)m * m; e += kQrThreads) {
+      const int i = (int)(e / m), j = (int)(e % m);
+      Qb[(long long)i * ldq + j] = (i == j) ? 1.0 : 0.0;
+    }
+  if (pb)
+    for (int c = tid; c < n; c += kQrThreads) pb[c] = c;
+  __syncthreads();
+
+  const int kmax = m < n ? m : n;
+  for (int j = 0; j < kmax; ++j) {
+    if (pb) {
+      for (int c = j + wave; c < n; c += kQrWaves) {
+        double s = 0.0;
+        for (int i = j + lane; i < m; i += 64) {
+          const double a = Ab[(long long)i * lda + c];
+          s += a * a;
+        }
+        s = wave_sum(s);
+        if (lane == 0) cn[c] = s;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int p = j;
+        double best = cn[j];
+        for (int c = j + 1; c < n; ++c)
+          if (cn[c] > best) {
+            best = cn[c];
+            p = c;
+          }
+        s_piv = p;
+        if (p != j) {
+          const int t = pb[j];
+          pb[j] = pb[p];
+          pb[p] = t;
+        }
+      }
+      __syncthreads();
+      const int p = s_piv;
+      if (p != j)
+        for (int i = tid; i < m; i += kQrThreads) {
+          const double t = Ab[(long long)i * lda + j];
+          Ab[(long long)i * lda + j] = Ab[(long long)i * lda + p];
+          Ab[(long long)i * lda + p] = t;
+        }
+      __syncthreads();
+    }
+    // Reflector for x = A[j:, j]  (dlarfg): beta = -sign(alpha) ||x||, tau = (beta-alpha)/beta.
+    if (wave == 0) {
+      double s = 0.0;
+      for (int i = j + 1 + lane; i < m; i += 64) {
+        const double a = Ab[(long long)i * lda + j];
+        s += a * a;
+      }
+      s = wave_sum(s);
+      if (lane == 0) {
+        const double alpha = Ab[(long long)j * lda + j];
+        if (s == 0.0) {
+          s_tau = 0.0;
+          s_beta = alpha;
+          s_scale = 0.0;
+        } else {
+          const double nrm = sqrt(alpha * alpha + s);
+          const double beta = alpha >= 0.0 ? -nrm : nrm;
+          s_tau = (beta - alpha) / beta;
+          s_beta = beta;
+          s_scale = 1.0 / (alpha - beta);
+        }
+      }
+    }
+    __syncthreads();
+    const double tau = s_tau;
+    for (int i = j + tid; i < m; i += kQrThreads)
+      v[i] = (i == j) ? 1.0 : Ab[(long long)i * lda + j] * s_scale;
+    __syncthreads();
+    for (int i = j + tid; i < m; i += kQrThreads)
+      Ab[(long long)i * lda + j] = (i == j) ? s_beta : 0.0;
+    if (tau != 0.0) {
+      for (int c = j + 1 + wave; c < n; c += kQrWaves) {
+        double w = 0.0;
+        for (int i = j + lane; i < m; i += 64) w += v[i] * Ab[(long long)i * lda + c];
+        w = wave_sum(w) * tau;
+        for (int i = j + lane; i < m; i += 64) Ab[(long long)i * lda + c] -= w * v[i];
+      }
+      if (Qb)
+        for (int r = wave; r < m; r += kQrWaves) {
+          double w = 0.0;
+          for (int i = j + lane; i < m; i += 64) w += Qb[(long long)r * ldq + i] * v[i];
+          w = wave_sum(w) * tau;
+          for (int i = j + lane; i < m; i += 64) Qb[(long long)r * ldq + i] -= w * v[i];
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// Y[b][blk*bs + i][blk*bs + j] += X[b*nblk + blk][i][j]: add nblk diagonal bs x bs blocks.
+__global__ void k_add_diag_blocks(int bs, int nblk, const double* __restrict__ X,
+                                  double* __restrict__ Y, int ldy, long long sY) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  const long long c = blockIdx.z;  // child index
+  if (j >= bs) return;
+  const long long b = c / nblk, blk = c % nblk;
+  Y[b * sY + (blk * bs + i) * ldy + blk * bs + j] += X[c * bs * bs + (long long)i * bs + j];
+}
+
+}  // namespace
+
+int geqrf_batched(int m, int n, double* A, int lda, long long sA, double* Q, int ldq, long long sQ,
+                  int* perm, long long sPerm, int batch, hipStream_t st) {
+  SSLAB_REQUIRE(m >= 0 && n >= 0 && batch >= 0, "geqrf: negative size");
+  if (m == 0 || n == 0 || batch == 0) return kOk;
+  SSLAB_REQUIRE(m <= kQrMaxRows && n <= kQrMaxCols,
+                "geqrf: at most %d rows and %d columns (got %d x %d)", kQrMaxRows, kQrMaxCols, m,
+                n);
+  SSLAB_REQUIRE(lda >= n && (Q == nullptr || ldq >= m), "geqrf: bad leading dimension");
+  SSLAB_REQUIRE(batch <= 1 << 30, "geqrf: batch too large");
+  hipLaunchKernelGGL(k_geqrf, dim3(batch), dim3(kQrThreads), 0, st, m, n, A, lda, sA, Q, ldq, sQ,
+                     perm, sPerm);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int add_diag_blocks(int bs, int nblk, int nparent, const double* X, double* Y, int ldy,
+                    long long sY, hipStream_t st) {
+  if (bs <= 0 || nblk <= 0 || nparent <= 0) return kOk;
+  SSLAB_REQUIRE((long long)nblk * nparent <= 65535 && bs <= 65535,
+                "add_diag_blocks: too many blocks");
+  hipLaunchKernelGGL(k_add_diag_blocks, dim3(ceil_div(bs, 256), bs, nblk * nparent), dim3(256), 0,
+                     st, bs, nblk, X, Y, ldy, sY);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace sslab

@@ -1,0 +1,303 @@
+"""Device HBS matrices: randomized compression, telescoping apply and ULV solve.
+
+An HBS (hierarchically block-separable, weak admissibility) matrix in "reduced form"
+(matAssembly/HBS/HBSnew.py:65-139):
+
+    A = D_0 + U_0 (D_1 + U_1 ( ... D_root ... ) V_1^T) V_0^T
+
+with block-diagonal D_l (Nb_l blocks n_l x n_l), U_l, V_l (Nb_l blocks n_l x k), and
+n_{l+1} = fac * k.  Every level is one strided batch of small dense blocks, so each step
+of the compression (HBSnew.py:147-210), the apply (:303-388) and the ULV factorization
+and solve (ULVsparse.py:231-358) is a handful of batched launches of libsslab_hip.so:
+MFMA GEMMs, Householder / column-pivoted QR, triangular solves.
+
+Differences from the reference, on purpose:
+* the column basis of each box is taken from a column-pivoted QR of Y_i P_i (the
+  rank-revealing FP64 pivot selection) instead of an SVD; for an exactly rank-k block
+  both give the same subspace;
+* the triangular blocks of the ULV factors are inverted once at factorization time so
+  the solve is GEMMs only;
+* transposed solves factor A^T = D^T + V (...)^T U^T on first use.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_handle
+
+F64 = torch.float64
+I32 = torch.int32
+
+
+# ---------------------------------------------------------------------------
+# batched primitives on (nb, rows, cols) device tensors with unit last stride
+
+def _bstride(X):
+    return 0 if X.shape[0] == 1 else X.stride(0)
+
+
+def bgemm(A, B, out=None, alpha=1.0, beta=0.0, ta=False, tb=False):
+    """out_b = alpha op(A_b) op(B_b) + beta out_b for (nb, ., .) tensors (a batch of 1
+    broadcasts)."""
+    if A.stride(2) != 1:
+        A = A.contiguous()
+    if B.stride(2) != 1:
+        B = B.contiguous()
+    nb = max(A.shape[0], B.shape[0])
+    M = A.shape[2] if ta else A.shape[1]
+    K = A.shape[1] if ta else A.shape[2]
+    K2 = B.shape[2] if tb else B.shape[1]
+    N = B.shape[1] if tb else B.shape[2]
+    if K != K2:
+        raise ValueError(f"bgemm: inner dimensions {K} != {K2}")
+    if out is None:
+        out = torch.empty(nb, M, N, dtype=F64, device=A.device)
+        beta = 0.0
+    if out.stride(2) != 1 or tuple(out.shape) != (nb, M, N):
+        raise ValueError("bgemm: bad output")
+    call("sslab_dgemm_strided_batched", int(ta), int(tb), M, N, K, float(alpha), ptr(A),
+         A.stride(1), _bstride(A), ptr(B), B.stride(1), _bstride(B), float(beta), ptr(out),
+         out.stride(1), out.stride(0), nb, stream_handle())
+    return out
+
+
+def bqr(A, want_q=True, pivot=False):
+    """Batched Householder QR of (nb, m, n): returns (Q (nb, m, m) or None, R (nb, m, n),
+    perm (nb, n) or None) with A[:, :, perm] = Q R."""
+    R = A.contiguous().clone()
+    nb, m, n = R.shape
+    Q = torch.empty(nb, m, m, dtype=F64, device=A.device) if want_q else None
+    perm = torch.empty(nb, n, dtype=I32, device=A.device) if pivot else None
+    call("sslab_dgeqrf_batched", m, n, ptr(R), n, m * n, ptr(Q) if want_q else None, m, m * m,
+         ptr(perm) if pivot else None, n, nb, stream_handle())
+    return Q, R, perm
+
+
+def btri_inv(R):
+    """Inverse of a batch of upper-triangular (nb, n, n) blocks."""
+    nb, n, _ = R.shape
+    R = R.contiguous()
+    X = torch.eye(n, dtype=F64, device=R.device).expand(nb, n, n).contiguous()
+    if n:
+        call("sslab_dtrsm_upper_batched", n, n, ptr(R), n, n * n, ptr(X), n, n * n, nb,
+             stream_handle())
+    return X
+
+
+def _t(X):
+    return X.transpose(1, 2)
+
+
+# ---------------------------------------------------------------------------
+
+class DeviceHBS:
+    """HBS matrix with device-resident factors (lists ordered leaf level first)."""
+
+    def __init__(self, Umats, Dmats, Vmats, fac=2, perm=None):
+        self.Umats = list(Umats)
+        self.Dmats = list(Dmats)
+        self.Vmats = list(Vmats)
+        self.fac = fac
+        self.device = self.Dmats[0].device
+        N = self.Dmats[0].shape[0] * self.Dmats[0].shape[1]
+        self.N = N
+        self.perm = None if perm is None else torch.as_tensor(np.asarray(perm)).to(self.device).long()
+        self.Nbvec = [D.shape[0] for D in self.Dmats]
+        self._ulv = {}
+
+    # -- construction from sketches (HBSnew.py:147-210) --------------------------------
+    @staticmethod
+    def from_sketches(Om, Psi, Y, Z, Nb, L, rk, fac=2, perm=None):
+        """Om, Psi (N x s) test matrices and Y = A Om, Z = A^T Psi, all in leaf-permuted
+        order, on the device."""
+        N, s = Om.shape
+        nl = N // Nb
+        nb, n = Nb, nl
+        Oml, Psl, Yl, Zl = Om, Psi, Y, Z
+        Umats, Vmats, Dmats = [], [], []
+        U = V = D = None
+        for lvl in range(L - 1, -1, -1):
+            if lvl < L - 1:
+                O3 = Oml.view(nb, n, s)
+                P3 = Psl.view(nb, n, s)
+                Ynew = bgemm(D, O3, Yl.view(nb, n, s).clone(), -1.0, 1.0)
+                Yl = bgemm(U, Ynew, ta=True).reshape(-1, s)
+                Znew = bgemm(D, P3, Zl.view(nb, n, s).clone(), -1.0, 1.0, ta=True)
+                Zl = bgemm(V, Znew, ta=True).reshape(-1, s)
+                Oml = bgemm(V, O3, ta=True).reshape(-1, s)
+                Psl = bgemm(U, P3, ta=True).reshape(-1, s)
+                nb //= fac
+                n = Oml.shape[0] // nb
+            rkm = min(rk, nl * fac ** (L - 1 - lvl))
+            O3 = Oml.reshape(nb, n, s)
+            P3 = Psl.reshape(nb, n, s)
+            Y3 = Yl.reshape(nb, n, s)
+            Z3 = Zl.reshape(nb, n, s)
+            if lvl > 0:
+                U, pinvO = _basis_and_pinv(O3, Y3, rkm)
+                V, pinvP = _basis_and_pinv(P3, Z3, rkm)
+                T1 = bgemm(Y3, pinvO)
+                bgemm(U, bgemm(U, T1, ta=True), T1, -1.0, 1.0)
+                T2 = bgemm(Z3, pinvP)
+                bgemm(V, bgemm(V, T2, ta=True), T2, -1.0, 1.0)
+                D = bgemm(U, bgemm(U, T2, ta=True, tb=True), T1, 1.0, 1.0)
+                Umats.append(U)
+                Vmats.append(V)
+                Dmats.append(D)
+            else:
+                _, pinvO = _basis_and_pinv(O3, None, 0)
+                Dmats.append(bgemm(Y3, pinvO))
+        return DeviceHBS(Umats, Dmats, Vmats, fac, perm)
+
+    # -- apply (HBSnew.py:303-388) ----------------------------------------------------
+    def apply(self, x: torch.Tensor, mode="N") -> torch.Tensor:
+        one_d = x.dim() == 1
+        X = x.reshape(x.shape[0], -1).to(self.device, F64)
+        if self.perm is not None:
+            X = X[self.perm]
+        r = X.shape[1]
+        In, Out = (self.Vmats, self.Umats) if mode == "N" else (self.Umats, self.Vmats)
+        tD = mode != "N"
+        VV = [X.contiguous()]
+        for lvl, M in enumerate(In):
+            nb, n, k = M.shape
+            VV.append(bgemm(M, VV[lvl].view(nb, n, r), ta=True).reshape(nb * k, r))
+        Dr = self.Dmats[-1]
+        u = bgemm(Dr, VV[-1].view(Dr.shape[0], -1, r), ta=tD).reshape(-1, r)
+        for lvl in range(len(Out) - 1, -1, -1):
+            M = Out[lvl]
+            nb, n, k = M.shape
+            out = bgemm(M, u.view(nb, k, r))
+            bgemm(self.Dmats[lvl], VV[lvl].view(nb, n, r), out, 1.0, 1.0, ta=tD)
+            u = out.reshape(nb * n, r)
+        if self.perm is not None:
+            w = torch.empty_like(u)
+            w[self.perm] = u
+            u = w
+        return u[:, 0] if one_d else u
+
+    def dense(self):
+        return self.apply(torch.eye(self.N, dtype=F64, device=self.device))
+
+    @property
+    def nbytes(self):
+        return 8 * sum(M.numel() for M in self.Umats + self.Vmats + self.Dmats)
+
+    def transpose(self):
+        return DeviceHBS(self.Vmats, [_t(D).contiguous() for D in self.Dmats], self.Umats,
+                         self.fac, None if self.perm is None else self.perm.cpu().numpy())
+
+    # -- ULV (ULVsparse.py:231-358) -----------------------------------------------------
+    def ulv_factor(self, mode="N"):
+        if mode in self._ulv:
+            return self._ulv[mode]
+        if mode == "T":
+            f = self.transpose().ulv_factor("N")
+            self._ulv["T"] = f
+            return f
+        fac = self.fac
+        levels = []
+        Uhat = R22 = None
+        nlev = len(self.Dmats)
+        for lvl in range(nlev):
+            D = self.Dmats[lvl]
+            if lvl == 0:
+                Rhat = D
+            else:
+                nbp, nf, _ = D.shape
+                k = nf // fac
+                Rhat = bgemm(Uhat, D.reshape(nbp * fac, k, nf)).reshape(nbp, nf, nf)
+                call("sslab_add_diag_blocks", k, fac, nbp, ptr(R22.contiguous()), ptr(Rhat), nf,
+                     nf * nf, stream_handle())
+            if lvl == nlev - 1:
+                Q, R, _ = bqr(Rhat)
+                n = R.shape[1]
+                G = bgemm(btri_inv(R), Q, tb=True)       # R^{-1} Q^T
+                levels.append(dict(root=True, G=G, D=D))
+                break
+            V = self.Vmats[lvl]
+            U = self.Umats[lvl]
+            nb, n, k = V.shape
+            QV, _, _ = bqr(V)
+            W = torch.cat([QV[:, :, k:], V], dim=2).contiguous()
+            Q, R, _ = bqr(bgemm(Rhat, W))
+            if lvl == 0:
+                Uc = U
+            else:  # blkdiag(Ud of the fac children) U: one product per child row block
+                kc = Uhat.shape[2]
+                Uc = bgemm(Uhat, U.reshape(nb * fac, kc, k)).reshape(nb, n, k)
+            Uu = bgemm(Q[:, :, :n - k], Uc, ta=True) if n > k else None
+            Ud = bgemm(Q[:, :, n - k:], Uc, ta=True)
+            R22 = R[:, n - k:, n - k:].contiguous()
+            lev = dict(root=False, nb=nb, n=n, k=k, Q=Q, W=W, Uu=Uu,
+                       R12=R[:, :n - k, n - k:].contiguous(),
+                       R11inv=btri_inv(R[:, :n - k, :n - k].contiguous()), U=U, D=self.Dmats[lvl])
+            levels.append(lev)
+            Uhat = Ud
+        self._ulv[mode] = levels
+        return levels
+
+    def solve(self, b: torch.Tensor, mode="N") -> torch.Tensor:
+        """x with A x = b (mode N) or A^T x = b (mode T), b of shape (N,) or (N, r)."""
+        levels = self.ulv_factor(mode)
+        one_d = b.dim() == 1
+        B = b.reshape(b.shape[0], -1).to(self.device, F64)
+        if self.perm is not None:
+            B = B[self.perm]
+        r = B.shape[1]
+        cur = B.contiguous()
+        tops = []
+        for lev in levels[:-1]:
+            nb, n, k, Q = lev["nb"], lev["n"], lev["k"], lev["Q"]
+            c3 = cur.view(nb, n, r)
+            tops.append(bgemm(Q[:, :, :n - k], c3, ta=True) if n > k else None)
+            cur = bgemm(Q[:, :, n - k:], c3, ta=True).reshape(nb * k, r)
+        root = levels[-1]
+        y = bgemm(root["G"], cur.view(1, -1, r))
+        v = bgemm(root["D"], y)
+        y = y.reshape(-1, r)
+        v = v.reshape(-1, r)
+        for lvl in range(len(levels) - 2, -1, -1):
+            lev = levels[lvl]
+            nb, n, k = lev["nb"], lev["n"], lev["k"]
+            y2 = y.view(nb, k, r)
+            vb = v.view(nb, k, r)
+            x = torch.empty(nb, n, r, dtype=F64, device=self.device)
+            if n > k:
+                rhs0 = tops[lvl].clone()
+                bgemm(lev["Uu"], vb, rhs0, -1.0, 1.0)
+                bgemm(lev["R12"], y2, rhs0, -1.0, 1.0)
+                y1 = bgemm(lev["R11inv"], rhs0)
+                bgemm(lev["W"][:, :, :n - k], y1, x)
+                bgemm(lev["W"][:, :, n - k:], y2, x, 1.0, 1.0)
+            else:
+                bgemm(lev["W"], y2, x)
+            vn = bgemm(lev["U"], vb)
+            bgemm(lev["D"], x, vn, 1.0, 1.0)
+            v = vn.reshape(nb * n, r)
+            y = x.reshape(nb * n, r)
+        if self.perm is not None:
+            w = torch.empty_like(y)
+            w[self.perm] = y
+            y = w
+        return y[:, 0] if one_d else y
+
+
+def _basis_and_pinv(Om3, Y3, rk):
+    """Per box: complete QR of Om_i^T = Q R; the basis of Y_i Q[:, n:] (Y_i restricted
+    to the null space of Om_i, which removes the diagonal-block term, as in
+    matAssembly/HBS/HBStorch.py:73-82) from a column-pivoted QR (its first rk columns);
+    pinv(Om_i) = Q[:, :n] R^{-T}.  Needs s > n samples."""
+    nb, n, s = Om3.shape
+    if s <= n:
+        raise ValueError(f"HBS compression needs more samples than box rows ({s} <= {n})")
+    Q, R, _ = bqr(_t(Om3))                        # (nb, s, s), R (nb, s, n)
+    Rinv = btri_inv(R[:, :n, :n].contiguous())
+    pinv = bgemm(Q[:, :, :n], Rinv, tb=True)      # (nb, s, n)
+    if Y3 is None:
+        return None, pinv
+    M = bgemm(Y3, Q[:, :, n:])                    # (nb, n, s - n)
+    QM, _, _ = bqr(M, pivot=True)
+    return QM[:, :, :rk].contiguous(), pinv

@@ -132,9 +132,11 @@ def _is_pow2(k):
 
 
 class SlabGrid2D:
-    """Leaf tiling of [0,1]^2 by N single slabs of nx x ny leaves with q points."""
+    """Leaf tiling of the rectangle [x0, x0+width] x [y0, y0+height] (default the unit
+    square) by N single slabs of nx x ny leaves with q points."""
 
-    def __init__(self, N: int, nx: int, ny: int, q: int):
+    def __init__(self, N: int, nx: int, ny: int, q: int, x0: float = 0.0,
+                 width: float = 1.0, y0: float = 0.0, height: float = 1.0):
         if N < 2:
             raise ValueError("need at least two single slabs (one interface)")
         if not (_is_pow2(nx) and _is_pow2(ny)):
@@ -143,9 +145,10 @@ class SlabGrid2D:
             raise ValueError("q >= 3 Chebyshev points per leaf side")
         self.N, self.nx, self.ny, self.q = N, nx, ny, q
         self.p = q - 2
-        self.H = 1.0 / N
+        self.x0, self.y0, self.width, self.height = x0, y0, width, height
+        self.H = width / N
         self.hx = self.H / nx
-        self.hy = 1.0 / ny
+        self.hy = height / ny
         self.m = ny * self.p
         self.nb = N - 1
         self.nleaves = N * nx * ny
@@ -155,8 +158,8 @@ class SlabGrid2D:
     def leaf_origins(self):
         s, j, ix = np.meshgrid(np.arange(self.N), np.arange(self.ny), np.arange(self.nx),
                                indexing="ij")
-        x0 = s.ravel() * self.H + ix.ravel() * self.hx
-        y0 = j.ravel() * self.hy
+        x0 = self.x0 + s.ravel() * self.H + ix.ravel() * self.hx
+        y0 = self.y0 + j.ravel() * self.hy
         return x0, y0
 
     def interior_points(self):
@@ -174,18 +177,19 @@ class SlabGrid2D:
 
     def gb_points(self):
         """Global boundary nodes in the order [bottom | top | left | right]."""
-        xs = self.edge_coords(self.N * self.nx, self.hx)
-        ys = self.edge_coords(self.ny, self.hy)
-        bottom = np.stack([xs, np.zeros_like(xs)], 1)
-        top = np.stack([xs, np.ones_like(xs)], 1)
-        left = np.stack([np.zeros_like(ys), ys], 1)
-        right = np.stack([np.ones_like(ys), ys], 1)
+        xs = self.x0 + self.edge_coords(self.N * self.nx, self.hx)
+        ys = self.y0 + self.edge_coords(self.ny, self.hy)
+        bottom = np.stack([xs, np.full_like(xs, self.y0)], 1)
+        top = np.stack([xs, np.full_like(xs, self.y0 + self.height)], 1)
+        left = np.stack([np.full_like(ys, self.x0), ys], 1)
+        right = np.stack([np.full_like(ys, self.x0 + self.width), ys], 1)
         return np.concatenate([bottom, top, left, right])
 
     def interface_points(self):
-        """(nb*m, 2) interface nodes, interface i at x = (i+1) H, bottom to top."""
-        ys = self.edge_coords(self.ny, self.hy)
-        pts = [np.stack([np.full_like(ys, (i + 1) * self.H), ys], 1) for i in range(self.nb)]
+        """(nb*m, 2) interface nodes, interface i at x = x0 + (i+1) H, bottom to top."""
+        ys = self.y0 + self.edge_coords(self.ny, self.hy)
+        pts = [np.stack([np.full_like(ys, self.x0 + (i + 1) * self.H), ys], 1)
+               for i in range(self.nb)]
         return np.concatenate(pts)
 
 

@@ -1,0 +1,95 @@
+"""Matrix assembly of source-target maps: dense or HBS-compressed, on the MI355X.
+
+Mirror of matAssembly/matAssembler.py (SimonDirckx/SslabLU): matAssemblerOptions,
+matAssemblerStats, matAssembler.assemble(stMap, dbg) with the methods 'dense'
+(M = A I, matAssembler.py:93-98) and 'rkHBS' (binary slabTree on the target points,
+s = max(2k, leaf) + k + 10 Gaussian samples, HBS compression, :103-133), and the
+default assemblers denseMatAssembler / rkHMatAssembler (:213-219).  The strong-
+admissibility and legacy methods ('rkHBS_strong', 'rkHBSold', 'epsHBS') are not
+provided.
+
+The map stMap.A may be a float64 device tensor, a device operator (device_op = True)
+or a scipy LinearOperator; the result is a device tensor (dense) or an HBSMAT.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from .HBS import HBSnew
+from .HBS.HBSnew import _apply_op
+from .HBS.slabTree import slabTree
+
+
+class matAssemblerOptions:
+    """method: 'dense' (default) or 'rkHBS'; leaf_size; maxRank; tree; ndim; quad."""
+
+    def __init__(self, method: str = "dense", tol: float = 1e-5, leaf_size: int = 8,
+                 maxRank: int = 8, tree=None, ndim=3, quad=False, reduced=False):
+        self.method = method
+        self.tol = tol
+        self.maxRank = maxRank
+        self.tree = tree
+        self.leaf_size = leaf_size
+        self.ndim = ndim
+        self.reduced = reduced
+        self.quad = quad
+
+
+class matAssemblerStats:
+    def __init__(self):
+        self.timeSample = 0
+        self.timeCompress = 0
+        self.nbytes = 0
+
+
+class matAssembler:
+    def __init__(self, matOpts: matAssemblerOptions = None, seed: int = 0):
+        self.matOpts = matOpts if matOpts is not None else matAssemblerOptions()
+        self.nbytes = 0
+        self.stats = matAssemblerStats()
+        self.seed = seed
+
+    def assemble(self, stMap, dbg=0):
+        linOp = stMap.A
+        dev = torch.device("cuda", torch.cuda.current_device())
+        method = self.matOpts.method
+        if method == "dense":
+            t0 = time.time()
+            I = torch.eye(linOp.shape[1], dtype=torch.float64, device=dev)
+            M = _apply_op(linOp, I, False, dev)
+            torch.cuda.synchronize(dev)
+            self.stats.timeSample = time.time() - t0
+            self.stats.timeCompress = 0
+            self.stats.nbytes = M.numel() * 8
+            return M
+        if method == "rkHBS":
+            k = self.matOpts.maxRank
+            tree = slabTree(stMap.XXI, False, self.matOpts.leaf_size)
+            self.matOpts.tree = tree
+            if not tree.uniform():
+                raise ValueError("rkHBS needs a cluster tree with equally sized leaves")
+            H = HBSnew.HBSMAT(linOp, tree, quad=False, seed=self.seed)
+            s = max(2 * k, self.matOpts.leaf_size) + k + 10
+            H._samples = lambda rk, s=s: s
+            H.construct(k)
+            self.stats.timeSample = H.tSample
+            self.stats.timeCompress = H.tCompress
+            self.stats.nbytes = H.nbytes
+            if dbg > 0:
+                print("\t HBS: %d samples, sample %.2e s, compress %.2e s, %.2f MB"
+                      % (s, H.tSample, H.tCompress, H.nbytes / 1e6))
+            return H
+        raise NotImplementedError(f"assembly method {method!r} is not provided")
+
+
+class denseMatAssembler(matAssembler):
+    def __init__(self):
+        super().__init__(matAssemblerOptions())
+
+
+class rkHMatAssembler(matAssembler):
+    def __init__(self, leaf_size, rk, tree=None, ndim=3, quad=False):
+        super().__init__(matAssemblerOptions("rkHBS", 0, leaf_size, rk, tree, ndim, quad))

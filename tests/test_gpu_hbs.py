@@ -1,0 +1,89 @@
+"""GPU parity of the HBS engine: batched QR, compression of exactly-HBS operators,
+telescoping apply and ULV solves against the oracle (oracle/hbs.py).
+
+Tolerances: exactly-HBS reconstruction 1e-9 relative (randomized compression in FP64),
+ULV solve 1e-10 relative to numpy.linalg.solve of the dense HBS matrix.
+"""
+import numpy as np
+import pytest
+
+from oracle import hbs as oh
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if hasattr(a, "cpu") else a
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def T(gpu):
+    import torch
+    from sslablu_amd import _lib
+    _lib.load()
+    return torch
+
+
+@pytest.mark.parametrize("m,n,pivot", [(7, 5, False), (40, 40, False), (293, 72, False),
+                                       (72, 72, True), (50, 30, True), (30, 50, True)])
+def test_batched_qr(T, m, n, pivot):
+    from sslablu_amd.hbs import bqr
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((3, m, n))
+    if pivot:  # rank-deficient columns to exercise the pivoting
+        A[:, :, n // 2:] = A[:, :, : n - n // 2] @ rng.standard_normal((n - n // 2, n - n // 2))
+    Q, R, perm = bqr(T.tensor(A, device="cuda"), pivot=pivot)
+    Q, R = Q.cpu().numpy(), R.cpu().numpy()
+    for b in range(3):
+        Ab = A[b][:, perm[b].cpu().numpy()] if pivot else A[b]
+        assert np.allclose(Q[b].T @ Q[b], np.eye(m), atol=1e-12)
+        assert np.allclose(np.tril(R[b], -1), 0.0)
+        assert np.linalg.norm(Q[b] @ R[b] - Ab) / np.linalg.norm(Ab) < 1e-13
+        if pivot:
+            d = np.abs(np.diag(R[b]))
+            assert np.all(d[:-1] + 1e-9 * d[0] >= d[1:])  # non-increasing |R_jj|
+
+
+def _device_hbs(T, U, D, V, fac):
+    from sslablu_amd.hbs import DeviceHBS
+    t = lambda Xs: [T.tensor(X, device="cuda") for X in Xs]
+    return DeviceHBS(t(U), t(D), t(V), fac)
+
+
+@pytest.mark.parametrize("k,nl,L,fac", [(4, 8, 3, 2), (6, 10, 4, 2), (5, 8, 3, 4),
+                                        (16, 16, 4, 2)])
+def test_apply_and_compress(T, k, nl, L, fac):
+    from sslablu_amd.hbs import DeviceHBS
+    rng = np.random.default_rng(7)
+    U, D, V, Nbvec = oh.gen_random_hbs(k, nl, L, fac, rng)
+    A = oh.hbs_dense(U, D, V)
+    H = _device_hbs(T, U, D, V, fac)
+    x = rng.standard_normal((A.shape[0], 3))
+    xt = T.tensor(x, device="cuda")
+    assert rel(H.apply(xt), A @ x) < 1e-13
+    assert rel(H.apply(xt, "T"), A.T @ x) < 1e-13
+    # compression from sketches of the dense operator (same test matrices as the oracle)
+    N = A.shape[0]
+    s = 6 * k + 4 * nl + 5 if fac == 4 else 4 * k + 2 * nl + 5
+    Om = rng.standard_normal((N, s))
+    Ps = rng.standard_normal((N, s))
+    dev = lambda X: T.tensor(X, device="cuda")
+    Hc = DeviceHBS.from_sketches(dev(Om), dev(Ps), dev(A @ Om), dev(A.T @ Ps), Nbvec[0], L, k,
+                                 fac)
+    assert rel(Hc.dense(), A) < 1e-9
+
+
+@pytest.mark.parametrize("k,nl,L,fac", [(4, 8, 3, 2), (6, 10, 4, 2), (3, 6, 3, 4),
+                                        (8, 8, 2, 2), (16, 32, 4, 2)])
+def test_ulv_solve(T, k, nl, L, fac):
+    rng = np.random.default_rng(11)
+    U, D, V, Nbvec = oh.gen_random_hbs(k, nl, L, fac, rng)
+    D[0] = D[0] + 10 * np.eye(D[0].shape[1])
+    A = oh.hbs_dense(U, D, V)
+    H = _device_hbs(T, U, D, V, fac)
+    b = rng.standard_normal((A.shape[0], 4))
+    bt = T.tensor(b, device="cuda")
+    assert rel(H.solve(bt), np.linalg.solve(A, b)) < 1e-10
+    assert rel(H.solve(bt, "T"), np.linalg.solve(A.T, b)) < 1e-10
+    assert rel(H.solve(bt[:, 0]), np.linalg.solve(A, b[:, 0])) < 1e-10
