@@ -90,6 +90,26 @@ int sslab_rb_factor(int nb, int m, const double* T, const double* SiM, const dou
 int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
                    double* X, int ldx, void* stream);
 
+/* ---- mixed precision (FP32 red-black preconditioner + FP64 refinement) -------
+ * No reference counterpart: the reference is FP64 throughout.  The FP32 factor is
+ * the FP64 red-black factor rounded to float (same layout, factor_doubles floats). */
+int sslab_cast_f64_to_f32(long long n, const double* x, float* y, void* stream);
+/* Red-black solve with an FP32 factor; V (float, overwritten), X (float). */
+int sslab_rb_solve_f32(int nb, int m, int nrhs, const float* factor32, float* V, int ldv,
+                       float* X, int ldx, void* stream);
+/* R = B - S X (FP64) for S = diag(T) + sub/super-diagonal SiM / SiP blocks; T may be
+ * NULL for identity diagonal blocks (the oms S-system), then B, X, R must be
+ * contiguous (ld = nrhs). */
+int sslab_rb_residual(int nb, int m, int nrhs, const double* T, const double* SiM,
+                      const double* SiP, const double* B, int ldb, const double* X, int ldx,
+                      double* R, int ldr, void* stream);
+/* x = M b, then `iters` steps x += M (b - S x), M = the FP32 red-black solve.
+ * B, X: nb*m x nrhs contiguous doubles; work64: nb*m*nrhs doubles; work32:
+ * 2*nb*m*nrhs floats. */
+int sslab_rb_solve_mixed(int nb, int m, int nrhs, const float* factor32, const double* T,
+                         const double* SiM, const double* SiP, const double* B, double* X,
+                         double* work64, float* work32, int iters, void* stream);
+
 /* ---- Thomas block-tridiagonal solver ---------------------------------------
  * Same system; sequential block elimination with explicit block inverses.
  * Replaces BlockTridiagonalSolver.factorize_helper / .solve_helper

@@ -153,6 +153,50 @@ class BlockTriFactor:
         self.solve_(V, X, stream)
         return X[:, 0] if one_d else X
 
+    # -- mixed precision --------------------------------------------------------------
+    def enable_mixed(self, T=None, SiM=None, SiP=None, stream=None):
+        """Keep an FP32 copy of the red-black factor (the refinement preconditioner) and
+        the FP64 blocks the refinement residual needs (T = None: identity diagonal)."""
+        if self.method != "rb":
+            raise ValueError("mixed precision uses the red-black factorization")
+        if self.cyclic:
+            raise NotImplementedError("mixed-precision solves of periodic systems")
+        self.factor32 = torch.empty(self.factor.numel(), dtype=torch.float32, device=self.device)
+        call("sslab_cast_f64_to_f32", self.factor.numel(), ptr(self.factor), ptr(self.factor32),
+             stream_handle(stream))
+        nb, m = self.nb, self.m
+        self._T64 = None if T is None else _dev_blocks(T, nb, m, self.device)
+        self._M64 = _dev_blocks(SiM, nb, m, self.device)
+        self._P64 = _dev_blocks(SiP, nb, m, self.device)
+        return self
+
+    def solve_mixed_(self, B: torch.Tensor, X: torch.Tensor, iters: int = 2, stream=None,
+                     work=None):
+        """X = S^{-1} B by FP32 red-black solves refined with FP64 residuals
+        (`iters` refinement steps).  B, X: contiguous float64 (N, r) device tensors."""
+        nb, m = self.nb, self.m
+        N, r = B.shape
+        if N != nb * m or tuple(X.shape) != (N, r) or not (B.is_contiguous() and X.is_contiguous()):
+            raise ValueError("solve_mixed_: contiguous (nb*m, r) float64 B and X required")
+        if work is None:
+            work = (torch.empty(N * r, dtype=F64, device=self.device),
+                    torch.empty(2 * N * r, dtype=torch.float32, device=self.device))
+        w64, w32 = work
+        call("sslab_rb_solve_mixed", nb, m, r, ptr(self.factor32),
+             ptr(self._T64) if self._T64 is not None else None, ptr(self._M64), ptr(self._P64),
+             ptr(B), ptr(X), ptr(w64), ptr(w32), int(iters), stream_handle(stream))
+        return X
+
+    def residual(self, B: torch.Tensor, X: torch.Tensor, stream=None) -> torch.Tensor:
+        """R = B - S X in FP64 (requires enable_mixed for the stored blocks)."""
+        R = torch.empty_like(B)
+        nb, m = self.nb, self.m
+        r = B.shape[1]
+        call("sslab_rb_residual", nb, m, r, ptr(self._T64) if self._T64 is not None else None,
+             ptr(self._M64), ptr(self._P64), ptr(B), r, ptr(X), r, ptr(R), r,
+             stream_handle(stream))
+        return R
+
     # -- introspection ----------------------------------------------------------
     @property
     def nbytes(self) -> int:

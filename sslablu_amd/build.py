@@ -29,6 +29,8 @@ HIP_FLAGS = [
     "-Wno-unused-function",
     "-Wno-unused-variable",
     "-Wno-unused-but-set-variable",
+    # FP64 atomic add (split-K GEMM partial sums) as the native global atomic.
+    "-munsafe-fp-atomics",
     # Packed FP32 instructions are disabled for all device code.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}",

@@ -25,7 +25,7 @@
 
 #include <vector>
 
-#include "gemm_f64.hpp"
+#include "gemm.hpp"
 #include "lu_f64.hpp"
 
 namespace sslab {
@@ -217,8 +217,12 @@ int rb_info_count(int nb) {
   return c + 1;
 }
 
-int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, double* X, int ldx,
-             hipStream_t st) {
+// The solve sweeps, for factors (and right-hand sides) stored in FP64 or FP32.
+template <typename T>
+static int rb_solve_t(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
+                      hipStream_t st) {
+  using GemmLaunch = GemmLaunchT<T>;
+  using GemmDesc = GemmDescT<T>;
   SSLAB_REQUIRE(nb >= 1 && m >= 1 && r >= 0, "rb_solve: bad sizes");
   SSLAB_REQUIRE(ldv >= r && ldx >= r, "rb_solve: leading dimension < nrhs");
   SSLAB_REQUIRE(factor && V && X && V != X, "rb_solve: null or aliased buffers");
@@ -227,13 +231,17 @@ int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, dou
   const long long m2 = (long long)m * m;
   const long long bv = (long long)m * ldv;  // one block row of V
   const long long bx = (long long)m * ldx;
+  // Every slot of X is written exactly once by a product below; starting from zero
+  // lets those products accumulate (C += ...) so a split-K launch needs no init pass.
+  SSLAB_TRY(zero_2d<T>((long long)nb * m, r, X, ldx, st));
   for (int l = 0; l < p.L; ++l) {
     const RbLevel& v = p.lv[l];
     const int n = v.n, nE = v.nE, nO = v.nO;
-    const double* TG = factor + v.tg_off;
-    const double* WW = factor + v.ww_off;
+    const T* TG = factor + v.tg_off;
+    const T* WW = factor + v.ww_off;
     const long long sv = (2LL << l) * bv, sx = (2LL << l) * bx;  // stride between same-parity nodes
     const long long ov = (1LL << l) * bv, ox = (1LL << l) * bx;  // offset of the first odd node
+    (void)n;
     GemmLaunch G{};
     int k = 0;
     const int both = (nE < nO ? nE : nO) - 1;
@@ -256,20 +264,20 @@ int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, dou
                                  -1.0));
       G.d[k++] = d;
     }
-    {  // odd: y_j = Tinv_j v_j into X
-      GemmDesc d = gemm_desc(m, r, nO, X + ox, ldx, sx, nullptr, 0, 0, 0.0);
+    {  // odd: y_j = Tinv_j v_j into X (zero-initialised)
+      GemmDesc d = gemm_desc(m, r, nO, X + ox, ldx, sx, X + ox, ldx, sx, 1.0);
       gemm_add_term(d, gemm_term(TG, 3 * m, 3 * m2, 0, V + ov, ldv, sv, 0, m, 1.0));
       G.d[k++] = d;
     }
     G.nd = k;
     SSLAB_TRY(gemm_launch(G, st));
   }
-  SSLAB_TRY(gemm_strided(0, 0, m, r, m, 1.0, factor + p.top_off, m, m2, V, ldv, 0, 0.0, X, ldx, 0,
-                         1, st));
+  SSLAB_TRY(gemm_strided<T>(0, 0, m, r, m, 1.0, factor + p.top_off, m, m2, V, ldv, 0, 1.0, X, ldx,
+                            0, 1, st));
   for (int l = p.L - 1; l >= 0; --l) {
     const RbLevel& v = p.lv[l];
     const int n = v.n, nE = v.nE, nO = v.nO;
-    const double* TG = factor + v.tg_off;
+    const T* TG = factor + v.tg_off;
     const long long sx = (2LL << l) * bx;
     const long long ox = (1LL << l) * bx;
     GemmLaunch G{};
@@ -291,6 +299,89 @@ int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, dou
     }
     G.nd = k;
     SSLAB_TRY(gemm_launch(G, st));
+  }
+  return kOk;
+}
+
+int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, double* X, int ldx,
+             hipStream_t st) {
+  return rb_solve_t<double>(nb, m, r, factor, V, ldv, X, ldx, st);
+}
+
+int rb_solve_f32(int nb, int m, int r, const float* factor, float* V, int ldv, float* X, int ldx,
+                 hipStream_t st) {
+  return rb_solve_t<float>(nb, m, r, factor, V, ldv, X, ldx, st);
+}
+
+// r = b - S x for the block system (T = NULL means identity diagonal blocks), FP64.
+int rb_residual(int nb, int m, int r, const double* T, const double* SiM, const double* SiP,
+                const double* B, int ldb, const double* X, int ldx, double* R, int ldr,
+                hipStream_t st) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1 && r >= 0 && SiM && SiP && B && X && R,
+                "rb_residual: bad arguments");
+  if (r == 0) return kOk;
+  const long long m2 = (long long)m * m;
+  const long long bb = (long long)m * ldb, bx = (long long)m * ldx, br = (long long)m * ldr;
+  GemmLaunch G{};
+  int k = 0;
+  const double* Cin = B;
+  int ldcin = ldb;
+  long long scin = bb;
+  if (T == nullptr) {  // R = B - X first, then the couplings accumulate into R
+    SSLAB_REQUIRE(ldb == r && ldx == r && ldr == r, "rb_residual: contiguous buffers required");
+    SSLAB_TRY(sub_f64((long long)nb * m * r, B, X, R, st));
+    Cin = R;
+    ldcin = ldr;
+    scin = br;
+  }
+  auto add_diag = [&](GemmDesc& d, long long i0) {
+    if (T) gemm_add_term(d, gemm_term(T + i0 * m2, m, m2, 0, X + i0 * bx, ldx, bx, 0, m, -1.0));
+  };
+  if (nb >= 3) {  // interior blocks: both neighbours
+    GemmDesc d = gemm_desc(m, r, nb - 2, R + br, ldr, br, Cin + scin, ldcin, scin, 1.0);
+    gemm_add_term(d, gemm_term(SiM + m2, m, m2, 0, X, ldx, bx, 0, m, -1.0));
+    gemm_add_term(d, gemm_term(SiP + m2, m, m2, 0, X + 2 * bx, ldx, bx, 0, m, -1.0));
+    add_diag(d, 1);
+    G.d[k++] = d;
+  }
+  {
+    GemmDesc d = gemm_desc(m, r, 1, R, ldr, br, Cin, ldcin, scin, 1.0);
+    if (nb >= 2) gemm_add_term(d, gemm_term(SiP, m, m2, 0, X + bx, ldx, bx, 0, m, -1.0));
+    add_diag(d, 0);
+    if (d.nterms) G.d[k++] = d;
+  }
+  if (nb >= 2) {
+    const long long i = nb - 1;
+    GemmDesc d = gemm_desc(m, r, 1, R + i * br, ldr, br, Cin + i * scin, ldcin, scin, 1.0);
+    gemm_add_term(d, gemm_term(SiM + i * m2, m, m2, 0, X + (i - 1) * bx, ldx, bx, 0, m, -1.0));
+    add_diag(d, i);
+    G.d[k++] = d;
+  }
+  G.nd = k;
+  if (k) SSLAB_TRY(gemm_launch(G, st));
+  return kOk;
+}
+
+// Mixed-precision solve: the red-black factor stored in FP32 is the preconditioner of
+// an iterative refinement whose residual r = b - S x is FP64:
+//   x = M b;  repeat `iters` times:  r = b - S x (FP64),  x += M r.
+// B, X: N x r (contiguous, N = nb*m); R64: N x r doubles; V32, X32: N x r floats each.
+int rb_solve_mixed(int nb, int m, int r, const float* factor32, const double* T,
+                   const double* SiM, const double* SiP, const double* B, double* X, double* R64,
+                   float* V32, float* X32, int iters, hipStream_t st) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1 && r >= 0 && iters >= 0, "rb_solve_mixed: bad sizes");
+  SSLAB_REQUIRE(factor32 && B && X && V32 && X32 && (iters == 0 || (R64 && SiM && SiP)),
+                "rb_solve_mixed: null pointer");
+  if (r == 0) return kOk;
+  const long long n = (long long)nb * m * r;
+  SSLAB_TRY(cast_f64_to_f32(n, B, V32, st));
+  SSLAB_TRY(rb_solve_t<float>(nb, m, r, factor32, V32, r, X32, r, st));
+  SSLAB_TRY(cast_f32_to_f64(n, X32, X, st));
+  for (int it = 0; it < iters; ++it) {
+    SSLAB_TRY(rb_residual(nb, m, r, T, SiM, SiP, B, r, X, r, R64, r, st));
+    SSLAB_TRY(cast_f64_to_f32(n, R64, V32, st));
+    SSLAB_TRY(rb_solve_t<float>(nb, m, r, factor32, V32, r, X32, r, st));
+    SSLAB_TRY(acc_f32_into_f64(n, X32, X, st));
   }
   return kOk;
 }

@@ -12,6 +12,14 @@ int rb_factor(int nb, int m, const double* T, const double* SiM, const double* S
               double* factor, double* work, int* iwork, int* info, hipStream_t st);
 int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, double* X, int ldx,
              hipStream_t st);
+int rb_solve_f32(int nb, int m, int r, const float* factor, float* V, int ldv, float* X, int ldx,
+                 hipStream_t st);
+int rb_residual(int nb, int m, int r, const double* T, const double* SiM, const double* SiP,
+                const double* B, int ldb, const double* X, int ldx, double* R, int ldr,
+                hipStream_t st);
+int rb_solve_mixed(int nb, int m, int r, const float* factor32, const double* T,
+                   const double* SiM, const double* SiP, const double* B, double* X, double* R64,
+                   float* V32, float* X32, int iters, hipStream_t st);
 
 int thomas_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
                  long long* work_ints);

@@ -3,7 +3,7 @@
 
 #include "../../include/sslab_hip.h"
 #include "blocktri_f64.hpp"
-#include "gemm_f64.hpp"
+#include "gemm.hpp"
 #include "hps_f64.hpp"
 #include "lu_f64.hpp"
 #include "oms_f64.hpp"
@@ -101,6 +101,29 @@ int sslab_rb_factor(int nb, int m, const double* T, const double* SiM, const dou
 int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv, double* X,
                    int ldx, void* stream) {
   return sslab::rb_solve(nb, m, nrhs, factor, V, ldv, X, ldx, as_stream(stream));
+}
+
+int sslab_cast_f64_to_f32(long long n, const double* x, float* y, void* stream) {
+  return sslab::cast_f64_to_f32(n, x, y, as_stream(stream));
+}
+
+int sslab_rb_solve_f32(int nb, int m, int nrhs, const float* factor32, float* V, int ldv,
+                       float* X, int ldx, void* stream) {
+  return sslab::rb_solve_f32(nb, m, nrhs, factor32, V, ldv, X, ldx, as_stream(stream));
+}
+
+int sslab_rb_residual(int nb, int m, int nrhs, const double* T, const double* SiM,
+                      const double* SiP, const double* B, int ldb, const double* X, int ldx,
+                      double* R, int ldr, void* stream) {
+  return sslab::rb_residual(nb, m, nrhs, T, SiM, SiP, B, ldb, X, ldx, R, ldr, as_stream(stream));
+}
+
+int sslab_rb_solve_mixed(int nb, int m, int nrhs, const float* factor32, const double* T,
+                         const double* SiM, const double* SiP, const double* B, double* X,
+                         double* work64, float* work32, int iters, void* stream) {
+  const long long n = (long long)nb * m * nrhs;
+  return sslab::rb_solve_mixed(nb, m, nrhs, factor32, T, SiM, SiP, B, X, work64, work32,
+                               work32 ? work32 + n : nullptr, iters, as_stream(stream));
 }
 
 int sslab_thomas_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,

@@ -1,0 +1,474 @@
+// Grouped / batched GEMM for gfx950 on MFMA, FP64 and FP32.
+//
+// Tiling: a workgroup computes a BM x BN tile of C with WM x WN wavefronts (64
+// lanes each); a wavefront owns a (BM/WM) x (BN/WN) sub-tile made of 16x16 MFMA
+// fragments.  K is consumed in BK-wide slabs staged through LDS, k-major with a
+// pad, and the next slab is prefetched into registers while the current one is
+// multiplied.  Launches with few output tiles but long K split K over several
+// workgroups that accumulate with native global FP atomics.
+//
+// Operand lane layouts (one element per lane):
+//   A (16x4):  lane l holds A[l % 16][l / 16];   B (4x16): lane l holds B[l / 16][l % 16]
+//   D (16x16): lane l, item v holds
+//     v_mfma_f64_16x16x4_f64:  D[(l/16) + 4*v][l % 16]
+//     v_mfma_f32_16x16x4_f32:  D[4*(l/16) + v][l % 16]
+// (both measured on gfx950).
+#include "gemm.hpp"
+
+namespace sslab {
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+template <int BM_, int BN_, int BK_, int WM_, int WN_>
+struct GemmCfg {
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+  static constexpr int NT = WM * WN * 64;
+  static constexpr int WTM = BM / WM, WTN = BN / WN;  // wavefront tile
+  static constexpr int FM = WTM / 16, FN = WTN / 16;   // MFMA fragments per wave
+  static constexpr int LDA_S = BM + 16;                // padded k-major LDS rows
+  static constexpr int LDB_S = BN + 16;
+  static constexpr int A_PER_T = BM * BK / NT;
+  static constexpr int B_PER_T = BK * BN / NT;
+  static_assert(BM % (WM * 16) == 0 && BN % (WN * 16) == 0, "tile/wave mismatch");
+  static_assert(BK % 4 == 0, "BK must be a multiple of the MFMA K (4)");
+  static_assert((BM * BK) % NT == 0 && (BK * BN) % NT == 0, "load split");
+};
+
+// Map the hardware block id so that contiguous ranges of logical tiles land on the
+// same XCD (the dispatcher assigns workgroups round-robin to the 8 XCDs, each with
+// its own L2).  Bijective on [0, n).
+__device__ __forceinline__ int xcd_remap(int bid, int n) {
+  const int q = n >> 3;
+  if (bid < (q << 3)) return (bid & 7) * q + (bid >> 3);
+  return bid;
+}
+
+template <class Cfg, typename T>
+__device__ __forceinline__ void load_a(const GemmTermT<T>& t, const T* A, int M, int row0, int k0,
+                                       int K, int tid, T (&ra)[Cfg::A_PER_T]) {
+  if (!t.transA) {  // contiguous along k
+#pragma unroll
+    for (int r = 0; r < Cfg::A_PER_T; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int k = e % Cfg::BK, i = e / Cfg::BK;
+      const int gi = row0 + i, gk = k0 + k;
+      ra[r] = (gi < M && gk < K) ? A[(long long)gi * t.lda + gk] : T(0);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < Cfg::A_PER_T; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int i = e % Cfg::BM, k = e / Cfg::BM;
+      const int gi = row0 + i, gk = k0 + k;
+      ra[r] = (gi < M && gk < K) ? A[(long long)gk * t.lda + gi] : T(0);
+    }
+  }
+}
+
+template <class Cfg, typename T>
+__device__ __forceinline__ void store_a(const GemmTermT<T>& t, int tid, T alpha,
+                                        const T (&ra)[Cfg::A_PER_T], T* As) {
+#pragma unroll
+  for (int r = 0; r < Cfg::A_PER_T; ++r) {
+    const int e = tid + r * Cfg::NT;
+    int k, i;
+    if (!t.transA) {
+      k = e % Cfg::BK;
+      i = e / Cfg::BK;
+    } else {
+      i = e % Cfg::BM;
+      k = e / Cfg::BM;
+    }
+    As[k * Cfg::LDA_S + i] = alpha * ra[r];
+  }
+}
+
+template <class Cfg, typename T>
+__device__ __forceinline__ void load_b(const GemmTermT<T>& t, const T* B, int N, int col0, int k0,
+                                       int K, int tid, T (&rb)[Cfg::B_PER_T]) {
+  if (!t.transB) {  // contiguous along j
+#pragma unroll
+    for (int r = 0; r < Cfg::B_PER_T; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int j = e % Cfg::BN, k = e / Cfg::BN;
+      const int gj = col0 + j, gk = k0 + k;
+      rb[r] = (gj < N && gk < K) ? B[(long long)gk * t.ldb + gj] : T(0);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < Cfg::B_PER_T; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int k = e % Cfg::BK, j = e / Cfg::BK;
+      const int gj = col0 + j, gk = k0 + k;
+      rb[r] = (gj < N && gk < K) ? B[(long long)gj * t.ldb + gk] : T(0);
+    }
+  }
+}
+
+template <class Cfg, typename T>
+__device__ __forceinline__ void store_b(const GemmTermT<T>& t, int tid, const T (&rb)[Cfg::B_PER_T],
+                                        T* Bs) {
+#pragma unroll
+  for (int r = 0; r < Cfg::B_PER_T; ++r) {
+    const int e = tid + r * Cfg::NT;
+    int k, j;
+    if (!t.transB) {
+      j = e % Cfg::BN;
+      k = e / Cfg::BN;
+    } else {
+      k = e % Cfg::BK;
+      j = e / Cfg::BK;
+    }
+    Bs[k * Cfg::LDB_S + j] = rb[r];
+  }
+}
+
+template <class Cfg, typename T>
+__global__ void __launch_bounds__(Cfg::NT) k_gemm(const GemmLaunchT<T> L, int total_tiles) {
+  using M4 = Mfma<T>;
+  using v4 = typename M4::v4;
+  __shared__ T As[Cfg::BK * Cfg::LDA_S];
+  __shared__ T Bs[Cfg::BK * Cfg::LDB_S];
+
+  const int tile = xcd_remap(blockIdx.x, total_tiles);
+
+  // Select the descriptor with a fully unrolled scan (constant kernarg offsets).
+  GemmDescT<T> D = L.d[0];
+#pragma unroll
+  for (int d = 1; d < kMaxDesc; ++d)
+    if (d < L.nd && tile >= L.d[d].tile_begin) D = L.d[d];
+
+  const int per_part = D.tiles_m * D.tiles_n;
+  int local = tile - D.tile_begin;
+  const int bk = local / per_part;  // (batch, k part) pair
+  local -= bk * per_part;
+  const int b = bk / D.ksplit;
+  const int kp = bk - b * D.ksplit;
+  const int kbeg = kp * D.kchunk, kfin = kbeg + D.kchunk;  // K range over the summed terms
+  const int tm = local / D.tiles_n;
+  const int tn = local - tm * D.tiles_n;
+  const int row0 = tm * Cfg::BM, col0 = tn * Cfg::BN;
+  const int M = D.M, N = D.N;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave / Cfg::WN, wn = wave % Cfg::WN;
+  const int wrow = wm * Cfg::WTM, wcol = wn * Cfg::WTN;
+
+  v4 acc[Cfg::FM][Cfg::FN];
+#pragma unroll
+  for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = v4{T(0), T(0), T(0), T(0)};
+
+  T ra[Cfg::A_PER_T], rb[Cfg::B_PER_T];
+
+  int koff = 0;  // start of the current term in the summed K range
+#pragma unroll
+  for (int ti = 0; ti < kMaxTerms; ++ti) {
+    if (ti >= D.nterms) break;
+    const GemmTermT<T> Tm = D.term[ti];
+    const T* A = Tm.A + (long long)b * Tm.strideA;
+    const T* B = Tm.B + (long long)b * Tm.strideB;
+    // this workgroup's part of the term: [ks, ke) in the term's own K coordinates
+    const int ks = max(kbeg - koff, 0);
+    const int ke = min(kfin - koff, Tm.K);
+    koff += Tm.K;
+    if (ke <= ks) continue;
+    load_a<Cfg, T>(Tm, A, M, row0, ks, ke, tid, ra);
+    load_b<Cfg, T>(Tm, B, N, col0, ks, ke, tid, rb);
+    for (int k0 = ks; k0 < ke; k0 += Cfg::BK) {
+      __syncthreads();
+      store_a<Cfg, T>(Tm, tid, Tm.alpha, ra, As);
+      store_b<Cfg, T>(Tm, tid, rb, Bs);
+      __syncthreads();
+      if (k0 + Cfg::BK < ke) {
+        load_a<Cfg, T>(Tm, A, M, row0, k0 + Cfg::BK, ke, tid, ra);
+        load_b<Cfg, T>(Tm, B, N, col0, k0 + Cfg::BK, ke, tid, rb);
+      }
+#pragma unroll
+      for (int kk = 0; kk < Cfg::BK; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        T af[Cfg::FM], bf[Cfg::FN];
+#pragma unroll
+        for (int i = 0; i < Cfg::FM; ++i) af[i] = As[kr * Cfg::LDA_S + wrow + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < Cfg::FN; ++j) bf[j] = Bs[kr * Cfg::LDB_S + wcol + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+          for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = M4::op(af[i], bf[j], acc[i][j]);
+      }
+    }
+  }
+
+  T* C = D.C + (long long)b * D.strideC;
+  const T* Cin = D.Cin ? D.Cin + (long long)b * D.strideCin : nullptr;
+  const T beta = D.beta;
+  const bool use_cin = (Cin != nullptr) && (beta != T(0));
+  const bool split = D.ksplit > 1;  // C already holds beta*Cin: accumulate atomically
+#pragma unroll
+  for (int i = 0; i < Cfg::FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < Cfg::FN; ++j) {
+      const int col = col0 + wcol + j * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row0 + wrow + i * 16 + M4::row(lane, v);
+        if (row < M && col < N) {
+          T val = acc[i][j][v];
+          if (split) {
+            atomicAdd(&C[(long long)row * D.ldc + col], val);
+          } else {
+            if (use_cin) val += beta * Cin[(long long)row * D.ldcin + col];
+            C[(long long)row * D.ldc + col] = val;
+          }
+        }
+      }
+    }
+  }
+}
+
+using Cfg128 = GemmCfg<128, 128, 16, 2, 2>;
+using Cfg128x64 = GemmCfg<128, 64, 16, 2, 2>;
+using Cfg64 = GemmCfg<64, 64, 32, 2, 2>;
+using Cfg64x32 = GemmCfg<64, 32, 16, 2, 2>;
+using Cfg32 = GemmCfg<32, 32, 16, 2, 2>;
+
+// Workgroups a launch would create with BM x BN tiles.
+template <typename T>
+static long long count_tiles(const GemmLaunchT<T>& L, int BM, int BN) {
+  long long t = 0;
+  for (int i = 0; i < L.nd; ++i) {
+    const GemmDescT<T>& d = L.d[i];
+    if (d.batch > 0 && d.M > 0 && d.N > 0)
+      t += (long long)ceil_div(d.M, BM) * ceil_div(d.N, BN) * d.batch;
+  }
+  return t;
+}
+
+// C_b = beta * Cin_b (Cin may be null or alias C): the initial value of a split-K
+// output, which the K parts then accumulate into atomically.
+template <typename T>
+__global__ void k_scale_init(int M, int N, const T* __restrict__ Cin, int ldcin, long long sCin,
+                             T beta, T* __restrict__ C, int ldc, long long sC) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long b = blockIdx.z;
+  if (j >= N) return;
+  for (int i = blockIdx.y; i < M; i += gridDim.y) {
+    const T v = (Cin != nullptr && beta != T(0)) ? beta * Cin[b * sCin + (long long)i * ldcin + j]
+                                                  : T(0);
+    C[b * sC + (long long)i * ldc + j] = v;
+  }
+}
+
+template <typename T>
+static long long total_k(const GemmDescT<T>& d) {
+  long long k = 0;
+  for (int t = 0; t < d.nterms; ++t) k += d.term[t].K;
+  return k;
+}
+
+// ksplit > 1 splits every output tile's summed K range over that many workgroups.
+template <class Cfg, typename T>
+static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit) {
+  int total = 0;
+  GemmLaunchT<T> P{};
+  P.nd = 0;
+  for (int i = 0; i < L.nd; ++i) {
+    GemmDescT<T> d = L.d[i];
+    if (d.batch <= 0 || d.M <= 0 || d.N <= 0) continue;
+    if (d.beta != T(0) && d.Cin == nullptr) {
+      set_error("gemm: beta != 0 requires Cin");
+      return kBadArgument;
+    }
+    const long long K = total_k(d);
+    int ks = ksplit;
+    if (K < (long long)ks * 2 * Cfg::BK) ks = (int)(K / (2 * Cfg::BK));  // >= 2 slabs per part
+    if (ks < 1) ks = 1;
+    d.kchunk = (ceil_div((int)K, ks) + Cfg::BK - 1) / Cfg::BK * Cfg::BK;
+    if (d.kchunk < Cfg::BK) d.kchunk = Cfg::BK;
+    d.ksplit = ks > 1 ? ceil_div((int)K, d.kchunk) : 1;
+    if (d.ksplit == 1) d.kchunk = (int)(K > 0 ? K : 1);
+    if (d.ksplit > 1 && !(d.Cin == d.C && d.beta == T(1) && d.ldcin == d.ldc &&
+                          d.strideCin == d.strideC)) {
+      SSLAB_REQUIRE(d.batch <= 65535, "gemm: split-K init batch too large");
+      const int gy = d.M < 4096 ? d.M : 4096;
+      hipLaunchKernelGGL(k_scale_init<T>, dim3(ceil_div(d.N, 256), gy, d.batch), dim3(256), 0,
+                         stream, d.M, d.N, d.Cin, d.ldcin, d.strideCin, d.beta, d.C, d.ldc,
+                         d.strideC);
+      SSLAB_HIP_CHECK(hipGetLastError());
+    }
+    d.tiles_m = ceil_div(d.M, Cfg::BM);
+    d.tiles_n = ceil_div(d.N, Cfg::BN);
+    d.tile_begin = total;
+    long long add = (long long)d.tiles_m * d.tiles_n * d.batch * d.ksplit;
+    if (add + total > 0x7fffffffLL) {
+      set_error("gemm: too many tiles in one launch");
+      return kBadArgument;
+    }
+    total += (int)add;
+    P.d[P.nd++] = d;
+  }
+  if (total == 0) return kOk;
+  hipLaunchKernelGGL((k_gemm<Cfg, T>), dim3(total), dim3(Cfg::NT), 0, stream, P, total);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+template <typename T>
+int gemm_launch(GemmLaunchT<T>& L, hipStream_t stream) {
+  SSLAB_REQUIRE(L.nd >= 0 && L.nd <= kMaxDesc, "gemm: bad descriptor count %d", L.nd);
+  long long kmax = 0;
+  for (int i = 0; i < L.nd; ++i) {
+    const GemmDescT<T>& d = L.d[i];
+    SSLAB_REQUIRE(d.nterms >= 1 && d.nterms <= kMaxTerms, "gemm: bad term count");
+    for (int t = 0; t < d.nterms; ++t) {
+      const GemmTermT<T>& Tm = d.term[t];
+      SSLAB_REQUIRE(Tm.A && Tm.B, "gemm: null operand");
+      SSLAB_REQUIRE(Tm.K >= 0, "gemm: negative K");
+    }
+    SSLAB_REQUIRE(total_k(d) < 0x7fffffffLL, "gemm: K too large");
+    if (d.batch > 0 && total_k(d) > kmax) kmax = total_k(d);
+  }
+  // Largest tile that still gives every CU (256 on MI355X) at least two workgroups.
+  // Launches with few output tiles but long K (deep red-black levels, few right-hand
+  // sides) split K over several workgroups instead, so every level fills the chip;
+  // short-K launches fall back to smaller tiles.
+  constexpr long long kTarget = 2 * 256;
+  if (count_tiles(L, 128, 128) >= kTarget) return launch_cfg<Cfg128, T>(L, stream, 1);
+  if (count_tiles(L, 128, 64) >= kTarget) return launch_cfg<Cfg128x64, T>(L, stream, 1);
+  const long long t64 = count_tiles(L, 64, 64);
+  if (t64 >= kTarget) return launch_cfg<Cfg64, T>(L, stream, 1);
+  if (kmax >= 256) {
+    const long long want = (kTarget * 2 + t64 - 1) / (t64 > 0 ? t64 : 1);  // ~4 per CU
+    const int ks = (int)(want < 64 ? want : 64);
+    return launch_cfg<Cfg64, T>(L, stream, ks);
+  }
+  if (count_tiles(L, 64, 32) >= kTarget) return launch_cfg<Cfg64x32, T>(L, stream, 1);
+  return launch_cfg<Cfg32, T>(L, stream, 1);
+}
+
+template <typename T>
+int gemm_strided(int transA, int transB, int M, int N, int K, typename same_t<T>::type alpha,
+                 const T* A, int lda, long long sA, const T* B, int ldb, long long sB,
+                 typename same_t<T>::type beta, T* C, int ldc, long long sC, int batch,
+                 hipStream_t stream) {
+  GemmLaunchT<T> L{};
+  L.nd = 1;
+  L.d[0] = gemm_desc<T>(M, N, batch, C, ldc, sC, beta != T(0) ? C : nullptr, ldc, sC, beta);
+  gemm_add_term<T>(L.d[0], gemm_term<T>(A, lda, sA, transA, B, ldb, sB, transB, K, alpha));
+  return gemm_launch<T>(L, stream);
+}
+
+template int gemm_launch<double>(GemmLaunchT<double>&, hipStream_t);
+template int gemm_launch<float>(GemmLaunchT<float>&, hipStream_t);
+template int gemm_strided<double>(int, int, int, int, int, double, const double*, int, long long,
+                                  const double*, int, long long, double, double*, int, long long,
+                                  int, hipStream_t);
+template int gemm_strided<float>(int, int, int, int, int, float, const float*, int, long long,
+                                 const float*, int, long long, float, float*, int, long long, int,
+                                 hipStream_t);
+
+// ---- element-wise helpers -------------------------------------------------------------
+
+namespace {
+
+template <typename Tin, typename Tout>
+__global__ void k_cast(long long n, const Tin* __restrict__ x, Tout* __restrict__ y) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = (Tout)x[i];
+}
+
+__global__ void k_acc(long long n, const float* __restrict__ x, double* __restrict__ y) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] += (double)x[i];
+}
+
+__global__ void k_sub(long long n, const double* __restrict__ a, const double* __restrict__ b,
+                      double* __restrict__ z) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    z[i] = a[i] - b[i];
+}
+
+int grid_for(long long n) {
+  long long g = (n + 255) / 256;
+  return (int)(g < 8192 ? (g > 0 ? g : 1) : 8192);
+}
+
+template <typename T>
+__global__ void k_zero_2d(long long rows, int cols, T* __restrict__ X, int ld) {
+  const long long n = rows * cols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long i = e / cols;
+    X[i * ld + (e - i * cols)] = T(0);
+  }
+}
+
+}  // namespace
+
+template <typename T>
+int zero_2d(long long rows, int cols, T* X, int ld, hipStream_t st) {
+  if (rows <= 0 || cols <= 0) return kOk;
+  SSLAB_REQUIRE(ld >= cols && X != nullptr, "zero_2d: bad buffer");
+  hipLaunchKernelGGL(k_zero_2d<T>, dim3(grid_for(rows * cols)), dim3(256), 0, st, rows, cols, X, ld);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+template int zero_2d<double>(long long, int, double*, int, hipStream_t);
+template int zero_2d<float>(long long, int, float*, int, hipStream_t);
+
+int cast_f64_to_f32(long long n, const double* x, float* y, hipStream_t st) {
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL((k_cast<double, float>), dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int cast_f32_to_f64(long long n, const float* x, double* y, hipStream_t st) {
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL((k_cast<float, double>), dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int acc_f32_into_f64(long long n, const float* x, double* y, hipStream_t st) {
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL(k_acc, dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int sub_f64(long long n, const double* a, const double* b, double* z, hipStream_t st) {
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL(k_sub, dim3(grid_for(n)), dim3(256), 0, st, n, a, b, z);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace sslab

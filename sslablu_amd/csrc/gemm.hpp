@@ -1,0 +1,112 @@
+// Grouped / batched GEMM on gfx950 MFMA, FP64 (v_mfma_f64_16x16x4_f64) and FP32
+// (v_mfma_f32_16x16x4_f32, FP32 accumulation).
+//
+//   C_b = beta * Cin_b + sum_t alpha_t * op(A_t,b) * op(B_t,b)        (row-major)
+//
+// One launch executes up to kMaxDesc descriptors; each descriptor is a strided
+// batch of equally-shaped products with up to kMaxTerms summed terms (the terms
+// share M and N but may have different K).  Summed terms let one launch express
+// the block-tridiagonal sweeps  v_i - W^L_i v_{i-1} - W^R_i v_{i+1}  without a
+// second pass over the output.  Cin may alias C (in-place update).
+#pragma once
+
+#include "sslab_common.hpp"
+
+namespace sslab {
+
+constexpr int kMaxTerms = 3;
+constexpr int kMaxDesc = 6;
+
+template <class T>
+struct same_t {
+  using type = T;
+};
+
+template <typename T>
+struct GemmTermT {
+  const T* A;   // op(A): M x K
+  const T* B;   // op(B): K x N
+  long long strideA;  // batch strides in elements
+  long long strideB;
+  int lda, ldb;
+  int K;
+  int transA, transB;  // 0: as stored (row-major), 1: transposed
+  T alpha;
+};
+
+template <typename T>
+struct GemmDescT {
+  GemmTermT<T> term[kMaxTerms];
+  int nterms;
+  int M, N, batch;
+  const T* Cin;  // may be null when beta == 0
+  long long strideCin;
+  int ldcin;
+  T* C;
+  long long strideC;
+  int ldc;
+  T beta;
+  int tile_begin;  // filled in by gemm_launch
+  int tiles_m, tiles_n;
+  int ksplit;  // workgroups sharing one output tile along K (filled in by gemm_launch)
+  int kchunk;  // K range of one of them, a multiple of the tile's BK
+};
+
+template <typename T>
+struct GemmLaunchT {
+  GemmDescT<T> d[kMaxDesc];
+  int nd;
+};
+
+using GemmTerm = GemmTermT<double>;
+using GemmDesc = GemmDescT<double>;
+using GemmLaunch = GemmLaunchT<double>;
+
+template <typename T>
+inline GemmTermT<T> gemm_term(const T* A, int lda, long long sA, int transA, const T* B, int ldb,
+                              long long sB, int transB, int K,
+                              typename same_t<T>::type alpha) {
+  GemmTermT<T> t;
+  t.A = A; t.B = B; t.strideA = sA; t.strideB = sB; t.lda = lda; t.ldb = ldb;
+  t.K = K; t.transA = transA; t.transB = transB; t.alpha = alpha;
+  return t;
+}
+
+template <typename T>
+inline GemmDescT<T> gemm_desc(int M, int N, int batch, T* C, int ldc, long long sC,
+                              const typename same_t<T>::type* Cin, int ldcin, long long sCin,
+                              typename same_t<T>::type beta) {
+  GemmDescT<T> d{};
+  d.M = M; d.N = N; d.batch = batch; d.C = C; d.ldc = ldc; d.strideC = sC;
+  d.Cin = Cin; d.ldcin = ldcin; d.strideCin = sCin; d.beta = beta; d.nterms = 0;
+  return d;
+}
+
+template <typename T>
+inline void gemm_add_term(GemmDescT<T>& d, const GemmTermT<T>& t) { d.term[d.nterms++] = t; }
+
+// Launch all descriptors in `L` (L.nd <= kMaxDesc) on `stream`.  Descriptors with
+// batch == 0, M == 0 or N == 0 are skipped.  Returns a Status.  (double, float)
+template <typename T>
+int gemm_launch(GemmLaunchT<T>& L, hipStream_t stream);
+
+// Convenience: single strided-batched product with one term.
+template <typename T>
+int gemm_strided(int transA, int transB, int M, int N, int K, typename same_t<T>::type alpha,
+                 const T* A, int lda, long long sA, const T* B, int ldb, long long sB,
+                 typename same_t<T>::type beta, T* C, int ldc, long long sC, int batch,
+                 hipStream_t stream);
+
+// X[i][j] = 0 for i < rows, j < cols (leading dimension ld); a kernel, never a memset.
+template <typename T>
+int zero_2d(long long rows, int cols, T* X, int ld, hipStream_t st);
+
+// Element-wise helpers for the mixed-precision paths (a kernel each, no memsets).
+int cast_f64_to_f32(long long n, const double* x, float* y, hipStream_t st);
+int cast_f32_to_f64(long long n, const float* x, double* y, hipStream_t st);
+// y += x (x float, y double)
+int acc_f32_into_f64(long long n, const float* x, double* y, hipStream_t st);
+// z = a - b (double)
+int sub_f64(long long n, const double* a, const double* b, double* z, hipStream_t st);
+
+}  // namespace sslab

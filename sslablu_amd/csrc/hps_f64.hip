@@ -14,7 +14,7 @@
 // canonical order, so one gather kernel builds all four operands.
 #include "hps_f64.hpp"
 
-#include "gemm_f64.hpp"
+#include "gemm.hpp"
 #include "lu_f64.hpp"
 
 namespace sslab {

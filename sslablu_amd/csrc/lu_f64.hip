@@ -12,7 +12,7 @@
 // triangle in LDS, the off-diagonal updates are MFMA GEMMs.
 #include "lu_f64.hpp"
 
-#include "gemm_f64.hpp"
+#include "gemm.hpp"
 
 namespace sslab {
 
@@ -241,11 +241,11 @@ __global__ void k_copy_block(int n, const double* __restrict__ X, int ldx, long 
       X[(long long)blockIdx.z * sX + (long long)i * ldx + j];
 }
 
-__global__ void k_zero_block(int n, double* __restrict__ Y, int ldy, long long sY) {
-  const int i = blockIdx.y;
+__global__ void k_zero_block(int m, int n, double* __restrict__ Y, int ldy, long long sY) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  Y[(long long)blockIdx.z * sY + (long long)i * ldy + j] = 0.0;
+  for (int i = blockIdx.y; i < m; i += gridDim.y)
+    Y[(long long)blockIdx.z * sY + (long long)i * ldy + j] = 0.0;
 }
 
 int max_lds_bytes() {
@@ -409,9 +409,10 @@ int copy_block_batched(int m, int n, const double* X, int ldx, long long sX, dou
 int zero_block_batched(int m, int n, double* Y, int ldy, long long sY, int batch,
                        hipStream_t stream) {
   if (m <= 0 || n <= 0 || batch <= 0) return kOk;
-  SSLAB_REQUIRE(batch <= 65535 && m <= 65535, "zero_block: size too large");
-  hipLaunchKernelGGL(k_zero_block, dim3(ceil_div(n, 256), m, batch), dim3(256), 0, stream, n, Y,
-                     ldy, sY);
+  SSLAB_REQUIRE(batch <= 65535, "zero_block: batch too large");
+  const int gy = m < 4096 ? m : 4096;
+  hipLaunchKernelGGL(k_zero_block, dim3(ceil_div(n, 256), gy, batch), dim3(256), 0, stream, m, n,
+                     Y, ldy, sY);
   SSLAB_HIP_CHECK(hipGetLastError());
   return kOk;
 }

@@ -7,7 +7,8 @@
 // grouped GEMM launch with strided terms, no gather.
 #include "oms_f64.hpp"
 
-#include "gemm_f64.hpp"
+#include "gemm.hpp"
+#include "lu_f64.hpp"
 
 namespace sslab {
 
@@ -32,16 +33,18 @@ int oms_rhs(int nb, int m, int nxp, int ncols, const double* Y, const double* F,
     return gemm_term(Y + i * sY + 2 * m + 2 * nxp, ncols, sY, 0, Ftop + i * sF, ldF, sF, 0,
                      2 * nxp, -1.0);
   };
+  // Accumulate into a zeroed output so split-K launches need no init pass.
+  SSLAB_TRY(zero_block_batched((int)((long long)nb * m), r, out, ldo, 0, 1, st));
   GemmLaunch G{};
   int k = 0;
   if (nb > 2) {
-    GemmDesc d = gemm_desc(m, r, nb - 2, out + sO, ldo, sO, nullptr, 0, 0, 0.0);
+    GemmDesc d = gemm_desc(m, r, nb - 2, out + sO, ldo, sO, out + sO, ldo, sO, 1.0);
     gemm_add_term(d, bottom(1));
     gemm_add_term(d, top(1));
     G.d[k++] = d;
   }
   {
-    GemmDesc d = gemm_desc(m, r, 1, out, ldo, sO, nullptr, 0, 0, 0.0);
+    GemmDesc d = gemm_desc(m, r, 1, out, ldo, sO, out, ldo, sO, 1.0);
     gemm_add_term(d, bottom(0));
     gemm_add_term(d, top(0));
     gemm_add_term(d, gemm_term(Y, ncols, sY, 0, Fleft, ldF, 0, 0, m, -1.0));
@@ -49,7 +52,7 @@ int oms_rhs(int nb, int m, int nxp, int ncols, const double* Y, const double* F,
   }
   if (nb >= 2) {
     const long long i = nb - 1;
-    GemmDesc d = gemm_desc(m, r, 1, out + i * sO, ldo, sO, nullptr, 0, 0, 0.0);
+    GemmDesc d = gemm_desc(m, r, 1, out + i * sO, ldo, sO, out + i * sO, ldo, sO, 1.0);
     gemm_add_term(d, bottom(i));
     gemm_add_term(d, top(i));
     gemm_add_term(d, gemm_term(Y + i * sY + m, ncols, sY, 0, Fright, ldF, 0, 0, m, -1.0));
