@@ -115,6 +115,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box here has a single GPU and the "
@@ -185,6 +186,51 @@ def main():
     res[:-1] += torch.bmm(op.SiP[:-1], Xb[1:])
     resid = (torch.linalg.norm(res.reshape(nb * m, -1) - rhs) / torch.linalg.norm(rhs)).item()
 
+    # ---- mixed precision: FP32 red-black preconditioner + FP64 residual refinement ----
+    mixed = None
+    if not args.no_mixed:
+        fac.enable_mixed(None, op.SiM, op.SiP)
+        Bm = torch.empty(nb * m, NRHS, dtype=torch.float64, device=dev)
+        Xm = torch.empty_like(Bm)
+        wk = (torch.empty(nb * m * NRHS, dtype=torch.float64, device=dev),
+              torch.empty(2 * nb * m * NRHS, dtype=torch.float32, device=dev))
+        errs = []
+        iters = None
+        for it in range(4):  # fewest refinement steps that meet 1e-8 vs the FP64 solve
+            op.rhs_from_boundary(F, out=Bm)
+            fac.solve_mixed_(Bm, Xm, iters=it, work=wk)
+            errs.append((torch.linalg.norm(Xm - X) / torch.linalg.norm(X)).item())
+            if errs[-1] <= 1e-8:
+                iters = it
+                break
+        if iters is not None:
+            def step_mixed():
+                op.rhs_from_boundary(F, out=Bm)
+                fac.solve_mixed_(Bm, Xm, iters=iters, work=wk)
+
+            for _ in range(args.warmup):
+                step_mixed()
+            torch.cuda.synchronize()
+            evm = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                   for _ in range(args.steps)]
+            t0 = time.perf_counter()
+            for e0, e1 in evm:
+                e0.record(stream)
+                step_mixed()
+                e1.record(stream)
+            torch.cuda.synchronize()
+            wall_m = time.perf_counter() - t0
+            lat_m = np.array([e0.elapsed_time(e1) for e0, e1 in evm])
+            mixed = {"value": round(NRHS * args.steps / wall_m, 1), "unit": "RHS solves/s",
+                     "precision": "fp32 MFMA solve + fp64 residual refinement",
+                     "iters": iters, "rel_err_vs_fp64": errs[-1], "err_by_iters": errs,
+                     "p50_ms": round(float(np.percentile(lat_m, 50)), 4),
+                     "p99_ms": round(float(np.percentile(lat_m, 99)), 4),
+                     "speedup_vs_fp64": round(float(np.mean(lat_ms) / np.mean(lat_m)), 3)}
+        else:
+            mixed = {"value": None, "err_by_iters": errs,
+                     "note": "refinement did not reach 1e-8 within 3 steps"}
+
     # ---- roofline: the step is 12 launches of the FP64 MFMA GEMM kernel ----------------
     g_sum = nb * 4 * g.nx * g.p + 2 * m
     flops, nbytes = algorithmic_work(nb, m, g_sum, NRHS)
@@ -214,6 +260,7 @@ def main():
         "rel_err_vs_fp64_thomas": rel_err,
         "residual": resid,
         "roofline": roofline,
+        "mixed": mixed,
     }
     if not args.no_cpu_baseline:
         cb = cpu_baseline(op, F_host, args.cpu_budget)

@@ -249,6 +249,237 @@ __global__ void __launch_bounds__(Cfg::NT) k_gemm(const GemmLaunchT<T> L, int to
   }
 }
 
+// ---- fast path: every term as stored (row-major), 16-byte aligned operands ------------
+//
+// Same descriptors, split-K and epilogue as k_gemm, with
+//   * 16-byte global loads; M and N edges clamp the row / column (the clamped values
+//     only feed outputs that are never stored), K tails are zero-filled per vector;
+//   * a two-stage LDS ring: one barrier per K slab, the next slab's global loads in
+//     flight while the current slab is multiplied;
+//   * both operands stored k-contiguous in LDS ([row][k] for A, [col][k] for B) and a
+//     permuted k order: at MFMA step s, lane group g = lane / 16 supplies
+//     k = g * BK/4 + s.  The MFMA sums over its four k lanes, so any assignment of k to
+//     (step, lane group) shared by A and B gives the same product, and with this one a
+//     lane's operands for consecutive steps are adjacent: one ds_read_b128 fetches
+//     16 / sizeof(T) steps.
+// The LDS row length BK + 16 / sizeof(T) makes the 16 row reads of a lane group hit
+// disjoint banks.
+
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+
+template <class Cfg, typename T>
+struct FastGeom {
+  static constexpr int VEC = 16 / (int)sizeof(T);
+  static constexpr int KS = Cfg::BK / 4;  // k values per lane group per slab
+  static constexpr int LD = Cfg::BK + VEC;
+  static constexpr int A_VECS = Cfg::BM * Cfg::BK / (VEC * Cfg::NT);
+  static constexpr int B_VECS = Cfg::BK * Cfg::BN / (VEC * Cfg::NT);
+  static constexpr int STAGE = (Cfg::BM + Cfg::BN) * LD;
+  static constexpr bool ok = (Cfg::BM * Cfg::BK) % (VEC * Cfg::NT) == 0 &&
+                             (Cfg::BK * Cfg::BN) % (VEC * Cfg::NT) == 0 && KS % VEC == 0 &&
+                             A_VECS >= 1 && B_VECS >= 1;
+};
+
+template <class Cfg, typename T>
+__global__ void __launch_bounds__(Cfg::NT) k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
+  using G = FastGeom<Cfg, T>;
+  using M4 = Mfma<T>;
+  using v4 = typename M4::v4;
+  using vec = typename Vec16<T>::type;
+  constexpr int VEC = G::VEC, KS = G::KS, LD = G::LD, BK = Cfg::BK;
+  constexpr int KV = BK / VEC;       // vectors per A row slab
+  constexpr int NV = Cfg::BN / VEC;  // vectors per B k row
+  __shared__ __attribute__((aligned(16))) T smem[2 * G::STAGE];
+
+  const int tile = xcd_remap(blockIdx.x, total_tiles);
+  int di = 0;
+#pragma unroll
+  for (int d = 1; d < kMaxDesc; ++d)
+    if (d < L.nd && tile >= L.d[d].tile_begin) di = d;
+  const GemmDescT<T>& D = L.d[di];
+
+  const int per_part = D.tiles_m * D.tiles_n;
+  int local = tile - D.tile_begin;
+  const int bk = local / per_part;
+  local -= bk * per_part;
+  const int b = bk / D.ksplit;
+  const int kp = bk - b * D.ksplit;
+  const int kbeg = kp * D.kchunk, kfin = kbeg + D.kchunk;
+  const int tm = local / D.tiles_n;
+  const int tn = local - tm * D.tiles_n;
+  const int row0 = tm * Cfg::BM, col0 = tn * Cfg::BN;
+  const int M = D.M, N = D.N;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave / Cfg::WN, wn = wave % Cfg::WN;
+  const int wrow = wm * Cfg::WTM, wcol = wn * Cfg::WTN;
+  const int g = lane >> 4, li = lane & 15;
+
+  v4 acc[Cfg::FM][Cfg::FN];
+#pragma unroll
+  for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = v4{T(0), T(0), T(0), T(0)};
+
+  int stage = 0;
+  int koff = 0;
+  const int nterms = D.nterms;
+  for (int ti = 0; ti < nterms; ++ti) {
+    const GemmTermT<T>& Tm = D.term[ti];
+    const int ks = max(kbeg - koff, 0);
+    const int ke = min(kfin - koff, Tm.K);
+    koff += Tm.K;
+    if (ke <= ks) continue;
+    const T alpha = Tm.alpha;
+    const T* __restrict__ A = Tm.A + (long long)b * Tm.strideA;
+    const T* __restrict__ B = Tm.B + (long long)b * Tm.strideB;
+    const int lda = Tm.lda, ldb = Tm.ldb;
+    const int nslab = (ke - ks + BK - 1) / BK;
+    vec ra[G::A_VECS], rb[G::B_VECS];
+
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int r = 0; r < G::A_VECS; ++r) {
+        const int e = tid + r * Cfg::NT;
+        const int kv = e % KV, i = e / KV;
+        const int row = min(row0 + i, M - 1);
+        const int k = k0 + kv * VEC;
+        ra[r] = (k < ke) ? *reinterpret_cast<const vec*>(A + (long long)row * lda + k) : vec(T(0));
+      }
+#pragma unroll
+      for (int r = 0; r < G::B_VECS; ++r) {
+        const int e = tid + r * Cfg::NT;
+        const int jv = e % NV, kk = e / NV;
+        const int k = k0 + kk;
+        const int c = col0 + jv * VEC;
+        if (k < ke) {
+          const T* src = B + (long long)k * ldb;
+          if (c + VEC <= N) {
+            rb[r] = *reinterpret_cast<const vec*>(src + c);
+          } else {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) rb[r][q] = src[min(c + q, N - 1)];
+          }
+        } else {
+          rb[r] = vec(T(0));
+        }
+      }
+    };
+    auto store = [&](int st) {
+      T* As = smem + st * G::STAGE;
+      T* Bs = As + Cfg::BM * LD;
+#pragma unroll
+      for (int r = 0; r < G::A_VECS; ++r) {
+        const int e = tid + r * Cfg::NT;
+        const int kv = e % KV, i = e / KV;
+        *reinterpret_cast<vec*>(As + i * LD + kv * VEC) = alpha * ra[r];
+      }
+#pragma unroll
+      for (int r = 0; r < G::B_VECS; ++r) {
+        const int e = tid + r * Cfg::NT;
+        const int jv = e % NV, kk = e / NV;
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) Bs[(jv * VEC + q) * LD + kk] = rb[r][q];
+      }
+    };
+
+    load(ks);
+    store(stage);
+    __syncthreads();
+    for (int t = 0; t < nslab; ++t) {
+      const bool more = t + 1 < nslab;
+      if (more) load(ks + (t + 1) * BK);
+      {
+        const T* As = smem + stage * G::STAGE;
+        const T* Bs = As + Cfg::BM * LD;
+        vec af[Cfg::FM][KS / VEC], bf[Cfg::FN][KS / VEC];
+#pragma unroll
+        for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+          for (int q = 0; q < KS / VEC; ++q)
+            af[i][q] = *reinterpret_cast<const vec*>(As + (wrow + i * 16 + li) * LD + g * KS +
+                                                     q * VEC);
+#pragma unroll
+        for (int j = 0; j < Cfg::FN; ++j)
+#pragma unroll
+          for (int q = 0; q < KS / VEC; ++q)
+            bf[j][q] = *reinterpret_cast<const vec*>(Bs + (wcol + j * 16 + li) * LD + g * KS +
+                                                     q * VEC);
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+          for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+            for (int j = 0; j < Cfg::FN; ++j)
+              acc[i][j] = M4::op(af[i][s / VEC][s % VEC], bf[j][s / VEC][s % VEC], acc[i][j]);
+      }
+      if (more) store(stage ^ 1);
+      __syncthreads();
+      stage ^= 1;
+    }
+  }
+
+  T* C = D.C + (long long)b * D.strideC;
+  const T* Cin = D.Cin ? D.Cin + (long long)b * D.strideCin : nullptr;
+  const T beta = D.beta;
+  const bool use_cin = (Cin != nullptr) && (beta != T(0));
+  const bool split = D.ksplit > 1;
+  const int ldc = D.ldc, ldcin = D.ldcin;
+#pragma unroll
+  for (int i = 0; i < Cfg::FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < Cfg::FN; ++j) {
+      const int col = col0 + wcol + j * 16 + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row0 + wrow + i * 16 + M4::row(lane, v);
+        if (row < M && col < N) {
+          T val = acc[i][j][v];
+          if (split) {
+            atomicAdd(&C[(long long)row * ldc + col], val);
+          } else {
+            if (use_cin) val += beta * Cin[(long long)row * ldcin + col];
+            C[(long long)row * ldc + col] = val;
+          }
+        }
+      }
+    }
+  }
+}
+
+// True when every term of every descriptor can take the fast path.
+template <typename T>
+static bool fast_ok(const GemmLaunchT<T>& P) {
+  constexpr int VEC = 16 / (int)sizeof(T);
+  static const bool enabled = [] {
+    const char* s = getenv("SSLAB_GEMM_FAST");
+    return !(s && s[0] == '0');
+  }();
+  if (!enabled) return false;
+  for (int i = 0; i < P.nd; ++i) {
+    const GemmDescT<T>& d = P.d[i];
+    for (int t = 0; t < d.nterms; ++t) {
+      const GemmTermT<T>& m = d.term[t];
+      if (m.transA || m.transB) return false;
+      if ((reinterpret_cast<uintptr_t>(m.A) | reinterpret_cast<uintptr_t>(m.B)) & 15) return false;
+      if (m.lda % VEC || m.ldb % VEC || m.K % VEC) return false;
+      if (m.strideA % VEC || m.strideB % VEC) return false;
+    }
+  }
+  return true;
+}
+
 using Cfg128 = GemmCfg<128, 128, 16, 2, 2>;
 using Cfg128x64 = GemmCfg<128, 64, 16, 2, 2>;
 using Cfg64 = GemmCfg<64, 64, 32, 2, 2>;
@@ -331,6 +562,13 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit) {
     P.d[P.nd++] = d;
   }
   if (total == 0) return kOk;
+  if constexpr (FastGeom<Cfg, T>::ok) {
+    if (fast_ok(P)) {
+      hipLaunchKernelGGL((k_gemm_fast<Cfg, T>), dim3(total), dim3(Cfg::NT), 0, stream, P, total);
+      SSLAB_HIP_CHECK(hipGetLastError());
+      return kOk;
+    }
+  }
   hipLaunchKernelGGL((k_gemm<Cfg, T>), dim3(total), dim3(Cfg::NT), 0, stream, P, total);
   SSLAB_HIP_CHECK(hipGetLastError());
   return kOk;
