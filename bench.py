@@ -16,6 +16,7 @@ Prints one JSON line (rank 0, N = 1 only).
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -31,6 +32,12 @@ N_SLABS = 32
 NX, NY, Q = 1, 32, 18
 NRHS = 64
 SEED = 0
+CALIB_MIB = 256
+# HBM bytes per GEMM launch of one step, from rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE over
+# the step's 12 GEMM dispatches, each counter scaled by its reading for the calibration
+# copy (summary and commands: profiles/rb_solve_c2_helmholtz_hbm_traffic.md).  A
+# measured constant of the current kernels: re-measure after changing them.
+TRAFFIC = 43580587
 
 
 class PDO2d:
@@ -146,6 +153,15 @@ def main():
     torch.cuda.synchronize()
     t_factor = time.perf_counter() - t0
 
+    # A plain device copy of known size (untimed).  Under rocprofv3 --pmc its FETCH_SIZE
+    # and WRITE_SIZE (CALIB_MIB MiB each) calibrate the HBM byte counters of the solve
+    # kernels measured in the same run.
+    cal_src = torch.ones(CALIB_MIB << 17, dtype=torch.float64, device=dev)
+    cal_dst = torch.empty_like(cal_src)
+    cal_dst.copy_(cal_src)
+    torch.cuda.synchronize()
+    del cal_src, cal_dst
+
     g = op.g
     m, nb = g.m, g.nb
     ngb = g.gb_points().shape[0]
@@ -154,6 +170,11 @@ def main():
     F = torch.tensor(F_host, device=dev)
     V = torch.empty(nb * m, NRHS, dtype=torch.float64, device=dev)
     X = torch.empty_like(V)
+
+    # GEMM tile policy of the solve, picked by the autotuner on this factor (setup, untimed)
+    t0 = time.perf_counter()
+    pol, _ = fac.autotune(NRHS)
+    t_tune = time.perf_counter() - t0
 
     def step():
         op.rhs_from_boundary(F, out=V)
@@ -204,6 +225,8 @@ def main():
                 iters = it
                 break
         if iters is not None:
+            fac.autotune(NRHS, "mixed", iters=iters)
+
             def step_mixed():
                 op.rhs_from_boundary(F, out=Bm)
                 fac.solve_mixed_(Bm, Xm, iters=iters, work=wk)
@@ -231,15 +254,20 @@ def main():
             mixed = {"value": None, "err_by_iters": errs,
                      "note": "refinement did not reach 1e-8 within 3 steps"}
 
-    # ---- roofline: the step is 12 launches of the FP64 MFMA GEMM kernel ----------------
+    # ---- roofline: the step is 12 launches of the FP64 MFMA GEMM kernel (k_gemm_fast)
+    # plus two fill kernels (~3% of the step); the per-launch average below is the HIP-event
+    # step time divided by the 12 GEMM launches, so it slightly overstates a launch.
     g_sum = nb * 4 * g.nx * g.p + 2 * m
     flops, nbytes = algorithmic_work(nb, m, g_sum, NRHS)
+    n_launch = 12
     step_s = float(np.mean(lat_ms)) / 1e3
-    achieved_tf = flops / step_s / 1e12
+    achieved_tf = (flops / n_launch) / (step_s / n_launch) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": 78.6,
-                "unit": "TFLOP/s", "frac": round(achieved_tf / 78.6, 4), "traffic": None,
-                "kernel": "k_gemm_f64 (12 launches per step: 1 rhs-map + 2*5 red-black "
-                          "levels + coarsest block)",
+                "unit": "TFLOP/s", "frac": round(achieved_tf / 78.6, 4), "traffic": TRAFFIC,
+                "kernel": "k_gemm_fast<64x64x16, f64> (12 launches per step: 1 rhs-map + 2*5 "
+                          "red-black levels + coarsest block)",
+                "launches_per_step": n_launch, "avg_launch_us": round(step_s / n_launch * 1e6, 2),
+                "algorithmic_flops_per_launch": flops / n_launch,
                 "algorithmic_flops_per_step": flops, "algorithmic_bytes_per_step": nbytes}
 
     out = {
@@ -257,6 +285,7 @@ def main():
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "factor_time_s": round(t_factor, 4),
+        "gemm_policy": list(dataclasses.astuple(pol)), "autotune_s": round(t_tune, 3),
         "rel_err_vs_fp64_thomas": rel_err,
         "residual": resid,
         "roofline": roofline,

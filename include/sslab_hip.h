@@ -90,6 +90,17 @@ int sslab_rb_factor(int nb, int m, const double* T, const double* SiM, const dou
 int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
                    double* X, int ldx, void* stream);
 
+/* ---- GEMM tile policy (autotuning hook) ----------------------------------------
+ * No reference counterpart (the reference calls BLAS).  Sets / reads the calling
+ * thread's tile and split-K selection for every later GEMM launch made by this
+ * library on that thread.  p = {target_wgs, split_wgs, max_ksplit, min_split_k,
+ * cfg_mid, cfg_split, max_tile}, n = 7.  Tile ids: 0 128x128, 1 128x64, 2 64x64/BK32,
+ * 3 64x64/BK16, 4 64x32, 5 32x32, 6 32x64, 7 64x128 (8 32x16 and 9 32x32/BK32 are
+ * chosen automatically for launches with at most 16 / 32 columns).  Results do not depend on the
+ * policy beyond floating-point summation order. */
+int sslab_gemm_set_policy(const int* p, int n);
+int sslab_gemm_get_policy(int* p, int n);
+
 /* ---- mixed precision (FP32 red-black preconditioner + FP64 refinement) -------
  * No reference counterpart: the reference is FP64 throughout.  The FP32 factor is
  * the FP64 red-black factor rounded to float (same layout, factor_doubles floats). */

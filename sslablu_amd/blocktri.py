@@ -21,7 +21,7 @@ import warnings
 
 import torch
 
-from . import _lib
+from . import _lib, tuning
 from ._lib import call, ptr, stream_handle
 
 F64 = torch.float64
@@ -131,8 +131,9 @@ class BlockTriFactor:
             raise ValueError("solve_: float64 buffers with unit column stride required")
         r = V.shape[1]
         s = stream_handle(stream)
-        call(f"sslab_{self.method}_solve", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
-             ptr(X), X.stride(0), s)
+        with tuning.policy(tuning.lookup(self.tune_key(r))):
+            call(f"sslab_{self.method}_solve", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
+                 ptr(X), X.stride(0), s)
         if self.cyclic:
             # t = K [x_{nb-1}; x_0];  x -= Z t
             t = torch.empty(2 * m, r, dtype=F64, device=X.device)
@@ -182,10 +183,41 @@ class BlockTriFactor:
             work = (torch.empty(N * r, dtype=F64, device=self.device),
                     torch.empty(2 * N * r, dtype=torch.float32, device=self.device))
         w64, w32 = work
-        call("sslab_rb_solve_mixed", nb, m, r, ptr(self.factor32),
-             ptr(self._T64) if self._T64 is not None else None, ptr(self._M64), ptr(self._P64),
-             ptr(B), ptr(X), ptr(w64), ptr(w32), int(iters), stream_handle(stream))
+        with tuning.policy(tuning.lookup(self.tune_key(r, "mixed"))):
+            call("sslab_rb_solve_mixed", nb, m, r, ptr(self.factor32),
+                 ptr(self._T64) if self._T64 is not None else None, ptr(self._M64),
+                 ptr(self._P64), ptr(B), ptr(X), ptr(w64), ptr(w32), int(iters),
+                 stream_handle(stream))
         return X
+
+    # -- autotuning ---------------------------------------------------------------------
+    def tune_key(self, r: int, precision: str = "fp64") -> str:
+        return f"blocktri:{self.method}:{self.nb}:{self.m}:{r}:{precision}"
+
+    def autotune(self, r: int, precision: str = "fp64", iters: int = 1, reps: int = 10):
+        """Pick the GEMM tile policy for solves with `r` right-hand sides by timing every
+        candidate on this factor; later solve_ / solve_mixed_ calls of that width use
+        it.  Returns (policy, microseconds per solve)."""
+        N = self.nb * self.m
+        V = torch.randn(N, r, dtype=F64, device=self.device)
+        X = torch.empty_like(V)
+        if precision == "mixed":
+            work = (torch.empty(N * r, dtype=F64, device=self.device),
+                    torch.empty(2 * N * r, dtype=torch.float32, device=self.device))
+            fn = lambda: self.solve_mixed_(V, X, iters=iters, work=work)  # noqa: E731
+        else:
+            V0 = V.clone()
+            fn = lambda: (V.copy_(V0), self.solve_(V, X))  # noqa: E731
+        key = self.tune_key(r, precision)
+        with tuning.policy(tuning.DEFAULT):
+            tuning.lookup(key)  # load a persisted cache first, then drop a stale choice
+            _saved = tuning._cache.pop(key, None)
+            try:
+                return tuning.autotune(fn, key, reps=reps)
+            except Exception:
+                if _saved is not None:
+                    tuning._cache[key] = _saved
+                raise
 
     def residual(self, B: torch.Tensor, X: torch.Tensor, stream=None) -> torch.Tensor:
         """R = B - S X in FP64 (requires enable_mixed for the stored blocks)."""

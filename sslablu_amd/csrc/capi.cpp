@@ -55,6 +55,22 @@ int sslab_dgemm_strided_batched(int transA, int transB, int M, int N, int K, dou
                              beta, C, ldc, strideC, batch, as_stream(stream));
 }
 
+int sslab_gemm_set_policy(const int* p, int n) {
+  SSLAB_REQUIRE(p != nullptr && n == 7, "gemm_set_policy: expected 7 integers");
+  sslab::GemmPolicy q;
+  q.target_wgs = p[0]; q.split_wgs = p[1]; q.max_ksplit = p[2]; q.min_split_k = p[3];
+  q.cfg_mid = p[4]; q.cfg_split = p[5]; q.max_tile = p[6];
+  return sslab::gemm_set_policy(q);
+}
+
+int sslab_gemm_get_policy(int* p, int n) {
+  SSLAB_REQUIRE(p != nullptr && n == 7, "gemm_get_policy: expected 7 integers");
+  const sslab::GemmPolicy q = sslab::gemm_get_policy();
+  p[0] = q.target_wgs; p[1] = q.split_wgs; p[2] = q.max_ksplit; p[3] = q.min_split_k;
+  p[4] = q.cfg_mid; p[5] = q.cfg_split; p[6] = q.max_tile;
+  return 0;
+}
+
 int sslab_dgetrf_batched(int n, double* A, int lda, long long strideA, int* ipiv,
                          long long strideP, int* info, int batch, void* stream) {
   return sslab::getrf_batched(n, A, lda, strideA, ipiv, strideP, info, batch, as_stream(stream));

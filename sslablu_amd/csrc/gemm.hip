@@ -346,6 +346,11 @@ __global__ void __launch_bounds__(Cfg::NT) k_gemm_fast(const GemmLaunchT<T> L, i
     const T* __restrict__ B = Tm.B + (long long)b * Tm.strideB;
     const int lda = Tm.lda, ldb = Tm.ldb;
     const int nslab = (ke - ks + BK - 1) / BK;
+    // 16-byte loads need aligned rows and, for A, K-range ends on vector boundaries
+    // (then a vector is wholly inside or wholly outside [ks, ke)); otherwise element loads.
+    const bool va = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % VEC == 0) &&
+                    (ks % VEC == 0) && (ke % VEC == 0);
+    const bool vb = ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (ldb % VEC == 0);
     vec ra[G::A_VECS], rb[G::B_VECS];
 
     auto load = [&](int k0) {
@@ -355,7 +360,13 @@ __global__ void __launch_bounds__(Cfg::NT) k_gemm_fast(const GemmLaunchT<T> L, i
         const int kv = e % KV, i = e / KV;
         const int row = min(row0 + i, M - 1);
         const int k = k0 + kv * VEC;
-        ra[r] = (k < ke) ? *reinterpret_cast<const vec*>(A + (long long)row * lda + k) : vec(T(0));
+        const T* src = A + (long long)row * lda + k;
+        if (va) {
+          ra[r] = (k < ke) ? *reinterpret_cast<const vec*>(src) : vec(T(0));
+        } else {
+#pragma unroll
+          for (int q = 0; q < VEC; ++q) ra[r][q] = (k + q < ke) ? src[q] : T(0);
+        }
       }
 #pragma unroll
       for (int r = 0; r < G::B_VECS; ++r) {
@@ -365,7 +376,7 @@ __global__ void __launch_bounds__(Cfg::NT) k_gemm_fast(const GemmLaunchT<T> L, i
         const int c = col0 + jv * VEC;
         if (k < ke) {
           const T* src = B + (long long)k * ldb;
-          if (c + VEC <= N) {
+          if (vb && c + VEC <= N) {
             rb[r] = *reinterpret_cast<const vec*>(src + c);
           } else {
 #pragma unroll
@@ -458,10 +469,10 @@ __global__ void __launch_bounds__(Cfg::NT) k_gemm_fast(const GemmLaunchT<T> L, i
   }
 }
 
-// True when every term of every descriptor can take the fast path.
+// True when every term of every descriptor uses its operands as stored (the
+// vector-load kernel; unaligned operands fall back to element loads inside it).
 template <typename T>
 static bool fast_ok(const GemmLaunchT<T>& P) {
-  constexpr int VEC = 16 / (int)sizeof(T);
   static const bool enabled = [] {
     const char* s = getenv("SSLAB_GEMM_FAST");
     return !(s && s[0] == '0');
@@ -469,22 +480,29 @@ static bool fast_ok(const GemmLaunchT<T>& P) {
   if (!enabled) return false;
   for (int i = 0; i < P.nd; ++i) {
     const GemmDescT<T>& d = P.d[i];
-    for (int t = 0; t < d.nterms; ++t) {
-      const GemmTermT<T>& m = d.term[t];
-      if (m.transA || m.transB) return false;
-      if ((reinterpret_cast<uintptr_t>(m.A) | reinterpret_cast<uintptr_t>(m.B)) & 15) return false;
-      if (m.lda % VEC || m.ldb % VEC || m.K % VEC) return false;
-      if (m.strideA % VEC || m.strideB % VEC) return false;
-    }
+    for (int t = 0; t < d.nterms; ++t)
+      if (d.term[t].transA || d.term[t].transB) return false;
   }
   return true;
 }
 
+// Tile configurations, by id (the ids of GemmPolicy and sslab_gemm_set_policy):
+//   0: 128x128 BK16   1: 128x64 BK16   2: 64x64 BK32   3: 64x64 BK16
+//   4: 64x32 BK16     5: 32x32 BK16    6: 32x64 BK16 (2 waves)   7: 64x128 BK16
+//   8: 32x16 BK32 (2 waves)   9: 32x32 BK32 (2 waves)     -- few right-hand sides
 using Cfg128 = GemmCfg<128, 128, 16, 2, 2>;
 using Cfg128x64 = GemmCfg<128, 64, 16, 2, 2>;
 using Cfg64 = GemmCfg<64, 64, 32, 2, 2>;
+using Cfg64s = GemmCfg<64, 64, 16, 2, 2>;
 using Cfg64x32 = GemmCfg<64, 32, 16, 2, 2>;
 using Cfg32 = GemmCfg<32, 32, 16, 2, 2>;
+using Cfg32x64 = GemmCfg<32, 64, 16, 1, 2>;
+using Cfg64x128 = GemmCfg<64, 128, 16, 2, 2>;
+using Cfg32x16 = GemmCfg<32, 16, 32, 2, 1>;
+using Cfg32w = GemmCfg<32, 32, 32, 2, 1>;
+constexpr int kNumCfg = 10;
+constexpr int kCfgBM[kNumCfg] = {128, 128, 64, 64, 64, 32, 32, 64, 32, 32};
+constexpr int kCfgBN[kNumCfg] = {128, 64, 64, 64, 32, 32, 64, 128, 16, 32};
 
 // Workgroups a launch would create with BM x BN tiles.
 template <typename T>
@@ -575,6 +593,36 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit) {
 }
 
 template <typename T>
+static int launch_id(int id, GemmLaunchT<T>& L, hipStream_t stream, int ks) {
+  switch (id) {
+    case 0: return launch_cfg<Cfg128, T>(L, stream, ks);
+    case 1: return launch_cfg<Cfg128x64, T>(L, stream, ks);
+    case 2: return launch_cfg<Cfg64, T>(L, stream, ks);
+    case 3: return launch_cfg<Cfg64s, T>(L, stream, ks);
+    case 4: return launch_cfg<Cfg64x32, T>(L, stream, ks);
+    case 5: return launch_cfg<Cfg32, T>(L, stream, ks);
+    case 6: return launch_cfg<Cfg32x64, T>(L, stream, ks);
+    case 7: return launch_cfg<Cfg64x128, T>(L, stream, ks);
+    case 8: return launch_cfg<Cfg32x16, T>(L, stream, ks);
+    default: return launch_cfg<Cfg32w, T>(L, stream, ks);
+  }
+}
+
+static thread_local GemmPolicy g_policy;
+
+int gemm_set_policy(const GemmPolicy& p) {
+  SSLAB_REQUIRE(p.target_wgs >= 1 && p.split_wgs >= 1 && p.max_ksplit >= 1 && p.min_split_k >= 0,
+                "gemm policy: counts must be positive");
+  SSLAB_REQUIRE(p.cfg_mid >= 0 && p.cfg_mid < kNumCfg && p.cfg_split >= 0 &&
+                    p.cfg_split < kNumCfg && p.max_tile >= 0 && p.max_tile <= 2,
+                "gemm policy: tile id out of range");
+  g_policy = p;
+  return kOk;
+}
+
+GemmPolicy gemm_get_policy() { return g_policy; }
+
+template <typename T>
 int gemm_launch(GemmLaunchT<T>& L, hipStream_t stream) {
   SSLAB_REQUIRE(L.nd >= 0 && L.nd <= kMaxDesc, "gemm: bad descriptor count %d", L.nd);
   long long kmax = 0;
@@ -589,22 +637,33 @@ int gemm_launch(GemmLaunchT<T>& L, hipStream_t stream) {
     SSLAB_REQUIRE(total_k(d) < 0x7fffffffLL, "gemm: K too large");
     if (d.batch > 0 && total_k(d) > kmax) kmax = total_k(d);
   }
-  // Largest tile that still gives every CU (256 on MI355X) at least two workgroups.
+  // The largest tile that still gives the chip (256 CUs) policy.target_wgs workgroups.
   // Launches with few output tiles but long K (deep red-black levels, few right-hand
   // sides) split K over several workgroups instead, so every level fills the chip;
   // short-K launches fall back to smaller tiles.
-  constexpr long long kTarget = 2 * 256;
-  if (count_tiles(L, 128, 128) >= kTarget) return launch_cfg<Cfg128, T>(L, stream, 1);
-  if (count_tiles(L, 128, 64) >= kTarget) return launch_cfg<Cfg128x64, T>(L, stream, 1);
-  const long long t64 = count_tiles(L, 64, 64);
-  if (t64 >= kTarget) return launch_cfg<Cfg64, T>(L, stream, 1);
-  if (kmax >= 256) {
-    const long long want = (kTarget * 2 + t64 - 1) / (t64 > 0 ? t64 : 1);  // ~4 per CU
-    const int ks = (int)(want < 64 ? want : 64);
-    return launch_cfg<Cfg64, T>(L, stream, ks);
+  // Few right-hand sides (N <= 32) use the 16- and 32-column tiles, so the MFMA work
+  // padded onto empty columns stays small and the launch streams A at HBM rate.
+  const GemmPolicy& P = g_policy;
+  const long long target = P.target_wgs;
+  int nmax = 0;
+  for (int i = 0; i < L.nd; ++i)
+    if (L.d[i].batch > 0 && L.d[i].N > nmax) nmax = L.d[i].N;
+  if (nmax > 64 && P.max_tile == 0 && count_tiles(L, 128, 128) >= target)
+    return launch_id<T>(0, L, stream, 1);
+  if (nmax > 32 && P.max_tile <= 1 && count_tiles(L, 128, 64) >= target)
+    return launch_id<T>(1, L, stream, 1);
+  const int mid = nmax <= 16 ? 8 : (nmax <= 32 ? 9 : P.cfg_mid);
+  const int spl = nmax <= 16 ? 8 : (nmax <= 32 ? 9 : P.cfg_split);
+  if (count_tiles(L, kCfgBM[mid], kCfgBN[mid]) >= target) return launch_id<T>(mid, L, stream, 1);
+  if (kmax >= P.min_split_k) {
+    const long long tn = count_tiles(L, kCfgBM[spl], kCfgBN[spl]);
+    const long long want = (P.split_wgs + tn - 1) / (tn > 0 ? tn : 1);
+    const int ks = (int)(want < P.max_ksplit ? want : P.max_ksplit);
+    return launch_id<T>(spl, L, stream, ks);
   }
-  if (count_tiles(L, 64, 32) >= kTarget) return launch_cfg<Cfg64x32, T>(L, stream, 1);
-  return launch_cfg<Cfg32, T>(L, stream, 1);
+  if (nmax <= 32) return launch_id<T>(mid, L, stream, 1);
+  if (count_tiles(L, 64, 32) >= target) return launch_id<T>(4, L, stream, 1);
+  return launch_id<T>(5, L, stream, 1);
 }
 
 template <typename T>

@@ -85,6 +85,20 @@ inline GemmDescT<T> gemm_desc(int M, int N, int batch, T* C, int ldc, long long 
 template <typename T>
 inline void gemm_add_term(GemmDescT<T>& d, const GemmTermT<T>& t) { d.term[d.nterms++] = t; }
 
+// Tile / split-K selection of gemm_launch, per calling thread (an autotuner sets it
+// before the calls it has tuned).  Defaults reproduce the built-in heuristic.
+struct GemmPolicy {
+  int target_wgs = 1024;  // largest tile that still yields this many workgroups
+  int split_wgs = 512;    // split-K launches aim at this many workgroups
+  int max_ksplit = 64;    // at most this many K parts per output tile
+  int min_split_k = 256;  // split K only when the longest summed K reaches this
+  int cfg_mid = 3;        // tile id below 128x64 (2: 64x64 BK32, 3: 64x64 BK16, ...)
+  int cfg_split = 3;      // tile id of split-K launches (both for more than 32 columns)
+  int max_tile = 0;       // 0: up to 128x128, 1: up to 128x64, 2: no 128-row tiles
+};
+int gemm_set_policy(const GemmPolicy& p);
+GemmPolicy gemm_get_policy();
+
 // Launch all descriptors in `L` (L.nd <= kMaxDesc) on `stream`.  Descriptors with
 // batch == 0, M == 0 or N == 0 are skipped.  Returns a Status.  (double, float)
 template <typename T>
