@@ -60,6 +60,8 @@ class HBSMAT:
         if tree is not None:
             self.perm = np.asarray(tree.perm_leaf)
             self.Nb = tree.nleaves
+            if A is None:
+                self.shape = (len(self.perm), len(self.perm))
             self.nl = self.shape[0] // self.Nb
             self.L = tree.nlevels
             self.tree = tree
