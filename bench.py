@@ -115,6 +115,69 @@ def cpu_baseline(op, F_host, budget_s=20.0):
                 x=x)
 
 
+def serving_bench(dev, n_req=2000, seed=0):
+    """Serving workload: `n_req` requests, each uniform over {Poisson, Helmholtz} x slabs
+    {8,16,32,64} x leaf order {8,12,16} (32 leaves per interface, m = 32 p) x r in
+    [1, 32], N(0,1) boundary data resident on the device; factor cache of <= 24 operators.
+    Runs (1) all requests at once (saturation) through the plain FP64 path (one request
+    at a time), the continuous-batching server in FP64 and in mixed precision, and
+    (2) Poisson arrivals at a fixed rate (0.7 x the plain path's saturation rate) through
+    the plain path and the FP64 server."""
+    import torch
+    from sslablu_amd import serving as sv
+    specs = [sv.ProblemSpec(pde, n, p) for pde in ("poisson", "helmholtz")
+             for n in (8, 16, 32, 64) for p in (8, 12, 16)]
+    cache = sv.FactorCache(dev, hbm_budget_bytes=96 << 30, max_operators=24, mixed=True)
+    t0 = time.perf_counter()
+    srv64 = sv.BatchingServer(cache, slots=4, max_cols=128, precision="fp64")
+    srv64.warm(specs)
+    srvmx = sv.BatchingServer(cache, slots=4, max_cols=128, precision="mixed", iters=1)
+    srvmx.warm(specs)
+    t_warm = time.perf_counter() - t0
+
+    def fresh(reqs, rate_t=None):
+        return [sv.Request(q.rid, q.spec, q.F, q.t_arrive if rate_t is None else rate_t[i])
+                for i, q in enumerate(reqs)]
+
+    base = sv.make_requests(specs, n_req, float("inf"), rmax=32, seed=seed, device=dev)
+    out = {"workload": f"{n_req} requests over {len(specs)} operators (poisson/helmholtz x "
+                       "slabs 8,16,32,64 x p 8,12,16, 32 leaves per interface), r ~ U[1,32]",
+           "cache_operators": len(cache.ops), "cache_gb": round(cache.nbytes / 1e9, 2),
+           "warm_s (factor + graph capture)": round(t_warm, 2),
+           "factor_s_total": round(cache.factor_s, 2)}
+    runs = {}
+    plain = fresh(base)
+    runs["plain_fp64"] = sv.serve_plain(cache, plain, realtime=False)
+    b64 = fresh(base)
+    runs["batched_fp64"] = srv64.serve(b64, realtime=False)
+    bmx = fresh(base)
+    runs["batched_mixed"] = srvmx.serve(bmx, realtime=False)
+    err64 = max(((a.X - b.X).norm() / b.X.norm()).item() for a, b in zip(b64, plain))
+    errmx = max(((a.X - b.X).norm() / b.X.norm()).item() for a, b in zip(bmx, plain))
+    sat = {k: {"solves_per_s": round(v.solves_per_s, 1), "requests_per_s": round(v.requests_per_s, 1),
+               "p50_ms": round(v.p50_ms, 3), "p99_ms": round(v.p99_ms, 3),
+               "batches": v.batches} for k, v in runs.items()}
+    sat["batched_fp64"]["max_rel_err_vs_plain"] = err64
+    sat["batched_mixed"]["max_rel_err_vs_plain"] = errmx
+    out["saturation"] = sat
+    out["speedup_batched_fp64_vs_plain"] = round(runs["batched_fp64"].solves_per_s /
+                                                 runs["plain_fp64"].solves_per_s, 2)
+    out["speedup_batched_mixed_vs_plain"] = round(runs["batched_mixed"].solves_per_s /
+                                                  runs["plain_fp64"].solves_per_s, 2)
+    # fixed-rate arrivals
+    rate = 0.7 * runs["plain_fp64"].requests_per_s
+    rng = np.random.default_rng(seed + 1)
+    arr = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    rt = {}
+    for name, fn in (("plain_fp64", lambda q: sv.serve_plain(cache, q)),
+                     ("batched_fp64", lambda q: srv64.serve(q))):
+        st = fn(fresh(base, arr))
+        rt[name] = {"solves_per_s": round(st.solves_per_s, 1), "p50_ms": round(st.p50_ms, 3),
+                    "p99_ms": round(st.p99_ms, 3), "batches": st.batches}
+    out["poisson_arrivals"] = {"rate_per_s": round(rate, 1), **rt}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +186,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box here has a single GPU and the "
@@ -291,6 +355,8 @@ def main():
         "roofline": roofline,
         "mixed": mixed,
     }
+    if not args.no_serving:
+        out["serving"] = serving_bench(dev)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(op, F_host, args.cpu_budget)
         out["cpu_baseline"] = {
