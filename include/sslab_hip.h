@@ -90,6 +90,15 @@ int sslab_rb_factor(int nb, int m, const double* T, const double* SiM, const dou
 int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
                    double* X, int ldx, void* stream);
 
+/* Several independent systems (any mix of nb, m, nrhs) solved together: each
+ * elimination level of all systems shares one grouped GEMM launch.  Arrays of length
+ * nsys (host memory) give each system's sizes, factor (from sslab_rb_factor), V and X
+ * exactly as for sslab_rb_solve.  No reference counterpart (the reference solves one
+ * system per call, direct_solve/omsdirectsolve.py:221-279); nsys <= 64. */
+int sslab_rb_solve_multi(int nsys, const int* nb, const int* m, const int* nrhs,
+                         const double* const* factor, double* const* V, const int* ldv,
+                         double* const* X, const int* ldx, void* stream);
+
 /* ---- GEMM tile policy (autotuning hook) ----------------------------------------
  * No reference counterpart (the reference calls BLAS).  Sets / reads the calling
  * thread's tile and split-K selection for every later GEMM launch made by this

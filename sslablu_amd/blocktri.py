@@ -127,9 +127,12 @@ class BlockTriFactor:
         N = nb * m
         if V.shape[0] != N or X.shape[0] != N or V.dim() != 2 or X.shape[1] != V.shape[1]:
             raise ValueError("solve_: V and X must be (nb*m, r)")
-        if V.stride(1) != 1 or X.stride(1) != 1 or V.dtype != F64 or X.dtype != F64:
+        if V.dtype != F64 or X.dtype != F64 or (V.shape[1] > 0 and (V.stride(1) != 1 or
+                                                                    X.stride(1) != 1)):
             raise ValueError("solve_: float64 buffers with unit column stride required")
         r = V.shape[1]
+        if r == 0:
+            return X
         s = stream_handle(stream)
         with tuning.policy(tuning.lookup(self.tune_key(r))):
             call(f"sslab_{self.method}_solve", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
@@ -236,3 +239,39 @@ class BlockTriFactor:
         if self.Z is not None:
             n += (self.Z.numel() + self.K.numel()) * 8
         return n
+
+
+def solve_multi(factors, Vs, Xs, stream=None):
+    """Solve several red-black-factored systems together: every elimination level of all
+    systems shares one grouped GEMM launch (sslab_rb_solve_multi), so systems of
+    different block sizes and right-hand-side counts go through the sweeps at once.
+    Vs[i] (overwritten) / Xs[i]: float64 device tensors (nb_i*m_i, r_i), unit column
+    stride, as for BlockTriFactor.solve_."""
+    import ctypes
+    n = len(factors)
+    if not (len(Vs) == len(Xs) == n):
+        raise ValueError("solve_multi: one V and one X per factor")
+    for f, V, X in zip(factors, Vs, Xs):
+        if f.method != "rb" or f.cyclic:
+            raise ValueError("solve_multi: non-cyclic red-black factors only")
+        N = f.nb * f.m
+        if V.shape[0] != N or X.shape[0] != N or V.dim() != 2 or X.shape[1] != V.shape[1]:
+            raise ValueError("solve_multi: V and X must be (nb*m, r)")
+        if V.dtype != F64 or X.dtype != F64 or (V.shape[1] > 0 and (V.stride(1) != 1 or
+                                                                    X.stride(1) != 1)):
+            raise ValueError("solve_multi: float64 buffers with unit column stride required")
+    ints = lambda xs: (ctypes.c_int * n)(*xs)  # noqa: E731
+    ptrs = lambda ts: (ctypes.c_void_p * n)(*[ptr(t) for t in ts])  # noqa: E731
+    nb = ints([f.nb for f in factors])
+    m = ints([f.m for f in factors])
+    r = ints([V.shape[1] for V in Vs])
+    fp = ptrs([f.factor for f in factors])
+    vp = ptrs(Vs)
+    xp = ptrs(Xs)
+    ldv = ints([max(V.stride(0), V.shape[1]) for V in Vs])
+    ldx = ints([max(X.stride(0), X.shape[1]) for X in Xs])
+    A = ctypes.addressof
+    call("sslab_rb_solve_multi", n, A(nb), A(m), A(r), A(fp), A(vp), A(ldv), A(xp), A(ldx),
+         stream_handle(stream))
+    return Xs
+

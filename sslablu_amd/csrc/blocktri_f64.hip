@@ -218,89 +218,185 @@ int rb_info_count(int nb) {
 }
 
 // The solve sweeps, for factors (and right-hand sides) stored in FP64 or FP32.
+// One system of a (possibly multi-system) red-black solve.
+template <typename T>
+struct RbSolveSys {
+  int nb, m, r;
+  const T* factor;
+  T* V;
+  int ldv;
+  T* X;
+  int ldx;
+  RbPlan p;
+};
+
+// Append a descriptor, launching the pending ones first when the launch is full.  The
+// descriptors of one level write disjoint outputs from inputs no descriptor of that
+// level writes, so a level may be split over several launches.
+template <typename T>
+static int rb_push(GemmLaunchT<T>& G, const GemmDescT<T>& d, hipStream_t st) {
+  if (G.nd == kMaxDesc) {
+    SSLAB_TRY(gemm_launch(G, st));
+    G.nd = 0;
+  }
+  G.d[G.nd++] = d;
+  return kOk;
+}
+
+template <typename T>
+static int rb_flush(GemmLaunchT<T>& G, hipStream_t st) {
+  if (G.nd > 0) {
+    SSLAB_TRY(gemm_launch(G, st));
+    G.nd = 0;
+  }
+  return kOk;
+}
+
+// Forward sweep of level l:  v_i -= W^L v_{i-1} + W^R v_{i+1} (even i),  y_j = Tinv_j v_j
+// (odd j, accumulated into the zero-initialised X).
+template <typename T>
+static int rb_fwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStream_t st) {
+  const int m = S.m, r = S.r, ldv = S.ldv, ldx = S.ldx;
+  T* V = S.V;
+  T* X = S.X;
+  const long long m2 = (long long)m * m;
+  const long long bv = (long long)m * ldv, bx = (long long)m * ldx;
+  const RbLevel& v = S.p.lv[l];
+  const int nE = v.nE, nO = v.nO;
+  const T* TG = S.factor + v.tg_off;
+  const T* WW = S.factor + v.ww_off;
+  const long long sv = (2LL << l) * bv, sx = (2LL << l) * bx;  // same-parity node stride
+  const long long ov = (1LL << l) * bv, ox = (1LL << l) * bx;  // first odd node
+  const int both = (nE < nO ? nE : nO) - 1;
+  if (both > 0) {  // even b in [1, min(nE,nO))
+    GemmDescT<T> d = gemm_desc<T>(m, r, both, V + sv, ldv, sv, V + sv, ldv, sv, 1.0);
+    gemm_add_term<T>(d, gemm_term<T>(WW + 2 * m2, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
+    gemm_add_term<T>(d, gemm_term<T>(WW + 2 * m2 + m, 2 * m, 2 * m2, 0, V + sv + ov, ldv, sv, 0, m,
+                                     -1.0));
+    SSLAB_TRY(rb_push(G, d, st));
+  }
+  {  // b = 0
+    GemmDescT<T> d = gemm_desc<T>(m, r, 1, V, ldv, sv, V, ldv, sv, 1.0);
+    gemm_add_term<T>(d, gemm_term<T>(WW + m, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
+    SSLAB_TRY(rb_push(G, d, st));
+  }
+  if (nE > nO && nE - 1 >= 1) {
+    const long long b = nE - 1;
+    GemmDescT<T> d = gemm_desc<T>(m, r, 1, V + b * sv, ldv, sv, V + b * sv, ldv, sv, 1.0);
+    gemm_add_term<T>(d, gemm_term<T>(WW + b * 2 * m2, 2 * m, 2 * m2, 0, V + b * sv - ov, ldv, sv,
+                                     0, m, -1.0));
+    SSLAB_TRY(rb_push(G, d, st));
+  }
+  {  // odd: y_j = Tinv_j v_j
+    GemmDescT<T> d = gemm_desc<T>(m, r, nO, X + ox, ldx, sx, X + ox, ldx, sx, 1.0);
+    gemm_add_term<T>(d, gemm_term<T>(TG, 3 * m, 3 * m2, 0, V + ov, ldv, sv, 0, m, 1.0));
+    SSLAB_TRY(rb_push(G, d, st));
+  }
+  return kOk;
+}
+
+// Coarsest block: x_0 = Tinv v_0.
+template <typename T>
+static int rb_top(GemmLaunchT<T>& G, const RbSolveSys<T>& S, hipStream_t st) {
+  const int m = S.m;
+  GemmDescT<T> d = gemm_desc<T>(m, S.r, 1, S.X, S.ldx, 0, S.X, S.ldx, 0, 1.0);
+  gemm_add_term<T>(d, gemm_term<T>(S.factor + S.p.top_off, m, (long long)m * m, 0, S.V, S.ldv, 0,
+                                   0, m, 1.0));
+  return rb_push(G, d, st);
+}
+
+// Backward sweep of level l:  x_j = y_j - (Tinv SiM)_j x_{j-1} - (Tinv SiP)_j x_{j+1}.
+template <typename T>
+static int rb_bwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStream_t st) {
+  const int m = S.m, r = S.r, ldx = S.ldx;
+  T* X = S.X;
+  const long long m2 = (long long)m * m;
+  const long long bx = (long long)m * ldx;
+  const RbLevel& v = S.p.lv[l];
+  const int n = v.n, nO = v.nO;
+  const T* TG = S.factor + v.tg_off;
+  const long long sx = (2LL << l) * bx;
+  const long long ox = (1LL << l) * bx;
+  const int nboth = (n % 2 == 0) ? nO - 1 : nO;  // odd nodes with both neighbours
+  if (nboth > 0) {
+    GemmDescT<T> d = gemm_desc<T>(m, r, nboth, X + ox, ldx, sx, X + ox, ldx, sx, 1.0);
+    gemm_add_term<T>(d, gemm_term<T>(TG + m, 3 * m, 3 * m2, 0, X, ldx, sx, 0, m, -1.0));
+    gemm_add_term<T>(d, gemm_term<T>(TG + 2 * m, 3 * m, 3 * m2, 0, X + sx, ldx, sx, 0, m, -1.0));
+    SSLAB_TRY(rb_push(G, d, st));
+  }
+  if (nboth < nO) {  // last odd node: left neighbour only
+    const long long c = nO - 1;
+    GemmDescT<T> d = gemm_desc<T>(m, r, 1, X + ox + c * sx, ldx, sx, X + ox + c * sx, ldx, sx, 1.0);
+    gemm_add_term<T>(d, gemm_term<T>(TG + c * 3 * m2 + m, 3 * m, 3 * m2, 0, X + c * sx, ldx, sx, 0,
+                                     m, -1.0));
+    SSLAB_TRY(rb_push(G, d, st));
+  }
+  return kOk;
+}
+
+// Solve several independent systems together: at every step the descriptors of all
+// systems share one grouped GEMM launch (more when they exceed kMaxDesc).  Forward step
+// t runs level t of every system that has it and the coarsest block of every system
+// whose levels ended at t; backward step u runs level L_s - 1 - u of system s, so the
+// systems finish together.  Every slot of X is written exactly once by a product that
+// accumulates into X, which starts at zero (then split-K launches need no init pass).
+template <typename T>
+static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
+  int Lmax = 0;
+  bool any = false;
+  for (int s = 0; s < nsys; ++s) {
+    RbSolveSys<T>& S = sys[s];
+    SSLAB_REQUIRE(S.nb >= 1 && S.m >= 1 && S.r >= 0, "rb_solve: bad sizes (system %d)", s);
+    S.p = rb_plan(S.nb, S.m);
+    if (S.r == 0) continue;  // nothing to solve (empty buffers may be null)
+    SSLAB_REQUIRE(S.ldv >= S.r && S.ldx >= S.r, "rb_solve: leading dimension < nrhs");
+    SSLAB_REQUIRE(S.factor && S.V && S.X && (void*)S.V != (void*)S.X,
+                  "rb_solve: null or aliased buffers");
+    any = true;
+    if (S.p.L > Lmax) Lmax = S.p.L;
+    SSLAB_TRY(zero_2d<T>((long long)S.nb * S.m, S.r, S.X, S.ldx, st));
+  }
+  if (!any) return kOk;
+  GemmLaunchT<T> G{};
+  G.nd = 0;
+  for (int t = 0; t <= Lmax; ++t) {
+    for (int s = 0; s < nsys; ++s) {
+      const RbSolveSys<T>& S = sys[s];
+      if (S.r == 0) continue;
+      if (t < S.p.L) SSLAB_TRY(rb_fwd_level(G, S, t, st));
+      else if (t == S.p.L) SSLAB_TRY(rb_top(G, S, st));
+    }
+    SSLAB_TRY(rb_flush(G, st));
+  }
+  for (int u = 0; u < Lmax; ++u) {
+    for (int s = 0; s < nsys; ++s) {
+      const RbSolveSys<T>& S = sys[s];
+      const int l = S.p.L - 1 - u;
+      if (S.r == 0 || l < 0) continue;
+      SSLAB_TRY(rb_bwd_level(G, S, l, st));
+    }
+    SSLAB_TRY(rb_flush(G, st));
+  }
+  return kOk;
+}
+
 template <typename T>
 static int rb_solve_t(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
                       hipStream_t st) {
-  using GemmLaunch = GemmLaunchT<T>;
-  using GemmDesc = GemmDescT<T>;
-  SSLAB_REQUIRE(nb >= 1 && m >= 1 && r >= 0, "rb_solve: bad sizes");
-  SSLAB_REQUIRE(ldv >= r && ldx >= r, "rb_solve: leading dimension < nrhs");
-  SSLAB_REQUIRE(factor && V && X && V != X, "rb_solve: null or aliased buffers");
-  if (r == 0) return kOk;
-  const RbPlan p = rb_plan(nb, m);
-  const long long m2 = (long long)m * m;
-  const long long bv = (long long)m * ldv;  // one block row of V
-  const long long bx = (long long)m * ldx;
-  // Every slot of X is written exactly once by a product below; starting from zero
-  // lets those products accumulate (C += ...) so a split-K launch needs no init pass.
-  SSLAB_TRY(zero_2d<T>((long long)nb * m, r, X, ldx, st));
-  for (int l = 0; l < p.L; ++l) {
-    const RbLevel& v = p.lv[l];
-    const int n = v.n, nE = v.nE, nO = v.nO;
-    const T* TG = factor + v.tg_off;
-    const T* WW = factor + v.ww_off;
-    const long long sv = (2LL << l) * bv, sx = (2LL << l) * bx;  // stride between same-parity nodes
-    const long long ov = (1LL << l) * bv, ox = (1LL << l) * bx;  // offset of the first odd node
-    (void)n;
-    GemmLaunch G{};
-    int k = 0;
-    const int both = (nE < nO ? nE : nO) - 1;
-    if (both > 0) {  // even b in [1, min(nE,nO))
-      GemmDesc d = gemm_desc(m, r, both, V + sv, ldv, sv, V + sv, ldv, sv, 1.0);
-      gemm_add_term(d, gemm_term(WW + 2 * m2, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
-      gemm_add_term(d, gemm_term(WW + 2 * m2 + m, 2 * m, 2 * m2, 0, V + sv + ov, ldv, sv, 0, m,
-                                 -1.0));
-      G.d[k++] = d;
-    }
-    {  // b = 0
-      GemmDesc d = gemm_desc(m, r, 1, V, ldv, sv, V, ldv, sv, 1.0);
-      gemm_add_term(d, gemm_term(WW + m, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
-      G.d[k++] = d;
-    }
-    if (nE > nO && nE - 1 >= 1) {
-      const long long b = nE - 1;
-      GemmDesc d = gemm_desc(m, r, 1, V + b * sv, ldv, sv, V + b * sv, ldv, sv, 1.0);
-      gemm_add_term(d, gemm_term(WW + b * 2 * m2, 2 * m, 2 * m2, 0, V + b * sv - ov, ldv, sv, 0, m,
-                                 -1.0));
-      G.d[k++] = d;
-    }
-    {  // odd: y_j = Tinv_j v_j into X (zero-initialised)
-      GemmDesc d = gemm_desc(m, r, nO, X + ox, ldx, sx, X + ox, ldx, sx, 1.0);
-      gemm_add_term(d, gemm_term(TG, 3 * m, 3 * m2, 0, V + ov, ldv, sv, 0, m, 1.0));
-      G.d[k++] = d;
-    }
-    G.nd = k;
-    SSLAB_TRY(gemm_launch(G, st));
-  }
-  SSLAB_TRY(gemm_strided<T>(0, 0, m, r, m, 1.0, factor + p.top_off, m, m2, V, ldv, 0, 1.0, X, ldx,
-                            0, 1, st));
-  for (int l = p.L - 1; l >= 0; --l) {
-    const RbLevel& v = p.lv[l];
-    const int n = v.n, nE = v.nE, nO = v.nO;
-    const T* TG = factor + v.tg_off;
-    const long long sx = (2LL << l) * bx;
-    const long long ox = (1LL << l) * bx;
-    GemmLaunch G{};
-    int k = 0;
-    const int nboth = (n % 2 == 0) ? nO - 1 : nO;  // odd nodes with both neighbours
-    (void)nE;
-    if (nboth > 0) {
-      GemmDesc d = gemm_desc(m, r, nboth, X + ox, ldx, sx, X + ox, ldx, sx, 1.0);
-      gemm_add_term(d, gemm_term(TG + m, 3 * m, 3 * m2, 0, X, ldx, sx, 0, m, -1.0));
-      gemm_add_term(d, gemm_term(TG + 2 * m, 3 * m, 3 * m2, 0, X + sx, ldx, sx, 0, m, -1.0));
-      G.d[k++] = d;
-    }
-    if (nboth < nO) {  // last odd node: left neighbour only
-      const long long c = nO - 1;
-      GemmDesc d = gemm_desc(m, r, 1, X + ox + c * sx, ldx, sx, X + ox + c * sx, ldx, sx, 1.0);
-      gemm_add_term(d, gemm_term(TG + c * 3 * m2 + m, 3 * m, 3 * m2, 0, X + c * sx, ldx, sx, 0, m,
-                                 -1.0));
-      G.d[k++] = d;
-    }
-    G.nd = k;
-    SSLAB_TRY(gemm_launch(G, st));
-  }
-  return kOk;
+  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}};
+  return rb_solve_multi_t<T>(1, &S, st);
+}
+
+int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,
+                   double* const* V, const int* ldv, double* const* X, const int* ldx,
+                   hipStream_t st) {
+  SSLAB_REQUIRE(nsys >= 0 && nsys <= 64, "rb_solve_multi: 0 <= nsys <= 64");
+  if (nsys == 0) return kOk;
+  SSLAB_REQUIRE(nb && m && r && factor && V && ldv && X && ldx, "rb_solve_multi: null array");
+  std::vector<RbSolveSys<double>> sys(nsys);
+  for (int s = 0; s < nsys; ++s)
+    sys[s] = RbSolveSys<double>{nb[s], m[s], r[s], factor[s], V[s], ldv[s], X[s], ldx[s], RbPlan{}};
+  return rb_solve_multi_t<double>(nsys, sys.data(), st);
 }
 
 int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, double* X, int ldx,

@@ -12,6 +12,9 @@ int rb_factor(int nb, int m, const double* T, const double* SiM, const double* S
               double* factor, double* work, int* iwork, int* info, hipStream_t st);
 int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, double* X, int ldx,
              hipStream_t st);
+int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,
+                   double* const* V, const int* ldv, double* const* X, const int* ldx,
+                   hipStream_t st);
 int rb_solve_f32(int nb, int m, int r, const float* factor, float* V, int ldv, float* X, int ldx,
                  hipStream_t st);
 int rb_residual(int nb, int m, int r, const double* T, const double* SiM, const double* SiP,

@@ -119,6 +119,12 @@ int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int
   return sslab::rb_solve(nb, m, nrhs, factor, V, ldv, X, ldx, as_stream(stream));
 }
 
+int sslab_rb_solve_multi(int nsys, const int* nb, const int* m, const int* nrhs,
+                         const double* const* factor, double* const* V, const int* ldv,
+                         double* const* X, const int* ldx, void* stream) {
+  return sslab::rb_solve_multi(nsys, nb, m, nrhs, factor, V, ldv, X, ldx, as_stream(stream));
+}
+
 int sslab_cast_f64_to_f32(long long n, const double* x, float* y, void* stream) {
   return sslab::cast_f64_to_f32(n, x, y, as_stream(stream));
 }

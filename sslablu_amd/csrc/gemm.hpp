@@ -15,7 +15,7 @@
 namespace sslab {
 
 constexpr int kMaxTerms = 3;
-constexpr int kMaxDesc = 6;
+constexpr int kMaxDesc = 12;  // 12 descriptors = 3.5 KB of kernel arguments
 
 template <class T>
 struct same_t {
