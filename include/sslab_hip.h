@@ -83,6 +83,8 @@ int sslab_rb_sizes(int nb, int m, long long* factor_doubles, long long* work_dou
                    long long* work_ints);
 /* Number of ints written to `info` by sslab_rb_factor (one per eliminated block). */
 int sslab_rb_info_count(int nb);
+/* T may be NULL for identity diagonal blocks (the oms S-system); the first level then
+ * needs no LU. */
 int sslab_rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
                     double* factor, double* work, int* iwork, int* info, void* stream);
 /* V (nb*m x nrhs, leading dim ldv) holds the right-hand sides and is overwritten;

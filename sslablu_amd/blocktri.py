@@ -59,13 +59,17 @@ class BlockTriFactor:
 
     # -- factorization ---------------------------------------------------------
     def factorize(self, T, SiM, SiP, E=None, F=None, stream=None):
-        """T, SiM, SiP: (nb, m, m) blocks (SiM[0] and SiP[nb-1] are ignored).
-        E, F: optional corner blocks for a periodic system."""
-        nb = len(T)
-        m = T[0].shape[0]
+        """T, SiM, SiP: (nb, m, m) blocks (SiM[0] and SiP[nb-1] are ignored); T = None
+        means identity diagonal blocks (the oms S-system; red-black then skips the
+        first level's LU).  E, F: optional corner blocks for a periodic system."""
+        ref = T if T is not None else SiM
+        nb = len(ref)
+        m = ref[0].shape[0]
         dev = self.device
         with torch.cuda.device(dev):
-            Td = _dev_blocks(T, nb, m, dev)
+            if T is None and self.method == "thomas":
+                T = torch.eye(m, dtype=F64, device=dev).expand(nb, m, m)
+            Td = _dev_blocks(T, nb, m, dev) if T is not None else None
             Md = _dev_blocks(SiM, nb, m, dev)
             Pd = _dev_blocks(SiP, nb, m, dev)
             nfac, nwork, niwork = _lib.sizes(self.method, nb, m)
@@ -75,8 +79,8 @@ class BlockTriFactor:
             ninfo = _lib.load().sslab_rb_info_count(nb) if self.method == "rb" else nb
             info = torch.zeros(ninfo, dtype=torch.int32, device=dev)
             s = stream_handle(stream)
-            call(f"sslab_{self.method}_factor", nb, m, ptr(Td), ptr(Md), ptr(Pd),
-                 ptr(self.factor), ptr(work), ptr(iwork), ptr(info), s)
+            call(f"sslab_{self.method}_factor", nb, m, ptr(Td) if Td is not None else None,
+                 ptr(Md), ptr(Pd), ptr(self.factor), ptr(work), ptr(iwork), ptr(info), s)
             self.nb, self.m = nb, m
             if self.method == "thomas":
                 # keep the pivots of LU(B_i) for the reference's `B` attribute

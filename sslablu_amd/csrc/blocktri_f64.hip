@@ -83,6 +83,15 @@ RbPlan rb_plan(int nb, int m) {
   return p;
 }
 
+__global__ void k_fill_int(int* __restrict__ p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void zero_int_async(int* p, int n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(n, 256)), dim3(256), 0, st, p, n, 0);
+}
+
 }  // namespace
 
 int rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
@@ -98,7 +107,9 @@ int rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
 int rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
               double* factor, double* work, int* iwork, int* info, hipStream_t st) {
   SSLAB_REQUIRE(nb >= 1 && m >= 1, "rb_factor: nb and m must be positive");
-  SSLAB_REQUIRE(T && factor && work && iwork, "rb_factor: null pointer");
+  SSLAB_REQUIRE(factor && work && iwork, "rb_factor: null pointer");
+  // T == NULL: identity diagonal blocks (the oms S-system u_i + S u = rhs).  Level 0 then
+  // needs no LU: T^{-1} = I, W = [SiM | SiP], T' = I - W^L SiP - W^R SiM.
   SSLAB_REQUIRE(nb == 1 || (SiM && SiP), "rb_factor: null off-diagonal blocks");
   const RbPlan p = rb_plan(nb, m);
   const long long m2 = (long long)m * m;
@@ -124,28 +135,56 @@ int rb_factor(int nb, int m, const double* T, const double* SiM, const double* S
     double* nM = nT + p.set_blocks * m2;
     double* nP = nM + p.set_blocks * m2;
 
-    // 1-2. LU of the odd diagonal blocks.
-    SSLAB_TRY(copy_block_batched(m, m, cT + m2, m, 2 * m2, LUw, m, m2, nO, st));
-    SSLAB_TRY(getrf_batched(m, LUw, m, m2, ipiv, m, linfo, nO, st));
-    if (info) {
-      SSLAB_HIP_CHECK(hipMemcpyAsync(info + info_base, linfo, sizeof(int) * nO,
-                                     hipMemcpyDeviceToDevice, st));
-    }
-    info_base += nO;
-    // 3. Right-hand sides [I | SiM_j | SiP_j] for odd j (SiP of a last odd node
-    //    without a right neighbour is zero).
-    SSLAB_TRY(set_identity_batched(m, Rs, 3 * m, 3 * m2, nO, st));
-    SSLAB_TRY(copy_block_batched(m, m, cM + m2, m, 2 * m2, Rs + m, 3 * m, 3 * m2, nO, st));
+    const bool ident = (l == 0 && T == nullptr);
     const int nOr = (n % 2 == 0) ? nO - 1 : nO;  // odd nodes with a right neighbour
-    SSLAB_TRY(copy_block_batched(m, m, cP + m2, m, 2 * m2, Rs + 2 * m, 3 * m, 3 * m2, nOr, st));
-    if (nOr < nO)
-      SSLAB_TRY(zero_block_batched(m, m, Rs + (long long)nOr * 3 * m2 + 2 * m, 3 * m, 3 * m2, 1,
-                                   st));
-    // 4. TG = T_odd^{-1} [I | SiM | SiP].
-    SSLAB_TRY(getrs_batched(m, 3 * m, LUw, m, m2, ipiv, m, Rs, 3 * m, 3 * m2, TG, 3 * m, 3 * m2,
-                            perm, nO, st));
+    if (ident) {
+      // TG_j = [I | SiM_j | SiP_j],  W_b = [SiM_{2b} | SiP_{2b}],  T'_b = I (then updated).
+      SSLAB_TRY(set_identity_batched(m, TG, 3 * m, 3 * m2, nO, st));
+      SSLAB_TRY(copy_block_batched(m, m, cM + m2, m, 2 * m2, TG + m, 3 * m, 3 * m2, nO, st));
+      SSLAB_TRY(copy_block_batched(m, m, cP + m2, m, 2 * m2, TG + 2 * m, 3 * m, 3 * m2, nOr, st));
+      if (nOr < nO)
+        SSLAB_TRY(zero_block_batched(m, m, TG + (long long)nOr * 3 * m2 + 2 * m, 3 * m, 3 * m2, 1,
+                                     st));
+      if (nE > 1)
+        SSLAB_TRY(copy_block_batched(m, m, cM + 2 * m2, m, 2 * m2, WW + 2 * m2, 2 * m, 2 * m2,
+                                     nE - 1, st));
+      SSLAB_TRY(copy_block_batched(m, m, cP, m, 2 * m2, WW + m, 2 * m, 2 * m2, nO, st));
+      if (info) zero_int_async(info + info_base, nO, st);
+      info_base += nO;
+      SSLAB_TRY(set_identity_batched(m, nT, m, m2, nE, st));
+    } else {
+      // 1-2. LU of the odd diagonal blocks.
+      SSLAB_TRY(copy_block_batched(m, m, cT + m2, m, 2 * m2, LUw, m, m2, nO, st));
+      SSLAB_TRY(getrf_batched(m, LUw, m, m2, ipiv, m, linfo, nO, st));
+      if (info) {
+        SSLAB_HIP_CHECK(hipMemcpyAsync(info + info_base, linfo, sizeof(int) * nO,
+                                       hipMemcpyDeviceToDevice, st));
+      }
+      info_base += nO;
+      // 3-4. Tinv_j (first block of TG) from the LU, then
+      //      TG_j[:, m:3m] = Tinv_j [SiM_j | SiP_j] as one MFMA GEMM launch (SiP of a
+      //      last odd node without a right neighbour is zero).
+      SSLAB_TRY(set_identity_batched(m, Rs, m, m2, nO, st));
+      SSLAB_TRY(getrs_batched(m, m, LUw, m, m2, ipiv, m, Rs, m, m2, TG, 3 * m, 3 * m2, perm, nO,
+                              st));
+      {
+        GemmLaunch G{};
+        G.nd = 0;
+        G.d[G.nd] = gemm_desc(m, m, nO, TG + m, 3 * m, 3 * m2, nullptr, 0, 0, 0.0);
+        gemm_add_term(G.d[G.nd++], gemm_term(TG, 3 * m, 3 * m2, 0, cM + m2, m, 2 * m2, 0, m, 1.0));
+        if (nOr > 0) {
+          G.d[G.nd] = gemm_desc(m, m, nOr, TG + 2 * m, 3 * m, 3 * m2, nullptr, 0, 0, 0.0);
+          gemm_add_term(G.d[G.nd++],
+                        gemm_term(TG, 3 * m, 3 * m2, 0, cP + m2, m, 2 * m2, 0, m, 1.0));
+        }
+        SSLAB_TRY(gemm_launch(G, st));
+      }
+      if (nOr < nO)
+        SSLAB_TRY(zero_block_batched(m, m, TG + (long long)nOr * 3 * m2 + 2 * m, 3 * m, 3 * m2, 1,
+                                     st));
+    }
     // 5. W^L_b = SiM_{2b} Tinv_{b-1} (b >= 1),  W^R_b = SiP_{2b} Tinv_b (b < nO).
-    {
+    if (!ident) {
       GemmLaunch G{};
       G.nd = 2;
       G.d[0] = gemm_desc(m, m, nE - 1, WW + 2 * m2, 2 * m, 2 * m2, nullptr, 0, 0, 0.0);
@@ -159,21 +198,24 @@ int rb_factor(int nb, int m, const double* T, const double* SiM, const double* S
       GemmLaunch G{};
       int k = 0;
       const int both = (nE < nO ? nE : nO) - 1;  // b in [1, min(nE,nO))
+      // Cin: T_{2b} (strided by two blocks), or the identity just written into nT.
+      const double* cin = ident ? nT : cT;
+      const long long scin = ident ? m2 : 2 * m2;
       if (both > 0) {
-        GemmDesc d = gemm_desc(m, m, both, nT + m2, m, m2, cT + 2 * m2, m, 2 * m2, 1.0);
+        GemmDesc d = gemm_desc(m, m, both, nT + m2, m, m2, cin + scin, m, scin, 1.0);
         gemm_add_term(d, gemm_term(WW + 2 * m2, 2 * m, 2 * m2, 0, cP + m2, m, 2 * m2, 0, m, -1.0));
         gemm_add_term(d, gemm_term(WW + 2 * m2 + m, 2 * m, 2 * m2, 0, cM + 3 * m2, m, 2 * m2, 0, m,
                                    -1.0));
         G.d[k++] = d;
       }
       {  // b = 0: right neighbour only (n >= 2 so node 1 exists)
-        GemmDesc d = gemm_desc(m, m, 1, nT, m, m2, cT, m, m2, 1.0);
+        GemmDesc d = gemm_desc(m, m, 1, nT, m, m2, cin, m, m2, 1.0);
         gemm_add_term(d, gemm_term(WW + m, 2 * m, 2 * m2, 0, cM + m2, m, m2, 0, m, -1.0));
         G.d[k++] = d;
       }
       if (nE > nO && nE - 1 >= 1) {  // n odd: last even node has a left neighbour only
         const long long b = nE - 1;
-        GemmDesc d = gemm_desc(m, m, 1, nT + b * m2, m, m2, cT + 2 * b * m2, m, m2, 1.0);
+        GemmDesc d = gemm_desc(m, m, 1, nT + b * m2, m, m2, cin + b * scin, m, m2, 1.0);
         gemm_add_term(d, gemm_term(WW + b * 2 * m2, 2 * m, 2 * m2, 0, cP + (2 * b - 1) * m2, m, m2,
                                    0, m, -1.0));
         G.d[k++] = d;
@@ -195,7 +237,11 @@ int rb_factor(int nb, int m, const double* T, const double* SiM, const double* S
     cM = nM;
     cP = nP;
   }
-  // Coarsest block: Tinv_top.
+  // Coarsest block: Tinv_top (the identity when the only block is an identity T).
+  if (cT == nullptr) {
+    if (info) zero_int_async(info + info_base, 1, st);
+    return set_identity_batched(m, factor + p.top_off, m, m2, 1, st);
+  }
   SSLAB_TRY(copy_block_batched(m, m, cT, m, m2, LUw, m, m2, 1, st));
   SSLAB_TRY(getrf_batched(m, LUw, m, m2, ipiv, m, linfo, 1, st));
   if (info) {

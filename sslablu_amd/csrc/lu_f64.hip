@@ -1,15 +1,16 @@
 // Batched FP64 LU (partial pivoting) and triangular solves for gfx950.
 //
-// getrf: right-looking blocked LU.  Each panel step is four launches over the
-// whole batch: the panel factorization (one workgroup per matrix, the panel held
-// column-major in LDS so the pivot search and the rank-1 updates read
-// consecutive LDS words), the row interchanges on the remaining columns, a
-// unit-lower triangular solve for the U12 block row, and the trailing Schur
-// update A22 -= L21 U12 on the MFMA GEMM.
+// getrf: right-looking blocked LU with 128-column outer blocks, so the trailing
+// Schur update A22 -= L21 U12 is an MFMA GEMM with K = 128.  Each 128-column panel
+// is factored recursively (halves: left half, its interchanges on the right half,
+// U12 solve, panel-local GEMM update, right half, its interchanges on the left
+// half) down to a base panel that one workgroup per matrix factors with the panel
+// held column-major in LDS (pivot search and rank-1 updates on consecutive LDS
+// words).
 //
-// getrs: gather rows by the pivot permutation, then block forward / backward
-// substitution: the diagonal-block solves run one column per lane with the
-// triangle in LDS, the off-diagonal updates are MFMA GEMMs.
+// Triangular solves (getrs, the U12 block rows): two-level blocking -- 32-row
+// diagonal solves, one column per lane with the triangle in LDS, inside 128-row
+// blocks whose off-block updates are MFMA GEMMs with K = 128.
 #include "lu_f64.hpp"
 
 #include "gemm.hpp"
@@ -130,13 +131,16 @@ k_getrf_panel(int n, int j0, int w, double* __restrict__ A, int lda, long long s
   }
 }
 
-// Apply the panel's interchanges (rows j0..j0+w-1) to columns [0,j0) U [j0+w,n).
-__global__ void k_laswp(int n, int j0, int w, double* __restrict__ A, int lda, long long sA,
-                        const int* __restrict__ ipiv, long long sP) {
-  const int ncols = n - w;
+// Apply the interchanges of rows r0..r0+w-1 (in order) to the columns
+// [a0, a1) U [b0, b1).
+__global__ void k_laswp(int r0, int w, int a0, int a1, int b0, int b1, double* __restrict__ A,
+                        int lda, long long sA, const int* __restrict__ ipiv, long long sP) {
+  const int na = a1 - a0;
+  const int ncols = na + (b1 - b0);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncols) return;
-  const int col = t < j0 ? t : t + w;
+  const int col = t < na ? a0 + t : b0 + (t - na);
+  const int j0 = r0;
   double* Ab = A + (long long)blockIdx.y * sA;
   const int* pv = ipiv + (long long)blockIdx.y * sP;
   for (int c = 0; c < w; ++c) {
@@ -152,43 +156,49 @@ __global__ void k_laswp(int n, int j0, int w, double* __restrict__ A, int lda, l
 
 // B[0:nr, col] <- T^{-1} B[0:nr, col] for the nr x nr triangle T (nr <= NB), one
 // column per lane.  kLower: unit lower triangle; else upper with its diagonal.
+// Both run the same backward (upper) substitution: the lower solve works in reversed
+// coordinates (J L J is upper triangular for the reversal J), because the compiler
+// schedules the unrolled backward loop without hoisting the whole triangle into
+// registers, while the unrolled forward loop spills.
 template <int NB, bool kLower>
 __global__ void __launch_bounds__(256)
 k_trsm_cols(int nr, int ncols, const double* __restrict__ T, int ldt, long long sT,
             double* __restrict__ B, int ldb, long long sB) {
-  __shared__ double Ts[NB * NB];
+  __shared__ double Ts[NB * NB];  // the triangle in solve coordinates
   const double* Tb = T + (long long)blockIdx.y * sT;
   double* Bb = B + (long long)blockIdx.y * sB;
   for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
     const int i = e / NB, j = e % NB;
-    Ts[e] = (i < nr && j < nr) ? Tb[(long long)i * ldt + j] : (i == j ? 1.0 : 0.0);
+    const int ri = kLower ? NB - 1 - i : i, rj = kLower ? NB - 1 - j : j;
+    // the unit lower triangle's diagonal holds U's in the packed LU: use 1
+    Ts[e] = (ri < nr && rj < nr && !(kLower && ri == rj)) ? Tb[(long long)ri * ldt + rj]
+                                                          : (i == j ? 1.0 : 0.0);
   }
   __syncthreads();
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncols) return;
+  const int col0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col0 >= ncols) return;
+  // x[i] is row ri(i) of the column; rows past nr re-read row nr-1 and are zeroed.
+  const double* bcol = Bb + col0;
   double x[NB];
 #pragma unroll
-  for (int i = 0; i < NB; ++i) x[i] = (i < nr) ? Bb[(long long)i * ldb + col] : 0.0;
-  if (kLower) {
-#pragma unroll
-    for (int i = 1; i < NB; ++i) {
-      double s = x[i];
-#pragma unroll
-      for (int j = 0; j < i; ++j) s -= Ts[i * NB + j] * x[j];
-      x[i] = s;
-    }
-  } else {
-#pragma unroll
-    for (int i = NB - 1; i >= 0; --i) {
-      double s = x[i];
-#pragma unroll
-      for (int j = i + 1; j < NB; ++j) s -= Ts[i * NB + j] * x[j];
-      x[i] = s / Ts[i * NB + i];
-    }
+  for (int i = 0; i < NB; ++i) {
+    const int ri = kLower ? NB - 1 - i : i;
+    const double v = bcol[(long long)(ri < nr ? ri : nr - 1) * ldb];
+    x[i] = (ri < nr) ? v : 0.0;
   }
 #pragma unroll
-  for (int i = 0; i < NB; ++i)
-    if (i < nr) Bb[(long long)i * ldb + col] = x[i];
+  for (int i = NB - 1; i >= 0; --i) {
+    double s = x[i];
+#pragma unroll
+    for (int j = i + 1; j < NB; ++j) s -= Ts[i * NB + j] * x[j];
+    x[i] = s / Ts[i * NB + i];
+  }
+  double* ocol = Bb + col0;
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int ri = kLower ? NB - 1 - i : i;
+    if (ri < nr) ocol[(long long)ri * ldb] = x[i];
+  }
 }
 
 // perm = the row order produced by applying ipiv[0..n) to the identity.
@@ -264,6 +274,118 @@ int max_lds_bytes() {
 
 }  // namespace
 
+namespace {
+
+constexpr int kOuterNB = 128;  // outer block of getrf and of the triangular solves
+
+int laswp(int r0, int w, int a0, int a1, int b0, int b1, double* A, int lda, long long sA,
+          const int* ipiv, long long sP, int batch, hipStream_t st) {
+  const int ncols = (a1 - a0) + (b1 - b0);
+  if (w <= 0 || ncols <= 0) return kOk;
+  hipLaunchKernelGGL(k_laswp, dim3(ceil_div(ncols, 256), batch), dim3(256), 0, st, r0, w, a0, a1,
+                     b0, b1, A, lda, sA, ipiv, sP);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+// X (n x nrhs) <- T^{-1} X for the unit-lower (lower = true) or upper n x n triangle T.
+int trsm_2level(bool lower, int n, int nrhs, const double* T, int ldt, long long sT, double* X,
+                int ldx, long long sX, int batch, hipStream_t st) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return kOk;
+  constexpr int NB = kTrsmNB;
+  const dim3 grid(ceil_div(nrhs, 256), batch);
+  auto diag = [&](int ib, int ni) -> int {
+    if (lower)
+      hipLaunchKernelGGL((k_trsm_cols<NB, true>), grid, dim3(256), 0, st, ni, nrhs,
+                         T + (long long)ib * ldt + ib, ldt, sT, X + (long long)ib * ldx, ldx, sX);
+    else
+      hipLaunchKernelGGL((k_trsm_cols<NB, false>), grid, dim3(256), 0, st, ni, nrhs,
+                         T + (long long)ib * ldt + ib, ldt, sT, X + (long long)ib * ldx, ldx, sX);
+    SSLAB_HIP_CHECK(hipGetLastError());
+    return kOk;
+  };
+  // X[r0:r0+nr) -= T[r0:r0+nr, c0:c0+nc) X[c0:c0+nc)
+  auto upd = [&](int r0, int nr, int c0, int nc) -> int {
+    if (nr <= 0 || nc <= 0) return kOk;
+    return gemm_strided<double>(0, 0, nr, nrhs, nc, -1.0, T + (long long)r0 * ldt + c0, ldt, sT,
+                                X + (long long)c0 * ldx, ldx, sX, 1.0, X + (long long)r0 * ldx,
+                                ldx, sX, batch, st);
+  };
+  if (lower) {
+    for (int kb = 0; kb < n; kb += kOuterNB) {
+      const int nr = n - kb < kOuterNB ? n - kb : kOuterNB;
+      for (int ib = kb; ib < kb + nr; ib += NB) {
+        const int ni = kb + nr - ib < NB ? kb + nr - ib : NB;
+        SSLAB_TRY(diag(ib, ni));
+        SSLAB_TRY(upd(ib + ni, kb + nr - ib - ni, ib, ni));
+      }
+      SSLAB_TRY(upd(kb + nr, n - kb - nr, kb, nr));
+    }
+  } else {
+    const int nob = ceil_div(n, kOuterNB);
+    for (int ob = nob - 1; ob >= 0; --ob) {
+      const int kb = ob * kOuterNB;
+      const int nr = n - kb < kOuterNB ? n - kb : kOuterNB;
+      const int nib = ceil_div(nr, NB);
+      for (int q = nib - 1; q >= 0; --q) {
+        const int ib = kb + q * NB;
+        const int ni = kb + nr - ib < NB ? kb + nr - ib : NB;
+        SSLAB_TRY(diag(ib, ni));
+        SSLAB_TRY(upd(kb, ib - kb, ib, ni));
+      }
+      SSLAB_TRY(upd(0, kb, kb, nr));
+    }
+  }
+  return kOk;
+}
+
+struct PanelCtx {
+  int n, lda, batch, lds, base_lds, base_glb;
+  long long sA, sP;
+  double* A;
+  int* ipiv;
+  int* info;
+  hipStream_t st;
+};
+
+// Widest base panel (8, 16 or 32 columns) whose rows fit in LDS; 0 = none does.
+int base_width(const PanelCtx& c, int rows) {
+  for (int w = 32; w >= 8; w >>= 1)
+    if ((long long)rows * w * 8 <= c.lds) return w;
+  return 0;
+}
+
+// Factor rows [j0, n) of columns [j0, j0 + w).
+int panel_rec(const PanelCtx& c, int j0, int w) {
+  const int rows = c.n - j0;
+  const int bw = base_width(c, rows);
+  if (w <= (bw > 0 ? bw : 32)) {
+    if (bw > 0) {
+      const size_t shm = (size_t)rows * w * sizeof(double);
+      hipLaunchKernelGGL(k_getrf_panel<true>, dim3(c.batch), dim3(kPanelThreads), shm, c.st, c.n,
+                         j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+    } else {
+      hipLaunchKernelGGL(k_getrf_panel<false>, dim3(c.batch), dim3(kPanelThreads), 0, c.st, c.n,
+                         j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+    }
+    SSLAB_HIP_CHECK(hipGetLastError());
+    return kOk;
+  }
+  const int w1 = (w / 2 + 7) / 8 * 8;
+  const int w2 = w - w1;
+  double* A11 = c.A + (long long)j0 * c.lda + j0;
+  SSLAB_TRY(panel_rec(c, j0, w1));
+  SSLAB_TRY(laswp(j0, w1, j0 + w1, j0 + w, 0, 0, c.A, c.lda, c.sA, c.ipiv, c.sP, c.batch, c.st));
+  SSLAB_TRY(trsm_2level(true, w1, w2, A11, c.lda, c.sA, A11 + w1, c.lda, c.sA, c.batch, c.st));
+  SSLAB_TRY(gemm_strided<double>(0, 0, rows - w1, w2, w1, -1.0, A11 + (long long)w1 * c.lda,
+                                 c.lda, c.sA, A11 + w1, c.lda, c.sA, 1.0,
+                                 A11 + (long long)w1 * c.lda + w1, c.lda, c.sA, c.batch, c.st));
+  SSLAB_TRY(panel_rec(c, j0 + w1, w2));
+  return laswp(j0 + w1, w2, j0, j0 + w1, 0, 0, c.A, c.lda, c.sA, c.ipiv, c.sP, c.batch, c.st);
+}
+
+}  // namespace
+
 int getrf_batched(int n, double* A, int lda, long long sA, int* ipiv, long long sP, int* info,
                   int batch, hipStream_t stream) {
   SSLAB_REQUIRE(n >= 0 && batch >= 0, "getrf: negative size");
@@ -275,43 +397,30 @@ int getrf_batched(int n, double* A, int lda, long long sA, int* ipiv, long long 
     hipLaunchKernelGGL(k_zero_int, dim3(ceil_div(batch, 256)), dim3(256), 0, stream, info, batch);
     SSLAB_HIP_CHECK(hipGetLastError());
   }
-  // Widest panel (<= 32 columns) whose column-major copy fits in LDS; reserve a
-  // little room for the static reduction scratch.
-  const int lds = max_lds_bytes() - 512;
-  int nb = 32;
-  while (nb > 8 && (long long)n * nb * 8 > lds) nb >>= 1;
-  const bool use_lds = (long long)n * nb * 8 <= lds;
-  if (!use_lds) nb = 32;
-  if (use_lds) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_getrf_panel<true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  }
-  for (int j0 = 0; j0 < n; j0 += nb) {
-    const int w = (n - j0) < nb ? (n - j0) : nb;
-    const size_t shm = use_lds ? (size_t)(n - j0) * w * sizeof(double) : 0;
-    if (use_lds)
-      hipLaunchKernelGGL(k_getrf_panel<true>, dim3(batch), dim3(kPanelThreads), shm, stream, n, j0,
-                         w, A, lda, sA, ipiv, sP, info);
-    else
-      hipLaunchKernelGGL(k_getrf_panel<false>, dim3(batch), dim3(kPanelThreads), 0, stream, n, j0,
-                         w, A, lda, sA, ipiv, sP, info);
-    SSLAB_HIP_CHECK(hipGetLastError());
-    if (n - w > 0) {
-      hipLaunchKernelGGL(k_laswp, dim3(ceil_div(n - w, 256), batch), dim3(256), 0, stream, n, j0,
-                         w, A, lda, sA, ipiv, sP);
-      SSLAB_HIP_CHECK(hipGetLastError());
-    }
+  PanelCtx c{};
+  c.n = n;
+  c.lda = lda;
+  c.batch = batch;
+  c.sA = sA;
+  c.sP = sP;
+  c.A = A;
+  c.ipiv = ipiv;
+  c.info = info;
+  c.st = stream;
+  c.lds = max_lds_bytes() - 512;  // room for the static reduction scratch
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_getrf_panel<true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+  for (int j0 = 0; j0 < n; j0 += kOuterNB) {
+    const int w = (n - j0) < kOuterNB ? (n - j0) : kOuterNB;
+    SSLAB_TRY(panel_rec(c, j0, w));
+    SSLAB_TRY(laswp(j0, w, 0, j0, j0 + w, n, A, lda, sA, ipiv, sP, batch, stream));
     const int rest = n - j0 - w;
     if (rest > 0) {
-      double* L11 = A + (long long)j0 * lda + j0;
-      double* A12 = A + (long long)j0 * lda + j0 + w;
-      hipLaunchKernelGGL((k_trsm_cols<32, true>), dim3(ceil_div(rest, 256), batch), dim3(256), 0,
-                         stream, w, rest, L11, lda, sA, A12, lda, sA);
-      SSLAB_HIP_CHECK(hipGetLastError());
-      double* A21 = A + (long long)(j0 + w) * lda + j0;
-      double* A22 = A + (long long)(j0 + w) * lda + j0 + w;
-      SSLAB_TRY(gemm_strided(0, 0, rest, rest, w, -1.0, A21, lda, sA, A12, lda, sA, 1.0, A22, lda,
-                             sA, batch, stream));
+      double* A11 = A + (long long)j0 * lda + j0;
+      SSLAB_TRY(trsm_2level(true, w, rest, A11, lda, sA, A11 + w, lda, sA, batch, stream));
+      SSLAB_TRY(gemm_strided<double>(0, 0, rest, rest, w, -1.0, A11 + (long long)w * lda, lda, sA,
+                                     A11 + w, lda, sA, 1.0, A11 + (long long)w * lda + w, lda, sA,
+                                     batch, stream));
     }
   }
   return kOk;
@@ -332,36 +441,8 @@ int getrs_batched(int n, int nrhs, const double* LU, int lda, long long sA, cons
   hipLaunchKernelGGL(k_gather_rows, dim3(ceil_div(nrhs, 256), n, batch), dim3(256), 0, stream, n,
                      nrhs, perm, B, ldb, sB, X, ldx, sX);
   SSLAB_HIP_CHECK(hipGetLastError());
-  constexpr int NB = kTrsmNB;
-  // Forward: L Y = P B.
-  for (int kb = 0; kb < n; kb += NB) {
-    const int nr = (n - kb) < NB ? (n - kb) : NB;
-    hipLaunchKernelGGL((k_trsm_cols<NB, true>), dim3(ceil_div(nrhs, 256), batch), dim3(256), 0,
-                       stream, nr, nrhs, LU + (long long)kb * lda + kb, lda, sA,
-                       X + (long long)kb * ldx, ldx, sX);
-    SSLAB_HIP_CHECK(hipGetLastError());
-    const int below = n - kb - nr;
-    if (below > 0) {
-      SSLAB_TRY(gemm_strided(0, 0, below, nrhs, nr, -1.0, LU + (long long)(kb + nr) * lda + kb, lda,
-                             sA, X + (long long)kb * ldx, ldx, sX, 1.0,
-                             X + (long long)(kb + nr) * ldx, ldx, sX, batch, stream));
-    }
-  }
-  // Backward: U X = Y.
-  const int nblk = ceil_div(n, NB);
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int kb = bi * NB;
-    const int nr = (n - kb) < NB ? (n - kb) : NB;
-    hipLaunchKernelGGL((k_trsm_cols<NB, false>), dim3(ceil_div(nrhs, 256), batch), dim3(256), 0,
-                       stream, nr, nrhs, LU + (long long)kb * lda + kb, lda, sA,
-                       X + (long long)kb * ldx, ldx, sX);
-    SSLAB_HIP_CHECK(hipGetLastError());
-    if (kb > 0) {
-      SSLAB_TRY(gemm_strided(0, 0, kb, nrhs, nr, -1.0, LU + kb, lda, sA, X + (long long)kb * ldx,
-                             ldx, sX, 1.0, X, ldx, sX, batch, stream));
-    }
-  }
-  return kOk;
+  SSLAB_TRY(trsm_2level(true, n, nrhs, LU, lda, sA, X, ldx, sX, batch, stream));   // L Y = P B
+  return trsm_2level(false, n, nrhs, LU, lda, sA, X, ldx, sX, batch, stream);      // U X = Y
 }
 
 int trsm_upper_batched(int n, int nrhs, const double* U, int ldu, long long sU, double* X, int ldx,
@@ -370,21 +451,7 @@ int trsm_upper_batched(int n, int nrhs, const double* U, int ldu, long long sU, 
   SSLAB_REQUIRE(ldu >= n && ldx >= nrhs, "trsm: bad leading dimension");
   if (n == 0 || nrhs == 0 || batch == 0) return kOk;
   SSLAB_REQUIRE(batch <= 65535, "trsm: batch too large");
-  constexpr int NB = kTrsmNB;
-  const int nblk = ceil_div(n, NB);
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int kb = bi * NB;
-    const int nr = (n - kb) < NB ? (n - kb) : NB;
-    hipLaunchKernelGGL((k_trsm_cols<NB, false>), dim3(ceil_div(nrhs, 256), batch), dim3(256), 0,
-                       stream, nr, nrhs, U + (long long)kb * ldu + kb, ldu, sU,
-                       X + (long long)kb * ldx, ldx, sX);
-    SSLAB_HIP_CHECK(hipGetLastError());
-    if (kb > 0) {
-      SSLAB_TRY(gemm_strided(0, 0, kb, nrhs, nr, -1.0, U + kb, ldu, sU, X + (long long)kb * ldx,
-                             ldx, sX, 1.0, X, ldx, sX, batch, stream));
-    }
-  }
-  return kOk;
+  return trsm_2level(false, n, nrhs, U, ldu, sU, X, ldx, sX, batch, stream);
 }
 
 int set_identity_batched(int n, double* X, int ldx, long long sX, int batch, hipStream_t stream) {

@@ -92,8 +92,7 @@ class CachedOperator:
         op = HPSSlabOperator(make_pdo(spec), g, device).build()
         op.Ts = None  # slab DtN maps are not needed after assembly
         nb, m = g.nb, g.m
-        eye = torch.eye(m, dtype=F64, device=device).expand(nb, m, m)
-        fac = BlockTriFactor("rb", device).factorize(eye, op.SiM, op.SiP)
+        fac = BlockTriFactor("rb", device).factorize(None, op.SiM, op.SiP)  # T = I
         if mixed:
             fac.enable_mixed(None, op.SiM, op.SiP)
         torch.cuda.synchronize(device)

@@ -79,7 +79,7 @@ def test_dgemm_strided_batched(torch_dev):
 
 
 @pytest.mark.parametrize("n,batch", [(1, 1), (5, 3), (33, 2), (64, 4), (200, 3), (512, 2),
-                                     (1000, 1)])
+                                     (1000, 1), (1100, 2), (2600, 1)])
 def test_getrf_getrs(torch_dev, n, batch):
     torch, dev = torch_dev
     from sslablu_amd._lib import call, ptr, stream_handle
@@ -207,3 +207,26 @@ def test_rb_level_api(torch_dev):
                 assert np.linalg.norm(got[j][k].cpu().numpy()) <= 1e-12
             else:
                 assert rel(got[j][k].cpu().numpy(), ref[j][k]) < 1e-12
+
+
+@pytest.mark.parametrize("nb", [1, 2, 3, 7, 16])
+@pytest.mark.parametrize("method", ["rb", "thomas"])
+def test_identity_diagonal_shortcut(torch_dev, nb, method):
+    """T = None (identity diagonal blocks, the oms S-system) equals passing T = I."""
+    torch, dev = torch_dev
+    from sslablu_amd.blocktri import BlockTriFactor
+    m, r = 40, 3
+    rng = np.random.default_rng(nb)
+    SiM = 0.3 * rng.standard_normal((nb, m, m)) / np.sqrt(m)
+    SiP = 0.3 * rng.standard_normal((nb, m, m)) / np.sqrt(m)
+    SiM[0] = 0
+    SiP[-1] = 0
+    B = torch.tensor(rng.standard_normal((nb * m, r)), device=dev)
+    f0 = BlockTriFactor(method).factorize(None, SiM, SiP)
+    f1 = BlockTriFactor(method).factorize(np.stack([np.eye(m)] * nb), SiM, SiP)
+    x0, x1 = f0.solve(B), f1.solve(B)
+    assert int((f0.info != 0).sum()) == 0
+    assert (torch.linalg.norm(x0 - x1) / torch.linalg.norm(x1)).item() < 1e-12
+    S = [(SiM[i], SiP[i]) for i in range(nb)]
+    ref = ob.thomas_direct(S, [np.eye(m)] * nb, B.cpu().numpy())
+    assert rel(x0.cpu().numpy(), ref) < TOL
