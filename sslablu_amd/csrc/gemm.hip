@@ -289,8 +289,11 @@ struct FastGeom {
                              A_VECS >= 1 && B_VECS >= 1;
 };
 
+// Two waves per SIMD at least (256 registers per lane): the 128x128 tile otherwise
+// takes 268 and runs one wave per SIMD.
 template <class Cfg, typename T>
-__global__ void __launch_bounds__(Cfg::NT) k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
+__global__ void __launch_bounds__(Cfg::NT) __attribute__((amdgpu_waves_per_eu(2)))
+k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
   using G = FastGeom<Cfg, T>;
   using M4 = Mfma<T>;
   using v4 = typename M4::v4;
@@ -414,26 +417,25 @@ __global__ void __launch_bounds__(Cfg::NT) k_gemm_fast(const GemmLaunchT<T> L, i
       {
         const T* As = smem + stage * G::STAGE;
         const T* Bs = As + Cfg::BM * LD;
-        vec af[Cfg::FM][KS / VEC], bf[Cfg::FN][KS / VEC];
+        // one 16-byte group of k steps at a time keeps the operand registers small
 #pragma unroll
-        for (int i = 0; i < Cfg::FM; ++i)
-#pragma unroll
-          for (int q = 0; q < KS / VEC; ++q)
-            af[i][q] = *reinterpret_cast<const vec*>(As + (wrow + i * 16 + li) * LD + g * KS +
-                                                     q * VEC);
-#pragma unroll
-        for (int j = 0; j < Cfg::FN; ++j)
-#pragma unroll
-          for (int q = 0; q < KS / VEC; ++q)
-            bf[j][q] = *reinterpret_cast<const vec*>(Bs + (wcol + j * 16 + li) * LD + g * KS +
-                                                     q * VEC);
-#pragma unroll
-        for (int s = 0; s < KS; ++s)
+        for (int q = 0; q < KS / VEC; ++q) {
+          vec af[Cfg::FM], bf[Cfg::FN];
 #pragma unroll
           for (int i = 0; i < Cfg::FM; ++i)
+            af[i] = *reinterpret_cast<const vec*>(As + (wrow + i * 16 + li) * LD + g * KS +
+                                                  q * VEC);
 #pragma unroll
-            for (int j = 0; j < Cfg::FN; ++j)
-              acc[i][j] = M4::op(af[i][s / VEC][s % VEC], bf[j][s / VEC][s % VEC], acc[i][j]);
+          for (int j = 0; j < Cfg::FN; ++j)
+            bf[j] = *reinterpret_cast<const vec*>(Bs + (wcol + j * 16 + li) * LD + g * KS +
+                                                  q * VEC);
+#pragma unroll
+          for (int e = 0; e < VEC; ++e)
+#pragma unroll
+            for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+              for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = M4::op(af[i][e], bf[j][e], acc[i][j]);
+        }
       }
       if (more) store(stage ^ 1);
       __syncthreads();

@@ -1,7 +1,7 @@
 // Batched FP64 LU (partial pivoting) and triangular solves for gfx950.
 //
-// getrf: right-looking blocked LU with 128-column outer blocks, so the trailing
-// Schur update A22 -= L21 U12 is an MFMA GEMM with K = 128.  Each 128-column panel
+// getrf: right-looking blocked LU with 256-column outer blocks, so the trailing
+// Schur update A22 -= L21 U12 is an MFMA GEMM with K = 256.  Each 256-column panel
 // is factored recursively (halves: left half, its interchanges on the right half,
 // U12 solve, panel-local GEMM update, right half, its interchanges on the left
 // half) down to a base panel that one workgroup per matrix factors with the panel
@@ -9,8 +9,8 @@
 // words).
 //
 // Triangular solves (getrs, the U12 block rows): two-level blocking -- 32-row
-// diagonal solves, one column per lane with the triangle in LDS, inside 128-row
-// blocks whose off-block updates are MFMA GEMMs with K = 128.
+// diagonal solves, one column per lane with the triangle in LDS, inside 256-row
+// blocks whose off-block updates are MFMA GEMMs with K = 256.
 #include "lu_f64.hpp"
 
 #include "gemm.hpp"
@@ -276,7 +276,7 @@ int max_lds_bytes() {
 
 namespace {
 
-constexpr int kOuterNB = 128;  // outer block of getrf and of the triangular solves
+constexpr int kOuterNB = 256;  // outer block of getrf and of the triangular solves
 
 int laswp(int r0, int w, int a0, int a1, int b0, int b1, double* A, int lda, long long sA,
           const int* ipiv, long long sP, int batch, hipStream_t st) {
