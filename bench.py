@@ -178,6 +178,97 @@ def serving_bench(dev, n_req=2000, seed=0):
     return out
 
 
+def c4_bench(dev, seed=0, reps=5):
+    """Synthetic block-tridiagonal workload with T != I: 64 diagonal blocks of n = 2048
+    (Gaussian-kernel entries exp(-|x_i - x_j|^2 / 0.01) on sorted U(0,1) points, coupling
+    blocks x 0.1, diagonal + n), 128 N(0,1) right-hand sides: red-black factor time and
+    solve throughput, residual; then a batch of three systems n in {512, 1024, 2048}
+    solved separately and in shared grouped launches (default and autotuned tile policy)."""
+    import torch
+    from sslablu_amd import tuning
+    from sslablu_amd.blocktri import BlockTriFactor, solve_multi
+    from sslablu_amd.synthetic import gaussian_blocktri
+    nb, n, r = 64, 2048, 128
+    out = {"workload": f"{nb} blocks of n={n}, Gaussian kernel h=0.01, coupling 0.1, "
+                       f"diagonal +n, {r} RHS, red-black FP64 (T != I)"}
+    Tw, Mw, Pw = gaussian_blocktri(8, 256, seed, device=dev)  # warm-up (code objects)
+    BlockTriFactor("rb", dev).factorize(Tw, Mw, Pw)
+    del Tw, Mw, Pw
+    T, M, P = gaussian_blocktri(nb, n, seed, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fac = BlockTriFactor("rb", dev).factorize(T, M, P)
+    torch.cuda.synchronize()
+    t_fac = time.perf_counter() - t0
+    g = torch.Generator(device=dev).manual_seed(seed)
+    B = torch.randn(nb * n, r, dtype=torch.float64, device=dev, generator=g)
+    V, X = B.clone(), torch.empty_like(B)
+    fac.solve_(V, X)
+    ts = []
+    for _ in range(reps):
+        V.copy_(B)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fac.solve_(V, X)
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    t_solve = float(np.median(ts)) / 1e3
+    fac.enable_mixed(T, M, P)
+    res = (fac.residual(B, X).norm() / B.norm()).item()
+    flops_rhs = 2.0 * (3 * nb - 2) * n * n
+    fac_flops = n ** 3 * (4 * (nb - 1) + 2 * nb / 3)
+    out.update({"factor_s": round(t_fac, 3),
+                "factor_tflops_algorithmic": round(fac_flops / t_fac / 1e12, 2),
+                "solve_ms": round(t_solve * 1e3, 3),
+                "rhs_solves_per_s": round(r / t_solve, 1),
+                "solve_tflops_algorithmic": round(flops_rhs * r / t_solve / 1e12, 2),
+                "solve_roofline_frac": round(flops_rhs * r / t_solve / 78.6e12, 4),
+                "residual": res, "factor_gb": round(fac.nbytes / 1e9, 2)})
+    del fac, T, M, P, B, V, X
+    torch.cuda.empty_cache()
+    facs, Bs = [], []
+    for k, nk in enumerate((512, 1024, 2048)):
+        Tk, Mk, Pk = gaussian_blocktri(nb, nk, seed + k, device=dev)
+        facs.append(BlockTriFactor("rb", dev).factorize(Tk, Mk, Pk))
+        Bs.append(torch.randn(nb * nk, r, dtype=torch.float64, device=dev, generator=g))
+        del Tk, Mk, Pk
+    Vs = [b.clone() for b in Bs]
+    Xs = [torch.empty_like(b) for b in Bs]
+
+    def separate():
+        for f, V_, X_ in zip(facs, Vs, Xs):
+            f.solve_(V_, X_)
+
+    def grouped():
+        solve_multi(facs, Vs, Xs)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    cands = [tuning.Policy(t, sp, 64, 256, cm, cs, mt) for t, sp, cm, cs, mt in
+             ((1024, 512, 3, 3, 0), (1024, 512, 3, 3, 1), (512, 1024, 2, 3, 1), (512, 512, 3, 2, 2),
+              (1024, 1024, 2, 2, 2), (2048, 512, 3, 3, 1))]
+    pol, _ = tuning.autotune(grouped, "c4:multi", cands=cands, reps=2)
+    t_sep = timed(separate)
+    t_grp = timed(grouped)
+    with tuning.policy(pol):
+        t_tuned = timed(grouped)
+    out["mixed_n_batch"] = {"systems": "n in {512, 1024, 2048}, 64 blocks each, 128 RHS each",
+                            "separate_ms": round(t_sep, 3), "grouped_ms": round(t_grp, 3),
+                            "grouped_autotuned_ms": round(t_tuned, 3),
+                            "autotuned_policy": list(dataclasses.astuple(pol))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +278,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box here has a single GPU and the "
@@ -356,6 +448,8 @@ def main():
     }
     if not args.no_serving:
         out["serving"] = serving_bench(dev)
+    if not args.no_c4:
+        out["c4_blocktri"] = c4_bench(dev)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(op, F_host, args.cpu_budget)
         out["cpu_baseline"] = {
