@@ -289,18 +289,132 @@ struct FastGeom {
                              A_VECS >= 1 && B_VECS >= 1;
 };
 
+// The K loop of one term [ks, ke) for one workgroup, accumulating into acc.  VA / VB:
+// 16-byte loads of A / B (chosen once per term; see k_gemm_fast).  Every load is
+// unconditional -- out-of-range rows / columns / k read a clamped in-range address and
+// the value is replaced by zero (k) or never stored (rows, columns) -- so the loads of
+// a slab carry no branches, which would make the compiler wait for each load in turn.
+template <class Cfg, typename T, bool VA, bool VB>
+__device__ __forceinline__ void fast_term(const T* __restrict__ A, int lda,
+                                          const T* __restrict__ B, int ldb, T alpha, int ks,
+                                          int ke, int M, int N, int row0, int col0, int tid,
+                                          int wrow, int wcol, int g, int li, T* smem, int& stage,
+                                          typename Mfma<T>::v4 (&acc)[Cfg::FM][Cfg::FN]) {
+  using G = FastGeom<Cfg, T>;
+  using M4 = Mfma<T>;
+  using vec = typename Vec16<T>::type;
+  constexpr int VEC = G::VEC, KS = G::KS, LD = G::LD, BK = Cfg::BK;
+  constexpr int KV = BK / VEC;       // vectors per A row slab
+  constexpr int NV = Cfg::BN / VEC;  // vectors per B k row
+  const int nslab = (ke - ks + BK - 1) / BK;
+  vec ra[G::A_VECS], rb[G::B_VECS];
+
+  // Raw loaded values stay in ra / rb while the current slab is multiplied; the
+  // zero-fill of out-of-range k is applied when they are written to LDS, so nothing
+  // consumes a load before the compute.
+  bool oka[G::A_VECS], okb[G::B_VECS];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < G::A_VECS; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int kv = e % KV, i = e / KV;
+      const T* src = A + (long long)min(row0 + i, M - 1) * lda;
+      const int k = k0 + kv * VEC;
+      if (VA) {  // [ks, ke) holds whole vectors: a vector is all in or all out
+        ra[r] = *reinterpret_cast<const vec*>(src + min(k, ke - VEC));
+        oka[r] = k < ke;
+      } else {
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) ra[r][q] = src[min(k + q, ke - 1)];
+        oka[r] = true;  // per-element k checks at the store
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < G::B_VECS; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int jv = e % NV, kk = e / NV;
+      const int k = k0 + kk;
+      const T* src = B + (long long)min(k, ke - 1) * ldb;
+      const int c = col0 + jv * VEC;
+      if (VB) {  // N is a multiple of VEC: a vector is all in or all out of [0, N)
+        rb[r] = *reinterpret_cast<const vec*>(src + min(c, N - VEC));
+      } else {
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) rb[r][q] = src[min(c + q, N - 1)];
+      }
+      okb[r] = k < ke;
+    }
+  };
+  auto store = [&](int st, int k0) {
+    T* As = smem + st * G::STAGE;
+    T* Bs = As + Cfg::BM * LD;
+#pragma unroll
+    for (int r = 0; r < G::A_VECS; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int kv = e % KV, i = e / KV;
+      vec v = alpha * ra[r];
+      if (VA) {
+        v = oka[r] ? v : vec(T(0));
+      } else {
+        const int k = k0 + kv * VEC;
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) v[q] = (k + q < ke) ? v[q] : T(0);
+      }
+      *reinterpret_cast<vec*>(As + i * LD + kv * VEC) = v;
+    }
+#pragma unroll
+    for (int r = 0; r < G::B_VECS; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int jv = e % NV, kk = e / NV;
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) Bs[(jv * VEC + q) * LD + kk] = okb[r] ? rb[r][q] : T(0);
+    }
+  };
+
+  load(ks);
+  store(stage, ks);
+  __syncthreads();
+  for (int t = 0; t < nslab; ++t) {
+    const bool more = t + 1 < nslab;
+    if (more) load(ks + (t + 1) * BK);
+    {
+      const T* As = smem + stage * G::STAGE;
+      const T* Bs = As + Cfg::BM * LD;
+      // one 16-byte group of k steps at a time keeps the operand registers small
+#pragma unroll
+      for (int q = 0; q < KS / VEC; ++q) {
+        vec af[Cfg::FM], bf[Cfg::FN];
+#pragma unroll
+        for (int i = 0; i < Cfg::FM; ++i)
+          af[i] = *reinterpret_cast<const vec*>(As + (wrow + i * 16 + li) * LD + g * KS + q * VEC);
+#pragma unroll
+        for (int j = 0; j < Cfg::FN; ++j)
+          bf[j] = *reinterpret_cast<const vec*>(Bs + (wcol + j * 16 + li) * LD + g * KS + q * VEC);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+#pragma unroll
+          for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+            for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = M4::op(af[i][e], bf[j][e], acc[i][j]);
+      }
+    }
+    if (more) store(stage ^ 1, ks + (t + 1) * BK);
+    __syncthreads();
+    stage ^= 1;
+  }
+}
+
 // Two waves per SIMD at least (256 registers per lane): the 128x128 tile otherwise
 // takes 268 and runs one wave per SIMD.
-template <class Cfg, typename T>
+// kAligned: the host has checked that every term takes 16-byte loads for A and B (then
+// only that path is compiled, which keeps the register count of the common case low).
+template <class Cfg, typename T, bool kAligned>
 __global__ void __launch_bounds__(Cfg::NT) __attribute__((amdgpu_waves_per_eu(2)))
 k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
   using G = FastGeom<Cfg, T>;
   using M4 = Mfma<T>;
   using v4 = typename M4::v4;
-  using vec = typename Vec16<T>::type;
-  constexpr int VEC = G::VEC, KS = G::KS, LD = G::LD, BK = Cfg::BK;
-  constexpr int KV = BK / VEC;       // vectors per A row slab
-  constexpr int NV = Cfg::BN / VEC;  // vectors per B k row
+  constexpr int VEC = G::VEC;
   __shared__ __attribute__((aligned(16))) T smem[2 * G::STAGE];
 
   const int tile = xcd_remap(blockIdx.x, total_tiles);
@@ -344,103 +458,27 @@ k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
     const int ke = min(kfin - koff, Tm.K);
     koff += Tm.K;
     if (ke <= ks) continue;
-    const T alpha = Tm.alpha;
-    const T* __restrict__ A = Tm.A + (long long)b * Tm.strideA;
-    const T* __restrict__ B = Tm.B + (long long)b * Tm.strideB;
-    const int lda = Tm.lda, ldb = Tm.ldb;
-    const int nslab = (ke - ks + BK - 1) / BK;
-    // 16-byte loads need aligned rows and, for A, K-range ends on vector boundaries
-    // (then a vector is wholly inside or wholly outside [ks, ke)); otherwise element loads.
-    const bool va = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % VEC == 0) &&
+    const T* A = Tm.A + (long long)b * Tm.strideA;
+    const T* B = Tm.B + (long long)b * Tm.strideB;
+    // 16-byte loads need aligned rows; for A also K-range ends on vector boundaries,
+    // for B a column count that is a multiple of the vector (uniform per term).
+    const bool va = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (Tm.lda % VEC == 0) &&
                     (ks % VEC == 0) && (ke % VEC == 0);
-    const bool vb = ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (ldb % VEC == 0);
-    vec ra[G::A_VECS], rb[G::B_VECS];
-
-    auto load = [&](int k0) {
-#pragma unroll
-      for (int r = 0; r < G::A_VECS; ++r) {
-        const int e = tid + r * Cfg::NT;
-        const int kv = e % KV, i = e / KV;
-        const int row = min(row0 + i, M - 1);
-        const int k = k0 + kv * VEC;
-        const T* src = A + (long long)row * lda + k;
-        if (va) {
-          ra[r] = (k < ke) ? *reinterpret_cast<const vec*>(src) : vec(T(0));
-        } else {
-#pragma unroll
-          for (int q = 0; q < VEC; ++q) ra[r][q] = (k + q < ke) ? src[q] : T(0);
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < G::B_VECS; ++r) {
-        const int e = tid + r * Cfg::NT;
-        const int jv = e % NV, kk = e / NV;
-        const int k = k0 + kk;
-        const int c = col0 + jv * VEC;
-        if (k < ke) {
-          const T* src = B + (long long)k * ldb;
-          if (vb && c + VEC <= N) {
-            rb[r] = *reinterpret_cast<const vec*>(src + c);
-          } else {
-#pragma unroll
-            for (int q = 0; q < VEC; ++q) rb[r][q] = src[min(c + q, N - 1)];
-          }
-        } else {
-          rb[r] = vec(T(0));
-        }
-      }
-    };
-    auto store = [&](int st) {
-      T* As = smem + st * G::STAGE;
-      T* Bs = As + Cfg::BM * LD;
-#pragma unroll
-      for (int r = 0; r < G::A_VECS; ++r) {
-        const int e = tid + r * Cfg::NT;
-        const int kv = e % KV, i = e / KV;
-        *reinterpret_cast<vec*>(As + i * LD + kv * VEC) = alpha * ra[r];
-      }
-#pragma unroll
-      for (int r = 0; r < G::B_VECS; ++r) {
-        const int e = tid + r * Cfg::NT;
-        const int jv = e % NV, kk = e / NV;
-#pragma unroll
-        for (int q = 0; q < VEC; ++q) Bs[(jv * VEC + q) * LD + kk] = rb[r][q];
-      }
-    };
-
-    load(ks);
-    store(stage);
-    __syncthreads();
-    for (int t = 0; t < nslab; ++t) {
-      const bool more = t + 1 < nslab;
-      if (more) load(ks + (t + 1) * BK);
-      {
-        const T* As = smem + stage * G::STAGE;
-        const T* Bs = As + Cfg::BM * LD;
-        // one 16-byte group of k steps at a time keeps the operand registers small
-#pragma unroll
-        for (int q = 0; q < KS / VEC; ++q) {
-          vec af[Cfg::FM], bf[Cfg::FN];
-#pragma unroll
-          for (int i = 0; i < Cfg::FM; ++i)
-            af[i] = *reinterpret_cast<const vec*>(As + (wrow + i * 16 + li) * LD + g * KS +
-                                                  q * VEC);
-#pragma unroll
-          for (int j = 0; j < Cfg::FN; ++j)
-            bf[j] = *reinterpret_cast<const vec*>(Bs + (wcol + j * 16 + li) * LD + g * KS +
-                                                  q * VEC);
-#pragma unroll
-          for (int e = 0; e < VEC; ++e)
-#pragma unroll
-            for (int i = 0; i < Cfg::FM; ++i)
-#pragma unroll
-              for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = M4::op(af[i][e], bf[j][e], acc[i][j]);
-        }
-      }
-      if (more) store(stage ^ 1);
-      __syncthreads();
-      stage ^= 1;
-    }
+    const bool vb = ((reinterpret_cast<uintptr_t>(B) & 15) == 0) && (Tm.ldb % VEC == 0) &&
+                    (N % VEC == 0);
+    if (kAligned || (va && vb))
+      fast_term<Cfg, T, true, true>(A, Tm.lda, B, Tm.ldb, Tm.alpha, ks, ke, M, N, row0, col0, tid,
+                                    wrow, wcol, g, li, smem, stage, acc);
+    else if (kAligned) {
+    } else if (va)
+      fast_term<Cfg, T, true, false>(A, Tm.lda, B, Tm.ldb, Tm.alpha, ks, ke, M, N, row0, col0, tid,
+                                     wrow, wcol, g, li, smem, stage, acc);
+    else if (vb)
+      fast_term<Cfg, T, false, true>(A, Tm.lda, B, Tm.ldb, Tm.alpha, ks, ke, M, N, row0, col0, tid,
+                                     wrow, wcol, g, li, smem, stage, acc);
+    else
+      fast_term<Cfg, T, false, false>(A, Tm.lda, B, Tm.ldb, Tm.alpha, ks, ke, M, N, row0, col0,
+                                      tid, wrow, wcol, g, li, smem, stage, acc);
   }
 
   T* C = D.C + (long long)b * D.strideC;
@@ -471,21 +509,31 @@ k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
   }
 }
 
-// True when every term of every descriptor uses its operands as stored (the
-// vector-load kernel; unaligned operands fall back to element loads inside it).
+// 0: some term uses a transposed operand (the LDS-transposing kernel k_gemm);
+// 1: the vector-load kernel, per-term choice of 16-byte or element loads;
+// 2: the vector-load kernel with 16-byte loads everywhere (aligned operands, leading
+//    dimensions, every K and N multiples of the vector width).
 template <typename T>
-static bool fast_ok(const GemmLaunchT<T>& P) {
+static int fast_kind(const GemmLaunchT<T>& P) {
+  constexpr int VEC = 16 / (int)sizeof(T);
   static const bool enabled = [] {
     const char* s = getenv("SSLAB_GEMM_FAST");
     return !(s && s[0] == '0');
   }();
-  if (!enabled) return false;
+  if (!enabled) return 0;
+  bool aligned = true;
   for (int i = 0; i < P.nd; ++i) {
     const GemmDescT<T>& d = P.d[i];
-    for (int t = 0; t < d.nterms; ++t)
-      if (d.term[t].transA || d.term[t].transB) return false;
+    if (d.N % VEC) aligned = false;
+    for (int t = 0; t < d.nterms; ++t) {
+      const GemmTermT<T>& m = d.term[t];
+      if (m.transA || m.transB) return 0;
+      if ((reinterpret_cast<uintptr_t>(m.A) | reinterpret_cast<uintptr_t>(m.B)) & 15) aligned = false;
+      if (m.lda % VEC || m.ldb % VEC || m.K % VEC || m.strideA % VEC || m.strideB % VEC)
+        aligned = false;
+    }
   }
-  return true;
+  return aligned ? 2 : 1;
 }
 
 // Tile configurations, by id (the ids of GemmPolicy and sslab_gemm_set_policy):
@@ -583,8 +631,16 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit) {
   }
   if (total == 0) return kOk;
   if constexpr (FastGeom<Cfg, T>::ok) {
-    if (fast_ok(P)) {
-      hipLaunchKernelGGL((k_gemm_fast<Cfg, T>), dim3(total), dim3(Cfg::NT), 0, stream, P, total);
+    const int kind = fast_kind(P);
+    if (kind == 2) {
+      hipLaunchKernelGGL((k_gemm_fast<Cfg, T, true>), dim3(total), dim3(Cfg::NT), 0, stream, P,
+                         total);
+      SSLAB_HIP_CHECK(hipGetLastError());
+      return kOk;
+    }
+    if (kind == 1) {
+      hipLaunchKernelGGL((k_gemm_fast<Cfg, T, false>), dim3(total), dim3(Cfg::NT), 0, stream, P,
+                         total);
       SSLAB_HIP_CHECK(hipGetLastError());
       return kOk;
     }
