@@ -269,6 +269,98 @@ def c4_bench(dev, seed=0, reps=5):
     return out
 
 
+def c3_bench(dev, seed=0, rk=40, nrhs=16):
+    """3-D Poisson on [0,1]^3: 16 slabs, every double slab one 9 x 98 x 98 Chebyshev grid,
+    so each interface carries m = 96^2 = 9216 nodes; S blocks HBS-compressed on a 128-leaf
+    face tree (72 points per leaf, rank <= 40, s = max(2k, leaf) + k + 10 seeded Gaussian
+    sketch columns), ULV-factored HBS Thomas elimination, 16 right-hand sides of Dirichlet
+    data sum_j a_j sin(pi k_j . x), k_j in {1,2,3}^3 (8 modes, seeded); checked against the
+    dense FP64 red-black solve of the same S-system."""
+    import torch
+    from sslablu_amd.blocktri import BlockTriFactor
+    from sslablu_amd.direct_solve.omsdirectsolveHBS import ThomasSolverHBS
+    from sslablu_amd.matAssembly.HBS import HBSnew
+    from sslablu_amd.spectral3d import FaceTree, SpectralSlabs3D
+    N, nx, ny = 16, 9, 98
+    rng = np.random.default_rng(seed)
+    modes = rng.integers(1, 4, size=(nrhs, 8, 3))
+    amps = rng.standard_normal((nrhs, 8))
+
+    def g(P):
+        return np.stack([(amps[c][None, :] * np.sin(np.pi * P @ modes[c].T)).sum(1)
+                         for c in range(nrhs)], axis=1)
+
+    out = {"workload": f"3-D Poisson, {N} slabs, Chebyshev {nx}x{ny}x{ny} per double slab, "
+                       f"m = {(ny - 2) ** 2}, HBS rank {rk}, leaf 72, {nrhs} RHS"}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s = SpectralSlabs3D(N, nx, ny, ny, device=dev).build()
+    B = s.rhs_from_function(g)
+    torch.cuda.synchronize()
+    out["assembly_s"] = round(time.perf_counter() - t0, 3)
+    tree = FaceTree(ny - 2, ny - 2, 72)
+    ns = max(2 * rk, 72) + rk + 10
+    t0 = time.perf_counter()
+    H = {}
+    for side in ("left", "right"):
+        h = HBSnew.HBSMAT(s.S[side], tree, quad=False, seed=seed)
+        h._samples = lambda rk_, s_=ns: s_
+        h.construct(rk)
+        H[side] = h
+    torch.cuda.synchronize()
+    out["compress_s"] = round(time.perf_counter() - t0, 3)
+    X = torch.randn(s.m, 16, dtype=torch.float64, device=dev)
+    out["compression_rel_err"] = max(
+        ((H[sd].matmat(X) - s.S[sd] @ X).norm() / (s.S[sd] @ X).norm()).item()
+        for sd in ("left", "right"))
+    out["hbs_bytes_per_block"] = int(H["left"].nbytes)
+    out["dense_bytes_per_block"] = int(s.m * s.m * 8)
+    # how the weak-admissibility compression error falls with rank (larger leaves allow it)
+    sweep = {}
+    for rk2, leaf2 in ((72, 144), (120, 288)):
+        tr2 = FaceTree(ny - 2, ny - 2, leaf2)
+        h2 = HBSnew.HBSMAT(s.S["left"], tr2, quad=False, seed=seed)
+        h2._samples = lambda rk_, s_=max(2 * rk2, leaf2) + rk2 + 10: s_
+        h2.construct(rk2)
+        sweep[f"rank {rk2}, leaf {leaf2}"] = {
+            "rel_err": ((h2.matmat(X) - s.S["left"] @ X).norm() / (s.S["left"] @ X).norm()).item(),
+            "bytes": int(h2.nbytes)}
+        del h2
+    out["compression_by_rank"] = sweep
+    nb = s.nb
+    S_rk = [[H["right"]]] + [[H["left"], H["right"]] for _ in range(nb - 2)] + [[H["left"]]]
+    t0 = time.perf_counter()
+    solver = ThomasSolverHBS(s.m, rk)
+    solver.factorize(S_rk)
+    torch.cuda.synchronize()
+    out["hbs_factor_s"] = round(time.perf_counter() - t0, 3)
+    solver.solve(B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x_hbs = solver.solve(B)
+    torch.cuda.synchronize()
+    t_hbs = time.perf_counter() - t0
+    out["hbs_solve_ms"] = round(t_hbs * 1e3, 2)
+    out["hbs_rhs_solves_per_s"] = round(nrhs / t_hbs, 1)
+    # dense FP64 reference of the same S-system (red-black, identity diagonal)
+    SiM, SiP = s.blocks()
+    t0 = time.perf_counter()
+    fac = BlockTriFactor("rb", dev).factorize(None, SiM, SiP)
+    torch.cuda.synchronize()
+    out["dense_factor_s"] = round(time.perf_counter() - t0, 3)
+    V, Xd = B.clone(), torch.empty_like(B)
+    fac.solve_(V, Xd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    V.copy_(B)
+    fac.solve_(V, Xd)
+    torch.cuda.synchronize()
+    out["dense_solve_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    out["dense_factor_gb"] = round(fac.nbytes / 1e9, 2)
+    out["hbs_vs_dense_rel_err"] = ((x_hbs - Xd).norm() / Xd.norm()).item()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +371,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box here has a single GPU and the "
@@ -450,6 +543,8 @@ def main():
         out["serving"] = serving_bench(dev)
     if not args.no_c4:
         out["c4_blocktri"] = c4_bench(dev)
+    if not args.no_c3:
+        out["c3_spectral3d_hbs"] = c3_bench(dev)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(op, F_host, args.cpu_budget)
         out["cpu_baseline"] = {
