@@ -116,6 +116,12 @@ int sslab_gemm_get_policy(int* p, int n);
  * No reference counterpart: the reference is FP64 throughout.  The FP32 factor is
  * the FP64 red-black factor rounded to float (same layout, factor_doubles floats). */
 int sslab_cast_f64_to_f32(long long n, const double* x, float* y, void* stream);
+/* Red-black factorization computed in FP32 (FP32 LU, FP32 MFMA Schur updates) from FP32
+ * copies of the blocks: the low-precision factor used as the refinement preconditioner
+ * by sslab_rb_solve_mixed.  Same layout and sizes as sslab_rb_factor (sizes counted in
+ * floats); T may be NULL for identity diagonal blocks. */
+int sslab_rb_factor_f32(int nb, int m, const float* T, const float* SiM, const float* SiP,
+                        float* factor32, float* work32, int* iwork, int* info, void* stream);
 /* Red-black solve with an FP32 factor; V (float, overwritten), X (float). */
 int sslab_rb_solve_f32(int nb, int m, int nrhs, const float* factor32, float* V, int ldv,
                        float* X, int ldx, void* stream);

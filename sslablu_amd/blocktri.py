@@ -129,6 +129,9 @@ class BlockTriFactor:
         Both float64, row-major with unit column stride."""
         nb, m = self.nb, self.m
         N = nb * m
+        if self.factor is None:
+            raise ValueError("solve_: no FP64 factor (factorize_mixed keeps only the FP32 one; "
+                             "use solve_mixed_)")
         if V.shape[0] != N or X.shape[0] != N or V.dim() != 2 or X.shape[1] != V.shape[1]:
             raise ValueError("solve_: V and X must be (nb*m, r)")
         if V.dtype != F64 or X.dtype != F64 or (V.shape[1] > 0 and (V.stride(1) != 1 or
@@ -176,6 +179,48 @@ class BlockTriFactor:
         self._T64 = None if T is None else _dev_blocks(T, nb, m, self.device)
         self._M64 = _dev_blocks(SiM, nb, m, self.device)
         self._P64 = _dev_blocks(SiP, nb, m, self.device)
+        return self
+
+    def factorize_mixed(self, T, SiM, SiP, stream=None):
+        """Factor the system in FP32 (FP32 LU and FP32 MFMA Schur updates of the red-black
+        elimination) as the low-precision preconditioner of solve_mixed_; the FP64 blocks
+        are kept for the refinement residual.  T = None: identity diagonal blocks.  Only
+        solve_mixed_ is available afterwards (there is no FP64 factor)."""
+        if self.method != "rb":
+            raise ValueError("mixed precision uses the red-black factorization")
+        ref = T if T is not None else SiM
+        nb = len(ref)
+        m = ref[0].shape[0]
+        dev = self.device
+        with torch.cuda.device(dev):
+            Td = _dev_blocks(T, nb, m, dev) if T is not None else None
+            Md = _dev_blocks(SiM, nb, m, dev)
+            Pd = _dev_blocks(SiP, nb, m, dev)
+            s = stream_handle(stream)
+
+            def f32(x):
+                y = torch.empty(x.shape, dtype=torch.float32, device=dev)
+                call("sslab_cast_f64_to_f32", x.numel(), ptr(x), ptr(y), s)
+                return y
+
+            T32 = f32(Td) if Td is not None else None
+            M32, P32 = f32(Md), f32(Pd)
+            nfac, nwork, niwork = _lib.sizes("rb", nb, m)
+            self.factor32 = torch.empty(nfac, dtype=torch.float32, device=dev)
+            work = torch.empty(max(nwork, 1), dtype=torch.float32, device=dev)
+            iwork = torch.empty(max(niwork, 1), dtype=torch.int32, device=dev)
+            info = torch.zeros(_lib.load().sslab_rb_info_count(nb), dtype=torch.int32, device=dev)
+            call("sslab_rb_factor_f32", nb, m, ptr(T32) if T32 is not None else None, ptr(M32),
+                 ptr(P32), ptr(self.factor32), ptr(work), ptr(iwork), ptr(info), s)
+            self.nb, self.m = nb, m
+            self.factor = None
+            self.info = info
+            self.cyclic = False
+            self._T64, self._M64, self._P64 = Td, Md, Pd
+            bad = int((info != 0).sum().item())
+            if bad:
+                warnings.warn(f"{bad} exactly singular pivot block(s) in the FP32 block-tridiagonal "
+                              "factorization", RuntimeWarning)
         return self
 
     def solve_mixed_(self, B: torch.Tensor, X: torch.Tensor, iters: int = 2, stream=None,
@@ -240,6 +285,8 @@ class BlockTriFactor:
     @property
     def nbytes(self) -> int:
         n = self.factor.numel() * 8 if self.factor is not None else 0
+        if getattr(self, "factor32", None) is not None:
+            n += self.factor32.numel() * 4
         if self.Z is not None:
             n += (self.Z.numel() + self.K.numel()) * 8
         return n

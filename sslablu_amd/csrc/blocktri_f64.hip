@@ -104,8 +104,11 @@ int rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
   return kOk;
 }
 
-int rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
-              double* factor, double* work, int* iwork, int* info, hipStream_t st) {
+template <typename E>
+static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E* factor, E* work,
+                       int* iwork, int* info, hipStream_t st) {
+  using GemmLaunch = GemmLaunchT<E>;
+  using GemmDesc = GemmDescT<E>;
   SSLAB_REQUIRE(nb >= 1 && m >= 1, "rb_factor: nb and m must be positive");
   SSLAB_REQUIRE(factor && work && iwork, "rb_factor: null pointer");
   // T == NULL: identity diagonal blocks (the oms S-system u_i + S u = rhs).  Level 0 then
@@ -114,26 +117,26 @@ int rb_factor(int nb, int m, const double* T, const double* SiM, const double* S
   const RbPlan p = rb_plan(nb, m);
   const long long m2 = (long long)m * m;
   const long long setsz = 3 * p.set_blocks * m2;
-  double* sets[2] = {work, work + setsz};
-  double* LUw = work + 2 * setsz;
-  double* Rs = LUw + (long long)p.max_odd * m2;
+  E* sets[2] = {work, work + setsz};
+  E* LUw = work + 2 * setsz;
+  E* Rs = LUw + (long long)p.max_odd * m2;
   int* ipiv = iwork;
   int* perm = iwork + (long long)p.max_odd * m;
   int* linfo = perm + (long long)p.max_odd * m;
   // info layout: one int per odd node per level, then one for the top block.
 
-  const double* cT = T;
-  const double* cM = SiM;
-  const double* cP = SiP;
+  const E* cT = T;
+  const E* cM = SiM;
+  const E* cP = SiP;
   int info_base = 0;
   for (int l = 0; l < p.L; ++l) {
     const RbLevel& v = p.lv[l];
     const int n = v.n, nE = v.nE, nO = v.nO;
-    double* TG = factor + v.tg_off;
-    double* WW = factor + v.ww_off;
-    double* nT = sets[(l + 1) & 1];
-    double* nM = nT + p.set_blocks * m2;
-    double* nP = nM + p.set_blocks * m2;
+    E* TG = factor + v.tg_off;
+    E* WW = factor + v.ww_off;
+    E* nT = sets[(l + 1) & 1];
+    E* nM = nT + p.set_blocks * m2;
+    E* nP = nM + p.set_blocks * m2;
 
     const bool ident = (l == 0 && T == nullptr);
     const int nOr = (n % 2 == 0) ? nO - 1 : nO;  // odd nodes with a right neighbour
@@ -199,7 +202,7 @@ int rb_factor(int nb, int m, const double* T, const double* SiM, const double* S
       int k = 0;
       const int both = (nE < nO ? nE : nO) - 1;  // b in [1, min(nE,nO))
       // Cin: T_{2b} (strided by two blocks), or the identity just written into nT.
-      const double* cin = ident ? nT : cT;
+      const E* cin = ident ? nT : cT;
       const long long scin = ident ? m2 : 2 * m2;
       if (both > 0) {
         GemmDesc d = gemm_desc(m, m, both, nT + m2, m, m2, cin + scin, m, scin, 1.0);
@@ -252,6 +255,16 @@ int rb_factor(int nb, int m, const double* T, const double* SiM, const double* S
   SSLAB_TRY(getrs_batched(m, m, LUw, m, m2, ipiv, m, Rs, m, m2, factor + p.top_off, m, m2, perm, 1,
                           st));
   return kOk;
+}
+
+int rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
+              double* factor, double* work, int* iwork, int* info, hipStream_t st) {
+  return rb_factor_t<double>(nb, m, T, SiM, SiP, factor, work, iwork, info, st);
+}
+
+int rb_factor_f32(int nb, int m, const float* T, const float* SiM, const float* SiP,
+                  float* factor, float* work, int* iwork, int* info, hipStream_t st) {
+  return rb_factor_t<float>(nb, m, T, SiM, SiP, factor, work, iwork, info, st);
 }
 
 int rb_info_count(int nb) {

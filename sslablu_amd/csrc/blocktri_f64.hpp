@@ -10,6 +10,8 @@ int rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
 int rb_info_count(int nb);
 int rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
               double* factor, double* work, int* iwork, int* info, hipStream_t st);
+int rb_factor_f32(int nb, int m, const float* T, const float* SiM, const float* SiP,
+                  float* factor, float* work, int* iwork, int* info, hipStream_t st);
 int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, double* X, int ldx,
              hipStream_t st);
 int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,

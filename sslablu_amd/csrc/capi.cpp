@@ -125,6 +125,11 @@ int sslab_rb_solve_multi(int nsys, const int* nb, const int* m, const int* nrhs,
   return sslab::rb_solve_multi(nsys, nb, m, nrhs, factor, V, ldv, X, ldx, as_stream(stream));
 }
 
+int sslab_rb_factor_f32(int nb, int m, const float* T, const float* SiM, const float* SiP,
+                        float* factor32, float* work32, int* iwork, int* info, void* stream) {
+  return sslab::rb_factor_f32(nb, m, T, SiM, SiP, factor32, work32, iwork, info, as_stream(stream));
+}
+
 int sslab_cast_f64_to_f32(long long n, const double* x, float* y, void* stream) {
   return sslab::cast_f64_to_f32(n, x, y, as_stream(stream));
 }

@@ -22,30 +22,34 @@ namespace {
 constexpr int kPanelThreads = 256;
 constexpr int kTrsmNB = 32;  // diagonal block of the getrs substitutions
 
+template <typename E>
+__device__ __forceinline__ E abs_e(E v) { return v < E(0) ? -v : v; }
+
 __global__ void k_zero_int(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
 
 // Factor panel columns [j0, j0+w) over rows [j0, n) of every batch matrix.
-// kLds: the panel is staged column-major in dynamic LDS (rows x w doubles);
+// kLds: the panel is staged column-major in dynamic LDS (rows x w elements);
 // otherwise it is factored in place in global memory.
-template <bool kLds>
+template <typename E, bool kLds>
 __global__ void __launch_bounds__(kPanelThreads)
-k_getrf_panel(int n, int j0, int w, double* __restrict__ A, int lda, long long sA,
+k_getrf_panel(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
               int* __restrict__ ipiv, long long sP, int* __restrict__ info) {
-  extern __shared__ double smem[];
-  __shared__ double red_v[kPanelThreads / 64];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  E* smem = reinterpret_cast<E*>(smem_raw);
+  __shared__ E red_v[kPanelThreads / 64];
   __shared__ int red_i[kPanelThreads / 64];
   __shared__ int s_piv;
 
   const int b = blockIdx.x;
-  double* Ab = A + (long long)b * sA;
+  E* Ab = A + (long long)b * sA;
   int* pv = ipiv + (long long)b * sP;
   const int rows = n - j0;
   const int tid = threadIdx.x;
 
-  double* P;
+  E* P;
   long long rs, cs;
   if (kLds) {
     P = smem;
@@ -64,10 +68,10 @@ k_getrf_panel(int n, int j0, int w, double* __restrict__ A, int lda, long long s
 
   for (int c = 0; c < w; ++c) {
     // Pivot search: largest |P(i,c)| for i >= c, first index on ties (idamax).
-    double best = -1.0;
+    E best = -1.0;
     int bi = rows;
     for (int i = c + tid; i < rows; i += kPanelThreads) {
-      const double v = fabs(P[i * rs + c * cs]);
+      const E v = abs_e(P[i * rs + c * cs]);
       if (v > best) {
         best = v;
         bi = i;
@@ -75,7 +79,7 @@ k_getrf_panel(int n, int j0, int w, double* __restrict__ A, int lda, long long s
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(best, off);
+      const E ov = __shfl_xor(best, off);
       const int oi = __shfl_xor(bi, off);
       if (ov > best || (ov == best && oi < bi)) {
         best = ov;
@@ -88,7 +92,7 @@ k_getrf_panel(int n, int j0, int w, double* __restrict__ A, int lda, long long s
     }
     __syncthreads();
     if (tid == 0) {
-      double bv = red_v[0];
+      E bv = red_v[0];
       int bx = red_i[0];
       for (int q = 1; q < kPanelThreads / 64; ++q) {
         if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bx)) {
@@ -104,16 +108,16 @@ k_getrf_panel(int n, int j0, int w, double* __restrict__ A, int lda, long long s
     const int p = s_piv;
     if (p != c) {
       for (int j = tid; j < w; j += kPanelThreads) {
-        const double t = P[c * rs + j * cs];
+        const E t = P[c * rs + j * cs];
         P[c * rs + j * cs] = P[p * rs + j * cs];
         P[p * rs + j * cs] = t;
       }
     }
     __syncthreads();
-    const double piv = P[c * rs + c * cs];
+    const E piv = P[c * rs + c * cs];
     if (piv == 0.0 && tid == 0 && info != nullptr && info[b] == 0) info[b] = j0 + c + 1;
     for (int i = c + 1 + tid; i < rows; i += kPanelThreads) {
-      double l = P[i * rs + c * cs];
+      E l = P[i * rs + c * cs];
       if (piv != 0.0) {
         l = l / piv;
         P[i * rs + c * cs] = l;
@@ -133,7 +137,8 @@ k_getrf_panel(int n, int j0, int w, double* __restrict__ A, int lda, long long s
 
 // Apply the interchanges of rows r0..r0+w-1 (in order) to the columns
 // [a0, a1) U [b0, b1).
-__global__ void k_laswp(int r0, int w, int a0, int a1, int b0, int b1, double* __restrict__ A,
+template <typename E>
+__global__ void k_laswp(int r0, int w, int a0, int a1, int b0, int b1, E* __restrict__ A,
                         int lda, long long sA, const int* __restrict__ ipiv, long long sP) {
   const int na = a1 - a0;
   const int ncols = na + (b1 - b0);
@@ -141,13 +146,13 @@ __global__ void k_laswp(int r0, int w, int a0, int a1, int b0, int b1, double* _
   if (t >= ncols) return;
   const int col = t < na ? a0 + t : b0 + (t - na);
   const int j0 = r0;
-  double* Ab = A + (long long)blockIdx.y * sA;
+  E* Ab = A + (long long)blockIdx.y * sA;
   const int* pv = ipiv + (long long)blockIdx.y * sP;
   for (int c = 0; c < w; ++c) {
     const int r = j0 + c;
     const int p = pv[r];
     if (p != r) {
-      const double t0 = Ab[(long long)r * lda + col];
+      const E t0 = Ab[(long long)r * lda + col];
       Ab[(long long)r * lda + col] = Ab[(long long)p * lda + col];
       Ab[(long long)p * lda + col] = t0;
     }
@@ -160,13 +165,13 @@ __global__ void k_laswp(int r0, int w, int a0, int a1, int b0, int b1, double* _
 // coordinates (J L J is upper triangular for the reversal J), because the compiler
 // schedules the unrolled backward loop without hoisting the whole triangle into
 // registers, while the unrolled forward loop spills.
-template <int NB, bool kLower>
+template <typename E, int NB, bool kLower>
 __global__ void __launch_bounds__(256)
-k_trsm_cols(int nr, int ncols, const double* __restrict__ T, int ldt, long long sT,
-            double* __restrict__ B, int ldb, long long sB) {
-  __shared__ double Ts[NB * NB];  // the triangle in solve coordinates
-  const double* Tb = T + (long long)blockIdx.y * sT;
-  double* Bb = B + (long long)blockIdx.y * sB;
+k_trsm_cols(int nr, int ncols, const E* __restrict__ T, int ldt, long long sT,
+            E* __restrict__ B, int ldb, long long sB) {
+  __shared__ E Ts[NB * NB];  // the triangle in solve coordinates
+  const E* Tb = T + (long long)blockIdx.y * sT;
+  E* Bb = B + (long long)blockIdx.y * sB;
   for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
     const int i = e / NB, j = e % NB;
     const int ri = kLower ? NB - 1 - i : i, rj = kLower ? NB - 1 - j : j;
@@ -178,22 +183,22 @@ k_trsm_cols(int nr, int ncols, const double* __restrict__ T, int ldt, long long 
   const int col0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (col0 >= ncols) return;
   // x[i] is row ri(i) of the column; rows past nr re-read row nr-1 and are zeroed.
-  const double* bcol = Bb + col0;
-  double x[NB];
+  const E* bcol = Bb + col0;
+  E x[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     const int ri = kLower ? NB - 1 - i : i;
-    const double v = bcol[(long long)(ri < nr ? ri : nr - 1) * ldb];
+    const E v = bcol[(long long)(ri < nr ? ri : nr - 1) * ldb];
     x[i] = (ri < nr) ? v : 0.0;
   }
 #pragma unroll
   for (int i = NB - 1; i >= 0; --i) {
-    double s = x[i];
+    E s = x[i];
 #pragma unroll
     for (int j = i + 1; j < NB; ++j) s -= Ts[i * NB + j] * x[j];
     x[i] = s / Ts[i * NB + i];
   }
-  double* ocol = Bb + col0;
+  E* ocol = Bb + col0;
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     const int ri = kLower ? NB - 1 - i : i;
@@ -224,9 +229,10 @@ k_ipiv_to_perm(int n, const int* __restrict__ ipiv, long long sP, int* __restric
 }
 
 // X[i, :] = B[perm[i], :]
+template <typename E>
 __global__ void k_gather_rows(int n, int ncols, const int* __restrict__ perm,
-                              const double* __restrict__ B, int ldb, long long sB,
-                              double* __restrict__ X, int ldx, long long sX) {
+                              const E* __restrict__ B, int ldb, long long sB,
+                              E* __restrict__ X, int ldx, long long sX) {
   const int b = blockIdx.z;
   const int i = blockIdx.y;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -235,15 +241,17 @@ __global__ void k_gather_rows(int n, int ncols, const int* __restrict__ perm,
   X[(long long)b * sX + (long long)i * ldx + c] = B[(long long)b * sB + (long long)src * ldb + c];
 }
 
-__global__ void k_set_identity(int n, double* __restrict__ X, int ldx, long long sX) {
+template <typename E>
+__global__ void k_set_identity(int n, E* __restrict__ X, int ldx, long long sX) {
   const int i = blockIdx.y;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   X[(long long)blockIdx.z * sX + (long long)i * ldx + j] = (i == j) ? 1.0 : 0.0;
 }
 
-__global__ void k_copy_block(int n, const double* __restrict__ X, int ldx, long long sX,
-                             double* __restrict__ Y, int ldy, long long sY) {
+template <typename E>
+__global__ void k_copy_block(int n, const E* __restrict__ X, int ldx, long long sX,
+                             E* __restrict__ Y, int ldy, long long sY) {
   const int i = blockIdx.y;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -251,7 +259,8 @@ __global__ void k_copy_block(int n, const double* __restrict__ X, int ldx, long 
       X[(long long)blockIdx.z * sX + (long long)i * ldx + j];
 }
 
-__global__ void k_zero_block(int m, int n, double* __restrict__ Y, int ldy, long long sY) {
+template <typename E>
+__global__ void k_zero_block(int m, int n, E* __restrict__ Y, int ldy, long long sY) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   for (int i = blockIdx.y; i < m; i += gridDim.y)
@@ -278,28 +287,30 @@ namespace {
 
 constexpr int kOuterNB = 256;  // outer block of getrf and of the triangular solves
 
-int laswp(int r0, int w, int a0, int a1, int b0, int b1, double* A, int lda, long long sA,
+template <typename E>
+int laswp(int r0, int w, int a0, int a1, int b0, int b1, E* A, int lda, long long sA,
           const int* ipiv, long long sP, int batch, hipStream_t st) {
   const int ncols = (a1 - a0) + (b1 - b0);
   if (w <= 0 || ncols <= 0) return kOk;
-  hipLaunchKernelGGL(k_laswp, dim3(ceil_div(ncols, 256), batch), dim3(256), 0, st, r0, w, a0, a1,
+  hipLaunchKernelGGL(k_laswp<E>, dim3(ceil_div(ncols, 256), batch), dim3(256), 0, st, r0, w, a0, a1,
                      b0, b1, A, lda, sA, ipiv, sP);
   SSLAB_HIP_CHECK(hipGetLastError());
   return kOk;
 }
 
 // X (n x nrhs) <- T^{-1} X for the unit-lower (lower = true) or upper n x n triangle T.
-int trsm_2level(bool lower, int n, int nrhs, const double* T, int ldt, long long sT, double* X,
+template <typename E>
+int trsm_2level(bool lower, int n, int nrhs, const E* T, int ldt, long long sT, E* X,
                 int ldx, long long sX, int batch, hipStream_t st) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return kOk;
   constexpr int NB = kTrsmNB;
   const dim3 grid(ceil_div(nrhs, 256), batch);
   auto diag = [&](int ib, int ni) -> int {
     if (lower)
-      hipLaunchKernelGGL((k_trsm_cols<NB, true>), grid, dim3(256), 0, st, ni, nrhs,
+      hipLaunchKernelGGL((k_trsm_cols<E, NB, true>), grid, dim3(256), 0, st, ni, nrhs,
                          T + (long long)ib * ldt + ib, ldt, sT, X + (long long)ib * ldx, ldx, sX);
     else
-      hipLaunchKernelGGL((k_trsm_cols<NB, false>), grid, dim3(256), 0, st, ni, nrhs,
+      hipLaunchKernelGGL((k_trsm_cols<E, NB, false>), grid, dim3(256), 0, st, ni, nrhs,
                          T + (long long)ib * ldt + ib, ldt, sT, X + (long long)ib * ldx, ldx, sX);
     SSLAB_HIP_CHECK(hipGetLastError());
     return kOk;
@@ -307,7 +318,7 @@ int trsm_2level(bool lower, int n, int nrhs, const double* T, int ldt, long long
   // X[r0:r0+nr) -= T[r0:r0+nr, c0:c0+nc) X[c0:c0+nc)
   auto upd = [&](int r0, int nr, int c0, int nc) -> int {
     if (nr <= 0 || nc <= 0) return kOk;
-    return gemm_strided<double>(0, 0, nr, nrhs, nc, -1.0, T + (long long)r0 * ldt + c0, ldt, sT,
+    return gemm_strided<E>(0, 0, nr, nrhs, nc, -1.0, T + (long long)r0 * ldt + c0, ldt, sT,
                                 X + (long long)c0 * ldx, ldx, sX, 1.0, X + (long long)r0 * ldx,
                                 ldx, sX, batch, st);
   };
@@ -339,33 +350,36 @@ int trsm_2level(bool lower, int n, int nrhs, const double* T, int ldt, long long
   return kOk;
 }
 
+template <typename E>
 struct PanelCtx {
   int n, lda, batch, lds, base_lds, base_glb;
   long long sA, sP;
-  double* A;
+  E* A;
   int* ipiv;
   int* info;
   hipStream_t st;
 };
 
 // Widest base panel (8, 16 or 32 columns) whose rows fit in LDS; 0 = none does.
-int base_width(const PanelCtx& c, int rows) {
+template <typename E>
+int base_width(const PanelCtx<E>& c, int rows) {
   for (int w = 32; w >= 8; w >>= 1)
-    if ((long long)rows * w * 8 <= c.lds) return w;
+    if ((long long)rows * w * (long long)sizeof(E) <= c.lds) return w;
   return 0;
 }
 
 // Factor rows [j0, n) of columns [j0, j0 + w).
-int panel_rec(const PanelCtx& c, int j0, int w) {
+template <typename E>
+int panel_rec(const PanelCtx<E>& c, int j0, int w) {
   const int rows = c.n - j0;
-  const int bw = base_width(c, rows);
+  const int bw = base_width<E>(c, rows);
   if (w <= (bw > 0 ? bw : 32)) {
     if (bw > 0) {
-      const size_t shm = (size_t)rows * w * sizeof(double);
-      hipLaunchKernelGGL(k_getrf_panel<true>, dim3(c.batch), dim3(kPanelThreads), shm, c.st, c.n,
+      const size_t shm = (size_t)rows * w * sizeof(E);
+      hipLaunchKernelGGL((k_getrf_panel<E, true>), dim3(c.batch), dim3(kPanelThreads), shm, c.st, c.n,
                          j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
     } else {
-      hipLaunchKernelGGL(k_getrf_panel<false>, dim3(c.batch), dim3(kPanelThreads), 0, c.st, c.n,
+      hipLaunchKernelGGL((k_getrf_panel<E, false>), dim3(c.batch), dim3(kPanelThreads), 0, c.st, c.n,
                          j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
     }
     SSLAB_HIP_CHECK(hipGetLastError());
@@ -373,20 +387,21 @@ int panel_rec(const PanelCtx& c, int j0, int w) {
   }
   const int w1 = (w / 2 + 7) / 8 * 8;
   const int w2 = w - w1;
-  double* A11 = c.A + (long long)j0 * c.lda + j0;
-  SSLAB_TRY(panel_rec(c, j0, w1));
-  SSLAB_TRY(laswp(j0, w1, j0 + w1, j0 + w, 0, 0, c.A, c.lda, c.sA, c.ipiv, c.sP, c.batch, c.st));
-  SSLAB_TRY(trsm_2level(true, w1, w2, A11, c.lda, c.sA, A11 + w1, c.lda, c.sA, c.batch, c.st));
-  SSLAB_TRY(gemm_strided<double>(0, 0, rows - w1, w2, w1, -1.0, A11 + (long long)w1 * c.lda,
+  E* A11 = c.A + (long long)j0 * c.lda + j0;
+  SSLAB_TRY(panel_rec<E>(c, j0, w1));
+  SSLAB_TRY(laswp<E>(j0, w1, j0 + w1, j0 + w, 0, 0, c.A, c.lda, c.sA, c.ipiv, c.sP, c.batch, c.st));
+  SSLAB_TRY(trsm_2level<E>(true, w1, w2, A11, c.lda, c.sA, A11 + w1, c.lda, c.sA, c.batch, c.st));
+  SSLAB_TRY(gemm_strided<E>(0, 0, rows - w1, w2, w1, -1.0, A11 + (long long)w1 * c.lda,
                                  c.lda, c.sA, A11 + w1, c.lda, c.sA, 1.0,
                                  A11 + (long long)w1 * c.lda + w1, c.lda, c.sA, c.batch, c.st));
-  SSLAB_TRY(panel_rec(c, j0 + w1, w2));
-  return laswp(j0 + w1, w2, j0, j0 + w1, 0, 0, c.A, c.lda, c.sA, c.ipiv, c.sP, c.batch, c.st);
+  SSLAB_TRY(panel_rec<E>(c, j0 + w1, w2));
+  return laswp<E>(j0 + w1, w2, j0, j0 + w1, 0, 0, c.A, c.lda, c.sA, c.ipiv, c.sP, c.batch, c.st);
 }
 
 }  // namespace
 
-int getrf_batched(int n, double* A, int lda, long long sA, int* ipiv, long long sP, int* info,
+template <typename E>
+int getrf_batched(int n, E* A, int lda, long long sA, int* ipiv, long long sP, int* info,
                   int batch, hipStream_t stream) {
   SSLAB_REQUIRE(n >= 0 && batch >= 0, "getrf: negative size");
   SSLAB_REQUIRE(lda >= n, "getrf: lda < n");
@@ -397,7 +412,7 @@ int getrf_batched(int n, double* A, int lda, long long sA, int* ipiv, long long 
     hipLaunchKernelGGL(k_zero_int, dim3(ceil_div(batch, 256)), dim3(256), 0, stream, info, batch);
     SSLAB_HIP_CHECK(hipGetLastError());
   }
-  PanelCtx c{};
+  PanelCtx<E> c{};
   c.n = n;
   c.lda = lda;
   c.batch = batch;
@@ -408,17 +423,17 @@ int getrf_batched(int n, double* A, int lda, long long sA, int* ipiv, long long 
   c.info = info;
   c.st = stream;
   c.lds = max_lds_bytes() - 512;  // room for the static reduction scratch
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_getrf_panel<true>),
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_getrf_panel<E, true>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
   for (int j0 = 0; j0 < n; j0 += kOuterNB) {
     const int w = (n - j0) < kOuterNB ? (n - j0) : kOuterNB;
-    SSLAB_TRY(panel_rec(c, j0, w));
-    SSLAB_TRY(laswp(j0, w, 0, j0, j0 + w, n, A, lda, sA, ipiv, sP, batch, stream));
+    SSLAB_TRY(panel_rec<E>(c, j0, w));
+    SSLAB_TRY(laswp<E>(j0, w, 0, j0, j0 + w, n, A, lda, sA, ipiv, sP, batch, stream));
     const int rest = n - j0 - w;
     if (rest > 0) {
-      double* A11 = A + (long long)j0 * lda + j0;
-      SSLAB_TRY(trsm_2level(true, w, rest, A11, lda, sA, A11 + w, lda, sA, batch, stream));
-      SSLAB_TRY(gemm_strided<double>(0, 0, rest, rest, w, -1.0, A11 + (long long)w * lda, lda, sA,
+      E* A11 = A + (long long)j0 * lda + j0;
+      SSLAB_TRY(trsm_2level<E>(true, w, rest, A11, lda, sA, A11 + w, lda, sA, batch, stream));
+      SSLAB_TRY(gemm_strided<E>(0, 0, rest, rest, w, -1.0, A11 + (long long)w * lda, lda, sA,
                                      A11 + w, lda, sA, 1.0, A11 + (long long)w * lda + w, lda, sA,
                                      batch, stream));
     }
@@ -426,8 +441,9 @@ int getrf_batched(int n, double* A, int lda, long long sA, int* ipiv, long long 
   return kOk;
 }
 
-int getrs_batched(int n, int nrhs, const double* LU, int lda, long long sA, const int* ipiv,
-                  long long sP, const double* B, int ldb, long long sB, double* X, int ldx,
+template <typename E>
+int getrs_batched(int n, int nrhs, const E* LU, int lda, long long sA, const int* ipiv,
+                  long long sP, const E* B, int ldb, long long sB, E* X, int ldx,
                   long long sX, int* perm, int batch, hipStream_t stream) {
   SSLAB_REQUIRE(n >= 0 && nrhs >= 0 && batch >= 0, "getrs: negative size");
   SSLAB_REQUIRE(lda >= n && ldb >= nrhs && ldx >= nrhs, "getrs: bad leading dimension");
@@ -438,50 +454,69 @@ int getrs_batched(int n, int nrhs, const double* LU, int lda, long long sA, cons
   hipLaunchKernelGGL(k_ipiv_to_perm, dim3(batch), dim3(256), (size_t)n * sizeof(int), stream, n,
                      ipiv, sP, perm);
   SSLAB_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_gather_rows, dim3(ceil_div(nrhs, 256), n, batch), dim3(256), 0, stream, n,
+  hipLaunchKernelGGL(k_gather_rows<E>, dim3(ceil_div(nrhs, 256), n, batch), dim3(256), 0, stream, n,
                      nrhs, perm, B, ldb, sB, X, ldx, sX);
   SSLAB_HIP_CHECK(hipGetLastError());
-  SSLAB_TRY(trsm_2level(true, n, nrhs, LU, lda, sA, X, ldx, sX, batch, stream));   // L Y = P B
-  return trsm_2level(false, n, nrhs, LU, lda, sA, X, ldx, sX, batch, stream);      // U X = Y
+  SSLAB_TRY(trsm_2level<E>(true, n, nrhs, LU, lda, sA, X, ldx, sX, batch, stream));   // L Y = P B
+  return trsm_2level<E>(false, n, nrhs, LU, lda, sA, X, ldx, sX, batch, stream);      // U X = Y
 }
 
-int trsm_upper_batched(int n, int nrhs, const double* U, int ldu, long long sU, double* X, int ldx,
+template <typename E>
+int trsm_upper_batched(int n, int nrhs, const E* U, int ldu, long long sU, E* X, int ldx,
                        long long sX, int batch, hipStream_t stream) {
   SSLAB_REQUIRE(n >= 0 && nrhs >= 0 && batch >= 0, "trsm: negative size");
   SSLAB_REQUIRE(ldu >= n && ldx >= nrhs, "trsm: bad leading dimension");
   if (n == 0 || nrhs == 0 || batch == 0) return kOk;
   SSLAB_REQUIRE(batch <= 65535, "trsm: batch too large");
-  return trsm_2level(false, n, nrhs, U, ldu, sU, X, ldx, sX, batch, stream);
+  return trsm_2level<E>(false, n, nrhs, U, ldu, sU, X, ldx, sX, batch, stream);
 }
 
-int set_identity_batched(int n, double* X, int ldx, long long sX, int batch, hipStream_t stream) {
+template <typename E>
+int set_identity_batched(int n, E* X, int ldx, long long sX, int batch, hipStream_t stream) {
   if (n <= 0 || batch <= 0) return kOk;
   SSLAB_REQUIRE(batch <= 65535 && n <= 65535, "set_identity: size too large");
-  hipLaunchKernelGGL(k_set_identity, dim3(ceil_div(n, 256), n, batch), dim3(256), 0, stream, n, X,
+  hipLaunchKernelGGL(k_set_identity<E>, dim3(ceil_div(n, 256), n, batch), dim3(256), 0, stream, n, X,
                      ldx, sX);
   SSLAB_HIP_CHECK(hipGetLastError());
   return kOk;
 }
 
-int copy_block_batched(int m, int n, const double* X, int ldx, long long sX, double* Y, int ldy,
+template <typename E>
+int copy_block_batched(int m, int n, const E* X, int ldx, long long sX, E* Y, int ldy,
                        long long sY, int batch, hipStream_t stream) {
   if (m <= 0 || n <= 0 || batch <= 0) return kOk;
   SSLAB_REQUIRE(batch <= 65535 && m <= 65535, "copy_block: size too large");
-  hipLaunchKernelGGL(k_copy_block, dim3(ceil_div(n, 256), m, batch), dim3(256), 0, stream, n, X,
+  hipLaunchKernelGGL(k_copy_block<E>, dim3(ceil_div(n, 256), m, batch), dim3(256), 0, stream, n, X,
                      ldx, sX, Y, ldy, sY);
   SSLAB_HIP_CHECK(hipGetLastError());
   return kOk;
 }
 
-int zero_block_batched(int m, int n, double* Y, int ldy, long long sY, int batch,
+template <typename E>
+int zero_block_batched(int m, int n, E* Y, int ldy, long long sY, int batch,
                        hipStream_t stream) {
   if (m <= 0 || n <= 0 || batch <= 0) return kOk;
   SSLAB_REQUIRE(batch <= 65535, "zero_block: batch too large");
   const int gy = m < 4096 ? m : 4096;
-  hipLaunchKernelGGL(k_zero_block, dim3(ceil_div(n, 256), gy, batch), dim3(256), 0, stream, m, n,
+  hipLaunchKernelGGL(k_zero_block<E>, dim3(ceil_div(n, 256), gy, batch), dim3(256), 0, stream, m, n,
                      Y, ldy, sY);
   SSLAB_HIP_CHECK(hipGetLastError());
   return kOk;
 }
+
+#define SSLAB_LU_INSTANTIATE(E)                                                                \
+  template int getrf_batched<E>(int, E*, int, long long, int*, long long, int*, int, hipStream_t); \
+  template int getrs_batched<E>(int, int, const E*, int, long long, const int*, long long,       \
+                                const E*, int, long long, E*, int, long long, int*, int,       \
+                                hipStream_t);                                                  \
+  template int trsm_upper_batched<E>(int, int, const E*, int, long long, E*, int, long long,    \
+                                     int, hipStream_t);                                        \
+  template int set_identity_batched<E>(int, E*, int, long long, int, hipStream_t);             \
+  template int copy_block_batched<E>(int, int, const E*, int, long long, E*, int, long long,    \
+                                     int, hipStream_t);                                        \
+  template int zero_block_batched<E>(int, int, E*, int, long long, int, hipStream_t);
+SSLAB_LU_INSTANTIATE(double)
+SSLAB_LU_INSTANTIATE(float)
+#undef SSLAB_LU_INSTANTIATE
 
 }  // namespace sslab

@@ -1,4 +1,5 @@
-// Batched FP64 LU with partial pivoting and the solves built on it (gfx950).
+// Batched LU with partial pivoting and the solves built on it (gfx950), FP64 and FP32
+// (E = double or float; FP32 for the low-precision factorizations).
 //
 // Conventions follow LAPACK getrf/getrs as exposed by scipy.linalg.lu_factor:
 // row-major storage here, the packed factor holds unit-lower L below the diagonal
@@ -12,29 +13,35 @@ namespace sslab {
 
 // In-place blocked right-looking LU of `batch` n x n matrices A_b = A + b*sA.
 // ipiv: batch x n ints (stride sP); info: batch ints (may be null).
-int getrf_batched(int n, double* A, int lda, long long sA, int* ipiv, long long sP,
+template <typename E>
+int getrf_batched(int n, E* A, int lda, long long sA, int* ipiv, long long sP,
                   int* info, int batch, hipStream_t stream);
 
 // Solve A_b X_b = B_b with the factors from getrf_batched.  B (n x nrhs, row-major,
 // leading dimension ldb, stride sB) is read; X (same shape, ldx, sX) receives the
 // solution.  X may equal B.  perm: batch x n int scratch (stride n).
-int getrs_batched(int n, int nrhs, const double* LU, int lda, long long sA, const int* ipiv,
-                  long long sP, const double* B, int ldb, long long sB, double* X, int ldx,
+template <typename E>
+int getrs_batched(int n, int nrhs, const E* LU, int lda, long long sA, const int* ipiv,
+                  long long sP, const E* B, int ldb, long long sB, E* X, int ldx,
                   long long sX, int* perm, int batch, hipStream_t stream);
 
 // In place X_b <- U_b^{-1} X_b for upper-triangular U_b (n x n, non-unit diagonal).
-int trsm_upper_batched(int n, int nrhs, const double* U, int ldu, long long sU, double* X, int ldx,
+template <typename E>
+int trsm_upper_batched(int n, int nrhs, const E* U, int ldu, long long sU, E* X, int ldx,
                        long long sX, int batch, hipStream_t stream);
 
 // X_b = I (n x n, leading dimension ldx) for every batch entry.
-int set_identity_batched(int n, double* X, int ldx, long long sX, int batch, hipStream_t stream);
+template <typename E>
+int set_identity_batched(int n, E* X, int ldx, long long sX, int batch, hipStream_t stream);
 
 // Y_b[r, c] = X_b[r, c] for an m x n block (row-major, leading dims), batched.
-int copy_block_batched(int m, int n, const double* X, int ldx, long long sX, double* Y, int ldy,
+template <typename E>
+int copy_block_batched(int m, int n, const E* X, int ldx, long long sX, E* Y, int ldy,
                        long long sY, int batch, hipStream_t stream);
 
 // Fill an m x n block with zeros, batched (a kernel, never hipMemset).
-int zero_block_batched(int m, int n, double* Y, int ldy, long long sY, int batch,
+template <typename E>
+int zero_block_batched(int m, int n, E* Y, int ldy, long long sY, int batch,
                        hipStream_t stream);
 
 }  // namespace sslab

@@ -70,3 +70,30 @@ def test_mixed_refinement(T, nb, m, r):
     # and the FP64 solve itself matches the oracle
     x_ref = ob.thomas_direct(S, Tb, B.cpu().numpy())
     assert np.linalg.norm(x64.cpu().numpy() - x_ref) / np.linalg.norm(x_ref) < 1e-12
+
+
+@pytest.mark.parametrize("identity", [True, False])
+@pytest.mark.parametrize("nb,m,r", [(31, 128, 8), (9, 200, 3)])
+def test_fp32_factorization_refined(T, identity, nb, m, r):
+    """Factor in FP32 (FP32 LU + FP32 MFMA Schur updates), refine with FP64 residuals:
+    <= 1e-8 vs the FP64 solve after one step, <= 1e-10 after two."""
+    from sslablu_amd.blocktri import BlockTriFactor
+    Tb, S = _system(nb, m, 7 + nb, identity)
+    SiM = np.stack([s[0] for s in S])
+    SiP = np.stack([s[1] for s in S])
+    Tin = None if identity else np.stack(Tb)
+    f64 = BlockTriFactor("rb").factorize(Tin, SiM, SiP)
+    f32 = BlockTriFactor("rb").factorize_mixed(Tin, SiM, SiP)
+    assert int((f32.info != 0).sum()) == 0
+    B = T.tensor(np.random.default_rng(5).standard_normal((nb * m, r)), device="cuda")
+    x64 = f64.solve(B)
+    errs = []
+    for iters in (0, 1, 2):
+        X = T.empty_like(B)
+        f32.solve_mixed_(B, X, iters=iters)
+        errs.append((T.linalg.norm(X - x64) / T.linalg.norm(x64)).item())
+    assert 1e-10 < errs[0] < 1e-4
+    assert errs[1] < 1e-8
+    assert errs[2] < 1e-10
+    with pytest.raises(ValueError):
+        f32.solve(B)
