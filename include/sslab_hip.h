@@ -92,6 +92,14 @@ int sslab_rb_factor(int nb, int m, const double* T, const double* SiM, const dou
 int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
                    double* X, int ldx, void* stream);
 
+/* sslab_rb_solve with flags; bit 0 (SSLAB_RB_IDENTITY_DIAGONAL): the factor came from
+ * sslab_rb_factor with T = NULL, so the first level's T^{-1} = I and its product is
+ * skipped (y = v is read from V).  No reference counterpart (an optimisation of the oms
+ * S-system, whose diagonal blocks are identities, multislab/oms.py:292). */
+#define SSLAB_RB_IDENTITY_DIAGONAL 1
+int sslab_rb_solve_ex(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
+                      double* X, int ldx, int flags, void* stream);
+
 /* Several independent systems (any mix of nb, m, nrhs) solved together: each
  * elimination level of all systems shares one grouped GEMM launch.  Arrays of length
  * nsys (host memory) give each system's sizes, factor (from sslab_rb_factor), V and X

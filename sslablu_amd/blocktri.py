@@ -54,6 +54,7 @@ class BlockTriFactor:
         self.factor = None
         self.info = None
         self.cyclic = False
+        self.ident0 = False  # red-black factor of identity diagonal blocks (T = None)
         self.Z = None
         self.K = None
 
@@ -66,6 +67,7 @@ class BlockTriFactor:
         nb = len(ref)
         m = ref[0].shape[0]
         dev = self.device
+        self.ident0 = T is None and self.method == "rb"
         with torch.cuda.device(dev):
             if T is None and self.method == "thomas":
                 T = torch.eye(m, dtype=F64, device=dev).expand(nb, m, m)
@@ -142,8 +144,12 @@ class BlockTriFactor:
             return X
         s = stream_handle(stream)
         with tuning.policy(tuning.lookup(self.tune_key(r))):
-            call(f"sslab_{self.method}_solve", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
-                 ptr(X), X.stride(0), s)
+            if self.method == "rb":
+                call("sslab_rb_solve_ex", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
+                     ptr(X), X.stride(0), 1 if self.ident0 else 0, s)
+            else:
+                call("sslab_thomas_solve", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
+                     ptr(X), X.stride(0), s)
         if self.cyclic:
             # t = K [x_{nb-1}; x_0];  x -= Z t
             t = torch.empty(2 * m, r, dtype=F64, device=X.device)

@@ -287,6 +287,7 @@ struct RbSolveSys {
   T* X;
   int ldx;
   RbPlan p;
+  bool ident0;  // the first level's diagonal blocks are identities (T^{-1} = I there)
 };
 
 // Append a descriptor, launching the pending ones first when the launch is full.  The
@@ -346,7 +347,8 @@ static int rb_fwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStr
                                      0, m, -1.0));
     SSLAB_TRY(rb_push(G, d, st));
   }
-  {  // odd: y_j = Tinv_j v_j
+  if (!(l == 0 && S.ident0)) {  // odd: y_j = Tinv_j v_j (with Tinv = I, y_j = v_j is read
+                                 // from V by the backward sweep instead)
     GemmDescT<T> d = gemm_desc<T>(m, r, nO, X + ox, ldx, sx, X + ox, ldx, sx, 1.0);
     gemm_add_term<T>(d, gemm_term<T>(TG, 3 * m, 3 * m2, 0, V + ov, ldv, sv, 0, m, 1.0));
     SSLAB_TRY(rb_push(G, d, st));
@@ -377,15 +379,22 @@ static int rb_bwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStr
   const long long sx = (2LL << l) * bx;
   const long long ox = (1LL << l) * bx;
   const int nboth = (n % 2 == 0) ? nO - 1 : nO;  // odd nodes with both neighbours
+  // y_j: in X (written by the forward sweep), or v_j itself in V when Tinv_j = I.
+  const bool from_v = (l == 0 && S.ident0);
+  const T* Y = from_v ? (const T*)S.V : (const T*)X;
+  const int ldy = from_v ? S.ldv : ldx;
+  const long long by = (long long)m * ldy;
+  const long long sy = (2LL << l) * by, oy = (1LL << l) * by;
   if (nboth > 0) {
-    GemmDescT<T> d = gemm_desc<T>(m, r, nboth, X + ox, ldx, sx, X + ox, ldx, sx, 1.0);
+    GemmDescT<T> d = gemm_desc<T>(m, r, nboth, X + ox, ldx, sx, Y + oy, ldy, sy, 1.0);
     gemm_add_term<T>(d, gemm_term<T>(TG + m, 3 * m, 3 * m2, 0, X, ldx, sx, 0, m, -1.0));
     gemm_add_term<T>(d, gemm_term<T>(TG + 2 * m, 3 * m, 3 * m2, 0, X + sx, ldx, sx, 0, m, -1.0));
     SSLAB_TRY(rb_push(G, d, st));
   }
   if (nboth < nO) {  // last odd node: left neighbour only
     const long long c = nO - 1;
-    GemmDescT<T> d = gemm_desc<T>(m, r, 1, X + ox + c * sx, ldx, sx, X + ox + c * sx, ldx, sx, 1.0);
+    GemmDescT<T> d = gemm_desc<T>(m, r, 1, X + ox + c * sx, ldx, sx, Y + oy + c * sy, ldy, sy,
+                                  1.0);
     gemm_add_term<T>(d, gemm_term<T>(TG + c * 3 * m2 + m, 3 * m, 3 * m2, 0, X + c * sx, ldx, sx, 0,
                                      m, -1.0));
     SSLAB_TRY(rb_push(G, d, st));
@@ -441,9 +450,15 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
 
 template <typename T>
 static int rb_solve_t(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
-                      hipStream_t st) {
-  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}};
+                      hipStream_t st, bool ident0 = false) {
+  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0};
   return rb_solve_multi_t<T>(1, &S, st);
+}
+
+int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, double* X,
+                int ldx, int flags, hipStream_t st) {
+  SSLAB_REQUIRE((flags & ~1) == 0, "rb_solve: unknown flags 0x%x", flags);
+  return rb_solve_t<double>(nb, m, r, factor, V, ldv, X, ldx, st, (flags & 1) != 0);
 }
 
 int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,
@@ -454,7 +469,8 @@ int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const do
   SSLAB_REQUIRE(nb && m && r && factor && V && ldv && X && ldx, "rb_solve_multi: null array");
   std::vector<RbSolveSys<double>> sys(nsys);
   for (int s = 0; s < nsys; ++s)
-    sys[s] = RbSolveSys<double>{nb[s], m[s], r[s], factor[s], V[s], ldv[s], X[s], ldx[s], RbPlan{}};
+    sys[s] = RbSolveSys<double>{nb[s], m[s], r[s], factor[s], V[s], ldv[s], X[s], ldx[s], RbPlan{},
+                                false};
   return rb_solve_multi_t<double>(nsys, sys.data(), st);
 }
 

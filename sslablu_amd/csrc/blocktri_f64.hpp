@@ -14,6 +14,8 @@ int rb_factor_f32(int nb, int m, const float* T, const float* SiM, const float* 
                   float* factor, float* work, int* iwork, int* info, hipStream_t st);
 int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, double* X, int ldx,
              hipStream_t st);
+int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, double* X,
+                int ldx, int flags, hipStream_t st);
 int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,
                    double* const* V, const int* ldv, double* const* X, const int* ldx,
                    hipStream_t st);

@@ -119,6 +119,11 @@ int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int
   return sslab::rb_solve(nb, m, nrhs, factor, V, ldv, X, ldx, as_stream(stream));
 }
 
+int sslab_rb_solve_ex(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
+                      double* X, int ldx, int flags, void* stream) {
+  return sslab::rb_solve_ex(nb, m, nrhs, factor, V, ldv, X, ldx, flags, as_stream(stream));
+}
+
 int sslab_rb_solve_multi(int nsys, const int* nb, const int* m, const int* nrhs,
                          const double* const* factor, double* const* V, const int* ldv,
                          double* const* X, const int* ldx, void* stream) {
