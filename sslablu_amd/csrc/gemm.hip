@@ -286,7 +286,8 @@ struct FastGeom {
   static constexpr int STAGE = (Cfg::BM + Cfg::BN) * LD;
   static constexpr bool ok = (Cfg::BM * Cfg::BK) % (VEC * Cfg::NT) == 0 &&
                              (Cfg::BK * Cfg::BN) % (VEC * Cfg::NT) == 0 && KS % VEC == 0 &&
-                             A_VECS >= 1 && B_VECS >= 1;
+                             A_VECS >= 1 && B_VECS >= 1 &&
+                             2 * STAGE * (int)sizeof(T) <= 160 * 1024;  // LDS per CU
 };
 
 // The K loop of one term [ks, ke) for one workgroup, accumulating into acc.  VA / VB:
@@ -509,6 +510,193 @@ k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
   }
 }
 
+// ---- narrow N: A streamed from HBM straight into MFMA operand registers ----------------
+//
+// For few right-hand sides (N <= 64) every A element is used by exactly one wavefront,
+// so staging A in LDS only costs LDS space, which caps the bytes in flight per CU.
+// Here the wavefronts of a workgroup are stacked along M (WN = 1) and each loads its
+// own A fragments with 16-byte loads in the permuted-k order of k_gemm_fast (lane group
+// g holds k = g*BK/4 + s), one slab ahead in a second register buffer; only the B slab
+// (shared by all wavefronts) goes through a two-stage LDS ring, pre-scaled by alpha.
+// Aligned operands only (k ranges on vector boundaries); out-of-range k re-reads the
+// last in-range A vector and meets zero rows of B.
+template <class Cfg, typename T>
+struct ADGeom {
+  static constexpr int VEC = 16 / (int)sizeof(T);
+  static constexpr int KS = Cfg::BK / 4;
+  static constexpr int QV = KS / VEC;
+  static constexpr int LD = Cfg::BK + VEC;
+  static constexpr int NV = Cfg::BN / VEC;
+  static constexpr int B_VECS = Cfg::BK * Cfg::BN / (VEC * Cfg::NT);
+  static constexpr int STAGE = Cfg::BN * LD;
+  static constexpr bool ok = Cfg::WN == 1 && (Cfg::BK * Cfg::BN) % (VEC * Cfg::NT) == 0 &&
+                             KS % VEC == 0 && B_VECS >= 1;
+};
+
+template <class Cfg, typename T>
+__global__ void __launch_bounds__(Cfg::NT) __attribute__((amdgpu_waves_per_eu(2)))
+k_gemm_adirect(const GemmLaunchT<T> L, int total_tiles) {
+  using G = ADGeom<Cfg, T>;
+  using M4 = Mfma<T>;
+  using v4 = typename M4::v4;
+  using vec = typename Vec16<T>::type;
+  constexpr int VEC = G::VEC, KS = G::KS, QV = G::QV, LD = G::LD, NV = G::NV, BK = Cfg::BK;
+  constexpr int FM = Cfg::FM, FN = Cfg::FN;
+  __shared__ __attribute__((aligned(16))) T smem[2 * G::STAGE];
+
+  const int tile = xcd_remap(blockIdx.x, total_tiles);
+  int di = 0;
+#pragma unroll
+  for (int d = 1; d < kMaxDesc; ++d)
+    if (d < L.nd && tile >= L.d[d].tile_begin) di = d;
+  const GemmDescT<T>& D = L.d[di];
+  const int per_part = D.tiles_m * D.tiles_n;
+  int local = tile - D.tile_begin;
+  const int bk = local / per_part;
+  local -= bk * per_part;
+  const int b = bk / D.ksplit;
+  const int kp = bk - b * D.ksplit;
+  const int kbeg = kp * D.kchunk, kfin = kbeg + D.kchunk;
+  const int tm = local / D.tiles_n;
+  const int tn = local - tm * D.tiles_n;
+  const int row0 = tm * Cfg::BM, col0 = tn * Cfg::BN;
+  const int M = D.M, N = D.N;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wrow = (tid >> 6) * Cfg::WTM;
+  const int g = lane >> 4, li = lane & 15;
+
+  v4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v4{T(0), T(0), T(0), T(0)};
+
+  int stage = 0;
+  int koff = 0;
+  const int nterms = D.nterms;
+  for (int ti = 0; ti < nterms; ++ti) {
+    const GemmTermT<T>& Tm = D.term[ti];
+    const int ks = max(kbeg - koff, 0);
+    const int ke = min(kfin - koff, Tm.K);
+    koff += Tm.K;
+    if (ke <= ks) continue;
+    const T* A = Tm.A + (long long)b * Tm.strideA;
+    const T* B = Tm.B + (long long)b * Tm.strideB;
+    const int lda = Tm.lda, ldb = Tm.ldb;
+    const T alpha = Tm.alpha;
+    const int nslab = (ke - ks + BK - 1) / BK;
+    const T* arow[FM];
+#pragma unroll
+    for (int fi = 0; fi < FM; ++fi)
+      arow[fi] = A + (long long)min(row0 + wrow + fi * 16 + li, M - 1) * lda + g * KS;
+    vec ra0[FM][QV], ra1[FM][QV], rb[G::B_VECS];
+    bool okb[G::B_VECS];
+
+    auto load_a = [&](int k0, vec (&ra)[FM][QV]) {
+#pragma unroll
+      for (int fi = 0; fi < FM; ++fi)
+#pragma unroll
+        for (int q = 0; q < QV; ++q)
+          ra[fi][q] = *reinterpret_cast<const vec*>(
+              arow[fi] + min(k0 + q * VEC, ke - VEC - g * KS));
+    };
+    auto load_b = [&](int k0) {
+#pragma unroll
+      for (int r = 0; r < G::B_VECS; ++r) {
+        const int e = tid + r * Cfg::NT;
+        const int jv = e % NV, kk = e / NV;
+        const int k = k0 + kk;
+        rb[r] = *reinterpret_cast<const vec*>(B + (long long)min(k, ke - 1) * ldb +
+                                              min(col0 + jv * VEC, N - VEC));
+        okb[r] = k < ke;
+      }
+    };
+    auto store_b = [&](int st) {
+      T* Bs = smem + st * G::STAGE;
+#pragma unroll
+      for (int r = 0; r < G::B_VECS; ++r) {
+        const int e = tid + r * Cfg::NT;
+        const int jv = e % NV, kk = e / NV;
+        const vec v = okb[r] ? alpha * rb[r] : vec(T(0));
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) Bs[(jv * VEC + q) * LD + kk] = v[q];
+      }
+    };
+    auto compute = [&](const vec (&ra)[FM][QV], int st) {
+      const T* Bs = smem + st * G::STAGE;
+#pragma unroll
+      for (int q = 0; q < QV; ++q) {
+        vec bf[FN];
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          bf[j] = *reinterpret_cast<const vec*>(Bs + (j * 16 + li) * LD + g * KS + q * VEC);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = M4::op(ra[i][q][e], bf[j][e], acc[i][j]);
+      }
+    };
+
+    // The A address clamp min(k, ke - VEC) is taken relative to the lane's own offset
+    // g*KS (folded into arow): lane position g*KS + k0 + q*VEC <= ke - VEC.
+    load_a(ks, ra0);
+    load_b(ks);
+    store_b(stage);
+    __syncthreads();
+    for (int t = 0; t < nslab; t += 2) {
+      const bool m1 = t + 1 < nslab;
+      if (m1) {
+        load_a(ks + (t + 1) * BK, ra1);
+        load_b(ks + (t + 1) * BK);
+      }
+      compute(ra0, stage);
+      if (m1) store_b(stage ^ 1);
+      __syncthreads();
+      stage ^= 1;
+      if (!m1) break;
+      const bool m2 = t + 2 < nslab;
+      if (m2) {
+        load_a(ks + (t + 2) * BK, ra0);
+        load_b(ks + (t + 2) * BK);
+      }
+      compute(ra1, stage);
+      if (m2) store_b(stage ^ 1);
+      __syncthreads();
+      stage ^= 1;
+    }
+  }
+
+  T* C = D.C + (long long)b * D.strideC;
+  const T* Cin = D.Cin ? D.Cin + (long long)b * D.strideCin : nullptr;
+  const T beta = D.beta;
+  const bool use_cin = (Cin != nullptr) && (beta != T(0));
+  const bool split = D.ksplit > 1;
+  const int ldc = D.ldc, ldcin = D.ldcin;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = col0 + j * 16 + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row0 + wrow + i * 16 + M4::row(lane, v);
+        if (row < M && col < N) {
+          T val = acc[i][j][v];
+          if (split) {
+            atomicAdd(&C[(long long)row * ldc + col], val);
+          } else {
+            if (use_cin) val += beta * Cin[(long long)row * ldcin + col];
+            C[(long long)row * ldc + col] = val;
+          }
+        }
+      }
+    }
+  }
+}
+
 // 0: some term uses a transposed operand (the LDS-transposing kernel k_gemm);
 // 1: the vector-load kernel, per-term choice of 16-byte or element loads;
 // 2: the vector-load kernel with 16-byte loads everywhere (aligned operands, leading
@@ -540,6 +728,8 @@ static int fast_kind(const GemmLaunchT<T>& P) {
 //   0: 128x128 BK16   1: 128x64 BK16   2: 64x64 BK32   3: 64x64 BK16
 //   4: 64x32 BK16     5: 32x32 BK16    6: 32x64 BK16 (2 waves)   7: 64x128 BK16
 //   8: 32x16 BK32 (2 waves)   9: 32x32 BK32 (2 waves)     -- few right-hand sides
+//  10: 128x64 BK32, 11: 128x64 BK64 -- A streamed to registers (k_gemm_adirect), aligned
+//      launches with at most 64 columns; otherwise the same tile on k_gemm_fast
 using Cfg128 = GemmCfg<128, 128, 16, 2, 2>;
 using Cfg128x64 = GemmCfg<128, 64, 16, 2, 2>;
 using Cfg64 = GemmCfg<64, 64, 32, 2, 2>;
@@ -550,9 +740,11 @@ using Cfg32x64 = GemmCfg<32, 64, 16, 1, 2>;
 using Cfg64x128 = GemmCfg<64, 128, 16, 2, 2>;
 using Cfg32x16 = GemmCfg<32, 16, 32, 2, 1>;
 using Cfg32w = GemmCfg<32, 32, 32, 2, 1>;
-constexpr int kNumCfg = 10;
-constexpr int kCfgBM[kNumCfg] = {128, 128, 64, 64, 64, 32, 32, 64, 32, 32};
-constexpr int kCfgBN[kNumCfg] = {128, 64, 64, 64, 32, 32, 64, 128, 16, 32};
+using CfgAD32 = GemmCfg<128, 64, 32, 4, 1>;  // A in registers (k_gemm_adirect)
+using CfgAD64 = GemmCfg<128, 64, 64, 4, 1>;
+constexpr int kNumCfg = 12;
+constexpr int kCfgBM[kNumCfg] = {128, 128, 64, 64, 64, 32, 32, 64, 32, 32, 128, 128};
+constexpr int kCfgBN[kNumCfg] = {128, 64, 64, 64, 32, 32, 64, 128, 16, 32, 64, 64};
 
 // Workgroups a launch would create with BM x BN tiles.
 template <typename T>
@@ -589,7 +781,7 @@ static long long total_k(const GemmDescT<T>& d) {
 }
 
 // ksplit > 1 splits every output tile's summed K range over that many workgroups.
-template <class Cfg, typename T>
+template <class Cfg, typename T, bool kADirect = false>
 static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit) {
   int total = 0;
   GemmLaunchT<T> P{};
@@ -630,6 +822,16 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit) {
     P.d[P.nd++] = d;
   }
   if (total == 0) return kOk;
+  if constexpr (kADirect && ADGeom<Cfg, T>::ok) {
+    int nmax = 0;
+    for (int i = 0; i < P.nd; ++i) nmax = P.d[i].N > nmax ? P.d[i].N : nmax;
+    if (nmax <= Cfg::BN && fast_kind(P) == 2) {
+      hipLaunchKernelGGL((k_gemm_adirect<Cfg, T>), dim3(total), dim3(Cfg::NT), 0, stream, P,
+                         total);
+      SSLAB_HIP_CHECK(hipGetLastError());
+      return kOk;
+    }
+  }
   if constexpr (FastGeom<Cfg, T>::ok) {
     const int kind = fast_kind(P);
     if (kind == 2) {
@@ -662,7 +864,9 @@ static int launch_id(int id, GemmLaunchT<T>& L, hipStream_t stream, int ks) {
     case 6: return launch_cfg<Cfg32x64, T>(L, stream, ks);
     case 7: return launch_cfg<Cfg64x128, T>(L, stream, ks);
     case 8: return launch_cfg<Cfg32x16, T>(L, stream, ks);
-    default: return launch_cfg<Cfg32w, T>(L, stream, ks);
+    case 9: return launch_cfg<Cfg32w, T>(L, stream, ks);
+    case 10: return launch_cfg<CfgAD32, T, true>(L, stream, ks);
+    default: return launch_cfg<CfgAD64, T, true>(L, stream, ks);
   }
 }
 

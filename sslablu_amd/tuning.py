@@ -76,7 +76,7 @@ def candidates() -> list[Policy]:
     """The search space: workgroup targets, split-K targets and tile ids."""
     out = []
     for target, split, cmid, csplit, mt in itertools.product(
-            (512, 1024), (512, 1024, 2048), (2, 3), (1, 2, 3, 6), (0, 1)):
+            (512, 1024), (512, 1024, 2048), (2, 3, 10), (1, 2, 3, 6, 10, 11), (0, 1)):
         out.append(Policy(target, split, 64, 256, cmid, csplit, mt))
     return out
 

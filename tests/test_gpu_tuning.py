@@ -46,3 +46,33 @@ def test_solve_same_under_all_policies(T, r):
     assert tuning.lookup(fac.tune_key(r)) == pol and us > 0
     x = fac.solve(B)
     assert (T.linalg.norm(x - ref) / T.linalg.norm(ref)).item() < 1e-12
+
+
+@pytest.mark.parametrize("cfg", [10, 11])
+def test_register_streamed_tiles(T, cfg):
+    """Tiles 10 / 11 stream A from HBM into MFMA registers (aligned launches with at most
+    64 columns): GEMMs with ragged M, K tails and split-K, and a red-black solve, match."""
+    from sslablu_amd import tuning
+    from sslablu_amd._lib import call, ptr, stream_handle
+    from sslablu_amd.blocktri import BlockTriFactor
+    pol = tuning.Policy(100000, 1024, 64, 64, cfg, cfg, 0)
+    rng = np.random.default_rng(cfg)
+    for (b, M, N, K) in [(3, 200, 64, 1000), (2, 512, 48, 96), (1, 70, 64, 34), (5, 128, 2, 512)]:
+        A = rng.standard_normal((b, M, K))
+        Bm = rng.standard_normal((b, K, N))
+        C0 = rng.standard_normal((b, M, N))
+        At, Bt, Ct = (T.tensor(x, device="cuda") for x in (A, Bm, C0))
+        with tuning.policy(pol):
+            call("sslab_dgemm_strided_batched", 0, 0, M, N, K, -0.5, ptr(At), K, M * K, ptr(Bt), N,
+                 K * N, 1.0, ptr(Ct), N, M * N, b, stream_handle())
+        ref = -0.5 * (A @ Bm) + C0
+        assert np.linalg.norm(Ct.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-13
+    nb, m = 13, 96
+    SiM = 0.3 * rng.standard_normal((nb, m, m)) / np.sqrt(m)
+    SiP = 0.3 * rng.standard_normal((nb, m, m)) / np.sqrt(m)
+    fac = BlockTriFactor("rb").factorize(None, SiM, SiP)
+    Bv = T.tensor(rng.standard_normal((nb * m, 64)), device="cuda")
+    ref = fac.solve(Bv)
+    with tuning.policy(pol):
+        x = fac.solve(Bv)
+    assert (T.linalg.norm(x - ref) / T.linalg.norm(ref)).item() < 1e-12
