@@ -8,8 +8,10 @@ seeded N(0,1) Dirichlet data.
 
 * factor (timed once, reported as factor_time_s): HPS leaf DtN maps -> slab merges ->
   double-slab S maps -> red-black factorization, all on the GPU;
-* step (timed K times): boundary data -> S-system right-hand sides (oms rhs maps) ->
-  red-black solve, for the whole 64-column batch, inputs resident in HBM.
+* step (timed K times): boundary data -> S-system right-hand sides with the first
+  red-black level folded in (one grouped GEMM launch over precomputed boundary maps)
+  -> the remaining red-black levels, for the whole 64-column batch, inputs resident in
+  HBM.
 
 Prints one JSON line (rank 0, N = 1 only).
 """
@@ -406,7 +408,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from sslablu_amd import _lib
+    from sslablu_amd import _lib, tuning
     from sslablu_amd.blocktri import BlockTriFactor
     from sslablu_amd.hps import HPSSlabOperator, SlabGrid2D
 
@@ -446,14 +448,17 @@ def main():
     V = torch.empty(nb * m, NRHS, dtype=torch.float64, device=dev)
     X = torch.empty_like(V)
 
-    # GEMM tile policy of the solve, picked by the autotuner on this factor (setup, untimed)
-    t0 = time.perf_counter()
-    pol, _ = fac.autotune(NRHS)
-    t_tune = time.perf_counter() - t0
-
     def step():
-        op.rhs_from_boundary(F, out=V)
-        fac.solve_(V, X)
+        op.rhs_rb0_from_boundary(F, out=V)
+        fac.solve_(V, X, level0_reduced=True)
+
+    # GEMM tile policy of the step, picked by the autotuner on this factor (setup, untimed;
+    # the boundary maps of the fused first level are built here too)
+    t0 = time.perf_counter()
+    op.rb0_maps()
+    pol, _ = fac.autotune(NRHS, fn=step)
+    tuning.set_policy(pol)  # the right-hand-side launch runs under it as well
+    t_tune = time.perf_counter() - t0
 
     for _ in range(args.warmup):
         step()
@@ -469,6 +474,7 @@ def main():
     wall = time.perf_counter() - t0
     lat_ms = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
     ms_per_step = wall * 1e3 / args.steps
+    tuning.set_policy(tuning.DEFAULT)
 
     # ---- accuracy: independent FP64 Thomas path and the S-system residual --------
     fac_th = BlockTriFactor("thomas", dev).factorize(
@@ -543,18 +549,24 @@ def main():
             mixed = {"value": None, "err_by_iters": errs,
                      "note": "refinement did not reach 1e-8 within 3 steps"}
 
-    # ---- roofline: the step is 12 launches of the FP64 MFMA GEMM kernel (k_gemm_fast)
-    # plus two fill kernels (~3% of the step); the per-launch average below is the HIP-event
-    # step time divided by the 12 GEMM launches, so it slightly overstates a launch.
+    # ---- roofline: the step is GEMM launches of the FP64 MFMA kernel (k_gemm_fast) plus
+    # one fill kernel (X = 0, ~2% of the step): the fused right-hand side + first level,
+    # L - 1 forward levels, the coarsest block and L backward levels.  The per-launch
+    # average below is the HIP-event step time divided by the GEMM launches, so it
+    # slightly overstates a launch.
     g_sum = nb * 4 * g.nx * g.p + 2 * m
     flops, nbytes = algorithmic_work(nb, m, g_sum, NRHS)
-    n_launch = 12
+    levels, n_ = 0, nb
+    while n_ > 1:
+        n_, levels = (n_ + 1) // 2, levels + 1
+    n_launch = 1 + (levels - 1) + 1 + levels
     step_s = float(np.mean(lat_ms)) / 1e3
     achieved_tf = (flops / n_launch) / (step_s / n_launch) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": 78.6,
                 "unit": "TFLOP/s", "frac": round(achieved_tf / 78.6, 4), "traffic": TRAFFIC,
-                "kernel": "k_gemm_fast<64x64x16, f64> (12 launches per step: 1 rhs-map + 2*5 "
-                          "red-black levels + coarsest block)",
+                "kernel": f"k_gemm_fast<f64> ({n_launch} launches per step: fused rhs-map + "
+                          f"first level, {levels - 1} forward levels, coarsest block, "
+                          f"{levels} backward levels)",
                 "launches_per_step": n_launch, "avg_launch_us": round(step_s / n_launch * 1e6, 2),
                 "algorithmic_flops_per_launch": flops / n_launch,
                 "algorithmic_flops_per_step": flops, "algorithmic_bytes_per_step": nbytes}

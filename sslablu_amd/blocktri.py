@@ -125,10 +125,14 @@ class BlockTriFactor:
         self.Z, self.K = Z, K
 
     # -- solves -----------------------------------------------------------------
-    def solve_(self, V: torch.Tensor, X: torch.Tensor, stream=None):
+    def solve_(self, V: torch.Tensor, X: torch.Tensor, stream=None,
+               level0_reduced: bool = False):
         """Solve in place of preallocated device buffers: V (N, r) holds the
         right-hand sides and is overwritten, X (N, r) receives the solution.
-        Both float64, row-major with unit column stride."""
+        Both float64, row-major with unit column stride.  level0_reduced: V's even
+        block rows already carry the first red-black level's update (as written by
+        HPSSlabOperator.rhs_rb0_from_boundary); red-black factors built with T = None
+        only."""
         nb, m = self.nb, self.m
         N = nb * m
         if self.factor is None:
@@ -139,14 +143,17 @@ class BlockTriFactor:
         if V.dtype != F64 or X.dtype != F64 or (V.shape[1] > 0 and (V.stride(1) != 1 or
                                                                     X.stride(1) != 1)):
             raise ValueError("solve_: float64 buffers with unit column stride required")
+        if level0_reduced and not (self.method == "rb" and self.ident0 and not self.cyclic):
+            raise ValueError("solve_: level0_reduced needs a red-black factor with T = None")
         r = V.shape[1]
         if r == 0:
             return X
         s = stream_handle(stream)
+        flags = (1 if self.ident0 else 0) | (2 if level0_reduced else 0)
         with tuning.policy(tuning.lookup(self.tune_key(r))):
             if self.method == "rb":
                 call("sslab_rb_solve_ex", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
-                     ptr(X), X.stride(0), 1 if self.ident0 else 0, s)
+                     ptr(X), X.stride(0), flags, s)
             else:
                 call("sslab_thomas_solve", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
                      ptr(X), X.stride(0), s)
@@ -252,14 +259,19 @@ class BlockTriFactor:
     def tune_key(self, r: int, precision: str = "fp64") -> str:
         return f"blocktri:{self.method}:{self.nb}:{self.m}:{r}:{precision}"
 
-    def autotune(self, r: int, precision: str = "fp64", iters: int = 1, reps: int = 10):
+    def autotune(self, r: int, precision: str = "fp64", iters: int = 1, reps: int = 10,
+                 fn=None):
         """Pick the GEMM tile policy for solves with `r` right-hand sides by timing every
         candidate on this factor; later solve_ / solve_mixed_ calls of that width use
-        it.  Returns (policy, microseconds per solve)."""
+        it.  `fn` (optional) is the caller's own step to time instead of a bare solve
+        (e.g. right-hand-side assembly plus solve_); it runs under each candidate policy.
+        Returns (policy, microseconds per call)."""
         N = self.nb * self.m
         V = torch.randn(N, r, dtype=F64, device=self.device)
         X = torch.empty_like(V)
-        if precision == "mixed":
+        if fn is not None:
+            pass
+        elif precision == "mixed":
             work = (torch.empty(N * r, dtype=F64, device=self.device),
                     torch.empty(2 * N * r, dtype=torch.float32, device=self.device))
             fn = lambda: self.solve_mixed_(V, X, iters=iters, work=work)  # noqa: E731

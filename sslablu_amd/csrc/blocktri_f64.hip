@@ -288,6 +288,7 @@ struct RbSolveSys {
   int ldx;
   RbPlan p;
   bool ident0;  // the first level's diagonal blocks are identities (T^{-1} = I there)
+  bool even0;   // V's even slots already hold v_i - W^L v_{i-1} - W^R v_{i+1} (level 0)
 };
 
 // Append a descriptor, launching the pending ones first when the launch is full.  The
@@ -328,19 +329,20 @@ static int rb_fwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStr
   const long long sv = (2LL << l) * bv, sx = (2LL << l) * bx;  // same-parity node stride
   const long long ov = (1LL << l) * bv, ox = (1LL << l) * bx;  // first odd node
   const int both = (nE < nO ? nE : nO) - 1;
-  if (both > 0) {  // even b in [1, min(nE,nO))
+  const bool even = !(l == 0 && S.even0);  // else the caller applied it with the rhs
+  if (even && both > 0) {  // even b in [1, min(nE,nO))
     GemmDescT<T> d = gemm_desc<T>(m, r, both, V + sv, ldv, sv, V + sv, ldv, sv, 1.0);
     gemm_add_term<T>(d, gemm_term<T>(WW + 2 * m2, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
     gemm_add_term<T>(d, gemm_term<T>(WW + 2 * m2 + m, 2 * m, 2 * m2, 0, V + sv + ov, ldv, sv, 0, m,
                                      -1.0));
     SSLAB_TRY(rb_push(G, d, st));
   }
-  {  // b = 0
+  if (even) {  // b = 0
     GemmDescT<T> d = gemm_desc<T>(m, r, 1, V, ldv, sv, V, ldv, sv, 1.0);
     gemm_add_term<T>(d, gemm_term<T>(WW + m, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
     SSLAB_TRY(rb_push(G, d, st));
   }
-  if (nE > nO && nE - 1 >= 1) {
+  if (even && nE > nO && nE - 1 >= 1) {
     const long long b = nE - 1;
     GemmDescT<T> d = gemm_desc<T>(m, r, 1, V + b * sv, ldv, sv, V + b * sv, ldv, sv, 1.0);
     gemm_add_term<T>(d, gemm_term<T>(WW + b * 2 * m2, 2 * m, 2 * m2, 0, V + b * sv - ov, ldv, sv,
@@ -450,15 +452,16 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
 
 template <typename T>
 static int rb_solve_t(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
-                      hipStream_t st, bool ident0 = false) {
-  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0};
+                      hipStream_t st, bool ident0 = false, bool even0 = false) {
+  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0, even0};
   return rb_solve_multi_t<T>(1, &S, st);
 }
 
 int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, double* X,
                 int ldx, int flags, hipStream_t st) {
-  SSLAB_REQUIRE((flags & ~1) == 0, "rb_solve: unknown flags 0x%x", flags);
-  return rb_solve_t<double>(nb, m, r, factor, V, ldv, X, ldx, st, (flags & 1) != 0);
+  SSLAB_REQUIRE((flags & ~3) == 0, "rb_solve: unknown flags 0x%x", flags);
+  return rb_solve_t<double>(nb, m, r, factor, V, ldv, X, ldx, st, (flags & 1) != 0,
+                            (flags & 2) != 0);
 }
 
 int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,
@@ -470,7 +473,7 @@ int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const do
   std::vector<RbSolveSys<double>> sys(nsys);
   for (int s = 0; s < nsys; ++s)
     sys[s] = RbSolveSys<double>{nb[s], m[s], r[s], factor[s], V[s], ldv[s], X[s], ldx[s], RbPlan{},
-                                false};
+                                false, false};
   return rb_solve_multi_t<double>(nsys, sys.data(), st);
 }
 

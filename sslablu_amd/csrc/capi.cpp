@@ -210,6 +210,13 @@ int sslab_oms_rhs(int nb, int m, int nxp, int ncols, const double* Y, const doub
   return sslab::oms_rhs(nb, m, nxp, ncols, Y, F, ldF, r, N, out, ldo, as_stream(stream));
 }
 
+int sslab_oms_rhs_rb0(int nb, int m, int nxp, int ncols, const double* Y, const double* Ycb,
+                      const double* Yct, const double* Ycl, const double* Ycr, const double* F,
+                      int ldF, int r, int N, double* out, int ldo, void* stream) {
+  return sslab::oms_rhs_rb0(nb, m, nxp, ncols, Y, Ycb, Yct, Ycl, Ycr, F, ldF, r, N, out, ldo,
+                            as_stream(stream));
+}
+
 int sslab_gather2(int batch, int nrows, int ncols, const double* s0, int ld0, long long st0,
                   const double* s1, int ld1, long long st1, const int* rid0, const int* cid0,
                   const int* rid1, const int* cid1, double* out, int ldo, long long sto,

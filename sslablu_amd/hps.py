@@ -348,3 +348,67 @@ class HPSSlabOperator:
         call("sslab_oms_rhs", nb, m, nxp, ncols, ptr(self.Y), ptr(F), F.stride(0), r,
              g.N, ptr(out), out.stride(0), stream_handle(stream))
         return out
+
+    # -------------------------------------------------------------------------
+    def rb0_maps(self, stream=None):
+        """Boundary maps that fold the first red-black elimination level of the S-system
+        (identity diagonal blocks) into the right-hand side: for even i,
+            v_i - SiM_i v_{i-1} - SiP_i v_{i+1}
+              = Ycb_i Fbot[(i-1) nxp : (i+3) nxp] + Yct_i Ftop[...] (+ boundary terms),
+        since v_k = -(Y_k F)[Ic] reads bottom/top windows k, k+1 (window w = rows
+        [w nxp, (w+1) nxp)).  Built once per operator with the library's GEMM; cached."""
+        if getattr(self, "_rb0", None) is not None:
+            return self._rb0
+        from .hbs import bgemm
+        g = self.g
+        m, nb, nxp = g.m, g.nb, g.nx * g.p
+        if nb < 2:
+            raise ValueError("rb0_maps: needs at least two interfaces")
+        Y = self.Y
+        nE, nO = (nb + 1) // 2, nb // 2
+        w = 2 * nxp
+        out = []
+        with torch.cuda.device(self.device), torch.cuda.stream(
+                stream if stream is not None else torch.cuda.current_stream()):
+            for c0 in (2 * m, 2 * m + w):  # bottom, then top maps
+                Yc = torch.zeros(nE, m, 2 * w, dtype=F64, device=self.device)
+                Yc[:, :, nxp:nxp + w] = -Y[0::2, :, c0:c0 + w]          # own double slab
+                if nE > 1:                                               # left neighbour
+                    bgemm(Y[2::2, :, 0:m], Y[1:nb - 1:2, :, c0:c0 + w], out=Yc[1:, :, 0:w],
+                          beta=1.0)
+                bgemm(Y[0:2 * nO:2, :, m:2 * m], Y[1::2, :, c0:c0 + w],  # right neighbour
+                      out=Yc[:nO, :, w:2 * w], beta=1.0)
+                out.append(Yc)
+            Ycl = (-Y[0, :, 0:m]).contiguous()
+            if (nb - 1) % 2 == 0:
+                Ycr = (-Y[nb - 1, :, m:2 * m]).contiguous()
+            else:
+                Ycr = bgemm(Y[nb - 2:nb - 1, :, m:2 * m], Y[nb - 1:nb, :, m:2 * m])[0]
+        self._rb0 = (out[0], out[1], Ycl, Ycr)
+        return self._rb0
+
+    def rhs_rb0_from_boundary(self, F: torch.Tensor, out: torch.Tensor | None = None,
+                              stream=None):
+        """rhs_from_boundary with the first red-black level applied (one grouped launch):
+        odd block rows hold rhs_j, even ones rhs_i - SiM_i rhs_{i-1} - SiP_i rhs_{i+1}.
+        Solve with BlockTriFactor.solve_(..., level0_reduced=True) of the factor built
+        with T = None."""
+        g = self.g
+        m, nb, nxp = g.m, g.nb, g.nx * g.p
+        if F.dim() == 1:
+            F = F[:, None]
+        F = F.to(device=self.device, dtype=F64).contiguous()
+        r = F.shape[1]
+        ngb = 2 * g.N * nxp + 2 * m
+        if F.shape[0] != ngb:
+            raise ValueError(f"boundary data needs {ngb} rows, got {F.shape[0]}")
+        Ycb, Yct, Ycl, Ycr = self.rb0_maps(stream)
+        if out is None:
+            out = torch.empty(nb * m, r, dtype=F64, device=self.device)
+        if out.shape != (nb * m, r) or out.dtype != F64 or (r > 0 and out.stride(1) != 1):
+            raise ValueError("rhs_rb0_from_boundary: out must be float64 (nb*m, r), unit "
+                             "column stride")
+        call("sslab_oms_rhs_rb0", nb, m, nxp, self.Y.shape[2], ptr(self.Y), ptr(Ycb),
+             ptr(Yct), ptr(Ycl), ptr(Ycr), ptr(F), max(F.stride(0), r), r, g.N, ptr(out),
+             max(out.stride(0), r), stream_handle(stream))
+        return out
