@@ -409,7 +409,7 @@ def main():
 
     import torch
     from sslablu_amd import _lib, tuning
-    from sslablu_amd.blocktri import BlockTriFactor
+    from sslablu_amd.blocktri import BlockTriFactor, rb_levels
     from sslablu_amd.hps import HPSSlabOperator, SlabGrid2D
 
     _lib.require_gpu()
@@ -449,7 +449,7 @@ def main():
     X = torch.empty_like(V)
 
     def step():
-        op.rhs_rb0_from_boundary(F, out=V)
+        op.rhs_rb0_from_boundary(F, V, X)
         fac.solve_(V, X, level0_reduced=True)
 
     # GEMM tile policy of the step, picked by the autotuner on this factor (setup, untimed;
@@ -550,23 +550,21 @@ def main():
                      "note": "refinement did not reach 1e-8 within 3 steps"}
 
     # ---- roofline: the step is GEMM launches of the FP64 MFMA kernel (k_gemm_fast) plus
-    # one fill kernel (X = 0, ~2% of the step): the fused right-hand side + first level,
-    # L - 1 forward levels, the coarsest block and L backward levels.  The per-launch
-    # average below is the HIP-event step time divided by the GEMM launches, so it
-    # slightly overstates a launch.
+    # one fill kernel (V's even rows and X = 0, ~4% of the step): the fused right-hand
+    # side + first level, L - 1 forward levels, the dense coarsest level and L backward
+    # levels.  The per-launch average below is the HIP-event step time divided by the GEMM
+    # launches, so it slightly overstates a launch.
     g_sum = nb * 4 * g.nx * g.p + 2 * m
     flops, nbytes = algorithmic_work(nb, m, g_sum, NRHS)
-    levels, n_ = 0, nb
-    while n_ > 1:
-        n_, levels = (n_ + 1) // 2, levels + 1
+    levels, n_top = rb_levels(nb, m)
     n_launch = 1 + (levels - 1) + 1 + levels
     step_s = float(np.mean(lat_ms)) / 1e3
     achieved_tf = (flops / n_launch) / (step_s / n_launch) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": 78.6,
                 "unit": "TFLOP/s", "frac": round(achieved_tf / 78.6, 4), "traffic": TRAFFIC,
                 "kernel": f"k_gemm_fast<f64> ({n_launch} launches per step: fused rhs-map + "
-                          f"first level, {levels - 1} forward levels, coarsest block, "
-                          f"{levels} backward levels)",
+                          f"first level, {levels - 1} forward levels, dense {n_top}-node "
+                          f"coarsest level, {levels} backward levels)",
                 "launches_per_step": n_launch, "avg_launch_us": round(step_s / n_launch * 1e6, 2),
                 "algorithmic_flops_per_launch": flops / n_launch,
                 "algorithmic_flops_per_step": flops, "algorithmic_bytes_per_step": nbytes}

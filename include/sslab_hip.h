@@ -97,9 +97,11 @@ int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int
  * skipped (y = v is read from V).  No reference counterpart (an optimisation of the oms
  * S-system, whose diagonal blocks are identities, multislab/oms.py:292). */
 #define SSLAB_RB_IDENTITY_DIAGONAL 1
-/* bit 1 (SSLAB_RB_LEVEL0_REDUCED): V's even block rows already hold the first level's
- * reduced right-hand sides v_i - W^L_i v_{i-1} - W^R_i v_{i+1} (sslab_oms_rhs_rb0 writes
- * them), so that update is skipped; V's odd block rows hold v_j as usual. */
+/* bit 1 (SSLAB_RB_LEVEL0_REDUCED, requires bit 0 and nb >= 2): V and X hold the entry
+ * state written by sslab_oms_rhs_rb0 -- V's even block rows the first level's reduced
+ * right-hand sides v_i - W^L_i v_{i-1} - W^R_i v_{i+1}, X's odd block rows v_j, X's even
+ * block rows zero -- so the first level's forward sweep and the zeroing of X are
+ * skipped and its backward sweep updates X in place. */
 #define SSLAB_RB_LEVEL0_REDUCED 2
 int sslab_rb_solve_ex(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
                       double* X, int ldx, int flags, void* stream);
@@ -214,18 +216,21 @@ int sslab_oms_rhs(int nb, int m, int nxp, int ncols, const double* Y, const doub
                   int ldF, int r, int N, double* out, int ldo, void* stream);
 
 /* sslab_oms_rhs followed by the first red-black elimination level of the S-system
- * (identity diagonal blocks), in one grouped launch: odd block rows of out receive
- * rhs_j, even ones rhs_i - SiM_i rhs_{i-1} - SiP_i rhs_{i+1}.  Ycb/Yct (ceil(nb/2) x m x
- * 4 nxp each) are the even rows' combined maps over bottom/top rows [(i-1) nxp, (i+3)
- * nxp) of F; Ycl (m x m) maps the left boundary into row 0, Ycr (m x m) the right
- * boundary into the last even row that sees it.  They are built once per operator
- * (HPSSlabOperator.rb0_maps in the Python package).  Pass out to sslab_rb_solve_ex with
- * SSLAB_RB_IDENTITY_DIAGONAL | SSLAB_RB_LEVEL0_REDUCED.  nb >= 2.  No reference
- * counterpart: the reference forms rhs (multislab/oms.py:180-186) and eliminates in
- * separate steps (direct_solve/omsdirectsolve.py:221-279). */
+ * (identity diagonal blocks): one zero kernel and one grouped launch that write the
+ * entry state of sslab_rb_solve_ex with SSLAB_RB_IDENTITY_DIAGONAL |
+ * SSLAB_RB_LEVEL0_REDUCED -- V's even block rows rhs_i - SiM_i rhs_{i-1} - SiP_i
+ * rhs_{i+1}, X's odd block rows rhs_j, X's even block rows zero (V's odd rows are left
+ * as they are).  Ycb/Yct (ceil(nb/2) x m x 4 nxp each) are the even rows' combined maps
+ * over bottom/top rows [(i-1) nxp, (i+3) nxp) of F; Ycl (m x m) maps the left boundary
+ * into row 0, Ycr (m x m) the right boundary into the last even row that sees it; they
+ * are built once per operator (HPSSlabOperator.rb0_maps in the Python package).
+ * nb >= 2, V and X distinct.  No reference counterpart: the reference forms rhs
+ * (multislab/oms.py:180-186) and eliminates in separate steps
+ * (direct_solve/omsdirectsolve.py:221-279). */
 int sslab_oms_rhs_rb0(int nb, int m, int nxp, int ncols, const double* Y, const double* Ycb,
                       const double* Yct, const double* Ycl, const double* Ycr, const double* F,
-                      int ldF, int r, int N, double* out, int ldo, void* stream);
+                      int ldF, int r, int N, double* V, int ldv, double* X, int ldx,
+                      void* stream);
 
 /* out[b][i][j] = sum over s in {0,1} of src_s[b][rid_s[i]][cid_s[j]] where both
  * indices are >= 0 (index arrays may be NULL = absent). */

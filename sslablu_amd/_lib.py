@@ -65,7 +65,7 @@ _SIGS = {
     "sslab_oms_rhs": [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp,
                       _c_int, _vp],
     "sslab_oms_rhs_rb0": [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp,
-                          _c_int, _c_int, _c_int, _vp, _c_int, _vp],
+                          _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp],
     "sslab_gather2": [_c_int, _c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_int, _c_ll,
                       _vp, _vp, _vp, _vp, _vp, _c_int, _c_ll, _vp],
 }

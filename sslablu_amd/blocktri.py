@@ -129,9 +129,10 @@ class BlockTriFactor:
                level0_reduced: bool = False):
         """Solve in place of preallocated device buffers: V (N, r) holds the
         right-hand sides and is overwritten, X (N, r) receives the solution.
-        Both float64, row-major with unit column stride.  level0_reduced: V's even
-        block rows already carry the first red-black level's update (as written by
-        HPSSlabOperator.rhs_rb0_from_boundary); red-black factors built with T = None
+        Both float64, row-major with unit column stride.  level0_reduced: V and X
+        hold the entry state written by HPSSlabOperator.rhs_rb0_from_boundary (V's even
+        block rows reduced by the first red-black level, X's odd block rows the odd
+        right-hand sides, X's even rows zero); red-black factors built with T = None
         only."""
         nb, m = self.nb, self.m
         N = nb * m
@@ -143,8 +144,10 @@ class BlockTriFactor:
         if V.dtype != F64 or X.dtype != F64 or (V.shape[1] > 0 and (V.stride(1) != 1 or
                                                                     X.stride(1) != 1)):
             raise ValueError("solve_: float64 buffers with unit column stride required")
-        if level0_reduced and not (self.method == "rb" and self.ident0 and not self.cyclic):
-            raise ValueError("solve_: level0_reduced needs a red-black factor with T = None")
+        if level0_reduced and not (self.method == "rb" and self.ident0 and not self.cyclic
+                                   and nb >= 2):
+            raise ValueError("solve_: level0_reduced needs a red-black factor with T = None "
+                             "and at least two blocks")
         r = V.shape[1]
         if r == 0:
             return X
@@ -308,6 +311,17 @@ class BlockTriFactor:
         if self.Z is not None:
             n += (self.Z.numel() + self.K.numel()) * 8
         return n
+
+
+def rb_levels(nb: int, m: int) -> tuple[int, int]:
+    """(elimination levels, nodes of the coarsest level) of the red-black factor of nb
+    blocks of size m: at least one level when nb >= 2, then halving while more than
+    min(4, 2048 // m) nodes remain; the coarsest level is stored as a dense inverse."""
+    top_max = max(1, min(4, 2048 // m))
+    n, L = nb, 0
+    while n > 1 and (L == 0 or n > top_max):
+        n, L = (n + 1) // 2, L + 1
+    return L, n
 
 
 def solve_multi(factors, Vs, Xs, stream=None):

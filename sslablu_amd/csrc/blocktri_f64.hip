@@ -19,6 +19,11 @@
 // x_j = y_j - (Tinv SiM)_j x_{j-1} - (Tinv SiP)_j x_{j+1}.  Level-l node k lives
 // in global block slot k << l, so every level works in place on V and X.
 //
+// The reduction stops (after at least one level) at a coarsest level of
+// n_top <= min(4, 2048 / m) nodes, whose (n_top m)^2 system is stored as an explicit
+// inverse: the solve's deepest levels are a few small, latency-bound launches each, and
+// one grouped product over the dense inverse replaces all of them.
+//
 // Thomas (direct_solve/omsdirectsolve.py:48-141) is kept as the faithful
 // sequential variant: A_i = SiM_i B_{i-1}^{-1}, B_i = T_i - A_i SiP_{i-1}.
 #include "blocktri_f64.hpp"
@@ -41,7 +46,8 @@ struct RbLevel {
 struct RbPlan {
   int nb, m, L;
   std::vector<RbLevel> lv;
-  long long top_off;       // Tinv of the coarsest block (m x m)
+  int n_top;               // nodes of the coarsest level
+  long long top_off;       // inverse of the coarsest level's system ((n_top m)^2)
   long long factor_doubles;
   // workspace (doubles): two ping-pong level sets (T', SiM', SiP'), LU blocks,
   // the stacked [I | SiM | SiP] right-hand sides.
@@ -49,7 +55,14 @@ struct RbPlan {
   long long work_doubles;
   long long work_ints;     // ipiv + perm + info
   int max_odd;
+  int piv_blocks;          // ipiv / perm capacity in blocks of m: max(max_odd, n_top)
 };
+
+// Largest coarsest level inverted as one dense system: at most 4 nodes and 2048 rows.
+inline int rb_top_max(int m) {
+  const int k = 2048 / m;
+  return k < 1 ? 1 : (k > 4 ? 4 : k);
+}
 
 RbPlan rb_plan(int nb, int m) {
   RbPlan p{};
@@ -59,7 +72,8 @@ RbPlan rb_plan(int nb, int m) {
   long long off = 0;
   int n = nb;
   p.max_odd = 1;
-  while (n > 1) {
+  const int top_max = rb_top_max(m);
+  while (n > 1 && (p.lv.empty() || n > top_max)) {
     RbLevel l{};
     l.n = n;
     l.nE = (n + 1) / 2;
@@ -73,13 +87,18 @@ RbPlan rb_plan(int nb, int m) {
     n = l.nE;
   }
   p.L = (int)p.lv.size();
+  p.n_top = n;
   p.top_off = off;
-  off += m2;
+  const long long nt2 = (long long)n * n * m2;
+  off += nt2;
   p.factor_doubles = off;
   p.set_blocks = (nb + 1) / 2;
+  // ping-pong sets, LU blocks, [I | SiM | SiP] right-hand sides, then the coarsest
+  // level's dense matrix and its identity right-hand side
   p.work_doubles = 2 * 3 * p.set_blocks * m2 + (long long)p.max_odd * m2 +
-                   (long long)p.max_odd * 3 * m2;
-  p.work_ints = (long long)p.max_odd * m * 2 + p.max_odd;
+                   (long long)p.max_odd * 3 * m2 + 2 * nt2;
+  p.piv_blocks = p.max_odd > n ? p.max_odd : n;
+  p.work_ints = (long long)p.piv_blocks * m * 2 + p.piv_blocks;
   return p;
 }
 
@@ -120,9 +139,10 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
   E* sets[2] = {work, work + setsz};
   E* LUw = work + 2 * setsz;
   E* Rs = LUw + (long long)p.max_odd * m2;
+  E* Dw = Rs + (long long)p.max_odd * 3 * m2;  // coarsest level: dense matrix, then I
   int* ipiv = iwork;
-  int* perm = iwork + (long long)p.max_odd * m;
-  int* linfo = perm + (long long)p.max_odd * m;
+  int* perm = iwork + (long long)p.piv_blocks * m;
+  int* linfo = perm + (long long)p.piv_blocks * m;
   // info layout: one int per odd node per level, then one for the top block.
 
   const E* cT = T;
@@ -240,20 +260,34 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
     cM = nM;
     cP = nP;
   }
-  // Coarsest block: Tinv_top (the identity when the only block is an identity T).
+  // Coarsest level: the inverse of its dense (n_top m)^2 system (the identity when the
+  // only block is an identity T).  The info slots of levels this plan does not have are
+  // zero.
+  const int nt = p.n_top;
+  const int Nt = nt * m;
+  const long long nt2 = (long long)Nt * Nt;
+  if (info) zero_int_async(info + info_base + 1, rb_info_count(nb) - info_base - 1, st);
   if (cT == nullptr) {
     if (info) zero_int_async(info + info_base, 1, st);
     return set_identity_batched(m, factor + p.top_off, m, m2, 1, st);
   }
-  SSLAB_TRY(copy_block_batched(m, m, cT, m, m2, LUw, m, m2, 1, st));
-  SSLAB_TRY(getrf_batched(m, LUw, m, m2, ipiv, m, linfo, 1, st));
+  E* Iw = Dw + nt2;
+  const long long sdiag = (long long)(Nt + 1) * m;  // block (k, k) -> (k+1, k+1)
+  SSLAB_TRY(set_identity_batched(Nt, Dw, Nt, nt2, 1, st));
+  SSLAB_TRY(copy_block_batched(m, m, cT, m, m2, Dw, Nt, sdiag, nt, st));
+  if (nt > 1) {
+    SSLAB_TRY(copy_block_batched(m, m, cM + m2, m, m2, Dw + (long long)m * Nt, Nt, sdiag, nt - 1,
+                                 st));
+    SSLAB_TRY(copy_block_batched(m, m, cP, m, m2, Dw + m, Nt, sdiag, nt - 1, st));
+  }
+  SSLAB_TRY(getrf_batched(Nt, Dw, Nt, nt2, ipiv, Nt, linfo, 1, st));
   if (info) {
     SSLAB_HIP_CHECK(
         hipMemcpyAsync(info + info_base, linfo, sizeof(int), hipMemcpyDeviceToDevice, st));
   }
-  SSLAB_TRY(set_identity_batched(m, Rs, m, m2, 1, st));
-  SSLAB_TRY(getrs_batched(m, m, LUw, m, m2, ipiv, m, Rs, m, m2, factor + p.top_off, m, m2, perm, 1,
-                          st));
+  SSLAB_TRY(set_identity_batched(Nt, Iw, Nt, nt2, 1, st));
+  SSLAB_TRY(getrs_batched(Nt, Nt, Dw, Nt, nt2, ipiv, Nt, Iw, Nt, nt2, factor + p.top_off, Nt, nt2,
+                          perm, 1, st));
   return kOk;
 }
 
@@ -288,7 +322,9 @@ struct RbSolveSys {
   int ldx;
   RbPlan p;
   bool ident0;  // the first level's diagonal blocks are identities (T^{-1} = I there)
-  bool even0;   // V's even slots already hold v_i - W^L v_{i-1} - W^R v_{i+1} (level 0)
+  bool rb0;     // entry state of oms_rhs_rb0 (requires ident0): V's even slots hold the
+                // level-0 reduced v_i - W^L v_{i-1} - W^R v_{i+1}, X's odd slots y_j = v_j,
+                // X's even slots zero; level 0's forward sweep and X's zeroing are skipped
 };
 
 // Append a descriptor, launching the pending ones first when the launch is full.  The
@@ -329,7 +365,7 @@ static int rb_fwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStr
   const long long sv = (2LL << l) * bv, sx = (2LL << l) * bx;  // same-parity node stride
   const long long ov = (1LL << l) * bv, ox = (1LL << l) * bx;  // first odd node
   const int both = (nE < nO ? nE : nO) - 1;
-  const bool even = !(l == 0 && S.even0);  // else the caller applied it with the rhs
+  const bool even = !(l == 0 && S.rb0);  // else the caller applied it with the rhs
   if (even && both > 0) {  // even b in [1, min(nE,nO))
     GemmDescT<T> d = gemm_desc<T>(m, r, both, V + sv, ldv, sv, V + sv, ldv, sv, 1.0);
     gemm_add_term<T>(d, gemm_term<T>(WW + 2 * m2, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
@@ -358,14 +394,24 @@ static int rb_fwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStr
   return kOk;
 }
 
-// Coarsest block: x_0 = Tinv v_0.
+// Coarsest level: x_k = sum_j Inv_kj v_j over its n_top nodes (slots k << L), as
+// descriptors batched over k with up to kMaxTerms summed j each.  X starts at zero there;
+// with more than kMaxTerms nodes several descriptors add into every x_k, atomically.
 template <typename T>
 static int rb_top(GemmLaunchT<T>& G, const RbSolveSys<T>& S, hipStream_t st) {
-  const int m = S.m;
-  GemmDescT<T> d = gemm_desc<T>(m, S.r, 1, S.X, S.ldx, 0, S.X, S.ldx, 0, 1.0);
-  gemm_add_term<T>(d, gemm_term<T>(S.factor + S.p.top_off, m, (long long)m * m, 0, S.V, S.ldv, 0,
-                                   0, m, 1.0));
-  return rb_push(G, d, st);
+  const int m = S.m, nt = S.p.n_top, L = S.p.L;
+  const int Nt = nt * m;
+  const long long sv = ((long long)m << L) * S.ldv, sx = ((long long)m << L) * S.ldx;
+  const T* Inv = S.factor + S.p.top_off;
+  for (int j0 = 0; j0 < nt; j0 += kMaxTerms) {
+    GemmDescT<T> d = gemm_desc<T>(m, S.r, nt, S.X, S.ldx, sx, S.X, S.ldx, sx, 1.0);
+    d.atomic = nt > kMaxTerms;
+    for (int j = j0; j < nt && j < j0 + kMaxTerms; ++j)
+      gemm_add_term<T>(d, gemm_term<T>(Inv + (long long)j * m, Nt, (long long)m * Nt, 0,
+                                       S.V + j * sv, S.ldv, 0, 0, m, 1.0));
+    SSLAB_TRY(rb_push(G, d, st));
+  }
+  return kOk;
 }
 
 // Backward sweep of level l:  x_j = y_j - (Tinv SiM)_j x_{j-1} - (Tinv SiP)_j x_{j+1}.
@@ -381,8 +427,9 @@ static int rb_bwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStr
   const long long sx = (2LL << l) * bx;
   const long long ox = (1LL << l) * bx;
   const int nboth = (n % 2 == 0) ? nO - 1 : nO;  // odd nodes with both neighbours
-  // y_j: in X (written by the forward sweep), or v_j itself in V when Tinv_j = I.
-  const bool from_v = (l == 0 && S.ident0);
+  // y_j: in X (written by the forward sweep or, for rb0, by the caller), or v_j itself in
+  // V when Tinv_j = I.
+  const bool from_v = (l == 0 && S.ident0 && !S.rb0);
   const T* Y = from_v ? (const T*)S.V : (const T*)X;
   const int ldy = from_v ? S.ldv : ldx;
   const long long by = (long long)m * ldy;
@@ -424,7 +471,7 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
                   "rb_solve: null or aliased buffers");
     any = true;
     if (S.p.L > Lmax) Lmax = S.p.L;
-    SSLAB_TRY(zero_2d<T>((long long)S.nb * S.m, S.r, S.X, S.ldx, st));
+    if (!S.rb0) SSLAB_TRY(zero_2d<T>((long long)S.nb * S.m, S.r, S.X, S.ldx, st));
   }
   if (!any) return kOk;
   GemmLaunchT<T> G{};
@@ -452,14 +499,17 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
 
 template <typename T>
 static int rb_solve_t(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
-                      hipStream_t st, bool ident0 = false, bool even0 = false) {
-  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0, even0};
+                      hipStream_t st, bool ident0 = false, bool rb0 = false) {
+  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0, rb0};
   return rb_solve_multi_t<T>(1, &S, st);
 }
 
 int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, double* X,
                 int ldx, int flags, hipStream_t st) {
   SSLAB_REQUIRE((flags & ~3) == 0, "rb_solve: unknown flags 0x%x", flags);
+  SSLAB_REQUIRE(!(flags & 2) || ((flags & 1) && nb >= 2),
+                "rb_solve: the level-0-reduced entry state needs identity diagonal blocks and "
+                "nb >= 2");
   return rb_solve_t<double>(nb, m, r, factor, V, ldv, X, ldx, st, (flags & 1) != 0,
                             (flags & 2) != 0);
 }

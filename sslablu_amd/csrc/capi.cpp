@@ -212,9 +212,10 @@ int sslab_oms_rhs(int nb, int m, int nxp, int ncols, const double* Y, const doub
 
 int sslab_oms_rhs_rb0(int nb, int m, int nxp, int ncols, const double* Y, const double* Ycb,
                       const double* Yct, const double* Ycl, const double* Ycr, const double* F,
-                      int ldF, int r, int N, double* out, int ldo, void* stream) {
-  return sslab::oms_rhs_rb0(nb, m, nxp, ncols, Y, Ycb, Yct, Ycl, Ycr, F, ldF, r, N, out, ldo,
-                            as_stream(stream));
+                      int ldF, int r, int N, double* V, int ldv, double* X, int ldx,
+                      void* stream) {
+  return sslab::oms_rhs_rb0(nb, m, nxp, ncols, Y, Ycb, Yct, Ycl, Ycr, F, ldF, r, N, V, ldv, X,
+                            ldx, as_stream(stream));
 }
 
 int sslab_gather2(int batch, int nrows, int ncols, const double* s0, int ld0, long long st0,

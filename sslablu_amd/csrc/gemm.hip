@@ -15,6 +15,8 @@
 // (both measured on gfx950).
 #include "gemm.hpp"
 
+#include <type_traits>
+
 namespace sslab {
 
 template <typename T>
@@ -226,7 +228,8 @@ __global__ void __launch_bounds__(Cfg::NT) k_gemm(const GemmLaunchT<T> L, int to
   const T* Cin = D.Cin ? D.Cin + (long long)b * D.strideCin : nullptr;
   const T beta = D.beta;
   const bool use_cin = (Cin != nullptr) && (beta != T(0));
-  const bool split = D.ksplit > 1;  // C already holds beta*Cin: accumulate atomically
+  // C already holds beta*Cin: accumulate atomically
+  const bool split = D.ksplit > 1 || D.atomic;
 #pragma unroll
   for (int i = 0; i < Cfg::FM; ++i) {
 #pragma unroll
@@ -405,13 +408,158 @@ __device__ __forceinline__ void fast_term(const T* __restrict__ A, int lda,
   }
 }
 
+// Aligned operands (every term takes 16-byte loads): one slab sequence over all terms
+// of the workgroup's K range, with the global loads of two slabs in flight (register
+// sets 0 and 1 alternate) while a third is multiplied from LDS.  A slab therefore waits
+// for loads issued two compute phases earlier, and a term boundary does not drain the
+// pipeline.  Loads past the last slab re-read in-range vectors and are never stored to a
+// stage that is multiplied.
+template <class Cfg, typename T>
+__device__ __forceinline__ void fast_desc_pipe(const GemmDescT<T>& D, int b, int kbeg, int kfin,
+                                               int M, int N, int row0, int col0, int tid,
+                                               int wrow, int wcol, int g, int li, T* smem,
+                                               typename Mfma<T>::v4 (&acc)[Cfg::FM][Cfg::FN]) {
+  using G = FastGeom<Cfg, T>;
+  using M4 = Mfma<T>;
+  using vec = typename Vec16<T>::type;
+  constexpr int VEC = G::VEC, KS = G::KS, LD = G::LD, BK = Cfg::BK;
+  constexpr int KV = BK / VEC;       // vectors per A row slab
+  constexpr int NV = Cfg::BN / VEC;  // vectors per B k row
+  const int nterms = D.nterms;
+
+  int S = 0;  // slabs over all terms
+  {
+    int off = 0;
+    for (int ti = 0; ti < nterms; ++ti) {
+      const int K = D.term[ti].K;
+      const int ks = max(kbeg - off, 0), ke = min(kfin - off, K);
+      off += K;
+      if (ke > ks) S += (ke - ks + BK - 1) / BK;
+    }
+  }
+  if (S == 0) return;
+
+  // load cursor: operands of the current term, next k to load and the term's k end
+  int cti = -1, cstart = 0, ck = 0, cke = 0, clda = 0, cldb = 0;
+  const T* cA = nullptr;
+  const T* cB = nullptr;
+  T calpha = T(0);
+  auto next_term = [&]() {
+    for (int ti = cti + 1; ti < nterms; ++ti) {
+      const GemmTermT<T>& Tm = D.term[ti];
+      const int off = cstart;
+      cstart += Tm.K;
+      const int ks = max(kbeg - off, 0), ke = min(kfin - off, Tm.K);
+      if (ke > ks) {
+        cti = ti;
+        cA = Tm.A + (long long)b * Tm.strideA;
+        cB = Tm.B + (long long)b * Tm.strideB;
+        clda = Tm.lda;
+        cldb = Tm.ldb;
+        ck = ks;
+        cke = ke;
+        calpha = Tm.alpha;
+        return;
+      }
+    }
+    cti = nterms;  // exhausted: k runs on past cke, loads stay clamped in range
+  };
+
+  vec ra[2][G::A_VECS], rb[2][G::B_VECS];
+  bool oka[2][G::A_VECS], okb[2][G::B_VECS];
+  T al[2];
+  auto load = [&](auto set) {
+    constexpr int s = decltype(set)::value;
+    al[s] = calpha;
+#pragma unroll
+    for (int r = 0; r < G::A_VECS; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int kv = e % KV, i = e / KV;
+      const T* src = cA + (long long)min(row0 + i, M - 1) * clda;
+      const int k = ck + kv * VEC;
+      ra[s][r] = *reinterpret_cast<const vec*>(src + min(k, cke - VEC));
+      oka[s][r] = k < cke;
+    }
+#pragma unroll
+    for (int r = 0; r < G::B_VECS; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int jv = e % NV, kk = e / NV;
+      const int k = ck + kk;
+      const T* src = cB + (long long)min(k, cke - 1) * cldb;
+      rb[s][r] = *reinterpret_cast<const vec*>(src + min(col0 + jv * VEC, N - VEC));
+      okb[s][r] = k < cke;
+    }
+    ck += BK;
+    if (ck >= cke && cti < nterms) next_term();
+  };
+  auto store = [&](auto set) {
+    constexpr int s = decltype(set)::value;
+    T* As = smem + s * G::STAGE;
+    T* Bs = As + Cfg::BM * LD;
+#pragma unroll
+    for (int r = 0; r < G::A_VECS; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int kv = e % KV, i = e / KV;
+      const vec v = al[s] * ra[s][r];
+      *reinterpret_cast<vec*>(As + i * LD + kv * VEC) = oka[s][r] ? v : vec(T(0));
+    }
+#pragma unroll
+    for (int r = 0; r < G::B_VECS; ++r) {
+      const int e = tid + r * Cfg::NT;
+      const int jv = e % NV, kk = e / NV;
+#pragma unroll
+      for (int q = 0; q < VEC; ++q) Bs[(jv * VEC + q) * LD + kk] = okb[s][r] ? rb[s][r][q] : T(0);
+    }
+  };
+  auto compute = [&](int st) {
+    const T* As = smem + st * G::STAGE;
+    const T* Bs = As + Cfg::BM * LD;
+#pragma unroll
+    for (int q = 0; q < KS / VEC; ++q) {
+      vec af[Cfg::FM], bf[Cfg::FN];
+#pragma unroll
+      for (int i = 0; i < Cfg::FM; ++i)
+        af[i] = *reinterpret_cast<const vec*>(As + (wrow + i * 16 + li) * LD + g * KS + q * VEC);
+#pragma unroll
+      for (int j = 0; j < Cfg::FN; ++j)
+        bf[j] = *reinterpret_cast<const vec*>(Bs + (wcol + j * 16 + li) * LD + g * KS + q * VEC);
+#pragma unroll
+      for (int e = 0; e < VEC; ++e)
+#pragma unroll
+        for (int i = 0; i < Cfg::FM; ++i)
+#pragma unroll
+          for (int j = 0; j < Cfg::FN; ++j) acc[i][j] = M4::op(af[i][e], bf[j][e], acc[i][j]);
+    }
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+
+  next_term();
+  load(I0{});  // slab 0
+  load(I1{});  // slab 1
+  store(I0{});
+  __syncthreads();
+  for (int t = 0; t < S; t += 2) {
+    load(I0{});  // slab t+2 (register set 0 was stored to stage 0 before)
+    compute(0);  // slab t
+    store(I1{});  // slab t+1 into stage 1 (last read two phases ago)
+    __syncthreads();
+    if (t + 1 >= S) break;
+    load(I1{});  // slab t+3
+    compute(1);  // slab t+1
+    store(I0{});  // slab t+2
+    __syncthreads();
+  }
+}
+
 // Two waves per SIMD at least (256 registers per lane): the 128x128 tile otherwise
 // takes 268 and runs one wave per SIMD.
 // kAligned: the host has checked that every term takes 16-byte loads for A and B (then
 // only that path is compiled, which keeps the register count of the common case low).
-template <class Cfg, typename T, bool kAligned>
+template <class Cfg, typename T, int kMode>
 __global__ void __launch_bounds__(Cfg::NT) __attribute__((amdgpu_waves_per_eu(2)))
 k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
+  constexpr bool kAligned = kMode >= 1;
   using G = FastGeom<Cfg, T>;
   using M4 = Mfma<T>;
   using v4 = typename M4::v4;
@@ -452,7 +600,9 @@ k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
 
   int stage = 0;
   int koff = 0;
-  const int nterms = D.nterms;
+  const int nterms = kMode == 2 ? 0 : D.nterms;
+  if constexpr (kMode == 2)
+    fast_desc_pipe<Cfg, T>(D, b, kbeg, kfin, M, N, row0, col0, tid, wrow, wcol, g, li, smem, acc);
   for (int ti = 0; ti < nterms; ++ti) {
     const GemmTermT<T>& Tm = D.term[ti];
     const int ks = max(kbeg - koff, 0);
@@ -486,7 +636,7 @@ k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
   const T* Cin = D.Cin ? D.Cin + (long long)b * D.strideCin : nullptr;
   const T beta = D.beta;
   const bool use_cin = (Cin != nullptr) && (beta != T(0));
-  const bool split = D.ksplit > 1;
+  const bool split = D.ksplit > 1 || D.atomic;
   const int ldc = D.ldc, ldcin = D.ldcin;
 #pragma unroll
   for (int i = 0; i < Cfg::FM; ++i) {
@@ -673,7 +823,7 @@ k_gemm_adirect(const GemmLaunchT<T> L, int total_tiles) {
   const T* Cin = D.Cin ? D.Cin + (long long)b * D.strideCin : nullptr;
   const T beta = D.beta;
   const bool use_cin = (Cin != nullptr) && (beta != T(0));
-  const bool split = D.ksplit > 1;
+  const bool split = D.ksplit > 1 || D.atomic;
   const int ldc = D.ldc, ldcin = D.ldcin;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
@@ -780,12 +930,52 @@ static long long total_k(const GemmDescT<T>& d) {
   return k;
 }
 
-// ksplit > 1 splits every output tile's summed K range over that many workgroups.
+// Split-K chunk of a launch with BM x BN tiles that aims at ksplit x (output tiles)
+// workgroups: the shortest K range c (a multiple of BK, at least two BK slabs) for which
+// splitting every descriptor into min(ceil(K_d / c), maxparts) parts stays within that
+// budget.  Descriptors of equal K are then split ksplit ways; in a launch of mixed K the
+// long products get more parts than the short ones, so no workgroup runs much longer than
+// the rest.  Returns 0 when nothing is worth splitting.
+template <class Cfg, typename T>
+static long long split_chunk(const GemmLaunchT<T>& L, int ksplit, int maxparts) {
+  long long tiles = 0, kmax = 0;
+  for (int i = 0; i < L.nd; ++i) {
+    const GemmDescT<T>& d = L.d[i];
+    if (d.batch <= 0 || d.M <= 0 || d.N <= 0) continue;
+    tiles += (long long)ceil_div(d.M, Cfg::BM) * ceil_div(d.N, Cfg::BN) * d.batch;
+    const long long K = total_k(d);
+    kmax = K > kmax ? K : kmax;
+  }
+  const long long budget = tiles * ksplit;
+  auto wgs = [&](long long c) {
+    long long w = 0;
+    for (int i = 0; i < L.nd; ++i) {
+      const GemmDescT<T>& d = L.d[i];
+      if (d.batch <= 0 || d.M <= 0 || d.N <= 0) continue;
+      long long parts = (total_k(d) + c - 1) / c;
+      parts = parts < 1 ? 1 : (parts > maxparts ? maxparts : parts);
+      w += (long long)ceil_div(d.M, Cfg::BM) * ceil_div(d.N, Cfg::BN) * d.batch * parts;
+    }
+    return w;
+  };
+  long long lo = 2, hi = (kmax + Cfg::BK - 1) / Cfg::BK;  // chunk in BK slabs
+  if (ksplit <= 1 || hi <= lo) return 0;
+  while (lo < hi) {  // the workgroup count never grows with the chunk
+    const long long mid = (lo + hi) / 2;
+    if (wgs(mid * Cfg::BK) <= budget) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo * Cfg::BK;
+}
+
+// ksplit > 1 splits the output tiles' summed K ranges over about ksplit x (output tiles)
+// workgroups in all (split_chunk), at most maxparts per tile.
 template <class Cfg, typename T, bool kADirect = false>
-static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit) {
+static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit, int maxparts = 64) {
   int total = 0;
   GemmLaunchT<T> P{};
   P.nd = 0;
+  const long long chunk = split_chunk<Cfg, T>(L, ksplit, maxparts);
   for (int i = 0; i < L.nd; ++i) {
     GemmDescT<T> d = L.d[i];
     if (d.batch <= 0 || d.M <= 0 || d.N <= 0) continue;
@@ -793,16 +983,21 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit) {
       set_error("gemm: beta != 0 requires Cin");
       return kBadArgument;
     }
+    const bool inplace = d.Cin == d.C && d.beta == T(1) && d.ldcin == d.ldc &&
+                         d.strideCin == d.strideC;
+    if (d.atomic && !inplace) {
+      set_error("gemm: an atomic descriptor must accumulate in place (Cin == C, beta == 1)");
+      return kBadArgument;
+    }
     const long long K = total_k(d);
-    int ks = ksplit;
-    if (K < (long long)ks * 2 * Cfg::BK) ks = (int)(K / (2 * Cfg::BK));  // >= 2 slabs per part
-    if (ks < 1) ks = 1;
+    long long parts = chunk > 0 ? (K + chunk - 1) / chunk : 1;
+    parts = parts < 1 ? 1 : (parts > maxparts ? maxparts : parts);
+    const int ks = (int)parts;
     d.kchunk = (ceil_div((int)K, ks) + Cfg::BK - 1) / Cfg::BK * Cfg::BK;
     if (d.kchunk < Cfg::BK) d.kchunk = Cfg::BK;
     d.ksplit = ks > 1 ? ceil_div((int)K, d.kchunk) : 1;
     if (d.ksplit == 1) d.kchunk = (int)(K > 0 ? K : 1);
-    if (d.ksplit > 1 && !(d.Cin == d.C && d.beta == T(1) && d.ldcin == d.ldc &&
-                          d.strideCin == d.strideC)) {
+    if (d.ksplit > 1 && !inplace) {
       SSLAB_REQUIRE(d.batch <= 65535, "gemm: split-K init batch too large");
       const int gy = d.M < 4096 ? d.M : 4096;
       hipLaunchKernelGGL(k_scale_init<T>, dim3(ceil_div(d.N, 256), gy, d.batch), dim3(256), 0,
@@ -835,13 +1030,23 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit) {
   if constexpr (FastGeom<Cfg, T>::ok) {
     const int kind = fast_kind(P);
     if (kind == 2) {
-      hipLaunchKernelGGL((k_gemm_fast<Cfg, T, true>), dim3(total), dim3(Cfg::NT), 0, stream, P,
-                         total);
+      static const bool pipe = [] {
+        const char* e = getenv("SSLAB_GEMM_PIPE");
+        return !(e && e[0] == '0');
+      }();
+      // the second register set spills for FP64 tiles of more than 128 x 64
+      constexpr bool fits = !(sizeof(T) == 8 && Cfg::BM * Cfg::BN > 128 * 64);
+      if (pipe && fits)
+        hipLaunchKernelGGL((k_gemm_fast<Cfg, T, fits ? 2 : 1>), dim3(total), dim3(Cfg::NT), 0,
+                           stream, P, total);
+      else
+        hipLaunchKernelGGL((k_gemm_fast<Cfg, T, 1>), dim3(total), dim3(Cfg::NT), 0, stream, P,
+                           total);
       SSLAB_HIP_CHECK(hipGetLastError());
       return kOk;
     }
     if (kind == 1) {
-      hipLaunchKernelGGL((k_gemm_fast<Cfg, T, false>), dim3(total), dim3(Cfg::NT), 0, stream, P,
+      hipLaunchKernelGGL((k_gemm_fast<Cfg, T, 0>), dim3(total), dim3(Cfg::NT), 0, stream, P,
                          total);
       SSLAB_HIP_CHECK(hipGetLastError());
       return kOk;
@@ -853,20 +1058,20 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit) {
 }
 
 template <typename T>
-static int launch_id(int id, GemmLaunchT<T>& L, hipStream_t stream, int ks) {
+static int launch_id(int id, GemmLaunchT<T>& L, hipStream_t stream, int ks, int maxparts = 64) {
   switch (id) {
-    case 0: return launch_cfg<Cfg128, T>(L, stream, ks);
-    case 1: return launch_cfg<Cfg128x64, T>(L, stream, ks);
-    case 2: return launch_cfg<Cfg64, T>(L, stream, ks);
-    case 3: return launch_cfg<Cfg64s, T>(L, stream, ks);
-    case 4: return launch_cfg<Cfg64x32, T>(L, stream, ks);
-    case 5: return launch_cfg<Cfg32, T>(L, stream, ks);
-    case 6: return launch_cfg<Cfg32x64, T>(L, stream, ks);
-    case 7: return launch_cfg<Cfg64x128, T>(L, stream, ks);
-    case 8: return launch_cfg<Cfg32x16, T>(L, stream, ks);
-    case 9: return launch_cfg<Cfg32w, T>(L, stream, ks);
-    case 10: return launch_cfg<CfgAD32, T, true>(L, stream, ks);
-    default: return launch_cfg<CfgAD64, T, true>(L, stream, ks);
+    case 0: return launch_cfg<Cfg128, T>(L, stream, ks, maxparts);
+    case 1: return launch_cfg<Cfg128x64, T>(L, stream, ks, maxparts);
+    case 2: return launch_cfg<Cfg64, T>(L, stream, ks, maxparts);
+    case 3: return launch_cfg<Cfg64s, T>(L, stream, ks, maxparts);
+    case 4: return launch_cfg<Cfg64x32, T>(L, stream, ks, maxparts);
+    case 5: return launch_cfg<Cfg32, T>(L, stream, ks, maxparts);
+    case 6: return launch_cfg<Cfg32x64, T>(L, stream, ks, maxparts);
+    case 7: return launch_cfg<Cfg64x128, T>(L, stream, ks, maxparts);
+    case 8: return launch_cfg<Cfg32x16, T>(L, stream, ks, maxparts);
+    case 9: return launch_cfg<Cfg32w, T>(L, stream, ks, maxparts);
+    case 10: return launch_cfg<CfgAD32, T, true>(L, stream, ks, maxparts);
+    default: return launch_cfg<CfgAD64, T, true>(L, stream, ks, maxparts);
   }
 }
 
@@ -921,7 +1126,7 @@ int gemm_launch(GemmLaunchT<T>& L, hipStream_t stream) {
     const long long tn = count_tiles(L, kCfgBM[spl], kCfgBN[spl]);
     const long long want = (P.split_wgs + tn - 1) / (tn > 0 ? tn : 1);
     const int ks = (int)(want < P.max_ksplit ? want : P.max_ksplit);
-    return launch_id<T>(spl, L, stream, ks);
+    return launch_id<T>(spl, L, stream, ks, P.max_ksplit);
   }
   if (nmax <= 32) return launch_id<T>(mid, L, stream, 1);
   if (count_tiles(L, 64, 32) >= target) return launch_id<T>(4, L, stream, 1);

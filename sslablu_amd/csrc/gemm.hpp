@@ -50,6 +50,8 @@ struct GemmDescT {
   int tiles_m, tiles_n;
   int ksplit;  // workgroups sharing one output tile along K (filled in by gemm_launch)
   int kchunk;  // K range of one of them, a multiple of the tile's BK
+  int atomic;  // 1: accumulate into C atomically (requires Cin == C, beta == 1), so several
+               // descriptors (or launches) may add into the same output in any order
 };
 
 template <typename T>

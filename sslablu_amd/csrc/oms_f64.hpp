@@ -10,6 +10,6 @@ int oms_rhs(int nb, int m, int nxp, int ncols, const double* Y, const double* F,
 
 int oms_rhs_rb0(int nb, int m, int nxp, int ncols, const double* Y, const double* Ycb,
                 const double* Yct, const double* Ycl, const double* Ycr, const double* F, int ldF,
-                int r, int N, double* out, int ldo, hipStream_t st);
+                int r, int N, double* V, int ldv, double* X, int ldx, hipStream_t st);
 
 }  // namespace sslab

@@ -387,12 +387,13 @@ class HPSSlabOperator:
         self._rb0 = (out[0], out[1], Ycl, Ycr)
         return self._rb0
 
-    def rhs_rb0_from_boundary(self, F: torch.Tensor, out: torch.Tensor | None = None,
-                              stream=None):
-        """rhs_from_boundary with the first red-black level applied (one grouped launch):
-        odd block rows hold rhs_j, even ones rhs_i - SiM_i rhs_{i-1} - SiP_i rhs_{i+1}.
-        Solve with BlockTriFactor.solve_(..., level0_reduced=True) of the factor built
-        with T = None."""
+    def rhs_rb0_from_boundary(self, F: torch.Tensor, V: torch.Tensor | None = None,
+                              X: torch.Tensor | None = None, stream=None):
+        """rhs_from_boundary with the first red-black level applied; returns (V, X), the
+        entry state of BlockTriFactor.solve_(V, X, level0_reduced=True) for the factor
+        built with T = None: V's even block rows hold rhs_i - SiM_i rhs_{i-1} -
+        SiP_i rhs_{i+1}, X's odd block rows rhs_j and X's even block rows zero (one zero
+        kernel and one grouped GEMM launch)."""
         g = self.g
         m, nb, nxp = g.m, g.nb, g.nx * g.p
         if F.dim() == 1:
@@ -403,12 +404,15 @@ class HPSSlabOperator:
         if F.shape[0] != ngb:
             raise ValueError(f"boundary data needs {ngb} rows, got {F.shape[0]}")
         Ycb, Yct, Ycl, Ycr = self.rb0_maps(stream)
-        if out is None:
-            out = torch.empty(nb * m, r, dtype=F64, device=self.device)
-        if out.shape != (nb * m, r) or out.dtype != F64 or (r > 0 and out.stride(1) != 1):
-            raise ValueError("rhs_rb0_from_boundary: out must be float64 (nb*m, r), unit "
-                             "column stride")
+        if V is None:
+            V = torch.empty(nb * m, r, dtype=F64, device=self.device)
+        if X is None:
+            X = torch.empty(nb * m, r, dtype=F64, device=self.device)
+        for B in (V, X):
+            if B.shape != (nb * m, r) or B.dtype != F64 or (r > 0 and B.stride(1) != 1):
+                raise ValueError("rhs_rb0_from_boundary: V and X must be float64 (nb*m, r) "
+                                 "with unit column stride")
         call("sslab_oms_rhs_rb0", nb, m, nxp, self.Y.shape[2], ptr(self.Y), ptr(Ycb),
-             ptr(Yct), ptr(Ycl), ptr(Ycr), ptr(F), max(F.stride(0), r), r, g.N, ptr(out),
-             max(out.stride(0), r), stream_handle(stream))
-        return out
+             ptr(Yct), ptr(Ycl), ptr(Ycr), ptr(F), max(F.stride(0), r), r, g.N, ptr(V),
+             max(V.stride(0), r), ptr(X), max(X.stride(0), r), stream_handle(stream))
+        return V, X

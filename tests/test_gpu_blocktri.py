@@ -230,3 +230,54 @@ def test_identity_diagonal_shortcut(torch_dev, nb, method):
     S = [(SiM[i], SiP[i]) for i in range(nb)]
     ref = ob.thomas_direct(S, [np.eye(m)] * nb, B.cpu().numpy())
     assert rel(x0.cpu().numpy(), ref) < TOL
+
+
+def _rb_plan_shape(nb, m):
+    """Levels and coarsest-level size of the red-black factor: at least one level (nb >= 2),
+    then halving while more than min(4, 2048 // m) nodes remain."""
+    top_max = max(1, min(4, 2048 // m))
+    n, levels = nb, []
+    while n > 1 and (not levels or n > top_max):
+        levels.append((n, (n + 1) // 2, n // 2))
+        n = (n + 1) // 2
+    return levels, n
+
+
+@pytest.mark.parametrize("nb,m,ntop", [(2, 16, 1), (3, 16, 2), (5, 16, 3), (7, 16, 4),
+                                       (9, 16, 3), (15, 16, 4), (5, 700, 2), (7, 1100, 1),
+                                       (13, 512, 4)])
+def test_rb_dense_coarsest_level(torch_dev, nb, m, ntop):
+    """The coarsest red-black level (up to four nodes) is solved through the explicit
+    inverse of its dense system; with four nodes two descriptors add into every block
+    atomically.  Checked against the Thomas oracle for general diagonal blocks, for
+    T = None and through the FP32 factor with FP64 refinement; the factor size pins the
+    plan (levels of [Tinv | Tinv SiM | Tinv SiP] and [W^L | W^R] blocks, then the
+    (ntop m)^2 inverse)."""
+    torch, dev = torch_dev
+    from sslablu_amd import _lib
+    from sslablu_amd.blocktri import BlockTriFactor
+    levels, n_top = _rb_plan_shape(nb, m)
+    assert n_top == ntop
+    want = sum(nO * 3 * m * m + nE * 2 * m * m for _, nE, nO in levels) + (n_top * m) ** 2
+    assert _lib.sizes("rb", nb, m)[0] == want
+    r = 5
+    rng = np.random.default_rng(nb * 7 + m)
+    T = np.stack([np.eye(m) + 0.3 * rng.standard_normal((m, m)) / np.sqrt(m)
+                  for _ in range(nb)])
+    SiM = 0.4 * rng.standard_normal((nb, m, m)) / np.sqrt(m)
+    SiP = 0.4 * rng.standard_normal((nb, m, m)) / np.sqrt(m)
+    SiM[0] = 0
+    SiP[-1] = 0
+    B = rng.standard_normal((nb * m, r))
+    S = [(SiM[i], SiP[i]) for i in range(nb)]
+    for Tk, Tl in ((T, list(T)), (None, [np.eye(m)] * nb)):
+        fac = BlockTriFactor("rb").factorize(Tk, SiM, SiP)
+        assert int((fac.info != 0).sum()) == 0
+        x = fac.solve(torch.tensor(B, device=dev)).cpu().numpy()
+        ref = ob.thomas_direct(S, Tl, B)
+        assert rel(x, ref) < TOL
+    fm = BlockTriFactor("rb").factorize_mixed(T, SiM, SiP)
+    Bt = torch.tensor(B, device=dev)
+    X = torch.empty_like(Bt)
+    fm.solve_mixed_(Bt.clone(), X, iters=3)
+    assert rel(X.cpu().numpy(), ob.thomas_direct(S, list(T), B)) < 1e-10

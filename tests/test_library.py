@@ -45,8 +45,14 @@ def test_size_queries_and_argument_errors_without_gpu():
     from sslablu_amd import _lib
     f, w, i = _lib.sizes("rb", 31, 512)
     m2 = 512 * 512
-    # level sizes 31 -> 16 -> 8 -> 4 -> 2 -> 1: factor = sum(3*nO + 2*nE) m^2 + m^2
-    levels = [(31, 16, 15), (16, 8, 8), (8, 4, 4), (4, 2, 2), (2, 1, 1)]
-    assert f == sum(3 * nO + 2 * nE for _, nE, nO in levels) * m2 + m2
+    # levels 31 -> 16 -> 8 -> 4, then the 4-node coarsest level (4 * 512 = 2048 rows) as one
+    # dense inverse: factor = sum(3*nO + 2*nE) m^2 + (4m)^2
+    levels = [(31, 16, 15), (16, 8, 8), (8, 4, 4)]
+    assert f == sum(3 * nO + 2 * nE for _, nE, nO in levels) * m2 + 16 * m2
+    # m > 1024: the coarsest level is a single block; nb = 1: no level at all; nb >= 2
+    # always eliminates once, even when the whole system would fit the dense limit
+    assert _lib.sizes("rb", 3, 1500)[0] == (3 + 2 * 2 + 3 + 2) * 1500 ** 2 + 1500 ** 2
+    assert _lib.sizes("rb", 1, 64)[0] == 64 * 64
+    assert _lib.sizes("rb", 4, 16)[0] == (3 * 2 + 2 * 2) * 256 + (2 * 16) ** 2
     with pytest.raises(_lib.SslabError):
         _lib.sizes("rb", 0, 4)
