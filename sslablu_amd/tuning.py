@@ -73,10 +73,12 @@ def policy(p: Policy | None):
 
 
 def candidates() -> list[Policy]:
-    """The search space: workgroup targets, split-K targets and tile ids."""
+    """The search space: workgroup targets, split-K targets and tile ids (split launches
+    of few columns favour the small 64x32 / 32x32 tiles with many K parts: the
+    workgroups of a narrow launch are latency-bound, so more of them in flight help)."""
     out = []
     for target, split, cmid, csplit, mt in itertools.product(
-            (512, 1024), (512, 1024, 2048), (2, 3, 10), (1, 2, 3, 6, 10, 11), (0, 1)):
+            (512, 1024), (512, 1024, 2048), (2, 3, 10), (2, 3, 4, 5, 6, 9, 10), (0, 1)):
         out.append(Policy(target, split, 64, 256, cmid, csplit, mt))
     return out
 
