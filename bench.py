@@ -8,10 +8,10 @@ seeded N(0,1) Dirichlet data.
 
 * factor (timed once, reported as factor_time_s): HPS leaf DtN maps -> slab merges ->
   double-slab S maps -> red-black factorization, all on the GPU;
-* step (timed K times): boundary data -> S-system right-hand sides with the first
-  red-black level folded in (one grouped GEMM launch over precomputed boundary maps)
-  -> the remaining red-black levels, for the whole 64-column batch, inputs resident in
-  HBM.
+* step (timed K times): boundary data -> the red-black backward sweep's entry state
+  (the rhs maps and the whole forward sweep folded into precomputed boundary-window
+  maps: one grouped GEMM launch) -> the backward sweep, for the whole 64-column batch,
+  inputs resident in HBM.
 
 Prints one JSON line (rank 0, N = 1 only).
 """
@@ -409,7 +409,7 @@ def main():
 
     import torch
     from sslablu_amd import _lib, tuning
-    from sslablu_amd.blocktri import BlockTriFactor, rb_levels
+    from sslablu_amd.blocktri import BlockTriFactor, rb_backward_blocks, rb_levels
     from sslablu_amd.hps import HPSSlabOperator, SlabGrid2D
 
     _lib.require_gpu()
@@ -419,16 +419,22 @@ def main():
 
     # ---- factor ---------------------------------------------------------------
     def factor():
+        t = [time.perf_counter()]
         op = HPSSlabOperator(helmholtz_pdo(), SlabGrid2D(N_SLABS, NX, NY, Q), dev).build()
         fac = BlockTriFactor("rb", dev).factorize(None, op.SiM, op.SiP)  # T = I
-        return op, fac
+        torch.cuda.synchronize()
+        t.append(time.perf_counter())
+        fold = op.fold(fac)  # forward sweep folded into boundary maps
+        torch.cuda.synchronize()
+        t.append(time.perf_counter())
+        return op, fac, fold, t
 
     factor()  # warm-up (library load, allocator, code objects)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    op, fac = factor()
-    torch.cuda.synchronize()
-    t_factor = time.perf_counter() - t0
+    op, fac, fold, tf = factor()
+    t_factor = tf[2] - tf[0]
+    t_assembly_rb = tf[1] - tf[0]
+    t_fold = tf[2] - tf[1]
 
     # A plain device copy of known size (untimed).  Under rocprofv3 --pmc its FETCH_SIZE
     # and WRITE_SIZE (CALIB_MIB MiB each) calibrate the HBM byte counters of the solve
@@ -445,19 +451,15 @@ def main():
     rng = np.random.default_rng(SEED)
     F_host = rng.standard_normal((ngb, NRHS))
     F = torch.tensor(F_host, device=dev)
-    V = torch.empty(nb * m, NRHS, dtype=torch.float64, device=dev)
-    X = torch.empty_like(V)
+    X = torch.empty(nb * m, NRHS, dtype=torch.float64, device=dev)
 
     def step():
-        op.rhs_rb0_from_boundary(F, V, X)
-        fac.solve_(V, X, level0_reduced=True)
+        fold.solve_(F, X)  # boundary maps -> entry state (one launch), backward sweep
 
-    # GEMM tile policy of the step, picked by the autotuner on this factor (setup, untimed;
-    # the boundary maps of the fused first level are built here too)
+    # GEMM tile policy of the step, picked by the autotuner on this factor (setup, untimed)
     t0 = time.perf_counter()
-    op.rb0_maps()
     pol, _ = fac.autotune(NRHS, fn=step)
-    tuning.set_policy(pol)  # the right-hand-side launch runs under it as well
+    tuning.set_policy(pol)  # the boundary-map launch runs under it as well
     t_tune = time.perf_counter() - t0
 
     for _ in range(args.warmup):
@@ -550,24 +552,30 @@ def main():
                      "note": "refinement did not reach 1e-8 within 3 steps"}
 
     # ---- roofline: the step is GEMM launches of the FP64 MFMA kernel (k_gemm_fast) plus
-    # one fill kernel (V's even rows and X = 0, ~4% of the step): the fused right-hand
-    # side + first level, L - 1 forward levels, the dense coarsest level and L backward
-    # levels.  The per-launch average below is the HIP-event step time divided by the GEMM
-    # launches, so it slightly overstates a launch.
-    g_sum = nb * 4 * g.nx * g.p + 2 * m
-    flops, nbytes = algorithmic_work(nb, m, g_sum, NRHS)
+    # one fill kernel (X = 0): the boundary-map launch (the folded forward sweep) and the L
+    # backward levels.  Work is what those launches execute: every compact map element
+    # and every backward-sweep block is multiplied into all NRHS columns once.  The
+    # per-launch average is the HIP-event step time divided by the GEMM launches, so it
+    # slightly overstates a launch (the fill kernel is included).
     levels, n_top = rb_levels(nb, m)
-    n_launch = 1 + (levels - 1) + 1 + levels
+    n_launch = 1 + levels
+    map_elems = fold.nbytes // 8
+    bwd_elems = rb_backward_blocks(nb, m) * m * m
+    flops = 2.0 * NRHS * (map_elems + bwd_elems)
+    nbytes = 8.0 * (map_elems + bwd_elems) + 8.0 * NRHS * (fold.nF + 4 * nb * m)
+    g_sum = nb * 4 * g.nx * g.p + 2 * m
+    thomas_flops, _ = algorithmic_work(nb, m, g_sum, NRHS)
     step_s = float(np.mean(lat_ms)) / 1e3
-    achieved_tf = (flops / n_launch) / (step_s / n_launch) / 1e12
+    achieved_tf = flops / step_s / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": 78.6,
                 "unit": "TFLOP/s", "frac": round(achieved_tf / 78.6, 4), "traffic": TRAFFIC,
-                "kernel": f"k_gemm_fast<f64> ({n_launch} launches per step: fused rhs-map + "
-                          f"first level, {levels - 1} forward levels, dense {n_top}-node "
-                          f"coarsest level, {levels} backward levels)",
+                "kernel": f"k_gemm_fast<f64> ({n_launch} launches per step: boundary maps "
+                          f"(folded forward sweep and dense {n_top}-node coarsest level), "
+                          f"{levels} backward levels)",
                 "launches_per_step": n_launch, "avg_launch_us": round(step_s / n_launch * 1e6, 2),
                 "algorithmic_flops_per_launch": flops / n_launch,
-                "algorithmic_flops_per_step": flops, "algorithmic_bytes_per_step": nbytes}
+                "algorithmic_flops_per_step": flops, "algorithmic_bytes_per_step": nbytes,
+                "thomas_flops_per_step": thomas_flops}
 
     out = {
         "metric": "RHS solves/sec + p50/p99 latency at <=1e-8 rel err vs FP64; factor time",
@@ -584,6 +592,9 @@ def main():
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "factor_time_s": round(t_factor, 4),
+        "factor_breakdown_s": {"hps_assembly_and_rb_factor": round(t_assembly_rb, 4),
+                               "boundary_fold": round(t_fold, 4)},
+        "fold_maps_mb": round(fold.nbytes / 2 ** 20, 1),
         "gemm_policy": list(dataclasses.astuple(pol)), "autotune_s": round(t_tune, 3),
         "rel_err_vs_fp64_thomas": rel_err,
         "residual": resid,

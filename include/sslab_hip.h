@@ -103,6 +103,14 @@ int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int
  * block rows zero -- so the first level's forward sweep and the zeroing of X are
  * skipped and its backward sweep updates X in place. */
 #define SSLAB_RB_LEVEL0_REDUCED 2
+/* bit 2 (SSLAB_RB_FORWARD_ONLY): run the forward sweep and the coarsest level only,
+ * leaving in X the backward sweep's entry state -- y_j = T_j^{-1} v_j at every level's
+ * odd block rows, x at the coarsest level's block rows -- and V overwritten.  bit 3
+ * (SSLAB_RB_BACKWARD_ONLY): X holds that entry state (e.g. from sslab_oms_fold_apply);
+ * run the backward sweep only (V is not read and may be NULL).  No reference
+ * counterpart (the reference solve is one piece, direct_solve/omsdirectsolve.py:221-279). */
+#define SSLAB_RB_FORWARD_ONLY 4
+#define SSLAB_RB_BACKWARD_ONLY 8
 int sslab_rb_solve_ex(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
                       double* X, int ldx, int flags, void* stream);
 
@@ -231,6 +239,25 @@ int sslab_oms_rhs_rb0(int nb, int m, int nxp, int ncols, const double* Y, const 
                       const double* Yct, const double* Ycl, const double* Ycr, const double* F,
                       int ldF, int r, int N, double* V, int ldv, double* X, int ldx,
                       void* stream);
+
+/* Boundary-folded forward sweep of the oms solve.  maps (nb*m x nF, leading dim ldm) is
+ * the backward sweep's entry state for F = I: sslab_oms_rhs of the nF x nF identity,
+ * then sslab_rb_solve_ex(..., SSLAB_RB_IDENTITY_DIAGONAL | SSLAB_RB_FORWARD_ONLY) with
+ * the factor of identity diagonal blocks.  sslab_oms_fold_create keeps the nonzero
+ * column runs of its blocks (boundary windows) in a compact device copy and returns an
+ * opaque plan; sslab_oms_fold_apply writes X = maps F (nb*m x nrhs) for boundary data F
+ * (nF x nrhs) with one zero kernel and one grouped GEMM launch, after which
+ * sslab_rb_solve_ex(..., SSLAB_RB_IDENTITY_DIAGONAL | SSLAB_RB_BACKWARD_ONLY) finishes
+ * the solve.  No reference counterpart (the reference forms rhs, multislab/oms.py:180-186,
+ * and runs the whole elimination per solve, direct_solve/omsdirectsolve.py:221-279). */
+int sslab_oms_fold_create(int nb, int m, int nF, const double* maps, int ldm, void** plan,
+                          void* stream);
+int sslab_oms_fold_apply(const void* plan, const double* F, int ldF, int nrhs, double* X,
+                         int ldx, void* stream);
+/* device bytes held by a plan; number of descriptor groups of its launch */
+long long sslab_oms_fold_bytes(const void* plan);
+int sslab_oms_fold_groups(const void* plan);
+void sslab_oms_fold_destroy(void* plan);
 
 /* out[b][i][j] = sum over s in {0,1} of src_s[b][rid_s[i]][cid_s[j]] where both
  * indices are >= 0 (index arrays may be NULL = absent). */

@@ -66,9 +66,17 @@ _SIGS = {
                       _c_int, _vp],
     "sslab_oms_rhs_rb0": [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                           _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp],
+    "sslab_oms_fold_create": [_c_int, _c_int, _c_int, _vp, _c_int, _vp, _vp],
+    "sslab_oms_fold_apply": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp],
+    "sslab_oms_fold_bytes": [_vp],
+    "sslab_oms_fold_groups": [_vp],
+    "sslab_oms_fold_destroy": [_vp],
     "sslab_gather2": [_c_int, _c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_int, _c_ll,
                       _vp, _vp, _vp, _vp, _vp, _c_int, _c_ll, _vp],
 }
+
+# entry points that do not return a status
+_RESTYPES = {"sslab_oms_fold_bytes": ctypes.c_longlong, "sslab_oms_fold_destroy": None}
 
 EXPORTED = tuple(_SIGS) + ("sslab_last_error",)
 
@@ -95,7 +103,7 @@ def load(path: str | os.PathLike | None = None):
         if fn is None:  # stale build: calling this entry point raises (see call())
             continue
         fn.argtypes = args
-        fn.restype = _c_int
+        fn.restype = _RESTYPES.get(name, _c_int)
     lib.sslab_last_error.argtypes = []
     lib.sslab_last_error.restype = ctypes.c_char_p
     _lib = lib

@@ -125,17 +125,27 @@ class BlockTriFactor:
         self.Z, self.K = Z, K
 
     # -- solves -----------------------------------------------------------------
-    def solve_(self, V: torch.Tensor, X: torch.Tensor, stream=None,
-               level0_reduced: bool = False):
+    def solve_(self, V: torch.Tensor | None, X: torch.Tensor, stream=None,
+               level0_reduced: bool = False, phase: str = "all"):
         """Solve in place of preallocated device buffers: V (N, r) holds the
         right-hand sides and is overwritten, X (N, r) receives the solution.
         Both float64, row-major with unit column stride.  level0_reduced: V and X
         hold the entry state written by HPSSlabOperator.rhs_rb0_from_boundary (V's even
         block rows reduced by the first red-black level, X's odd block rows the odd
         right-hand sides, X's even rows zero); red-black factors built with T = None
-        only."""
+        only.  phase "forward": the forward sweep and coarsest level only, leaving the
+        backward sweep's entry state in X; "backward": X holds that state (V unused, may be
+        None) -- red-black factors only."""
         nb, m = self.nb, self.m
         N = nb * m
+        if phase not in ("all", "forward", "backward"):
+            raise ValueError(f"solve_: unknown phase {phase!r}")
+        if phase != "all" and (self.method != "rb" or self.cyclic):
+            raise ValueError("solve_: split phases need a non-cyclic red-black factor")
+        if phase == "backward":
+            if level0_reduced:
+                raise ValueError("solve_: backward phase excludes level0_reduced")
+            V = X  # shape / dtype checks only: the library does not read it
         if self.factor is None:
             raise ValueError("solve_: no FP64 factor (factorize_mixed keeps only the FP32 one; "
                              "use solve_mixed_)")
@@ -152,10 +162,12 @@ class BlockTriFactor:
         if r == 0:
             return X
         s = stream_handle(stream)
-        flags = (1 if self.ident0 else 0) | (2 if level0_reduced else 0)
+        flags = ((1 if self.ident0 else 0) | (2 if level0_reduced else 0) |
+                 {"all": 0, "forward": 4, "backward": 8}[phase])
         with tuning.policy(tuning.lookup(self.tune_key(r))):
             if self.method == "rb":
-                call("sslab_rb_solve_ex", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
+                call("sslab_rb_solve_ex", nb, m, r, ptr(self.factor),
+                     0 if phase == "backward" else ptr(V), V.stride(0),
                      ptr(X), X.stride(0), flags, s)
             else:
                 call("sslab_thomas_solve", nb, m, r, ptr(self.factor), ptr(V), V.stride(0),
@@ -322,6 +334,19 @@ def rb_levels(nb: int, m: int) -> tuple[int, int]:
     while n > 1 and (L == 0 or n > top_max):
         n, L = (n + 1) // 2, L + 1
     return L, n
+
+
+def rb_backward_blocks(nb: int, m: int) -> int:
+    """m x m blocks one right-hand-side column of the red-black backward sweep multiplies
+    (x_j -= (T^-1 SiM)_j x_{j-1} + (T^-1 SiP)_j x_{j+1} over every level's odd nodes)."""
+    L, _ = rb_levels(nb, m)
+    n, blocks = nb, 0
+    for _ in range(L):
+        nO = n // 2
+        nboth = nO - 1 if n % 2 == 0 else nO
+        blocks += 2 * nboth + (nO - nboth)
+        n = (n + 1) // 2
+    return blocks
 
 
 def solve_multi(factors, Vs, Xs, stream=None):

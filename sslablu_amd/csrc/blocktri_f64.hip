@@ -113,6 +113,15 @@ void zero_int_async(int* p, int n, hipStream_t st) {
 
 }  // namespace
 
+int rb_shape(int nb, int m, int* nodd, int nodd_cap, int* n_top) {
+  if (nb < 1 || m < 1) return -1;
+  const RbPlan p = rb_plan(nb, m);
+  if (nodd)
+    for (int l = 0; l < p.L && l < nodd_cap; ++l) nodd[l] = p.lv[l].nO;
+  if (n_top) *n_top = p.n_top;
+  return p.L;
+}
+
 int rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
              long long* work_ints) {
   SSLAB_REQUIRE(nb >= 1 && m >= 1, "rb_sizes: nb and m must be positive");
@@ -325,6 +334,9 @@ struct RbSolveSys {
   bool rb0;     // entry state of oms_rhs_rb0 (requires ident0): V's even slots hold the
                 // level-0 reduced v_i - W^L v_{i-1} - W^R v_{i+1}, X's odd slots y_j = v_j,
                 // X's even slots zero; level 0's forward sweep and X's zeroing are skipped
+  int phase;    // 0: whole solve; 1: forward sweep and coarsest level only, leaving in X
+                // the backward sweep's entry state (y_j at every level's odd nodes, x at
+                // the coarsest level's nodes); 2: backward sweep only, from that state
 };
 
 // Append a descriptor, launching the pending ones first when the launch is full.  The
@@ -427,9 +439,9 @@ static int rb_bwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStr
   const long long sx = (2LL << l) * bx;
   const long long ox = (1LL << l) * bx;
   const int nboth = (n % 2 == 0) ? nO - 1 : nO;  // odd nodes with both neighbours
-  // y_j: in X (written by the forward sweep or, for rb0, by the caller), or v_j itself in
-  // V when Tinv_j = I.
-  const bool from_v = (l == 0 && S.ident0 && !S.rb0);
+  // y_j: in X (written by the forward sweep or, for rb0 / a backward-only phase, by the
+  // caller), or v_j itself in V when Tinv_j = I.
+  const bool from_v = (l == 0 && S.ident0 && !S.rb0 && S.phase == 0);
   const T* Y = from_v ? (const T*)S.V : (const T*)X;
   const int ldy = from_v ? S.ldv : ldx;
   const long long by = (long long)m * ldy;
@@ -466,12 +478,13 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
     SSLAB_REQUIRE(S.nb >= 1 && S.m >= 1 && S.r >= 0, "rb_solve: bad sizes (system %d)", s);
     S.p = rb_plan(S.nb, S.m);
     if (S.r == 0) continue;  // nothing to solve (empty buffers may be null)
-    SSLAB_REQUIRE(S.ldv >= S.r && S.ldx >= S.r, "rb_solve: leading dimension < nrhs");
-    SSLAB_REQUIRE(S.factor && S.V && S.X && (void*)S.V != (void*)S.X,
+    const bool uses_v = S.phase != 2;
+    SSLAB_REQUIRE(S.ldx >= S.r && (!uses_v || S.ldv >= S.r), "rb_solve: leading dimension < nrhs");
+    SSLAB_REQUIRE(S.factor && S.X && (!uses_v || (S.V && (void*)S.V != (void*)S.X)),
                   "rb_solve: null or aliased buffers");
     any = true;
     if (S.p.L > Lmax) Lmax = S.p.L;
-    if (!S.rb0) SSLAB_TRY(zero_2d<T>((long long)S.nb * S.m, S.r, S.X, S.ldx, st));
+    if (!S.rb0 && S.phase != 2) SSLAB_TRY(zero_2d<T>((long long)S.nb * S.m, S.r, S.X, S.ldx, st));
   }
   if (!any) return kOk;
   GemmLaunchT<T> G{};
@@ -479,17 +492,24 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
   for (int t = 0; t <= Lmax; ++t) {
     for (int s = 0; s < nsys; ++s) {
       const RbSolveSys<T>& S = sys[s];
-      if (S.r == 0) continue;
+      if (S.r == 0 || S.phase == 2) continue;
       if (t < S.p.L) SSLAB_TRY(rb_fwd_level(G, S, t, st));
       else if (t == S.p.L) SSLAB_TRY(rb_top(G, S, st));
     }
     SSLAB_TRY(rb_flush(G, st));
   }
+  for (int s = 0; s < nsys; ++s) {  // forward only: level 0's y_j = v_j (T^{-1} = I) into X
+    const RbSolveSys<T>& S = sys[s];
+    if (S.r == 0 || S.phase != 1 || !S.ident0 || S.rb0 || S.p.L == 0) continue;
+    const long long bv = (long long)S.m * S.ldv, bx = (long long)S.m * S.ldx;
+    SSLAB_TRY(copy_block_batched<T>(S.m, S.r, S.V + bv, S.ldv, 2 * bv, S.X + bx, S.ldx, 2 * bx,
+                                    S.p.lv[0].nO, st));
+  }
   for (int u = 0; u < Lmax; ++u) {
     for (int s = 0; s < nsys; ++s) {
       const RbSolveSys<T>& S = sys[s];
       const int l = S.p.L - 1 - u;
-      if (S.r == 0 || l < 0) continue;
+      if (S.r == 0 || l < 0 || S.phase == 1) continue;
       SSLAB_TRY(rb_bwd_level(G, S, l, st));
     }
     SSLAB_TRY(rb_flush(G, st));
@@ -499,19 +519,23 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
 
 template <typename T>
 static int rb_solve_t(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
-                      hipStream_t st, bool ident0 = false, bool rb0 = false) {
-  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0, rb0};
+                      hipStream_t st, bool ident0 = false, bool rb0 = false, int phase = 0) {
+  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0, rb0, phase};
   return rb_solve_multi_t<T>(1, &S, st);
 }
 
 int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, double* X,
                 int ldx, int flags, hipStream_t st) {
-  SSLAB_REQUIRE((flags & ~3) == 0, "rb_solve: unknown flags 0x%x", flags);
+  SSLAB_REQUIRE((flags & ~15) == 0, "rb_solve: unknown flags 0x%x", flags);
   SSLAB_REQUIRE(!(flags & 2) || ((flags & 1) && nb >= 2),
                 "rb_solve: the level-0-reduced entry state needs identity diagonal blocks and "
                 "nb >= 2");
+  SSLAB_REQUIRE((flags & 12) != 12, "rb_solve: forward-only and backward-only exclude each other");
+  SSLAB_REQUIRE(!((flags & 2) && (flags & 8)),
+                "rb_solve: backward-only already starts from the forward sweep's result");
+  const int phase = (flags & 4) ? 1 : ((flags & 8) ? 2 : 0);
   return rb_solve_t<double>(nb, m, r, factor, V, ldv, X, ldx, st, (flags & 1) != 0,
-                            (flags & 2) != 0);
+                            (flags & 2) != 0, phase);
 }
 
 int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,
@@ -523,7 +547,7 @@ int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const do
   std::vector<RbSolveSys<double>> sys(nsys);
   for (int s = 0; s < nsys; ++s)
     sys[s] = RbSolveSys<double>{nb[s], m[s], r[s], factor[s], V[s], ldv[s], X[s], ldx[s], RbPlan{},
-                                false, false};
+                                false, false, 0};
   return rb_solve_multi_t<double>(nsys, sys.data(), st);
 }
 

@@ -8,6 +8,10 @@ namespace sslab {
 int rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
              long long* work_ints);
 int rb_info_count(int nb);
+// Shape of the red-black plan for nb blocks of size m: the number of elimination levels
+// (returned), the odd-node count of every level (nodd, at least that many entries; may be
+// null) and the node count of the coarsest level (n_top).
+int rb_shape(int nb, int m, int* nodd, int nodd_cap, int* n_top);
 int rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
               double* factor, double* work, int* iwork, int* info, hipStream_t st);
 int rb_factor_f32(int nb, int m, const float* T, const float* SiM, const float* SiP,

@@ -7,6 +7,7 @@
 #include "hps_f64.hpp"
 #include "lu_f64.hpp"
 #include "oms_f64.hpp"
+#include "oms_fold.hpp"
 #include "qr_f64.hpp"
 
 namespace sslab {
@@ -217,6 +218,30 @@ int sslab_oms_rhs_rb0(int nb, int m, int nxp, int ncols, const double* Y, const 
   return sslab::oms_rhs_rb0(nb, m, nxp, ncols, Y, Ycb, Yct, Ycl, Ycr, F, ldF, r, N, V, ldv, X,
                             ldx, as_stream(stream));
 }
+
+int sslab_oms_fold_create(int nb, int m, int nF, const double* maps, int ldm, void** plan,
+                          void* stream) {
+  sslab::FoldPlan* P = nullptr;
+  const int rc = sslab::oms_fold_create(nb, m, nF, maps, ldm, &P, as_stream(stream));
+  if (plan) *plan = P;
+  return rc;
+}
+
+int sslab_oms_fold_apply(const void* plan, const double* F, int ldF, int nrhs, double* X,
+                         int ldx, void* stream) {
+  return sslab::oms_fold_apply(static_cast<const sslab::FoldPlan*>(plan), F, ldF, nrhs, X, ldx,
+                               as_stream(stream));
+}
+
+long long sslab_oms_fold_bytes(const void* plan) {
+  return sslab::oms_fold_bytes(static_cast<const sslab::FoldPlan*>(plan));
+}
+
+int sslab_oms_fold_groups(const void* plan) {
+  return sslab::oms_fold_groups(static_cast<const sslab::FoldPlan*>(plan));
+}
+
+void sslab_oms_fold_destroy(void* plan) { sslab::oms_fold_destroy(static_cast<sslab::FoldPlan*>(plan)); }
 
 int sslab_gather2(int batch, int nrows, int ncols, const double* s0, int ld0, long long st0,
                   const double* s1, int ld1, long long st1, const int* rid0, const int* cid0,

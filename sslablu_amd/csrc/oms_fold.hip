@@ -1,0 +1,284 @@
+// Boundary-folded forward sweep of the oms S-system solve (FP64, gfx950).
+//
+// Everything the red-black solve computes before its backward sweep is linear in the
+// global boundary data F (nF rows ordered [bottom | top | left | right]): the backward
+// sweep's entry state E -- y_j = T_j^{-1} v_j at every level's odd nodes and x at the
+// coarsest level's nodes, one m-row block per interface -- equals M F for a single
+// (nb m) x nF matrix M.  The block of M of an odd node of level l reads only the bottom
+// and top windows of the few double slabs its eliminations touched (about 2^(l+2)
+// windows of nx*p rows), a few hundred of the nF columns, plus the left / right boundary
+// data for the nodes at the two ends.
+//
+// M is built once per operator by running the rhs maps and the forward sweep on F = I
+// (its columns).  Its nonzero column runs -- at most kMaxTerms per block after closing
+// small gaps -- are gathered into compact arrays, one per group, where a group is a
+// sequence of same-level blocks whose runs have equal lengths and shift by a constant
+// number of columns from one block to the next.  A solve then is one zero kernel and one
+// grouped GEMM launch, X_b = sum_t Mc_{b,t} F[run_t], ahead of the backward sweep: the
+// forward sweep costs O(window) instead of O(m) columns per block and no longer runs
+// level by level.
+#include "oms_fold.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <vector>
+
+#include "blocktri_f64.hpp"
+#include "gemm.hpp"
+#include "lu_f64.hpp"
+
+namespace sslab {
+
+struct FoldTerm {
+  long long a_off;  // compact map of the group's first block (offset into maps)
+  int lda;
+  long long sA;     // next block's map
+  int f_row;        // first row of F read by the group's first block
+  int f_rstride;    // shift of that row from one block to the next
+  int K;
+};
+
+struct FoldGroup {
+  int batch;      // blocks in the group
+  int x_row;      // first row of X written (block slot * m)
+  int x_rstride;  // rows from one block to the next
+  int nterms;
+  FoldTerm t[kMaxTerms];
+};
+
+struct FoldPlan {
+  int nb, m, nF;
+  double* maps;
+  long long maps_doubles;
+  std::vector<FoldGroup> groups;
+};
+
+namespace {
+
+// flag[b * nF + c] = 1 when column c of block b (rows [b m, (b+1) m)) has a nonzero.
+__global__ void k_col_nonzero(int nb, int m, int nF, const double* __restrict__ M, int ldm,
+                              int* __restrict__ flag) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= (long long)nb * nF) return;
+  const int b = (int)(e / nF);
+  const int c = (int)(e - (long long)b * nF);
+  const double* col = M + (long long)b * m * ldm + c;
+  int nz = 0;
+  for (int i = 0; i < m; ++i) nz |= (col[(long long)i * ldm] != 0.0) ? 1 : 0;
+  flag[e] = nz;
+}
+
+struct Run {
+  int s, e;
+};
+
+// Nonzero column runs of one block: aligned to even columns (16-byte operand loads),
+// gaps of at most 8 columns closed, then the two closest runs merged until at most
+// kMaxTerms remain (zero columns inside a run cost only multiply-adds of zeros).
+std::vector<Run> block_runs(const int* f, int nF) {
+  std::vector<Run> r;
+  for (int c = 0; c < nF;) {
+    if (!f[c]) {
+      ++c;
+      continue;
+    }
+    int e = c;
+    while (e < nF && f[e]) ++e;
+    r.push_back({c & ~1, std::min(nF, (e + 1) & ~1)});
+    c = e;
+  }
+  std::vector<Run> o;
+  for (const Run& x : r) {
+    if (!o.empty() && x.s <= o.back().e + 8)
+      o.back().e = std::max(o.back().e, x.e);
+    else
+      o.push_back(x);
+  }
+  while ((int)o.size() > kMaxTerms) {
+    size_t best = 0;
+    int gap = INT_MAX;
+    for (size_t i = 0; i + 1 < o.size(); ++i) {
+      if (o[i + 1].s - o[i].e < gap) {
+        gap = o[i + 1].s - o[i].e;
+        best = i;
+      }
+    }
+    o[best].e = o[best + 1].e;
+    o.erase(o.begin() + (long)best + 1);
+  }
+  return o;
+}
+
+bool same_lengths(const std::vector<Run>& a, const std::vector<Run>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t t = 0; t < a.size(); ++t)
+    if (a[t].e - a[t].s != b[t].e - b[t].s) return false;
+  return true;
+}
+
+}  // namespace
+
+int oms_fold_create(int nb, int m, int nF, const double* maps, int ldm, FoldPlan** out,
+                    hipStream_t st) {
+  SSLAB_REQUIRE(out != nullptr, "oms_fold_create: null plan pointer");
+  *out = nullptr;
+  SSLAB_REQUIRE(nb >= 1 && m >= 1 && nF >= 1 && maps && ldm >= nF, "oms_fold_create: bad sizes");
+  SSLAB_REQUIRE((long long)nb * m <= INT_MAX && (long long)nb * nF <= INT_MAX,
+                "oms_fold_create: problem too large");
+  int nodd[64];
+  int n_top = 0;
+  const int L = rb_shape(nb, m, nodd, 64, &n_top);
+  SSLAB_REQUIRE(L >= 0 && L < 64, "oms_fold_create: bad level count");
+
+  // 1. nonzero columns of every block, to the host
+  const long long nfl = (long long)nb * nF;
+  int* dflag = nullptr;
+  SSLAB_HIP_CHECK(hipMalloc(&dflag, sizeof(int) * nfl));
+  hipLaunchKernelGGL(k_col_nonzero, dim3((unsigned)((nfl + 255) / 256)), dim3(256), 0, st, nb, m,
+                     nF, maps, ldm, dflag);
+  std::vector<int> flag(nfl);
+  hipError_t e1 = hipGetLastError();
+  hipError_t e2 = e1 == hipSuccess
+                      ? hipMemcpyAsync(flag.data(), dflag, sizeof(int) * nfl, hipMemcpyDeviceToHost, st)
+                      : e1;
+  hipError_t e3 = e2 == hipSuccess ? hipStreamSynchronize(st) : e2;
+  (void)hipFree(dflag);
+  SSLAB_HIP_CHECK(e3);
+
+  std::vector<std::vector<Run>> runs(nb);
+  for (int b = 0; b < nb; ++b) runs[b] = block_runs(flag.data() + (long long)b * nF, nF);
+
+  // 2. blocks in level order (odd nodes of level 0, 1, ..., then the coarsest level),
+  //    cut into groups of equal run lengths and constant run shifts
+  std::vector<std::vector<int>> seqs;
+  std::vector<int> seq_stride;  // slots between consecutive blocks of a sequence
+  for (int l = 0; l < L; ++l) {
+    std::vector<int> s;
+    for (int c = 0; c < nodd[l]; ++c) s.push_back((2 * c + 1) << l);
+    seqs.push_back(s);
+    seq_stride.push_back(2 << l);
+  }
+  {
+    std::vector<int> s;
+    for (int c = 0; c < n_top; ++c) s.push_back(c << L);
+    seqs.push_back(s);
+    seq_stride.push_back(1 << L);
+  }
+  struct Proto {
+    int b0, count, stride;
+    std::vector<Run> r;
+    std::vector<int> d;
+  };
+  std::vector<Proto> protos;
+  for (size_t q = 0; q < seqs.size(); ++q) {
+    const std::vector<int>& s = seqs[q];
+    size_t i = 0;
+    while (i < s.size()) {
+      Proto p{s[i], 1, seq_stride[q], runs[s[i]], std::vector<int>(runs[s[i]].size(), 0)};
+      size_t j = i + 1;
+      while (j < s.size() && same_lengths(runs[s[j]], p.r)) {
+        bool ok = true;
+        for (size_t t = 0; t < p.r.size(); ++t) {
+          const int d = runs[s[j]][t].s - runs[s[j - 1]][t].s;
+          if (j == i + 1)
+            p.d[t] = d;
+          else if (d != p.d[t])
+            ok = false;
+        }
+        if (!ok) break;
+        ++p.count;
+        ++j;
+      }
+      if (!p.r.empty()) protos.push_back(p);  // a block without runs stays zero
+      i = j;
+    }
+  }
+
+  // 3. compact maps: group g holds count blocks of m x Ktot (its runs side by side)
+  FoldPlan* P = new FoldPlan{nb, m, nF, nullptr, 0, {}};
+  long long total = 0;
+  for (const Proto& p : protos) {
+    int kt = 0;
+    for (const Run& x : p.r) kt += x.e - x.s;
+    total += (long long)p.count * m * kt;
+  }
+  P->maps_doubles = total;
+  if (total > 0) {
+    hipError_t ea = hipMalloc(&P->maps, sizeof(double) * total);
+    if (ea != hipSuccess) {
+      delete P;
+      SSLAB_HIP_CHECK(ea);
+    }
+  }
+  long long off = 0;
+  for (const Proto& p : protos) {
+    int kt = 0;
+    for (const Run& x : p.r) kt += x.e - x.s;
+    FoldGroup g{};
+    g.batch = p.count;
+    g.x_row = p.b0 * m;
+    g.x_rstride = p.stride * m;
+    g.nterms = (int)p.r.size();
+    int coff = 0;
+    for (int t = 0; t < g.nterms; ++t) {
+      const int len = p.r[t].e - p.r[t].s;
+      g.t[t] = FoldTerm{off + coff, kt, (long long)m * kt, p.r[t].s, p.d[t], len};
+      const int rc = copy_block_batched<double>(
+          m, len, maps + (long long)p.b0 * m * ldm + p.r[t].s, ldm,
+          (long long)p.stride * m * ldm + p.d[t], P->maps + off + coff, kt, (long long)m * kt,
+          p.count, st);
+      if (rc != kOk) {
+        oms_fold_destroy(P);
+        return rc;
+      }
+      coff += len;
+    }
+    P->groups.push_back(g);
+    off += (long long)p.count * m * kt;
+  }
+  *out = P;
+  return kOk;
+}
+
+int oms_fold_apply(const FoldPlan* P, const double* F, int ldF, int r, double* X, int ldx,
+                   hipStream_t st) {
+  SSLAB_REQUIRE(P != nullptr, "oms_fold_apply: null plan");
+  SSLAB_REQUIRE(r >= 0, "oms_fold_apply: negative column count");
+  if (r == 0) return kOk;
+  SSLAB_REQUIRE(F && X && ldF >= r && ldx >= r, "oms_fold_apply: bad pointers / strides");
+  // every product accumulates into the zeroed X, so split-K needs no initialisation pass
+  SSLAB_TRY(zero_2d<double>((long long)P->nb * P->m, r, X, ldx, st));
+  GemmLaunch G{};
+  G.nd = 0;
+  for (const FoldGroup& g : P->groups) {
+    if (G.nd == kMaxDesc) {  // groups write disjoint blocks: any split into launches works
+      SSLAB_TRY(gemm_launch(G, st));
+      G.nd = 0;
+    }
+    double* C = X + (long long)g.x_row * ldx;
+    const long long sC = (long long)g.x_rstride * ldx;
+    GemmDesc d = gemm_desc(P->m, r, g.batch, C, ldx, sC, C, ldx, sC, 1.0);
+    for (int t = 0; t < g.nterms; ++t) {
+      const FoldTerm& T = g.t[t];
+      gemm_add_term(d, gemm_term(P->maps + T.a_off, T.lda, T.sA, 0, F + (long long)T.f_row * ldF,
+                                 ldF, (long long)T.f_rstride * ldF, 0, T.K, 1.0));
+    }
+    G.d[G.nd++] = d;
+  }
+  if (G.nd > 0) SSLAB_TRY(gemm_launch(G, st));
+  return kOk;
+}
+
+long long oms_fold_bytes(const FoldPlan* P) {
+  return P ? (long long)sizeof(double) * P->maps_doubles : 0;
+}
+
+int oms_fold_groups(const FoldPlan* P) { return P ? (int)P->groups.size() : 0; }
+
+void oms_fold_destroy(FoldPlan* P) {
+  if (!P) return;
+  if (P->maps) (void)hipFree(P->maps);
+  delete P;
+}
+
+}  // namespace sslab

@@ -26,6 +26,8 @@ maps and coefficient samples (user callables evaluated on the host).
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -350,6 +352,11 @@ class HPSSlabOperator:
         return out
 
     # -------------------------------------------------------------------------
+    def fold(self, fac, stream=None) -> "BoundaryFold":
+        """The forward sweep of `fac` (red-black, T = None, built from this operator's
+        SiM / SiP) folded into boundary maps: see BoundaryFold."""
+        return BoundaryFold(self, fac, stream)
+
     def rb0_maps(self, stream=None):
         """Boundary maps that fold the first red-black elimination level of the S-system
         (identity diagonal blocks) into the right-hand side: for even i,
@@ -416,3 +423,78 @@ class HPSSlabOperator:
              ptr(Yct), ptr(Ycl), ptr(Ycr), ptr(F), max(F.stride(0), r), r, g.N, ptr(V),
              max(V.stride(0), r), ptr(X), max(X.stride(0), r), stream_handle(stream))
         return V, X
+
+
+class BoundaryFold:
+    """Solve of an oms S-system straight from global boundary data, with the red-black
+    forward sweep folded into boundary maps (sslab_oms_fold_*).
+
+    Everything before the backward sweep is linear in the boundary data F: the sweep's
+    entry state (y_j = T_j^{-1} v_j at every level's odd block rows, x at the coarsest
+    level's rows) is M F.  M is built once here -- the rhs maps (multislab/oms.py:180-186)
+    and the forward sweep applied to F = I -- and its blocks keep only the boundary
+    windows they read.  solve_ is then one grouped GEMM launch and the backward sweep."""
+
+    def __init__(self, op: HPSSlabOperator, fac, stream=None):
+        if fac.method != "rb" or not fac.ident0 or fac.cyclic or fac.factor is None:
+            raise ValueError("BoundaryFold: needs the FP64 red-black factor built with T = None")
+        g = op.g
+        m, nb, nxp = g.m, g.nb, g.nx * g.p
+        if fac.nb != nb or fac.m != m:
+            raise ValueError("BoundaryFold: factor and operator sizes differ")
+        self.op, self.fac = op, fac
+        self.nF = 2 * g.N * nxp + 2 * m
+        self.handle = None
+        s = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.device(op.device), torch.cuda.stream(s):
+            R = op.rhs_from_boundary(torch.eye(self.nF, dtype=F64, device=op.device), stream=s)
+            M = torch.empty_like(R)
+            fac.solve_(R, M, stream=s, phase="forward")
+            del R
+            h = ctypes.c_void_p()
+            call("sslab_oms_fold_create", nb, m, self.nF, ptr(M), M.stride(0),
+                 ctypes.addressof(h), stream_handle(s))
+            s.synchronize()  # the compact copies have read M
+            del M
+        self.handle = h.value
+
+    @property
+    def nbytes(self) -> int:
+        return int(_lib.load().sslab_oms_fold_bytes(self.handle)) if self.handle else 0
+
+    @property
+    def groups(self) -> int:
+        return int(_lib.load().sslab_oms_fold_groups(self.handle)) if self.handle else 0
+
+    def apply_(self, F: torch.Tensor, X: torch.Tensor, stream=None):
+        """X = M F: the backward sweep's entry state for boundary data F (nF x r)."""
+        r = F.shape[1]
+        nbm = self.fac.nb * self.fac.m
+        if F.shape[0] != self.nF or F.dtype != F64 or X.shape != (nbm, r) or X.dtype != F64:
+            raise ValueError(f"BoundaryFold: F must be float64 ({self.nF}, r) and X ({nbm}, r)")
+        if r > 0 and (F.stride(1) != 1 or X.stride(1) != 1):
+            raise ValueError("BoundaryFold: unit column stride required")
+        call("sslab_oms_fold_apply", self.handle, ptr(F), max(F.stride(0), r), r, ptr(X),
+             max(X.stride(0), r), stream_handle(stream))
+        return X
+
+    def solve_(self, F: torch.Tensor, X: torch.Tensor, stream=None):
+        """X = S^{-1} rhs(F): interface values for boundary data F (nF x r)."""
+        self.apply_(F, X, stream)
+        self.fac.solve_(None, X, stream=stream, phase="backward")
+        return X
+
+    def solve(self, F: torch.Tensor, stream=None) -> torch.Tensor:
+        if F.dim() == 1:
+            F = F[:, None]
+        F = F.to(device=self.op.device, dtype=F64).contiguous()
+        X = torch.empty(self.fac.nb * self.fac.m, F.shape[1], dtype=F64, device=self.op.device)
+        return self.solve_(F, X, stream)
+
+    def __del__(self):
+        h, self.handle = getattr(self, "handle", None), None
+        if h and _lib is not None:
+            try:
+                _lib.load().sslab_oms_fold_destroy(h)
+            except Exception:
+                pass

@@ -281,3 +281,28 @@ def test_rb_dense_coarsest_level(torch_dev, nb, m, ntop):
     X = torch.empty_like(Bt)
     fm.solve_mixed_(Bt.clone(), X, iters=3)
     assert rel(X.cpu().numpy(), ob.thomas_direct(S, list(T), B)) < 1e-10
+
+
+@pytest.mark.parametrize("nb,m", [(1, 8), (2, 16), (7, 16), (12, 40), (31, 64)])
+@pytest.mark.parametrize("ident", [False, True])
+def test_rb_split_phases(torch_dev, nb, m, ident):
+    """Forward-only then backward-only equals the whole red-black solve (general and
+    identity diagonal blocks); the backward phase reads no V."""
+    torch, dev = torch_dev
+    from sslablu_amd.blocktri import BlockTriFactor
+    rng = np.random.default_rng(nb * 3 + m)
+    T = None if ident else np.stack([np.eye(m) + 0.3 * rng.standard_normal((m, m)) / np.sqrt(m)
+                                     for _ in range(nb)])
+    SiM = 0.4 * rng.standard_normal((nb, m, m)) / np.sqrt(m)
+    SiP = 0.4 * rng.standard_normal((nb, m, m)) / np.sqrt(m)
+    SiM[0] = 0
+    SiP[-1] = 0
+    B = torch.tensor(rng.standard_normal((nb * m, 6)), device=dev)
+    fac = BlockTriFactor("rb").factorize(T, SiM, SiP)
+    X0 = fac.solve(B)
+    X1 = torch.empty_like(B)
+    fac.solve_(B.clone(), X1, phase="forward")
+    fac.solve_(None, X1, phase="backward")
+    assert rel(X1.cpu().numpy(), X0.cpu().numpy()) < 1e-13
+    with pytest.raises(ValueError):
+        BlockTriFactor("thomas").factorize(T, SiM, SiP).solve_(B.clone(), X1, phase="forward")

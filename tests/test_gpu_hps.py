@@ -164,3 +164,48 @@ def test_rhs_rb0_exact_solution_and_errors(T):
         fac_t.solve_(*op.rhs_rb0_from_boundary(F), level0_reduced=True)
     V0, X0 = op.rhs_rb0_from_boundary(F[:, None][:, :0])
     assert V0.shape == (g.nb * g.m, 0) and X0.shape == (g.nb * g.m, 0)
+
+
+@pytest.mark.parametrize("N,nx,ny,q,r", [(2, 1, 2, 6, 2), (3, 1, 2, 6, 3), (4, 2, 2, 5, 1),
+                                         (5, 2, 4, 8, 7), (9, 2, 2, 6, 4), (10, 1, 2, 5, 2),
+                                         (17, 1, 4, 6, 3)])
+def test_boundary_fold_matches_plain(T, N, nx, ny, q, r):
+    """The folded forward sweep (sslab_oms_fold_*): its entry state equals the
+    forward-only red-black phase applied to the formed right-hand sides, and the folded
+    solve equals the plain rhs + solve, for one to sixteen interfaces (every parity of
+    the boundary nodes and of the coarsest level)."""
+    from sslablu_amd.blocktri import BlockTriFactor
+    op = build(helmholtz(6.0), N, nx, ny, q)
+    g = op.g
+    rng = np.random.default_rng(N * 31 + r)
+    F = T.tensor(rng.standard_normal((g.gb_points().shape[0], r)), device=op.device)
+    fac = BlockTriFactor("rb", op.device).factorize(None, op.SiM, op.SiP)
+    fold = op.fold(fac)
+    assert fold.groups >= 1 and fold.nbytes > 0
+    rhs = op.rhs_from_boundary(F)
+    E0 = T.empty_like(rhs)
+    fac.solve_(rhs.clone(), E0, phase="forward")
+    E1 = fold.apply_(F, T.full_like(rhs, float("nan")))
+    assert (T.linalg.norm(E1 - E0) / T.linalg.norm(E0)).item() < 1e-12
+    X0 = fac.solve(rhs)
+    X1 = fold.solve(F)
+    assert (T.linalg.norm(X1 - X0) / T.linalg.norm(X0)).item() < 1e-12
+
+
+def test_boundary_fold_exact_and_config2(T):
+    """Config 1 (u = exp(x) sin(y)) and the config-2 sized plane wave through the folded
+    solve reproduce the exact interface values."""
+    from sslablu_amd.blocktri import BlockTriFactor
+    op = build(LAPLACE, 8, 2, 16, 12)
+    g = op.g
+    fac = BlockTriFactor("rb", op.device).factorize(None, op.SiM, op.SiP)
+    u = op.fold(fac).solve(T.tensor(exp_sin(g.gb_points()), device=op.device))
+    assert np.max(np.abs(u.cpu().numpy()[:, 0] - exp_sin(g.interface_points()))) < 1e-10
+    kappa, theta = 20.0, 0.7
+    pd = PDO2d(ones, ones, c=lambda P: -kappa ** 2 * np.ones(P.shape[0]))
+    op = build(pd, 32, 1, 32, 18)
+    g = op.g
+    u_ex = lambda P: np.sin(kappa * (P[:, 0] * np.cos(theta) + P[:, 1] * np.sin(theta)))
+    fac = BlockTriFactor("rb", op.device).factorize(None, op.SiM, op.SiP)
+    u = op.fold(fac).solve(T.tensor(u_ex(g.gb_points()), device=op.device))
+    assert np.max(np.abs(u.cpu().numpy()[:, 0] - u_ex(g.interface_points()))) < 1e-9
