@@ -401,6 +401,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--policy", default=None,
+                    help="fixed GEMM tile policy (7 comma-separated ints, as reported in "
+                         "gemm_policy) instead of autotuning; for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box here has a single GPU and the "
@@ -458,7 +461,11 @@ def main():
 
     # GEMM tile policy of the step, picked by the autotuner on this factor (setup, untimed)
     t0 = time.perf_counter()
-    pol, _ = fac.autotune(NRHS, fn=step)
+    if args.policy:
+        pol = tuning.Policy(*[int(v) for v in args.policy.split(",")])
+        tuning._cache[fac.tune_key(NRHS)] = pol
+    else:
+        pol, _ = fac.autotune(NRHS, fn=step)
     tuning.set_policy(pol)  # the boundary-map launch runs under it as well
     t_tune = time.perf_counter() - t0
 

@@ -32,7 +32,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .blocktri import BlockTriFactor
+from .blocktri import BlockTriFactor, rb_levels
 from .hps import HPSSlabOperator, SlabGrid2D
 
 F64 = torch.float64
@@ -84,7 +84,8 @@ def make_pdo(spec: ProblemSpec):
 
 
 class CachedOperator:
-    """A built operator: rhs maps (op.Y), red-black factor, optional FP32 factor."""
+    """A built operator: rhs maps (op.Y), red-black factor, the boundary-folded forward
+    sweep (FP64 batches: one launch + the backward sweep), optional FP32 factor."""
 
     def __init__(self, spec: ProblemSpec, device, mixed: bool):
         t0 = time.perf_counter()
@@ -95,6 +96,7 @@ class CachedOperator:
         fac = BlockTriFactor("rb", device).factorize(None, op.SiM, op.SiP)  # T = I
         if mixed:
             fac.enable_mixed(None, op.SiM, op.SiP)
+        self.fold = None if mixed else op.fold(fac)
         torch.cuda.synchronize(device)
         self.spec, self.op, self.fac = spec, op, fac
         self.N = nb * m
@@ -110,6 +112,8 @@ class CachedOperator:
         n = self.fac.nbytes + (self.op.Y.numel() + self.op.SiM.numel() + self.op.SiP.numel()) * 8
         if getattr(self.fac, "factor32", None) is not None:
             n += self.fac.factor32.numel() * 4
+        if self.fold is not None:
+            n += self.fold.nbytes
         return n + self.graph_bytes
 
 
@@ -133,13 +137,24 @@ class FactorCache:
     def nbytes(self):
         return sum(e.nbytes for e in self.ops.values())
 
-    @staticmethod
-    def estimate_bytes(spec: ProblemSpec) -> int:
-        """Resident bytes of a built operator (FP64 factor + maps), before graphs."""
-        nb, m = spec.nb, spec.m
-        ncols = 2 * m + 4 * spec.p
+    def estimate_bytes(self, spec: ProblemSpec) -> int:
+        """Resident bytes of a built operator (FP64 factor + maps), before graphs; for FP64
+        caches an upper bound of the folded boundary maps (an odd node of level l reads
+        at most 2^(l+2) strip windows of bottom and top data plus both boundary segments,
+        a coarsest-level node all nF boundary values)."""
+        nb, m, p = spec.nb, spec.m, spec.p
+        ncols = 2 * m + 4 * p
         rb, _, _ = _lib.sizes("rb", nb, m)
-        return 8 * (rb + nb * m * ncols + 2 * nb * m * m)
+        n = 8 * (rb + nb * m * ncols + 2 * nb * m * m)
+        if not self.mixed:
+            L, n_top = rb_levels(nb, m)
+            nF = 2 * spec.slabs * p + 2 * m
+            cols, k = n_top * nF, nb
+            for lvl in range(L):
+                cols += (k // 2) * min(nF, (4 << lvl) * 2 * p + 2 * m)
+                k = (k + 1) // 2
+            n += 8 * m * cols
+        return n
 
     def _evict_for(self, need: int) -> bool:
         while (self.nbytes + need > self.budget or len(self.ops) >= self.max_ops) and self.ops:
@@ -228,13 +243,14 @@ class _BatchProgram:
         dev = e.op.device
         self.e, self.W, self.precision, self.iters = e, W, precision, iters
         self.Fb = torch.zeros(e.ngb, W, dtype=F64, device=dev)
-        self.V = torch.empty(e.N, W, dtype=F64, device=dev)
+        folded = precision != "mixed" and e.fold is not None
+        self.V = None if folded else torch.empty(e.N, W, dtype=F64, device=dev)
         self.Xb = torch.empty(e.N, W, dtype=F64, device=dev)
         self.work = None
         if precision == "mixed":
             self.work = (torch.empty(e.N * W, dtype=F64, device=dev),
                          torch.empty(2 * e.N * W, dtype=torch.float32, device=dev))
-        self.nbytes = (self.Fb.numel() + 2 * e.N * W) * 8 + (
+        self.nbytes = (self.Fb.numel() + (1 if folded else 2) * e.N * W) * 8 + (
             e.N * W * 16 if self.work is not None else 0)
         self.graph = None
         with torch.cuda.stream(stream):
@@ -249,6 +265,9 @@ class _BatchProgram:
 
     def _run(self):
         e = self.e
+        if self.V is None:  # FP64: folded forward sweep, then the backward sweep
+            e.fold.solve_(self.Fb, self.Xb)
+            return
         e.op.rhs_from_boundary(self.Fb, out=self.V)
         if self.precision == "mixed":
             e.fac.solve_mixed_(self.V, self.Xb, iters=self.iters, work=self.work)

@@ -88,6 +88,9 @@ def test_cache_eviction_and_budget(S):
     srv = serving.BatchingServer(cache, slots=1, max_cols=8)
     srv.serve(reqs, realtime=False)
     assert cache.evictions > 0 and len(cache.ops) <= 2
+    for e in cache.ops.values():  # admission estimate bounds what a built operator holds
+        assert e.fold is not None and e.fold.nbytes > 0
+        assert e.nbytes - e.graph_bytes <= cache.estimate_bytes(e.spec)
     ref = serving.FactorCache(max_operators=8)
     for q in reqs:
         e = ref.acquire(q.spec)
