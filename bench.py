@@ -36,10 +36,10 @@ NRHS = 64
 SEED = 0
 CALIB_MIB = 256
 # HBM bytes per GEMM launch of one step, from rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE over
-# the step's 12 GEMM dispatches, each counter scaled by its reading for the calibration
-# copy (summary and commands: profiles/rb_solve_c2_helmholtz_hbm_traffic.md).  A
+# the step's 4 GEMM dispatches, each counter scaled by its reading for the calibration
+# copy (summary and commands: profiles/rb_fold_solve_c2_helmholtz_hbm_traffic.md).  A
 # measured constant of the current kernels: re-measure after changing them.
-TRAFFIC = 43580587
+TRAFFIC = 56554044
 
 
 class PDO2d:
