@@ -22,7 +22,10 @@
 // The reduction stops (after at least one level) at a coarsest level of
 // n_top <= min(4, 2048 / m) nodes, whose (n_top m)^2 system is stored as an explicit
 // inverse: the solve's deepest levels are a few small, latency-bound launches each, and
-// one grouped product over the dense inverse replaces all of them.
+// one grouped product over the dense inverse replaces all of them.  The inverse itself
+// comes from a red-black factorization of that small system down to one block, solved
+// for the identity (n_top m right-hand sides: large, efficient products), not from one
+// LU of the whole (n_top m)^2 matrix, whose panels would run on a single workgroup.
 //
 // Thomas (direct_solve/omsdirectsolve.py:48-141) is kept as the faithful
 // sequential variant: A_i = SiM_i B_{i-1}^{-1}, B_i = T_i - A_i SiP_{i-1}.
@@ -64,7 +67,8 @@ inline int rb_top_max(int m) {
   return k < 1 ? 1 : (k > 4 ? 4 : k);
 }
 
-RbPlan rb_plan(int nb, int m) {
+// top_max: the largest coarsest level kept as a dense inverse (0: rb_top_max(m)).
+RbPlan rb_plan(int nb, int m, int top_max = 0) {
   RbPlan p{};
   p.nb = nb;
   p.m = m;
@@ -72,7 +76,7 @@ RbPlan rb_plan(int nb, int m) {
   long long off = 0;
   int n = nb;
   p.max_odd = 1;
-  const int top_max = rb_top_max(m);
+  if (top_max <= 0) top_max = rb_top_max(m);
   while (n > 1 && (p.lv.empty() || n > top_max)) {
     RbLevel l{};
     l.n = n;
@@ -93,12 +97,17 @@ RbPlan rb_plan(int nb, int m) {
   off += nt2;
   p.factor_doubles = off;
   p.set_blocks = (nb + 1) / 2;
-  // ping-pong sets, LU blocks, [I | SiM | SiP] right-hand sides, then the coarsest
-  // level's dense matrix and its identity right-hand side
+  // ping-pong sets, LU blocks, [I | SiM | SiP] right-hand sides, then (n > 1) the
+  // coarsest level's own red-black factor and workspace and its identity right-hand side
   p.work_doubles = 2 * 3 * p.set_blocks * m2 + (long long)p.max_odd * m2 +
-                   (long long)p.max_odd * 3 * m2 + 2 * nt2;
+                   (long long)p.max_odd * 3 * m2;
   p.piv_blocks = p.max_odd > n ? p.max_odd : n;
   p.work_ints = (long long)p.piv_blocks * m * 2 + p.piv_blocks;
+  if (n > 1) {
+    const RbPlan sp = rb_plan(n, m, 1);
+    p.work_doubles += sp.factor_doubles + sp.work_doubles + nt2;
+    if (sp.work_ints > p.work_ints) p.work_ints = sp.work_ints;
+  }
   return p;
 }
 
@@ -132,9 +141,15 @@ int rb_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
   return kOk;
 }
 
+// Whole red-black solve of a system planned with the given coarsest-level limit (defined
+// with the solve sweeps below; the factorization uses it for the coarsest level's inverse).
+template <typename T>
+static int rb_solve_sub(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
+                        hipStream_t st, int top_max = 1);
+
 template <typename E>
 static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E* factor, E* work,
-                       int* iwork, int* info, hipStream_t st) {
+                       int* iwork, int* info, hipStream_t st, int top_max = 0) {
   using GemmLaunch = GemmLaunchT<E>;
   using GemmDesc = GemmDescT<E>;
   SSLAB_REQUIRE(nb >= 1 && m >= 1, "rb_factor: nb and m must be positive");
@@ -142,7 +157,7 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
   // T == NULL: identity diagonal blocks (the oms S-system u_i + S u = rhs).  Level 0 then
   // needs no LU: T^{-1} = I, W = [SiM | SiP], T' = I - W^L SiP - W^R SiM.
   SSLAB_REQUIRE(nb == 1 || (SiM && SiP), "rb_factor: null off-diagonal blocks");
-  const RbPlan p = rb_plan(nb, m);
+  const RbPlan p = rb_plan(nb, m, top_max);
   const long long m2 = (long long)m * m;
   const long long setsz = 3 * p.set_blocks * m2;
   E* sets[2] = {work, work + setsz};
@@ -280,24 +295,28 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
     if (info) zero_int_async(info + info_base, 1, st);
     return set_identity_batched(m, factor + p.top_off, m, m2, 1, st);
   }
-  E* Iw = Dw + nt2;
-  const long long sdiag = (long long)(Nt + 1) * m;  // block (k, k) -> (k+1, k+1)
-  SSLAB_TRY(set_identity_batched(Nt, Dw, Nt, nt2, 1, st));
-  SSLAB_TRY(copy_block_batched(m, m, cT, m, m2, Dw, Nt, sdiag, nt, st));
-  if (nt > 1) {
-    SSLAB_TRY(copy_block_batched(m, m, cM + m2, m, m2, Dw + (long long)m * Nt, Nt, sdiag, nt - 1,
-                                 st));
-    SSLAB_TRY(copy_block_batched(m, m, cP, m, m2, Dw + m, Nt, sdiag, nt - 1, st));
+  if (nt == 1) {
+    SSLAB_TRY(copy_block_batched(m, m, cT, m, m2, LUw, m, m2, 1, st));
+    SSLAB_TRY(getrf_batched(m, LUw, m, m2, ipiv, m, linfo, 1, st));
+    if (info) {
+      SSLAB_HIP_CHECK(
+          hipMemcpyAsync(info + info_base, linfo, sizeof(int), hipMemcpyDeviceToDevice, st));
+    }
+    SSLAB_TRY(set_identity_batched(m, Rs, m, m2, 1, st));
+    return getrs_batched(m, m, LUw, m, m2, ipiv, m, Rs, m, m2, factor + p.top_off, m, m2, perm, 1,
+                         st);
   }
-  SSLAB_TRY(getrf_batched(Nt, Dw, Nt, nt2, ipiv, Nt, linfo, 1, st));
-  if (info) {
-    SSLAB_HIP_CHECK(
-        hipMemcpyAsync(info + info_base, linfo, sizeof(int), hipMemcpyDeviceToDevice, st));
-  }
+  // nt > 1: red-black factor of the nt-node system (its blocks are the current level set,
+  // stride m^2) down to one block -- its info entries fill this plan's remaining slots --
+  // then its solve for the identity gives the inverse.
+  const RbPlan sp = rb_plan(nt, m, 1);
+  E* subF = Dw;
+  E* subW = subF + sp.factor_doubles;
+  E* Iw = subW + sp.work_doubles;
+  SSLAB_TRY(rb_factor_t<E>(nt, m, cT, cM, cP, subF, subW, iwork, info ? info + info_base : nullptr,
+                           st, 1));
   SSLAB_TRY(set_identity_batched(Nt, Iw, Nt, nt2, 1, st));
-  SSLAB_TRY(getrs_batched(Nt, Nt, Dw, Nt, nt2, ipiv, Nt, Iw, Nt, nt2, factor + p.top_off, Nt, nt2,
-                          perm, 1, st));
-  return kOk;
+  return rb_solve_sub<E>(nt, m, Nt, subF, Iw, Nt, factor + p.top_off, Nt, st);
 }
 
 int rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
@@ -337,6 +356,7 @@ struct RbSolveSys {
   int phase;    // 0: whole solve; 1: forward sweep and coarsest level only, leaving in X
                 // the backward sweep's entry state (y_j at every level's odd nodes, x at
                 // the coarsest level's nodes); 2: backward sweep only, from that state
+  int top_max;  // the plan's coarsest-level limit (0: the default, rb_top_max(m))
 };
 
 // Append a descriptor, launching the pending ones first when the launch is full.  The
@@ -476,7 +496,7 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
   for (int s = 0; s < nsys; ++s) {
     RbSolveSys<T>& S = sys[s];
     SSLAB_REQUIRE(S.nb >= 1 && S.m >= 1 && S.r >= 0, "rb_solve: bad sizes (system %d)", s);
-    S.p = rb_plan(S.nb, S.m);
+    S.p = rb_plan(S.nb, S.m, S.top_max);
     if (S.r == 0) continue;  // nothing to solve (empty buffers may be null)
     const bool uses_v = S.phase != 2;
     SSLAB_REQUIRE(S.ldx >= S.r && (!uses_v || S.ldv >= S.r), "rb_solve: leading dimension < nrhs");
@@ -518,9 +538,16 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
 }
 
 template <typename T>
+static int rb_solve_sub(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
+                        hipStream_t st, int top_max) {
+  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, false, false, 0, top_max};
+  return rb_solve_multi_t<T>(1, &S, st);
+}
+
+template <typename T>
 static int rb_solve_t(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
                       hipStream_t st, bool ident0 = false, bool rb0 = false, int phase = 0) {
-  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0, rb0, phase};
+  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0, rb0, phase, 0};
   return rb_solve_multi_t<T>(1, &S, st);
 }
 
@@ -547,7 +574,7 @@ int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const do
   std::vector<RbSolveSys<double>> sys(nsys);
   for (int s = 0; s < nsys; ++s)
     sys[s] = RbSolveSys<double>{nb[s], m[s], r[s], factor[s], V[s], ldv[s], X[s], ldx[s], RbPlan{},
-                                false, false, 0};
+                                false, false, 0, 0};
   return rb_solve_multi_t<double>(nsys, sys.data(), st);
 }
 
