@@ -205,6 +205,15 @@ int sslab_hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double
                     long long sTa, const double* Tb, long long sTb, const int* ca,
                     const int* cb, const int* na, const int* nb_idx, double* Tnew,
                     long long sTnew, double* work, int* iwork, int* info, void* stream);
+/* sslab_hps_merge that also keeps every merge's solution operator
+ * Xsol_b = (T^a_cc + T^b_cc)^{-1} T_{c,new} (c x n_new, batch stride sXsol): the shared
+ * edge's values are -Xsol u_new, which the downward pass (interior reconstruction,
+ * solver/pde_solver.py:169-201 solve_dir) applies. */
+int sslab_hps_merge_ex(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta,
+                       long long sTa, const double* Tb, long long sTb, const int* ca,
+                       const int* cb, const int* na, const int* nb_idx, double* Tnew,
+                       long long sTnew, double* work, int* iwork, int* info, double* Xsol,
+                       long long sXsol, void* stream);
 
 /* Double-slab S maps: for pairs (slab i, slab i+1) of slab DtN maps Ts (n_s x n_s,
  * stride sTs), Y_i = (T^i_cc + T^{i+1}_cc)^{-1} [T^i(ca, cola) | T^{i+1}(cb, colb)]

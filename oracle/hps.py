@@ -354,3 +354,18 @@ def global_solution_on_lines(pdo, N, leaf_h, q, bc):
         vals.append(u[sl])
         pts.append(rect.XX[sl])
     return np.concatenate(vals), np.concatenate(pts)
+
+
+def global_solution(pdo, N, leaf_h, q, bc):
+    """Brute force: the single HPS system on all of [0,1]^2 with Dirichlet data bc, solved
+    by sparse LU; returns (values, points) at every node.  Pins interior reconstruction
+    (solver/pde_solver.py:169-201 solve_dir per slab): with the exact interface values,
+    each slab's interior solution is the global one restricted (exact domain
+    decomposition)."""
+    hx, hy = leaf_h
+    rect = HPSRect(pdo, 0.0, 0.0, hx, hy, int(round(1.0 / hx)), int(round(1.0 / hy)), q)
+    A = rect.assemble().tocsc()
+    outer = rect.outer_edge_nodes()
+    rhs = np.zeros(rect.n)
+    rhs[outer] = bc(rect.XX[outer])
+    return spla.spsolve(A, rhs), rect.XX

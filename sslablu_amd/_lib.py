@@ -60,6 +60,8 @@ _SIGS = {
     "sslab_hps_merge_sizes": [_c_int, _c_int, _c_int, _vp, _vp],
     "sslab_hps_merge": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll,
                         _vp, _vp, _vp, _vp, _vp, _c_ll, _vp, _vp, _vp, _vp],
+    "sslab_hps_merge_ex": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll,
+                           _vp, _vp, _vp, _vp, _vp, _c_ll, _vp, _vp, _vp, _vp, _c_ll, _vp],
     "sslab_hps_pair_solve": [_c_int, _c_int, _c_int, _c_int, _vp, _c_ll, _vp, _vp, _vp, _vp,
                              _vp, _vp, _vp, _vp, _vp],
     "sslab_oms_rhs": [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp,

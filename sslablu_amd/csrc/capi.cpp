@@ -199,6 +199,15 @@ int sslab_hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double
                           sTnew, work, iwork, info, as_stream(stream));
 }
 
+int sslab_hps_merge_ex(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta,
+                       long long sTa, const double* Tb, long long sTb, const int* ca,
+                       const int* cb, const int* na, const int* nb_idx, double* Tnew,
+                       long long sTnew, double* work, int* iwork, int* info, double* Xsol,
+                       long long sXsol, void* stream) {
+  return sslab::hps_merge(nmerge, n_a, n_b, n_new, c, Ta, sTa, Tb, sTb, ca, cb, na, nb_idx, Tnew,
+                          sTnew, work, iwork, info, as_stream(stream), Xsol, sXsol);
+}
+
 int sslab_hps_pair_solve(int npair, int n_s, int c, int ncols, const double* Ts, long long sTs,
                          const int* ca, const int* cb, const int* cola, const int* colb, double* Y,
                          double* work, int* iwork, int* info, void* stream) {

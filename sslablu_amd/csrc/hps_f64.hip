@@ -149,7 +149,7 @@ int hps_merge_sizes(int nmerge, int n_new, int c, long long* work_doubles, long 
 int hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta, long long sTa,
               const double* Tb, long long sTb, const int* ca, const int* cb, const int* na,
               const int* nbix, double* Tnew, long long sTnew, double* work, int* iwork, int* info,
-              hipStream_t st) {
+              hipStream_t st, double* Xsol, long long sXsol) {
   SSLAB_REQUIRE(nmerge >= 1 && n_a >= 1 && n_b >= 1 && n_new >= 1 && c >= 1,
                 "hps_merge: bad sizes");
   SSLAB_REQUIRE(Ta && Tb && ca && cb && na && nbix && Tnew && work && iwork,
@@ -157,8 +157,11 @@ int hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta, 
   const long long c2 = (long long)c * c, cn = (long long)c * n_new;
   double* Tcc = work;
   double* R = Tcc + nmerge * c2;
-  double* X = R + nmerge * cn;
-  double* E = X + nmerge * cn;
+  // X = (T^a_cc + T^b_cc)^{-1} [T_{c,new}]: the shared edge's values are -X u_new
+  // (kept in Xsol, stride sXsol, when the caller wants the merge's solution operator)
+  double* X = Xsol ? Xsol : R + nmerge * cn;
+  const long long sX = Xsol ? sXsol : cn;
+  double* E = R + 2 * nmerge * cn;
   int* ipiv = iwork;
   int* perm = iwork + (long long)nmerge * c;
   int* linfo = info ? info : perm + (long long)nmerge * c;
@@ -170,9 +173,9 @@ int hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta, 
   SSLAB_TRY(gather2(nmerge, n_new, n_new, Ta, n_a, sTa, Tb, n_b, sTb, na, na, nbix, nbix, Tnew,
                     n_new, sTnew, st));
   SSLAB_TRY(getrf_batched(c, Tcc, c, c2, ipiv, c, linfo, nmerge, st));
-  SSLAB_TRY(getrs_batched(c, n_new, Tcc, c, c2, ipiv, c, R, n_new, cn, X, n_new, cn, perm, nmerge,
+  SSLAB_TRY(getrs_batched(c, n_new, Tcc, c, c2, ipiv, c, R, n_new, cn, X, n_new, sX, perm, nmerge,
                           st));
-  SSLAB_TRY(gemm_strided(0, 0, n_new, n_new, c, -1.0, E, c, cn, X, n_new, cn, 1.0, Tnew, n_new,
+  SSLAB_TRY(gemm_strided(0, 0, n_new, n_new, c, -1.0, E, c, cn, X, n_new, sX, 1.0, Tnew, n_new,
                          sTnew, nmerge, st));
   return kOk;
 }

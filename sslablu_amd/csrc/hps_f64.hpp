@@ -20,7 +20,7 @@ int hps_merge_sizes(int nmerge, int n_new, int c, long long* work_doubles, long 
 int hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta, long long sTa,
               const double* Tb, long long sTb, const int* ca, const int* cb, const int* na,
               const int* nbix, double* Tnew, long long sTnew, double* work, int* iwork, int* info,
-              hipStream_t st);
+              hipStream_t st, double* Xsol = nullptr, long long sXsol = 0);
 
 int hps_pair_solve(int npair, int n_s, int c, int ncols, const double* Ts, long long sTs,
                    const int* ca, const int* cb, const int* cola, const int* colb, double* Y,

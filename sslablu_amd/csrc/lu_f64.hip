@@ -13,6 +13,8 @@
 // blocks whose off-block updates are MFMA GEMMs with K = 256.
 #include "lu_f64.hpp"
 
+#include <cstdlib>
+
 #include "gemm.hpp"
 
 namespace sslab {
@@ -33,10 +35,11 @@ __global__ void k_zero_int(int* p, int n) {
 // Factor panel columns [j0, j0+w) over rows [j0, n) of every batch matrix.
 // kLds: the panel is staged column-major in dynamic LDS (rows x w elements);
 // otherwise it is factored in place in global memory.
-template <typename E, bool kLds>
-__global__ void __launch_bounds__(kPanelThreads)
+template <typename E, bool kLds, int NT = kPanelThreads>
+__global__ void __launch_bounds__(NT)
 k_getrf_panel(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
               int* __restrict__ ipiv, long long sP, int* __restrict__ info) {
+  constexpr int kPanelThreads = NT;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   E* smem = reinterpret_cast<E*>(smem_raw);
   __shared__ E red_v[kPanelThreads / 64];
@@ -376,8 +379,22 @@ int panel_rec(const PanelCtx<E>& c, int j0, int w) {
   if (w <= (bw > 0 ? bw : 32)) {
     if (bw > 0) {
       const size_t shm = (size_t)rows * w * sizeof(E);
-      hipLaunchKernelGGL((k_getrf_panel<E, true>), dim3(c.batch), dim3(kPanelThreads), shm, c.st, c.n,
-                         j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      static const int pnt = [] {
+        const char* e = getenv("SSLAB_PANEL_NT");
+        return e ? atoi(e) : kPanelThreads;
+      }();
+      if (pnt == 64)
+        hipLaunchKernelGGL((k_getrf_panel<E, true, 64>), dim3(c.batch), dim3(64), shm, c.st, c.n,
+                           j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      else if (pnt == 128)
+        hipLaunchKernelGGL((k_getrf_panel<E, true, 128>), dim3(c.batch), dim3(128), shm, c.st,
+                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      else if (pnt == 512)
+        hipLaunchKernelGGL((k_getrf_panel<E, true, 512>), dim3(c.batch), dim3(512), shm, c.st,
+                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      else
+        hipLaunchKernelGGL((k_getrf_panel<E, true>), dim3(c.batch), dim3(kPanelThreads), shm, c.st, c.n,
+                           j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
     } else {
       hipLaunchKernelGGL((k_getrf_panel<E, false>), dim3(c.batch), dim3(kPanelThreads), 0, c.st, c.n,
                          j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
@@ -424,6 +441,12 @@ int getrf_batched(int n, E* A, int lda, long long sA, int* ipiv, long long sP, i
   c.st = stream;
   c.lds = max_lds_bytes() - 512;  // room for the static reduction scratch
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_getrf_panel<E, true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_getrf_panel<E, true, 64>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_getrf_panel<E, true, 128>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_getrf_panel<E, true, 512>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, c.lds);
   for (int j0 = 0; j0 < n; j0 += kOuterNB) {
     const int w = (n - j0) < kOuterNB ? (n - j0) : kOuterNB;

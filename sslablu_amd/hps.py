@@ -207,6 +207,21 @@ def _coef(fn, P):
     return v.reshape(-1)
 
 
+def _child_maps(ca, cb, na, nbm, n_old, dev):
+    """Gather maps of the downward pass through one merge level: child alpha's boundary
+    node k is parent boundary node inv_na[k] or shared-edge node inv_ca[k] (the other
+    index -1); likewise beta."""
+    def inv(idx):
+        out = -np.ones(n_old, dtype=np.int64)
+        for i, k in enumerate(idx):
+            if k >= 0:
+                out[k] = i
+        return out
+    ia, iac, ib, ibc = inv(na), inv(ca), inv(nbm), inv(cb)
+    assert np.all((ia >= 0) != (iac >= 0)) and np.all((ib >= 0) != (ibc >= 0))
+    return [dev(x, I32) for x in (ia, iac, ib, ibc)]
+
+
 class HPSSlabOperator:
     """Device-resident S-system of the oms method for one PDE on one SlabGrid2D.
 
@@ -215,14 +230,18 @@ class HPSSlabOperator:
     blocks, and rhs_from_boundary() to map boundary data to S-system right-hand sides.
     """
 
-    def __init__(self, pdo, grid: SlabGrid2D, device=None, keep_leaf_ops=False):
+    def __init__(self, pdo, grid: SlabGrid2D, device=None, keep_leaf_ops=False,
+                 keep_solution_ops=False):
         _lib.require_gpu()
         _lib.load()
         self.pdo = pdo
         self.g = grid
         self.device = torch.device(device) if device is not None else torch.device(
             "cuda", torch.cuda.current_device())
-        self.keep_leaf_ops = keep_leaf_ops
+        # the leaf and merge solution operators are what solve_dir needs
+        self.keep_solution_ops = keep_solution_ops
+        self.keep_leaf_ops = keep_leaf_ops or keep_solution_ops
+        self._down = []
         self.info_bad = 0
 
     # -------------------------------------------------------------------------
@@ -259,6 +278,7 @@ class HPSSlabOperator:
     def merge_tree(self, T, stream=None):
         g = self.g
         p = g.p
+        self._down = []
         nxb, nyb = 1, 1
         boxes = T
         while nxb < g.nx or nyb < g.ny:
@@ -274,9 +294,19 @@ class HPSSlabOperator:
             iwork = torch.empty(wi, dtype=I32, device=self.device)
             info = torch.zeros(nmerge, dtype=I32, device=self.device)
             s_old = n_old * n_old
-            call("sslab_hps_merge", nmerge, n_old, n_old, n_new, c, ptr(boxes), 2 * s_old,
-                 ptr(boxes) + 8 * s_old, 2 * s_old, *[ptr(x) for x in maps], ptr(out),
-                 n_new * n_new, ptr(work), ptr(iwork), ptr(info), stream_handle(stream))
+            if self.keep_solution_ops:
+                Xs = torch.empty(nmerge, c, n_new, dtype=F64, device=self.device)
+                call("sslab_hps_merge_ex", nmerge, n_old, n_old, n_new, c, ptr(boxes),
+                     2 * s_old, ptr(boxes) + 8 * s_old, 2 * s_old, *[ptr(x) for x in maps],
+                     ptr(out), n_new * n_new, ptr(work), ptr(iwork), ptr(info), ptr(Xs),
+                     c * n_new, stream_handle(stream))
+                self._down.append(dict(X=Xs, n_old=n_old, n_new=n_new, c=c,
+                                       maps=_child_maps(ca, cb, na, nbm, n_old, self._dev)))
+            else:
+                call("sslab_hps_merge", nmerge, n_old, n_old, n_new, c, ptr(boxes),
+                     2 * s_old, ptr(boxes) + 8 * s_old, 2 * s_old, *[ptr(x) for x in maps],
+                     ptr(out), n_new * n_new, ptr(work), ptr(iwork), ptr(info),
+                     stream_handle(stream))
             self._infos.append(info)
             boxes = out
             nxb, nyb = new
@@ -352,6 +382,81 @@ class HPSSlabOperator:
         return out
 
     # -------------------------------------------------------------------------
+    def solve_dir(self, x_iface: torch.Tensor, F: torch.Tensor, ff_body=None, stream=None):
+        """Solution at every leaf's interior Chebyshev nodes (rows in the order of
+        SlabGrid2D.interior_points()) from the interface solution x_iface (nb*m x r, as
+        the S-system solve returns it) and the global boundary data F (boundary-node
+        order of SlabGrid2D.gb_points()): the reference's solve_dir
+        (solver/pde_solver.py:169-201, u_i = A_ii^{-1}(ff_body - A_ix u_b)) per slab, by
+        the HPS downward pass -- shared-edge values -X u at every merge (top down), then
+        -X_leaf u_b at the leaves; batched GEMMs and gathers only.  Needs
+        keep_solution_ops=True at construction.  Body loads (ff_body) are not supported:
+        the leaf and merge operators here are those of the homogeneous equation."""
+        if ff_body is not None:
+            raise NotImplementedError("solve_dir: body loads are not supported")
+        if not self.keep_solution_ops or self.leaf_X is None or (
+                len(self._down) == 0 and (self.g.nx > 1 or self.g.ny > 1)):
+            raise ValueError("solve_dir: build the operator with keep_solution_ops=True")
+        from .hbs import bgemm
+        g = self.g
+        m, nb, p, N = g.m, g.nb, g.p, g.N
+        nxp = g.nx * p
+        one_d = x_iface.dim() == 1
+        X = x_iface.reshape(x_iface.shape[0], -1).to(self.device, F64).contiguous()
+        if F.dim() == 1:
+            F = F[:, None]
+        F = F.to(self.device, F64).contiguous()
+        r = X.shape[1]
+        ngb = 2 * N * nxp + 2 * m
+        if X.shape[0] != nb * m or F.shape != (ngb, r):
+            raise ValueError(f"solve_dir: x_iface ({nb * m}, r) and F ({ngb}, r) expected")
+        dev = self.device
+        ident = torch.arange(r, dtype=I32, device=dev)
+        n_s = 2 * m + 2 * nxp
+        U = torch.empty(N, n_s, r, dtype=F64, device=dev)
+        s = stream_handle(stream)
+        i = np.arange(n_s)
+        bot = (i >= 2 * m) & (i < 2 * m + nxp)
+        top = i >= 2 * m + nxp
+        sref = np.where(bot, i - 2 * m, np.where(top, N * nxp + i - 2 * m - nxp, -1))
+
+        keep = []  # index arrays stay referenced until every gather is enqueued
+
+        def gather(out, batch, s0, rid0, s1, rid1, s1_stride):
+            d0, d1 = self._dev(rid0, I32), self._dev(rid1, I32)
+            keep.extend((d0, d1))
+            call("sslab_gather2", batch, n_s, r, s0, r, m * r, s1, r, s1_stride,
+                 ptr(d0), ptr(ident), ptr(d1), ptr(ident), out, r, n_s * r, s)
+
+        # slab boundary [left | right | bottom | top]: interfaces from X, the rest from F
+        if N > 2:  # slabs 1 .. N-2: left = interface s-1, right = interface s
+            gather(ptr(U) + 8 * n_s * r, N - 2, ptr(X), np.where(i < 2 * m, i, -1),
+                   ptr(F) + 8 * nxp * r, np.where(bot | top, sref, -1), nxp * r)
+        left_f = np.where(i < m, 2 * N * nxp + i, -1)
+        gather(ptr(U), 1, ptr(X) - 8 * m * r, np.where((i >= m) & (i < 2 * m), i, -1),
+               ptr(F), np.where(bot | top, sref, left_f), 0)
+        right_f = np.where((i >= m) & (i < 2 * m), 2 * N * nxp + i, -1)
+        last = N - 1
+        sref_last = np.where(bot, sref + last * nxp, np.where(top, sref + last * nxp, -1))
+        gather(ptr(U) + 8 * last * n_s * r, 1, ptr(X) + 8 * (last - 1) * m * r,
+               np.where(i < m, i, -1), ptr(F),
+               np.where(bot | top, sref_last, right_f), 0)
+        # downward pass through the merge levels, top down
+        for lvl in reversed(self._down):
+            Xs, n_old, c = lvl["X"], lvl["n_old"], lvl["c"]
+            nmerge = Xs.shape[0]
+            Uc = bgemm(Xs, U, alpha=-1.0)                     # shared-edge values
+            Uch = torch.empty(2 * nmerge, n_old, r, dtype=F64, device=dev)
+            ia, iac, ib, ibc = lvl["maps"]
+            for half, (r0, r1) in enumerate(((ia, iac), (ib, ibc))):
+                call("sslab_gather2", nmerge, n_old, r, ptr(U), r, U.stride(0), ptr(Uc), r,
+                     Uc.stride(0), ptr(r0), ptr(ident), ptr(r1), ptr(ident),
+                     ptr(Uch) + 8 * half * n_old * r, r, 2 * n_old * r, s)
+            U = Uch
+        out = bgemm(self.leaf_X, U, alpha=-1.0)               # leaf interiors
+        out = out.reshape(-1, r)
+        return out[:, 0] if one_d else out
+
     def fold(self, fac, stream=None) -> "BoundaryFold":
         """The forward sweep of `fac` (red-black, T = None, built from this operator's
         SiM / SiP) folded into boundary maps: see BoundaryFold."""

@@ -209,3 +209,53 @@ def test_boundary_fold_exact_and_config2(T):
     fac = BlockTriFactor("rb", op.device).factorize(None, op.SiM, op.SiP)
     u = op.fold(fac).solve(T.tensor(u_ex(g.gb_points()), device=op.device))
     assert np.max(np.abs(u.cpu().numpy()[:, 0] - u_ex(g.interface_points()))) < 1e-9
+
+
+def _match(P, Q):
+    """Index of every row of P in Q (both point lists; coordinates agree to 1e-9)."""
+    from scipy.spatial import cKDTree
+    d, idx = cKDTree(Q).query(P)
+    assert np.max(d) < 1e-9
+    return idx
+
+
+def test_solve_dir_exact_and_global(T):
+    """Interior reconstruction (the reference's solve_dir by the HPS downward pass):
+    config 1's u = exp(x) sin(y) at every leaf-interior node; the variable-coefficient
+    Helmholtz operator against the oracle's brute-force global HPS solve at the same
+    nodes (exact domain decomposition); several columns at once equal one at a time;
+    errors when the operators were not kept or a body load is given."""
+    from sslablu_amd.blocktri import BlockTriFactor
+    from sslablu_amd.hps import HPSSlabOperator, SlabGrid2D
+    op = HPSSlabOperator(LAPLACE, SlabGrid2D(8, 2, 16, 12), keep_solution_ops=True).build()
+    g = op.g
+    F = T.tensor(exp_sin(g.gb_points()), device=op.device)
+    fac = BlockTriFactor("rb", op.device).factorize(None, op.SiM, op.SiP)
+    x = op.fold(fac).solve(F)
+    u = op.solve_dir(x, F).cpu().numpy()[:, 0]
+    assert u.shape[0] == g.interior_points().shape[0]
+    assert np.max(np.abs(u - exp_sin(g.interior_points()))) < 1e-10
+
+    pd = helmholtz(6.0)
+    op = HPSSlabOperator(pd, SlabGrid2D(4, 2, 2, 8), keep_solution_ops=True).build()
+    g = op.g
+    bc = lambda P: np.cos(P[:, 0] + 2 * P[:, 1]) + P[:, 0] ** 2
+    F1 = T.tensor(bc(g.gb_points()), device=op.device)
+    fac = BlockTriFactor("rb", op.device).factorize(None, op.SiM, op.SiP)
+    u = op.solve_dir(op.fold(fac).solve(F1)[:, 0], F1).cpu().numpy()
+    assert u.ndim == 1
+    ref, XX = oh.global_solution(pd, 4, (g.hx, g.hy), 8, bc)
+    u_ref = ref[_match(g.interior_points(), XX)]
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-10
+    rng = np.random.default_rng(3)
+    F3 = T.tensor(rng.standard_normal((g.gb_points().shape[0], 3)), device=op.device)
+    x3 = fac.solve(op.rhs_from_boundary(F3))
+    U3 = op.solve_dir(x3, F3)
+    for k in range(3):
+        uk = op.solve_dir(x3[:, k], F3[:, k])
+        assert (T.linalg.norm(U3[:, k] - uk) / T.linalg.norm(uk)).item() < 1e-13
+    with pytest.raises(NotImplementedError):
+        op.solve_dir(x3, F3, ff_body=np.zeros(3))
+    plain = HPSSlabOperator(pd, SlabGrid2D(4, 2, 2, 8)).build()
+    with pytest.raises(ValueError):
+        plain.solve_dir(x3, F3)
