@@ -298,6 +298,56 @@ def c4_bench(dev, seed=0, reps=5):
     return out
 
 
+def interior_bench(dev, reps=10):
+    """Interior reconstruction (solve_dir) on the headline workload: interface solution
+    of 64 RHS -> the solution at every leaf-interior node (32 x 32 leaves per slab side,
+    256 nodes each), by the HPS downward pass.  Reports the time beside the solve's and
+    the error of a plane wave (constant kappa = 20, closed form) at all those nodes."""
+    import torch
+    from sslablu_amd.blocktri import BlockTriFactor
+    from sslablu_amd.hps import HPSSlabOperator, SlabGrid2D
+    t0 = time.perf_counter()
+    op = HPSSlabOperator(helmholtz_pdo(), SlabGrid2D(N_SLABS, NX, NY, Q), dev,
+                         keep_solution_ops=True).build()
+    fac = BlockTriFactor("rb", dev).factorize(None, op.SiM, op.SiP)
+    fold = op.fold(fac)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    g = op.g
+    rng = np.random.default_rng(SEED)
+    F = torch.tensor(rng.standard_normal((g.gb_points().shape[0], NRHS)), device=dev)
+    X = fold.solve(F)
+    op.solve_dir(X, F)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        op.solve_dir(X, F)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    # accuracy on a closed form: plane wave of -Lap u - 400 u = 0
+    kappa, th = 20.0, 0.7
+
+    class _Const:
+        c11 = c22 = staticmethod(lambda P: np.ones(P.shape[0]))
+        c1 = c2 = c12 = None
+        c = staticmethod(lambda P: -kappa ** 2 * np.ones(P.shape[0]))
+
+    opw = HPSSlabOperator(_Const(), SlabGrid2D(N_SLABS, NX, NY, Q), dev,
+                          keep_solution_ops=True).build()
+    facw = BlockTriFactor("rb", dev).factorize(None, opw.SiM, opw.SiP)
+    u_ex = lambda P: np.sin(kappa * (P[:, 0] * np.cos(th) + P[:, 1] * np.sin(th)))
+    Fw = torch.tensor(u_ex(opw.g.gb_points()), device=dev)
+    uw = opw.solve_dir(opw.fold(facw).solve(Fw), Fw).cpu().numpy()[:, 0]
+    err = float(np.max(np.abs(uw - u_ex(opw.g.interior_points()))))
+    nodes = g.interior_points().shape[0]
+    return {"workload": f"{NRHS} RHS -> solution at all {nodes} leaf-interior nodes",
+            "build_s (with solution operators)": round(t_build, 4),
+            "solve_dir_ms": round(ms, 3), "node_values_per_s": round(nodes * NRHS / ms * 1e3),
+            "plane_wave_max_err": err}
+
+
 def c3_bench(dev, seed=0, rk=40, nrhs=16):
     """3-D Poisson on [0,1]^3: 16 slabs, every double slab one 9 x 98 x 98 Chebyshev grid,
     so each interface carries m = 96^2 = 9216 nodes; S blocks HBS-compressed on a 128-leaf
@@ -401,6 +451,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-interior", action="store_true")
     ap.add_argument("--policy", default=None,
                     help="fixed GEMM tile policy (7 comma-separated ints, as reported in "
                          "gemm_policy) instead of autotuning; for profiler runs")
@@ -614,6 +665,8 @@ def main():
         out["c4_blocktri"] = c4_bench(dev)
     if not args.no_c3:
         out["c3_spectral3d_hbs"] = c3_bench(dev)
+    if not args.no_interior:
+        out["interior_reconstruction"] = interior_bench(dev)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(op, F_host, args.cpu_budget)
         out["cpu_baseline"] = {
