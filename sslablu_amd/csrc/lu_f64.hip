@@ -138,6 +138,123 @@ k_getrf_panel(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
   }
 }
 
+// Base panel with its rows in registers: thread t holds panel rows t + k NT (k < R) over
+// the w <= 32 panel columns, so the rank-1 updates are register FMAs; per column the
+// pivot search is a wave shuffle plus one LDS exchange, and the pivot row (after the
+// interchange) is broadcast through LDS.  Same pivot choice as the LDS panel (largest
+// |a|, first index on ties; an all-NaN column keeps its diagonal).
+template <typename E, int NT, int R>
+__global__ void __launch_bounds__(NT)
+k_getrf_panel_reg(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
+                  int* __restrict__ ipiv, long long sP, int* __restrict__ info) {
+  constexpr int W = 32;
+  __shared__ E prow[W];  // pivot row: the row that becomes row c
+  __shared__ E crow[W];  // the old row c, which moves to the pivot's place
+  __shared__ E red_v[NT / 64];
+  __shared__ int red_i[NT / 64];
+  const int b = blockIdx.x;
+  E* Ab = A + (long long)b * sA + (long long)j0 * lda + j0;
+  int* pv = ipiv + (long long)b * sP;
+  const int rows = n - j0;
+  const int tid = threadIdx.x;
+  E P[R][W];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int i = tid + k * NT;
+#pragma unroll
+    for (int j = 0; j < W; ++j) P[k][j] = (i < rows && j < w) ? Ab[(long long)i * lda + j] : E(0);
+  }
+#pragma clang loop unroll(full)
+  for (int c = 0; c < W; ++c) {
+    if (c >= w) continue;  // uniform; a full unroll keeps every P index static
+    E best = -1.0;
+    int bi = rows;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int i = tid + k * NT;
+      const E v = abs_e(P[k][c]);
+      if (i >= c && i < rows && v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const E ov = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > best || (ov == best && oi < bi)) {
+        best = ov;
+        bi = oi;
+      }
+    }
+    if ((tid & 63) == 0) {
+      red_v[tid >> 6] = best;
+      red_i[tid >> 6] = bi;
+    }
+    __syncthreads();
+    E bv = red_v[0];
+    int p = red_i[0];
+#pragma unroll
+    for (int q = 1; q < NT / 64; ++q) {
+      if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < p)) {
+        bv = red_v[q];
+        p = red_i[q];
+      }
+    }
+    if (p >= rows) p = c;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int i = tid + k * NT;
+      if (i == p) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) prow[j] = P[k][j];
+      }
+      if (i == c) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) crow[j] = P[k][j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int i = tid + k * NT;
+      if (i == c) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) P[k][j] = prow[j];
+      } else if (i == p) {
+#pragma unroll
+        for (int j = 0; j < W; ++j) P[k][j] = crow[j];
+      }
+    }
+    if (tid == 0) pv[j0 + c] = j0 + p;
+    const E piv = prow[c];
+    if (piv == 0.0 && tid == 0 && info != nullptr && info[b] == 0) info[b] = j0 + c + 1;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int i = tid + k * NT;
+      if (i > c && i < rows) {
+        E l = P[k][c];
+        if (piv != 0.0) {
+          l = l / piv;
+          P[k][c] = l;
+        }
+#pragma unroll
+        for (int j = c + 1; j < W; ++j) P[k][j] -= l * prow[j];
+      }
+    }
+    __syncthreads();  // prow / crow / red_* are rewritten by the next column
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int i = tid + k * NT;
+    if (i < rows) {
+#pragma unroll
+      for (int j = 0; j < W; ++j)
+        if (j < w) Ab[(long long)i * lda + j] = P[k][j];
+    }
+  }
+}
+
 // Apply the interchanges of rows r0..r0+w-1 (in order) to the columns
 // [a0, a1) U [b0, b1).
 template <typename E>
@@ -377,6 +494,26 @@ int panel_rec(const PanelCtx<E>& c, int j0, int w) {
   const int rows = c.n - j0;
   const int bw = base_width<E>(c, rows);
   if (w <= (bw > 0 ? bw : 32)) {
+    static const bool reg = [] {
+      const char* e = getenv("SSLAB_PANEL_REG");
+      return !(e && e[0] == '0');
+    }();
+    if (reg && w <= 32 && rows <= 1024) {
+      if (rows <= 64)
+        hipLaunchKernelGGL((k_getrf_panel_reg<E, 64, 1>), dim3(c.batch), dim3(64), 0, c.st, c.n,
+                           j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      else if (rows <= 256)
+        hipLaunchKernelGGL((k_getrf_panel_reg<E, 256, 1>), dim3(c.batch), dim3(256), 0, c.st,
+                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      else if (rows <= 512)
+        hipLaunchKernelGGL((k_getrf_panel_reg<E, 512, 1>), dim3(c.batch), dim3(512), 0, c.st,
+                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      else
+        hipLaunchKernelGGL((k_getrf_panel_reg<E, 1024, 1>), dim3(c.batch), dim3(1024), 0, c.st,
+                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      SSLAB_HIP_CHECK(hipGetLastError());
+      return kOk;
+    }
     if (bw > 0) {
       const size_t shm = (size_t)rows * w * sizeof(E);
       static const int pnt = [] {
