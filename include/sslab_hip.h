@@ -268,6 +268,20 @@ long long sslab_oms_fold_bytes(const void* plan);
 int sslab_oms_fold_groups(const void* plan);
 void sslab_oms_fold_destroy(void* plan);
 
+/* Column-batched vector operations of the device GMRES (sslablu_amd/krylov.py), the
+ * outer iteration the reference runs with scipy's gmres and the direct solver as
+ * preconditioner (cubeTest.py:163-186).  A panel W (n x r, leading dim ldw) holds r
+ * independent vectors; a basis V is k panels at stride sV (elements).
+ *   sslab_col_dots:  H[j][c] = sum_i V_j[i][c] W[i][c]   (j < k; H is k x r, ldh)
+ *   sslab_col_axpy:  W[i][c] += alpha sum_j V_j[i][c] H[j][c]
+ *   sslab_col_scale: Out[i][c] = W[i][c] s[c]           (s: r device values) */
+int sslab_col_dots(int n, int r, int k, const double* V, long long sV, int ldv,
+                   const double* W, int ldw, double* H, int ldh, void* stream);
+int sslab_col_axpy(int n, int r, int k, double alpha, const double* V, long long sV, int ldv,
+                   const double* H, int ldh, double* W, int ldw, void* stream);
+int sslab_col_scale(int n, int r, const double* W, int ldw, const double* s, double* Out,
+                    int ldo, void* stream);
+
 /* out[b][i][j] = sum over s in {0,1} of src_s[b][rid_s[i]][cid_s[j]] where both
  * indices are >= 0 (index arrays may be NULL = absent). */
 int sslab_gather2(int batch, int nrows, int ncols, const double* s0, int ld0, long long st0,

@@ -73,6 +73,11 @@ _SIGS = {
     "sslab_oms_fold_bytes": [_vp],
     "sslab_oms_fold_groups": [_vp],
     "sslab_oms_fold_destroy": [_vp],
+    "sslab_col_dots": [_c_int, _c_int, _c_int, _vp, _c_ll, _c_int, _vp, _c_int, _vp, _c_int,
+                       _vp],
+    "sslab_col_axpy": [_c_int, _c_int, _c_int, ctypes.c_double, _vp, _c_ll, _c_int, _vp,
+                       _c_int, _vp, _c_int, _vp],
+    "sslab_col_scale": [_c_int, _c_int, _vp, _c_int, _vp, _vp, _c_int, _vp],
     "sslab_gather2": [_c_int, _c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_int, _c_ll,
                       _vp, _vp, _vp, _vp, _vp, _c_int, _c_ll, _vp],
 }

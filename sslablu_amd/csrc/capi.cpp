@@ -5,6 +5,7 @@
 #include "blocktri_f64.hpp"
 #include "gemm.hpp"
 #include "hps_f64.hpp"
+#include "krylov_f64.hpp"
 #include "lu_f64.hpp"
 #include "oms_f64.hpp"
 #include "oms_fold.hpp"
@@ -251,6 +252,21 @@ int sslab_oms_fold_groups(const void* plan) {
 }
 
 void sslab_oms_fold_destroy(void* plan) { sslab::oms_fold_destroy(static_cast<sslab::FoldPlan*>(plan)); }
+
+int sslab_col_dots(int n, int r, int k, const double* V, long long sV, int ldv,
+                   const double* W, int ldw, double* H, int ldh, void* stream) {
+  return sslab::col_dots(n, r, k, V, sV, ldv, W, ldw, H, ldh, as_stream(stream));
+}
+
+int sslab_col_axpy(int n, int r, int k, double alpha, const double* V, long long sV, int ldv,
+                   const double* H, int ldh, double* W, int ldw, void* stream) {
+  return sslab::col_axpy(n, r, k, alpha, V, sV, ldv, H, ldh, W, ldw, as_stream(stream));
+}
+
+int sslab_col_scale(int n, int r, const double* W, int ldw, const double* s, double* Out,
+                    int ldo, void* stream) {
+  return sslab::col_scale(n, r, W, ldw, s, Out, ldo, as_stream(stream));
+}
 
 int sslab_gather2(int batch, int nrows, int ncols, const double* s0, int ld0, long long st0,
                   const double* s1, int ld1, long long st1, const int* rid0, const int* cid0,
