@@ -437,6 +437,20 @@ def c3_bench(dev, seed=0, rk=40, nrhs=16):
     out["dense_solve_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
     out["dense_factor_gb"] = round(fac.nbytes / 1e9, 2)
     out["hbs_vs_dense_rel_err"] = ((x_hbs - Xd).norm() / Xd.norm()).item()
+    # the reference's closing step (cubeTest.py:163-186): GMRES on the exact S-system with
+    # the compressed direct solver as preconditioner
+    from sslablu_amd.krylov import gmres, tridiag_apply
+    A = tridiag_apply(SiM, SiP, nb, s.m)
+    M = lambda v: torch.as_tensor(solver.solve(v), device=dev).contiguous()  # noqa: E731
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Xg, info = gmres(A, B, apply_M=M, tol=1e-10, restart=30, maxiter=60)
+    torch.cuda.synchronize()
+    out["gmres_hbs_preconditioned"] = {
+        "iterations": info.iterations, "converged": bool(info.converged),
+        "time_s": round(time.perf_counter() - t0, 3),
+        "rel_err_vs_dense": ((Xg - Xd).norm() / Xd.norm()).item(),
+        "residual_history": [float(f"{h:.2e}") for h in info.history]}
     return out
 
 
