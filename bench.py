@@ -352,7 +352,8 @@ def c3_bench(dev, seed=0, rk=40, nrhs=16):
     """3-D Poisson on [0,1]^3: 16 slabs, every double slab one 9 x 98 x 98 Chebyshev grid,
     so each interface carries m = 96^2 = 9216 nodes; S blocks HBS-compressed on a 128-leaf
     face tree (72 points per leaf, rank <= 40, s = max(2k, leaf) + k + 10 seeded Gaussian
-    sketch columns), ULV-factored HBS Thomas elimination, 16 right-hand sides of Dirichlet
+    sketch columns, also for every S'_i of the elimination), ULV-factored HBS Thomas
+    elimination, 16 right-hand sides of Dirichlet
     data sum_j a_j sin(pi k_j . x), k_j in {1,2,3}^3 (8 modes, seeded); checked against the
     dense FP64 red-black solve of the same S-system."""
     import torch
@@ -409,7 +410,7 @@ def c3_bench(dev, seed=0, rk=40, nrhs=16):
     nb = s.nb
     S_rk = [[H["right"]]] + [[H["left"], H["right"]] for _ in range(nb - 2)] + [[H["left"]]]
     t0 = time.perf_counter()
-    solver = ThomasSolverHBS(s.m, rk)
+    solver = ThomasSolverHBS(s.m, rk, samples=ns)  # S'_i sketched like the S blocks
     solver.factorize(S_rk)
     torch.cuda.synchronize()
     out["hbs_factor_s"] = round(time.perf_counter() - t0, 3)

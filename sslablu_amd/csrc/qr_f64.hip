@@ -1,6 +1,6 @@
 // Batched FP64 Householder QR (optionally column-pivoted) for gfx950.
 //
-// One workgroup (4 wavefronts) factors one small matrix held in global memory (the
+// One workgroup (16 wavefronts) factors one small matrix held in global memory (the
 // HBS boxes are at most a few hundred rows, so a matrix stays L2-resident): per
 // column j, an optional pivot search over the remaining column norms, the
 // reflector H_j = I - tau v v^T (LAPACK dlarfg convention, v_0 = 1) built in LDS,
@@ -11,11 +11,13 @@
 // A_in[:, perm] = Q R.
 #include "qr_f64.hpp"
 
+#include <cstdlib>
+
 namespace sslab {
 
 namespace {
 
-constexpr int kQrThreads = 256;
+constexpr int kQrThreads = 1024;  // 16 wavefronts share the trailing columns
 constexpr int kQrWaves = kQrThreads / 64;
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -133,6 +135,130 @@ k_geqrf(int m, int n, double* __restrict__ A, int lda, long long sA, double* __r
   }
 }
 
+// The same factorization with the matrix staged in LDS, column-major (every column walk
+// -- norms, reflector products, updates -- reads consecutive LDS words instead of global
+// rows lda apart), for matrices whose m*n + m + n doubles fit; 16 wavefronts share the
+// trailing columns (and the rows of Q).  The complete Q stays in global memory, updated
+// row by row (coalesced).
+constexpr int kQrLdsThreads = 1024;
+constexpr int kQrLdsWaves = kQrLdsThreads / 64;
+
+__global__ void __launch_bounds__(kQrLdsThreads)
+k_geqrf_lds(int m, int n, double* __restrict__ A, int lda, long long sA, double* __restrict__ Q,
+            int ldq, long long sQ, int* __restrict__ perm, long long sPerm) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char qr_smem[];
+  double* L = reinterpret_cast<double*>(qr_smem);  // L[c * m + i] = A[i][c]
+  double* v = L + (long long)m * n;
+  double* cn = v + m;
+  __shared__ double s_tau, s_beta, s_scale;
+  __shared__ int s_piv;
+  const int b = blockIdx.x;
+  double* Ab = A + (long long)b * sA;
+  double* Qb = Q ? Q + (long long)b * sQ : nullptr;
+  int* pb = perm ? perm + (long long)b * sPerm : nullptr;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  for (int e = tid; e < m * n; e += kQrLdsThreads) {
+    const int i = e / n, c = e - (e / n) * n;
+    L[(long long)c * m + i] = Ab[(long long)i * lda + c];
+  }
+  if (Qb)
+    for (long long e = tid; e < (long long)m * m; e += kQrLdsThreads) {
+      const int i = (int)(e / m), j = (int)(e % m);
+      Qb[(long long)i * ldq + j] = (i == j) ? 1.0 : 0.0;
+    }
+  if (pb)
+    for (int c = tid; c < n; c += kQrLdsThreads) pb[c] = c;
+  __syncthreads();
+
+  const int kmax = m < n ? m : n;
+  for (int j = 0; j < kmax; ++j) {
+    if (pb) {
+      for (int c = j + wave; c < n; c += kQrLdsWaves) {
+        const double* col = L + (long long)c * m;
+        double s = 0.0;
+        for (int i = j + lane; i < m; i += 64) s += col[i] * col[i];
+        s = wave_sum(s);
+        if (lane == 0) cn[c] = s;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int p = j;
+        double best = cn[j];
+        for (int c = j + 1; c < n; ++c)
+          if (cn[c] > best) {
+            best = cn[c];
+            p = c;
+          }
+        s_piv = p;
+        if (p != j) {
+          const int t = pb[j];
+          pb[j] = pb[p];
+          pb[p] = t;
+        }
+      }
+      __syncthreads();
+      const int p = s_piv;
+      if (p != j) {
+        double* cj = L + (long long)j * m;
+        double* cp = L + (long long)p * m;
+        for (int i = tid; i < m; i += kQrLdsThreads) {
+          const double t = cj[i];
+          cj[i] = cp[i];
+          cp[i] = t;
+        }
+      }
+      __syncthreads();
+    }
+    double* colj = L + (long long)j * m;
+    if (wave == 0) {
+      double s = 0.0;
+      for (int i = j + 1 + lane; i < m; i += 64) s += colj[i] * colj[i];
+      s = wave_sum(s);
+      if (lane == 0) {
+        const double alpha = colj[j];
+        if (s == 0.0) {
+          s_tau = 0.0;
+          s_beta = alpha;
+          s_scale = 0.0;
+        } else {
+          const double nrm = sqrt(alpha * alpha + s);
+          const double beta = alpha >= 0.0 ? -nrm : nrm;
+          s_tau = (beta - alpha) / beta;
+          s_beta = beta;
+          s_scale = 1.0 / (alpha - beta);
+        }
+      }
+    }
+    __syncthreads();
+    const double tau = s_tau;
+    for (int i = j + tid; i < m; i += kQrLdsThreads) v[i] = (i == j) ? 1.0 : colj[i] * s_scale;
+    __syncthreads();
+    for (int i = j + tid; i < m; i += kQrLdsThreads) colj[i] = (i == j) ? s_beta : 0.0;
+    if (tau != 0.0) {
+      for (int c = j + 1 + wave; c < n; c += kQrLdsWaves) {
+        double* col = L + (long long)c * m;
+        double w = 0.0;
+        for (int i = j + lane; i < m; i += 64) w += v[i] * col[i];
+        w = wave_sum(w) * tau;
+        for (int i = j + lane; i < m; i += 64) col[i] -= w * v[i];
+      }
+      if (Qb)
+        for (int r = wave; r < m; r += kQrLdsWaves) {
+          double w = 0.0;
+          for (int i = j + lane; i < m; i += 64) w += Qb[(long long)r * ldq + i] * v[i];
+          w = wave_sum(w) * tau;
+          for (int i = j + lane; i < m; i += 64) Qb[(long long)r * ldq + i] -= w * v[i];
+        }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < m * n; e += kQrLdsThreads) {
+    const int i = e / n, c = e - (e / n) * n;
+    Ab[(long long)i * lda + c] = L[(long long)c * m + i];
+  }
+}
+
 // Y[b][blk*bs + i][blk*bs + j] += X[b*nblk + blk][i][j]: add nblk diagonal bs x bs blocks.
 __global__ void k_add_diag_blocks(int bs, int nblk, const double* __restrict__ X,
                                   double* __restrict__ Y, int ldy, long long sY) {
@@ -155,6 +281,23 @@ int geqrf_batched(int m, int n, double* A, int lda, long long sA, double* Q, int
                 n);
   SSLAB_REQUIRE(lda >= n && (Q == nullptr || ldq >= m), "geqrf: bad leading dimension");
   SSLAB_REQUIRE(batch <= 1 << 30, "geqrf: batch too large");
+  const size_t lds = sizeof(double) * ((size_t)m * n + m + n);
+  static const bool use_lds = [] {
+    const char* e = getenv("SSLAB_QR_LDS");
+    return !(e && e[0] == '0');
+  }();
+  if (use_lds && lds <= 150 * 1024) {
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_geqrf_lds),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+      attr = true;
+    }
+    hipLaunchKernelGGL(k_geqrf_lds, dim3(batch), dim3(kQrLdsThreads), lds, st, m, n, A, lda, sA,
+                       Q, ldq, sQ, perm, sPerm);
+    SSLAB_HIP_CHECK(hipGetLastError());
+    return kOk;
+  }
   hipLaunchKernelGGL(k_geqrf, dim3(batch), dim3(kQrThreads), 0, st, m, n, A, lda, sA, Q, ldq, sQ,
                      perm, sPerm);
   SSLAB_HIP_CHECK(hipGetLastError());

@@ -255,13 +255,23 @@ class ThomasSolverHBS(DirectSolver):
     """Thomas elimination where every S'_i = I - S_{i,i-1} S'_{i-1}^{-1} S_{i-1,i}
     is HBS-compressed and ULV-factored (omsdirectsolveHBS.py:189-351)."""
 
-    def __init__(self, m, rk, cyclic=False):
+    def __init__(self, m, rk, cyclic=False, samples=None):
         super().__init__(m, cyclic)
         if cyclic:
             raise NotImplementedError("periodic ThomasSolverHBS is not provided (the reference "
                                       "never sets its smw_block, omsdirectsolveHBS.py:348-349)")
         self.rk = rk
+        # sketch columns per S'_i compression: None keeps HBSnew's rule (as the reference),
+        # an int or a callable rk -> int overrides it (e.g. the sketch count of the S blocks)
+        self.samples = samples
         self.solve_method = None
+
+    def _hbsmat(self, op, tree, quad):
+        h = HBSnew.HBSMAT(op, tree, quad)
+        if self.samples is not None:
+            sm = self.samples
+            h._samples = sm if callable(sm) else (lambda rk_, s_=int(sm): s_)
+        return h
 
     def factorize_helper(self, S_rk_list, diagList=None):
         if diagList is None:
@@ -282,7 +292,7 @@ class ThomasSolverHBS(DirectSolver):
         Sprime = [I]
         for i in range(1, n + 1):
             op = Sprime_Linop(Sl[i - 1], Sprime[i - 1], Sr[i - 1], id=(i == 1))
-            h = HBSnew.HBSMAT(op, tree, quad)
+            h = self._hbsmat(op, tree, quad)
             h.construct(rk, compute_ULV=True)
             Sprime.append(h)
         self.A, self.B, self.C = Sl, Sprime, Sr
@@ -295,7 +305,7 @@ class ThomasSolverHBS(DirectSolver):
         tree, quad = _tree_of(*D_list, *A, *C)
         B = [D_list[0] if hasattr(D_list[0], "solve") else dense_to_linop(D_list[0])]
         for i in range(1, n + 1):
-            h = HBSnew.HBSMAT(Dprime_Linop(D_list[i], A[i - 1], C[i - 1], B[-1]), tree, quad)
+            h = self._hbsmat(Dprime_Linop(D_list[i], A[i - 1], C[i - 1], B[-1]), tree, quad)
             h.construct(rk, compute_ULV=True)
             B.append(h)
         self.A, self.B, self.C = A, B, C

@@ -26,7 +26,9 @@ def T(gpu):
 
 
 @pytest.mark.parametrize("m,n,pivot", [(7, 5, False), (40, 40, False), (293, 72, False),
-                                       (72, 72, True), (50, 30, True), (30, 50, True)])
+                                       (72, 72, True), (50, 30, True), (30, 50, True),
+                                       (130, 72, False), (72, 58, True), (226, 40, False),
+                                       (300, 200, True)])
 def test_batched_qr(T, m, n, pivot):
     from sslablu_amd.hbs import bqr
     rng = np.random.default_rng(m + n)
