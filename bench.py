@@ -422,6 +422,19 @@ def c3_bench(dev, seed=0, rk=40, nrhs=16):
     t_hbs = time.perf_counter() - t0
     out["hbs_solve_ms"] = round(t_hbs * 1e3, 2)
     out["hbs_rhs_solves_per_s"] = round(nrhs / t_hbs, 1)
+    # the same solve captured once as a HIP graph (thousands of small launches replayed)
+    from sslablu_amd.graphs import CapturedSolve
+    cs = CapturedSolve(solver.solve, tuple(B.shape), device=dev)
+    cs(B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        xg = cs(B)
+    torch.cuda.synchronize()
+    t_g = (time.perf_counter() - t0) / 5
+    out["hbs_solve_graph_ms"] = round(t_g * 1e3, 2)
+    out["hbs_graph_rhs_solves_per_s"] = round(nrhs / t_g, 1)
+    out["hbs_graph_vs_eager_rel_diff"] = ((xg - x_hbs).norm() / x_hbs.norm()).item()
     # dense FP64 reference of the same S-system (red-black, identity diagonal)
     SiM, SiP = s.blocks()
     t0 = time.perf_counter()

@@ -83,3 +83,37 @@ def test_hbs_compressed_thomas_solve(T):
     x_dense = BlockTriFactor("rb").factorize(None, SiM, SiP).solve(B)
     e = rel(x_hbs, x_dense.cpu().numpy())
     assert e < max(100 * max(errs), 1e-10), (errs, e)
+
+
+def test_hbs_solve_graph_replay(T):
+    """The HBS Thomas solve (ULV solves and HBS applies, thousands of small launches at
+    the C3 size) captured once as a HIP graph: every replay, with new right-hand sides
+    copied into the captured input and eager work in between, equals the eager solve."""
+    import torch
+    from sslablu_amd.direct_solve.omsdirectsolveHBS import ThomasSolverHBS
+    from sslablu_amd.matAssembly.HBS import HBSnew
+    from sslablu_amd.spectral3d import FaceTree, SpectralSlabs3D
+    s = SpectralSlabs3D(6, 9, 34, 34).build()
+    tree = FaceTree(32, 32, 64)
+    rk = 20
+    H = {}
+    for side in ("left", "right"):
+        h = HBSnew.HBSMAT(s.S[side], tree, quad=False, seed=2)
+        h._samples = lambda rk_, s_=max(2 * rk, 64) + rk + 10: s_
+        h.construct(rk)
+        H[side] = h
+    nb = s.nb
+    S_rk = [[H["right"]]] + [[H["left"], H["right"]] for _ in range(nb - 2)] + [[H["left"]]]
+    solver = ThomasSolverHBS(s.m, rk, samples=max(2 * rk, 64) + rk + 10)
+    solver.factorize(S_rk)
+    from sslablu_amd.graphs import CapturedSolve
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    cs = CapturedSolve(solver.solve, (nb * s.m, 4))
+    for rep in range(3):
+        B = torch.randn(nb * s.m, 4, dtype=torch.float64, device="cuda", generator=gen)
+        X_out = cs(B)
+        torch.cuda.synchronize()
+        X_eager = torch.as_tensor(solver.solve(B), device="cuda")
+        assert (torch.linalg.norm(X_out - X_eager) / torch.linalg.norm(X_eager)).item() < 1e-13
+    with pytest.raises(ValueError):
+        cs(B[:, :2])
