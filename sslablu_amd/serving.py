@@ -369,14 +369,17 @@ class BatchingServer:
         W = _bucket(R, self.buckets)
         prog = self._program(e, W, s)
         with torch.cuda.stream(s):
-            c = 0
-            for q in reqs:
-                prog.Fb[:, c:c + q.r].copy_(q.F)
-                c += q.r
+            # one gather in and one copy out per batch (columns past R keep stale data:
+            # every column is solved independently and those results are not read)
+            if len(reqs) == 1:
+                prog.Fb[:, :R].copy_(reqs[0].F)
+            else:
+                prog.Fb[:, :R].copy_(torch.cat([q.F for q in reqs], dim=1))
             prog.launch()
+            Xall = prog.Xb[:, :R].clone()
             c = 0
             for q in reqs:
-                q.X = prog.Xb[:, c:c + q.r].clone()
+                q.X = Xall[:, c:c + q.r]
                 q.batch_size = len(reqs)
                 c += q.r
             ev = torch.cuda.Event()
