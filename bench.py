@@ -562,6 +562,7 @@ def main():
     wall = time.perf_counter() - t0
     lat_ms = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
     ms_per_step = wall * 1e3 / args.steps
+
     tuning.set_policy(tuning.DEFAULT)
 
     # ---- accuracy: independent FP64 Thomas path and the S-system residual --------
