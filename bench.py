@@ -542,7 +542,7 @@ def main():
     t0 = time.perf_counter()
     if args.policy:
         pol = tuning.Policy(*[int(v) for v in args.policy.split(",")])
-        tuning._cache[fac.tune_key(NRHS)] = pol
+        tuning.remember(fac.tune_key(NRHS), pol)
     else:
         pol, _ = fac.autotune(NRHS, fn=step)
     tuning.set_policy(pol)  # the boundary-map launch runs under it as well

@@ -107,6 +107,15 @@ def _save():
             json.dump({k: list(astuple(v)) for k, v in _cache.items()}, f, indent=1)
 
 
+def remember(key: str, p: Policy) -> None:
+    """Record `p` as the policy for `key` (as autotune does), e.g. a policy measured
+    elsewhere; persisted when SSLAB_TUNE_CACHE is set."""
+    with _lock:
+        _load()
+        _cache[key] = p
+        _save()
+
+
 def lookup(key: str) -> Policy | None:
     with _lock:
         _load()
