@@ -952,7 +952,7 @@ static long long split_chunk(const GemmLaunchT<T>& L, int ksplit, int maxparts) 
     for (int i = 0; i < L.nd; ++i) {
       const GemmDescT<T>& d = L.d[i];
       if (d.batch <= 0 || d.M <= 0 || d.N <= 0) continue;
-      long long parts = (total_k(d) + c - 1) / c;
+      long long parts = d.nosplit ? 1 : (total_k(d) + c - 1) / c;
       parts = parts < 1 ? 1 : (parts > maxparts ? maxparts : parts);
       w += (long long)ceil_div(d.M, Cfg::BM) * ceil_div(d.N, Cfg::BN) * d.batch * parts;
     }
@@ -990,7 +990,7 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit, int max
       return kBadArgument;
     }
     const long long K = total_k(d);
-    long long parts = chunk > 0 ? (K + chunk - 1) / chunk : 1;
+    long long parts = chunk > 0 && !d.nosplit ? (K + chunk - 1) / chunk : 1;
     parts = parts < 1 ? 1 : (parts > maxparts ? maxparts : parts);
     const int ks = (int)parts;
     d.kchunk = (ceil_div((int)K, ks) + Cfg::BK - 1) / Cfg::BK * Cfg::BK;

@@ -52,6 +52,8 @@ struct GemmDescT {
   int kchunk;  // K range of one of them, a multiple of the tile's BK
   int atomic;  // 1: accumulate into C atomically (requires Cin == C, beta == 1), so several
                // descriptors (or launches) may add into the same output in any order
+  int nosplit;  // 1: never split this descriptor's K range (its C is then written once,
+                // by plain stores when beta == 0, with no initialisation pass)
 };
 
 template <typename T>

@@ -51,9 +51,28 @@ struct FoldPlan {
   double* maps;
   long long maps_doubles;
   std::vector<FoldGroup> groups;
+  int* zero_rows;    // device: first X row of every block of the split groups
+  int nzero;         // number of such blocks
 };
 
+// Groups whose summed K is below this are never split: their blocks are written once
+// with plain stores, so only the blocks of the long groups need zeroed rows.
+constexpr int kFoldNoSplitK = 512;
+
 namespace {
+
+// Zero rows [row_b, row_b + m) x [0, r) of X for every listed block b (one launch).
+__global__ void k_zero_blocks(int nblk, const int* __restrict__ rows, int m, int r,
+                              double* __restrict__ X, int ldx) {
+  const long long n = (long long)nblk * m * r;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long q = e / r;
+    const int c = (int)(e - q * r);
+    const int b = (int)(q / m), i = (int)(q - (long long)b * m);
+    X[(long long)(rows[b] + i) * ldx + c] = 0.0;
+  }
+}
 
 // flag[b * nF + c] = 1 when column c of block b (rows [b m, (b+1) m)) has a nonzero.
 __global__ void k_col_nonzero(int nb, int m, int nF, const double* __restrict__ M, int ldm,
@@ -195,7 +214,7 @@ int oms_fold_create(int nb, int m, int nF, const double* maps, int ldm, FoldPlan
   }
 
   // 3. compact maps: group g holds count blocks of m x Ktot (its runs side by side)
-  FoldPlan* P = new FoldPlan{nb, m, nF, nullptr, 0, {}};
+  FoldPlan* P = new FoldPlan{nb, m, nF, nullptr, 0, {}, nullptr, 0};
   long long total = 0;
   for (const Proto& p : protos) {
     int kt = 0;
@@ -236,6 +255,26 @@ int oms_fold_create(int nb, int m, int nF, const double* maps, int ldm, FoldPlan
     P->groups.push_back(g);
     off += (long long)p.count * m * kt;
   }
+  // the blocks of split-eligible groups start from zero rows (see kFoldNoSplitK)
+  std::vector<int> zr;
+  for (const FoldGroup& g : P->groups) {
+    int kt = 0;
+    for (int t = 0; t < g.nterms; ++t) kt += g.t[t].K;
+    if (kt >= kFoldNoSplitK)
+      for (int q = 0; q < g.batch; ++q) zr.push_back(g.x_row + q * g.x_rstride);
+  }
+  P->nzero = (int)zr.size();
+  if (!zr.empty()) {
+    hipError_t ez = hipMalloc(&P->zero_rows, sizeof(int) * zr.size());
+    if (ez == hipSuccess)
+      ez = hipMemcpyAsync(P->zero_rows, zr.data(), sizeof(int) * zr.size(),
+                          hipMemcpyHostToDevice, st);
+    if (ez == hipSuccess) ez = hipStreamSynchronize(st);
+    if (ez != hipSuccess) {
+      oms_fold_destroy(P);
+      SSLAB_HIP_CHECK(ez);
+    }
+  }
   *out = P;
   return kOk;
 }
@@ -246,8 +285,14 @@ int oms_fold_apply(const FoldPlan* P, const double* F, int ldF, int r, double* X
   SSLAB_REQUIRE(r >= 0, "oms_fold_apply: negative column count");
   if (r == 0) return kOk;
   SSLAB_REQUIRE(F && X && ldF >= r && ldx >= r, "oms_fold_apply: bad pointers / strides");
-  // every product accumulates into the zeroed X, so split-K needs no initialisation pass
-  SSLAB_TRY(zero_2d<double>((long long)P->nb * P->m, r, X, ldx, st));
+  // long groups accumulate (split-K) into zeroed blocks; short ones store once, unsplit
+  if (P->nzero > 0) {
+    const long long n = (long long)P->nzero * P->m * r;
+    const long long gsz = (n + 255) / 256;
+    hipLaunchKernelGGL(k_zero_blocks, dim3((unsigned)(gsz < 8192 ? gsz : 8192)), dim3(256), 0,
+                       st, P->nzero, P->zero_rows, P->m, r, X, ldx);
+    SSLAB_HIP_CHECK(hipGetLastError());
+  }
   GemmLaunch G{};
   G.nd = 0;
   for (const FoldGroup& g : P->groups) {
@@ -257,7 +302,12 @@ int oms_fold_apply(const FoldPlan* P, const double* F, int ldF, int r, double* X
     }
     double* C = X + (long long)g.x_row * ldx;
     const long long sC = (long long)g.x_rstride * ldx;
-    GemmDesc d = gemm_desc(P->m, r, g.batch, C, ldx, sC, C, ldx, sC, 1.0);
+    int kt = 0;
+    for (int t = 0; t < g.nterms; ++t) kt += g.t[t].K;
+    const bool split_ok = kt >= kFoldNoSplitK;
+    GemmDesc d = split_ok ? gemm_desc(P->m, r, g.batch, C, ldx, sC, C, ldx, sC, 1.0)
+                          : gemm_desc(P->m, r, g.batch, C, ldx, sC, nullptr, 0, 0, 0.0);
+    d.nosplit = split_ok ? 0 : 1;
     for (int t = 0; t < g.nterms; ++t) {
       const FoldTerm& T = g.t[t];
       gemm_add_term(d, gemm_term(P->maps + T.a_off, T.lda, T.sA, 0, F + (long long)T.f_row * ldF,
@@ -278,6 +328,7 @@ int oms_fold_groups(const FoldPlan* P) { return P ? (int)P->groups.size() : 0; }
 void oms_fold_destroy(FoldPlan* P) {
   if (!P) return;
   if (P->maps) (void)hipFree(P->maps);
+  if (P->zero_rows) (void)hipFree(P->zero_rows);
   delete P;
 }
 
