@@ -275,7 +275,7 @@ class BlockTriFactor:
         return f"blocktri:{self.method}:{self.nb}:{self.m}:{r}:{precision}"
 
     def autotune(self, r: int, precision: str = "fp64", iters: int = 1, reps: int = 10,
-                 fn=None):
+                 fn=None, cands=None):
         """Pick the GEMM tile policy for solves with `r` right-hand sides by timing every
         candidate on this factor; later solve_ / solve_mixed_ calls of that width use
         it.  `fn` (optional) is the caller's own step to time instead of a bare solve
@@ -298,7 +298,7 @@ class BlockTriFactor:
             tuning.lookup(key)  # load a persisted cache first, then drop a stale choice
             _saved = tuning._cache.pop(key, None)
             try:
-                return tuning.autotune(fn, key, reps=reps)
+                return tuning.autotune(fn, key, cands=cands, reps=reps)
             except Exception:
                 if _saved is not None:
                     tuning._cache[key] = _saved
