@@ -261,6 +261,9 @@ class _BatchProgram:
             with torch.cuda.graph(g, stream=stream):
                 self._run()
             stream.synchronize()
+            with torch.cuda.stream(stream):
+                g.replay()  # the first replay uploads the graph: pay it here, not in a batch
+            stream.synchronize()
             self.graph = g
 
     def _run(self):
