@@ -56,3 +56,22 @@ def test_size_queries_and_argument_errors_without_gpu():
     assert _lib.sizes("rb", 4, 16)[0] == (3 * 2 + 2 * 2) * 256 + (2 * 16) ** 2
     with pytest.raises(_lib.SslabError):
         _lib.sizes("rb", 0, 4)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_python_plan_mirror_matches_library():
+    """blocktri.rb_levels (used for bench accounting) describes the same plan as the
+    library: the factor size it implies equals sslab_rb_sizes for many (nb, m), and the
+    backward-block count follows the same levels."""
+    from sslablu_amd import _lib
+    from sslablu_amd.blocktri import rb_backward_blocks, rb_levels
+    for nb in (1, 2, 3, 4, 5, 7, 8, 9, 16, 17, 31, 33, 64, 100):
+        for m in (8, 100, 512, 700, 1024, 1500, 2048, 3000):
+            L, n_top = rb_levels(nb, m)
+            n, words = nb, 0
+            for _ in range(L):
+                words += 3 * (n // 2) + 2 * ((n + 1) // 2)
+                n = (n + 1) // 2
+            assert n == n_top
+            assert _lib.sizes("rb", nb, m)[0] == words * m * m + (n_top * m) ** 2
+            assert rb_backward_blocks(nb, m) <= 2 * sum(1 for _ in range(L)) * nb
