@@ -139,15 +139,14 @@ k_getrf_panel(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
 }
 
 // Base panel with its rows in registers: thread t holds panel rows t + k NT (k < R) over
-// the w <= 32 panel columns, so the rank-1 updates are register FMAs; per column the
+// the w <= W (32 or 64) panel columns, so the rank-1 updates are register FMAs; per column the
 // pivot search is a wave shuffle plus one LDS exchange, and the pivot row (after the
 // interchange) is broadcast through LDS.  Same pivot choice as the LDS panel (largest
 // |a|, first index on ties; an all-NaN column keeps its diagonal).
-template <typename E, int NT, int R>
+template <typename E, int NT, int R, int W = 32>
 __global__ void __launch_bounds__(NT)
 k_getrf_panel_reg(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
                   int* __restrict__ ipiv, long long sP, int* __restrict__ info) {
-  constexpr int W = 32;
   __shared__ E prow[W];  // pivot row: the row that becomes row c
   __shared__ E crow[W];  // the old row c, which moves to the pivot's place
   __shared__ E red_v[NT / 64];
@@ -493,11 +492,26 @@ template <typename E>
 int panel_rec(const PanelCtx<E>& c, int j0, int w) {
   const int rows = c.n - j0;
   const int bw = base_width<E>(c, rows);
-  if (w <= (bw > 0 ? bw : 32)) {
+  static const bool reg64 = [] {
+    const char* e = getenv("SSLAB_PANEL_REG");
+    return !(e && e[0] == '0');
+  }();
+  const int base = (reg64 && rows <= 512) ? 64 : (bw > 0 ? bw : 32);
+  if (w <= base) {
     static const bool reg = [] {
       const char* e = getenv("SSLAB_PANEL_REG");
       return !(e && e[0] == '0');
     }();
+    if (reg && w > 32 && w <= 64 && rows <= 512) {  // 64-column panel: half the recursion
+      if (rows <= 256)
+        hipLaunchKernelGGL((k_getrf_panel_reg<E, 256, 1, 64>), dim3(c.batch), dim3(256), 0, c.st,
+                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      else
+        hipLaunchKernelGGL((k_getrf_panel_reg<E, 512, 1, 64>), dim3(c.batch), dim3(512), 0, c.st,
+                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      SSLAB_HIP_CHECK(hipGetLastError());
+      return kOk;
+    }
     if (reg && w <= 32 && rows <= 1024) {
       if (rows <= 64)
         hipLaunchKernelGGL((k_getrf_panel_reg<E, 64, 1>), dim3(c.batch), dim3(64), 0, c.st, c.n,
