@@ -138,119 +138,152 @@ k_getrf_panel(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
   }
 }
 
-// Base panel with its rows in registers: thread t holds panel rows t + k NT (k < R) over
-// the w <= W (32 or 64) panel columns, so the rank-1 updates are register FMAs; per column the
-// pivot search is a wave shuffle plus one LDS exchange, and the pivot row (after the
-// interchange) is broadcast through LDS.  Same pivot choice as the LDS panel (largest
-// |a|, first index on ties; an all-NaN column keeps its diagonal).
-template <typename E, int NT, int R, int W = 32>
+// Cross-lane helpers of the register panel.  The wave-wide reductions run on DPP row
+// permutations (xor 1, xor 2, half-row and row mirrors reduce each row of 16 lanes) and four
+// readlanes combine the rows; every lane of the wave must be active.
+template <int kCtrl>
+__device__ __forceinline__ int dpp_i32(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, kCtrl, 0xF, 0xF, false);
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp_e(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_i32<kCtrl>((int)b);
+  const int hi = dpp_i32<kCtrl>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int kCtrl>
+__device__ __forceinline__ float dpp_e(float x) {
+  return __int_as_float(dpp_i32<kCtrl>(__float_as_int(x)));
+}
+__device__ __forceinline__ double readlane_e(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float readlane_e(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+constexpr int kDppXor1 = 0xB1;       // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;       // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;
+constexpr int kDppMirror = 0x140;
+
+template <typename E>
+__device__ __forceinline__ E max_e(E a, E b) { return a > b ? a : b; }
+
+// Wave maximum of a value that is never NaN (same result in every lane).
+template <typename E>
+__device__ __forceinline__ E wave_max(E v) {
+  v = max_e(v, dpp_e<kDppXor1>(v));
+  v = max_e(v, dpp_e<kDppXor2>(v));
+  v = max_e(v, dpp_e<kDppHalfMirror>(v));
+  v = max_e(v, dpp_e<kDppMirror>(v));
+  const E a = max_e(readlane_e(v, 0), readlane_e(v, 16));
+  const E b = max_e(readlane_e(v, 32), readlane_e(v, 48));
+  return max_e(a, b);
+}
+__device__ __forceinline__ int wave_min(int v) {
+  v = min(v, dpp_i32<kDppXor1>(v));
+  v = min(v, dpp_i32<kDppXor2>(v));
+  v = min(v, dpp_i32<kDppHalfMirror>(v));
+  v = min(v, dpp_i32<kDppMirror>(v));
+  const int a = min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16));
+  const int b = min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48));
+  return min(a, b);
+}
+
+// Base panel with its rows in registers: thread t holds panel row t (rows <= NT) over the
+// w <= W panel columns, so the rank-1 updates are register FMAs.  The column loop is fully
+// unrolled so that every register index is static.  Rows never move between lanes during
+// the factorization: each lane carries the row's current position (`pos`), an interchange
+// swaps two positions, and every row is stored at its final position at the end.
+//
+// Per column c: each wave finds its best candidate (largest |a| among positions >= c,
+// smallest position on ties) with DPP reductions and publishes (value, position) and the
+// candidate's U row (columns c..W-1) in LDS; after the column's single barrier every thread
+// picks the winning wave and reads the pivot row from its slot.  The LDS slots alternate
+// between two buffers, so a column's writes never meet the previous column's reads.  Same
+// pivot choice as the LDS panel (first index on ties; NaN entries lose to every number, so
+// an all-NaN column keeps its diagonal).  (A rolled loop over a row that shifts one column
+// per step measured slower: it does the full row width of work every column.)
+template <typename E, int NT, int W>
 __global__ void __launch_bounds__(NT)
 k_getrf_panel_reg(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
                   int* __restrict__ ipiv, long long sP, int* __restrict__ info) {
-  __shared__ E prow[W];  // pivot row: the row that becomes row c
-  __shared__ E crow[W];  // the old row c, which moves to the pivot's place
-  __shared__ E red_v[NT / 64];
-  __shared__ int red_i[NT / 64];
+  static_assert(W <= NT, "panel width <= threads");
+  constexpr int NW = NT / 64;
+  __shared__ E s_row[2][NW][W];
+  __shared__ E s_val[2][NW];
+  __shared__ int s_pos[2][NW];
   const int b = blockIdx.x;
   E* Ab = A + (long long)b * sA + (long long)j0 * lda + j0;
   int* pv = ipiv + (long long)b * sP;
   const int rows = n - j0;
   const int tid = threadIdx.x;
-  E P[R][W];
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+  const bool live = tid < rows;
+  int pos = tid;
+  int my_piv = 0;
+  E P[W];
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const int i = tid + k * NT;
-#pragma unroll
-    for (int j = 0; j < W; ++j) P[k][j] = (i < rows && j < w) ? Ab[(long long)i * lda + j] : E(0);
-  }
+  for (int j = 0; j < W; ++j) P[j] = (live && j < w) ? Ab[(long long)tid * lda + j] : E(0);
 #pragma clang loop unroll(full)
   for (int c = 0; c < W; ++c) {
     if (c >= w) continue;  // uniform; a full unroll keeps every P index static
-    E best = -1.0;
-    int bi = rows;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int i = tid + k * NT;
-      const E v = abs_e(P[k][c]);
-      if (i >= c && i < rows && v > best) {
-        best = v;
-        bi = i;
-      }
+    const int buf = c & 1;
+    // candidates: live rows at positions >= c; NaN ranks below every number (-0.5),
+    // non-candidates below that (-1)
+    const E a = abs_e(P[c]);
+    const E v = (live && pos >= c) ? (a == a ? a : E(-0.5)) : E(-1);
+    const E m = wave_max(v);
+    const int lp = wave_min(v == m ? pos : INT_MAX);
+    if (lane == 0) {
+      s_val[buf][wv] = m;
+      s_pos[buf][wv] = m < E(-0.75) ? INT_MAX : lp;
     }
+    if (m >= E(-0.75) && pos == lp) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const E ov = __shfl_xor(best, off);
-      const int oi = __shfl_xor(bi, off);
-      if (ov > best || (ov == best && oi < bi)) {
-        best = ov;
-        bi = oi;
-      }
-    }
-    if ((tid & 63) == 0) {
-      red_v[tid >> 6] = best;
-      red_i[tid >> 6] = bi;
+      for (int j = c; j < W; ++j) s_row[buf][wv][j] = P[j];
     }
     __syncthreads();
-    E bv = red_v[0];
-    int p = red_i[0];
+    int q = 0;
+    E bv = s_val[buf][0];
+    int p = s_pos[buf][0];
 #pragma unroll
-    for (int q = 1; q < NT / 64; ++q) {
-      if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < p)) {
-        bv = red_v[q];
-        p = red_i[q];
+    for (int k = 1; k < NW; ++k) {
+      const E ov = s_val[buf][k];
+      const int op = s_pos[buf][k];
+      if (ov > bv || (ov == bv && op < p)) {
+        bv = ov;
+        p = op;
+        q = k;
       }
     }
-    if (p >= rows) p = c;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int i = tid + k * NT;
-      if (i == p) {
-#pragma unroll
-        for (int j = 0; j < W; ++j) prow[j] = P[k][j];
-      }
-      if (i == c) {
-#pragma unroll
-        for (int j = 0; j < W; ++j) crow[j] = P[k][j];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int i = tid + k * NT;
-      if (i == c) {
-#pragma unroll
-        for (int j = 0; j < W; ++j) P[k][j] = prow[j];
-      } else if (i == p) {
-#pragma unroll
-        for (int j = 0; j < W; ++j) P[k][j] = crow[j];
-      }
-    }
-    if (tid == 0) pv[j0 + c] = j0 + p;
+    // the row at position c is always a candidate, so some wave published a row
+    const E* prow = s_row[buf][q];
+    if (pos == p) pos = c;
+    else if (pos == c) pos = p;
+    if (tid == c) my_piv = p;
     const E piv = prow[c];
-    if (piv == 0.0 && tid == 0 && info != nullptr && info[b] == 0) info[b] = j0 + c + 1;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int i = tid + k * NT;
-      if (i > c && i < rows) {
-        E l = P[k][c];
-        if (piv != 0.0) {
-          l = l / piv;
-          P[k][c] = l;
-        }
-#pragma unroll
-        for (int j = c + 1; j < W; ++j) P[k][j] -= l * prow[j];
+    if (piv == E(0) && tid == 0 && info != nullptr && info[b] == 0) info[b] = j0 + c + 1;
+    if (live && pos > c) {
+      E l = P[c];
+      if (piv != E(0)) {
+        l = l / piv;
+        P[c] = l;
       }
+#pragma unroll
+      for (int j = c + 1; j < W; ++j) P[j] -= l * prow[j];
     }
-    __syncthreads();  // prow / crow / red_* are rewritten by the next column
   }
+  if (tid < w) pv[j0 + tid] = j0 + my_piv;
+  if (live) {
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const int i = tid + k * NT;
-    if (i < rows) {
-#pragma unroll
-      for (int j = 0; j < W; ++j)
-        if (j < w) Ab[(long long)i * lda + j] = P[k][j];
-    }
+    for (int j = 0; j < W; ++j)
+      if (j < w) Ab[(long long)pos * lda + j] = P[j];
   }
 }
 
@@ -487,6 +520,23 @@ int base_width(const PanelCtx<E>& c, int rows) {
   return 0;
 }
 
+// Register base panel with the fewest waves that hold one row per lane.
+template <typename E, int W>
+int launch_panel_reg(const PanelCtx<E>& c, int j0, int w, int rows) {
+#define SSLAB_PANEL_REG_LAUNCH(NT)                                                          \
+  hipLaunchKernelGGL((k_getrf_panel_reg<E, NT, W>), dim3(c.batch), dim3(NT), 0, c.st, c.n, j0, \
+                     w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info)
+  if (rows <= 64) SSLAB_PANEL_REG_LAUNCH(64);
+  else if (rows <= 128) SSLAB_PANEL_REG_LAUNCH(128);
+  else if (rows <= 256) SSLAB_PANEL_REG_LAUNCH(256);
+  else if constexpr (W > 32) SSLAB_PANEL_REG_LAUNCH(512);  // callers: rows <= 512
+  else if (rows <= 512) SSLAB_PANEL_REG_LAUNCH(512);
+  else SSLAB_PANEL_REG_LAUNCH(1024);
+#undef SSLAB_PANEL_REG_LAUNCH
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
 // Factor rows [j0, n) of columns [j0, j0 + w).
 template <typename E>
 int panel_rec(const PanelCtx<E>& c, int j0, int w) {
@@ -503,29 +553,11 @@ int panel_rec(const PanelCtx<E>& c, int j0, int w) {
       return !(e && e[0] == '0');
     }();
     if (reg && w > 32 && w <= 64 && rows <= 512) {  // 64-column panel: half the recursion
-      if (rows <= 256)
-        hipLaunchKernelGGL((k_getrf_panel_reg<E, 256, 1, 64>), dim3(c.batch), dim3(256), 0, c.st,
-                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
-      else
-        hipLaunchKernelGGL((k_getrf_panel_reg<E, 512, 1, 64>), dim3(c.batch), dim3(512), 0, c.st,
-                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
-      SSLAB_HIP_CHECK(hipGetLastError());
+      SSLAB_TRY((launch_panel_reg<E, 64>(c, j0, w, rows)));
       return kOk;
     }
     if (reg && w <= 32 && rows <= 1024) {
-      if (rows <= 64)
-        hipLaunchKernelGGL((k_getrf_panel_reg<E, 64, 1>), dim3(c.batch), dim3(64), 0, c.st, c.n,
-                           j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
-      else if (rows <= 256)
-        hipLaunchKernelGGL((k_getrf_panel_reg<E, 256, 1>), dim3(c.batch), dim3(256), 0, c.st,
-                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
-      else if (rows <= 512)
-        hipLaunchKernelGGL((k_getrf_panel_reg<E, 512, 1>), dim3(c.batch), dim3(512), 0, c.st,
-                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
-      else
-        hipLaunchKernelGGL((k_getrf_panel_reg<E, 1024, 1>), dim3(c.batch), dim3(1024), 0, c.st,
-                           c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
-      SSLAB_HIP_CHECK(hipGetLastError());
+      SSLAB_TRY((launch_panel_reg<E, 32>(c, j0, w, rows)));
       return kOk;
     }
     if (bw > 0) {
