@@ -143,7 +143,8 @@ k_getrf_panel(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
 // readlanes combine the rows; every lane of the wave must be active.
 template <int kCtrl>
 __device__ __forceinline__ int dpp_i32(int x) {
-  return __builtin_amdgcn_update_dpp(x, x, kCtrl, 0xF, 0xF, false);
+  // every lane has a valid source lane for the permutations used here
+  return __builtin_amdgcn_mov_dpp(x, kCtrl, 0xF, 0xF, false);
 }
 template <int kCtrl>
 __device__ __forceinline__ double dpp_e(double x) {
@@ -214,9 +215,9 @@ k_getrf_panel_reg(int n, int j0, int w, E* __restrict__ A, int lda, long long sA
                   int* __restrict__ ipiv, long long sP, int* __restrict__ info) {
   static_assert(W <= NT, "panel width <= threads");
   constexpr int NW = NT / 64;
-  __shared__ E s_row[2][NW][W];
-  __shared__ E s_val[2][NW];
-  __shared__ int s_pos[2][NW];
+  __shared__ __attribute__((aligned(16))) E s_row[2][NW][W];
+  __shared__ __attribute__((aligned(16))) E s_val[2][NW];
+  __shared__ __attribute__((aligned(16))) int s_pos[2][NW];
   const int b = blockIdx.x;
   E* Ab = A + (long long)b * sA + (long long)j0 * lda + j0;
   int* pv = ipiv + (long long)b * sP;
@@ -245,23 +246,18 @@ k_getrf_panel_reg(int n, int j0, int w, E* __restrict__ A, int lda, long long sA
       s_pos[buf][wv] = m < E(-0.75) ? INT_MAX : lp;
     }
     if (m >= E(-0.75) && pos == lp) {
+      // the U row (columns c..W-1), from an even column so that the stores pair up
 #pragma unroll
-      for (int j = c; j < W; ++j) s_row[buf][wv][j] = P[j];
+      for (int j = c & ~1; j < W; ++j) s_row[buf][wv][j] = P[j];
     }
     __syncthreads();
-    int q = 0;
-    E bv = s_val[buf][0];
-    int p = s_pos[buf][0];
-#pragma unroll
-    for (int k = 1; k < NW; ++k) {
-      const E ov = s_val[buf][k];
-      const int op = s_pos[buf][k];
-      if (ov > bv || (ov == bv && op < p)) {
-        bv = ov;
-        p = op;
-        q = k;
-      }
-    }
+    // winning wave: lanes 0..NW-1 read the published candidates and the same DPP reductions
+    // pick the largest value, then the smallest position; a ballot names its wave
+    const E cv = lane < NW ? s_val[buf][lane] : E(-2);
+    const int cp = lane < NW ? s_pos[buf][lane] : INT_MAX;
+    const E cm = wave_max(cv);
+    const int p = wave_min(cv == cm ? cp : INT_MAX);
+    const int q = __ffsll((unsigned long long)__ballot(lane < NW && cv == cm && cp == p)) - 1;
     // the row at position c is always a candidate, so some wave published a row
     const E* prow = s_row[buf][q];
     if (pos == p) pos = c;
