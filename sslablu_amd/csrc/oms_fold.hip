@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <vector>
 
 #include "blocktri_f64.hpp"
@@ -53,11 +54,22 @@ struct FoldPlan {
   std::vector<FoldGroup> groups;
   int* zero_rows;    // device: first X row of every block of the split groups
   int nzero;         // number of such blocks
+  int nosplit_k;     // groups whose summed K is below this are never split
 };
 
 // Groups whose summed K is below this are never split: their blocks are written once
-// with plain stores, so only the blocks of the long groups need zeroed rows.
+// with plain stores, so only the blocks of the long groups need zeroed rows.  The
+// environment variable SSLAB_FOLD_NOSPLIT_K overrides it (read when a plan is built).
 constexpr int kFoldNoSplitK = 512;
+
+int fold_nosplit_k() {
+  static const int v = [] {
+    const char* e = getenv("SSLAB_FOLD_NOSPLIT_K");
+    const int x = e ? atoi(e) : 0;
+    return x > 0 ? x : kFoldNoSplitK;
+  }();
+  return v;
+}
 
 namespace {
 
@@ -214,7 +226,7 @@ int oms_fold_create(int nb, int m, int nF, const double* maps, int ldm, FoldPlan
   }
 
   // 3. compact maps: group g holds count blocks of m x Ktot (its runs side by side)
-  FoldPlan* P = new FoldPlan{nb, m, nF, nullptr, 0, {}, nullptr, 0};
+  FoldPlan* P = new FoldPlan{nb, m, nF, nullptr, 0, {}, nullptr, 0, kFoldNoSplitK};
   long long total = 0;
   for (const Proto& p : protos) {
     int kt = 0;
@@ -256,11 +268,12 @@ int oms_fold_create(int nb, int m, int nF, const double* maps, int ldm, FoldPlan
     off += (long long)p.count * m * kt;
   }
   // the blocks of split-eligible groups start from zero rows (see kFoldNoSplitK)
+  P->nosplit_k = fold_nosplit_k();
   std::vector<int> zr;
   for (const FoldGroup& g : P->groups) {
     int kt = 0;
     for (int t = 0; t < g.nterms; ++t) kt += g.t[t].K;
-    if (kt >= kFoldNoSplitK)
+    if (kt >= P->nosplit_k)
       for (int q = 0; q < g.batch; ++q) zr.push_back(g.x_row + q * g.x_rstride);
   }
   P->nzero = (int)zr.size();
@@ -304,7 +317,7 @@ int oms_fold_apply(const FoldPlan* P, const double* F, int ldF, int r, double* X
     const long long sC = (long long)g.x_rstride * ldx;
     int kt = 0;
     for (int t = 0; t < g.nterms; ++t) kt += g.t[t].K;
-    const bool split_ok = kt >= kFoldNoSplitK;
+    const bool split_ok = kt >= P->nosplit_k;
     GemmDesc d = split_ok ? gemm_desc(P->m, r, g.batch, C, ldx, sC, C, ldx, sC, 1.0)
                           : gemm_desc(P->m, r, g.batch, C, ldx, sC, nullptr, 0, 0, 0.0);
     d.nosplit = split_ok ? 0 : 1;

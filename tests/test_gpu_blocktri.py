@@ -107,6 +107,35 @@ def test_getrf_getrs(torch_dev, n, batch):
         assert rel(At[k].cpu().numpy(), lu) < 1e-12
 
 
+@pytest.mark.parametrize("n,tie_rows", [
+    (48, (5, 20, 40)),           # one wave, 64-column register panel
+    (200, (7, 30, 150, 199)),    # ties inside a wave and across waves
+    (450, (70, 71, 300, 449)),   # 512-thread panel
+    (900, (100, 640, 899)),      # 32-column register panel, 1024 threads
+])
+def test_getrf_pivot_ties_first_index(torch_dev, n, tie_rows):
+    """Exact ties for the largest |a| in the first column go to the first row (LAPACK's
+    idamax); random data elsewhere keeps the later pivots well separated."""
+    torch, dev = torch_dev
+    from sslablu_amd._lib import call, ptr, stream_handle
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((2, n, n))
+    A[:, :, 0] = np.clip(A[:, :, 0], -4.0, 4.0)
+    for k, r in enumerate(tie_rows):
+        A[:, r, 0] = 9.0 if k % 2 == 0 else -9.0
+    At = torch.tensor(A, device=dev)
+    piv = torch.empty(2, n, dtype=torch.int32, device=dev)
+    info = torch.zeros(2, dtype=torch.int32, device=dev)
+    call("sslab_dgetrf_batched", n, ptr(At), n, n * n, ptr(piv), n, ptr(info), 2, stream_handle())
+    assert int(info.abs().sum()) == 0
+    for k in range(2):
+        lu, p = sla.lu_factor(A[k])
+        got = piv[k].cpu().numpy()
+        assert got[0] == min(tie_rows)
+        assert np.array_equal(got, p)
+        assert rel(At[k].cpu().numpy(), lu) < 1e-12
+
+
 def test_getrf_reports_singular(torch_dev):
     torch, dev = torch_dev
     from sslablu_amd._lib import call, ptr, stream_handle
