@@ -386,19 +386,34 @@ def c3_bench(dev, seed=0, rk=40, nrhs=16):
     out["assembly_s"] = round(time.perf_counter() - t0, 3)
     tree = FaceTree(ny - 2, ny - 2, 72)
     ns = max(2 * rk, 72) + rk + 10
-    t0 = time.perf_counter()
-    H = {}
-    for side in ("left", "right"):
-        h = HBSnew.HBSMAT(s.S[side], tree, quad=False, seed=seed)
-        h._samples = lambda rk_, s_=ns: s_
-        h.construct(rk)
-        H[side] = h
-    torch.cuda.synchronize()
-    out["compress_s"] = round(time.perf_counter() - t0, 3)
+    def compress(fp32):
+        torch.cuda.synchronize()
+        t0_ = time.perf_counter()
+        H_ = {}
+        for side in ("left", "right"):
+            h = HBSnew.HBSMAT(s.S[side], tree, quad=False, seed=seed, sketch_fp32=fp32)
+            h._samples = lambda rk_, s_=ns: s_
+            h.construct(rk)
+            H_[side] = h
+        torch.cuda.synchronize()
+        return H_, time.perf_counter() - t0_
+
     X = torch.randn(s.m, 16, dtype=torch.float64, device=dev)
-    out["compression_rel_err"] = max(
-        ((H[sd].matmat(X) - s.S[sd] @ X).norm() / (s.S[sd] @ X).norm()).item()
-        for sd in ("left", "right"))
+
+    def comp_err(H_):
+        return max(((H_[sd].matmat(X) - s.S[sd] @ X).norm() / (s.S[sd] @ X).norm()).item()
+                   for sd in ("left", "right"))
+
+    # sketch products Y = S Omega, Z = S^T Psi in FP32 (MFMA), QR / bases in FP64; the
+    # FP64-sketch compression of the same blocks is reported beside it
+    H, t_c = compress(True)
+    out["sketch"] = "fp32 sketch GEMMs (v_mfma_f32), fp64 QR and interpolative bases"
+    out["compress_s"] = round(t_c, 3)
+    out["compression_rel_err"] = comp_err(H)
+    H64, t_c64 = compress(False)
+    out["compress_fp64_sketch_s"] = round(t_c64, 3)
+    out["compression_rel_err_fp64_sketch"] = comp_err(H64)
+    del H64
     out["hbs_bytes_per_block"] = int(H["left"].nbytes)
     out["dense_bytes_per_block"] = int(s.m * s.m * 8)
     # how the weak-admissibility compression error falls with rank (larger leaves allow it)

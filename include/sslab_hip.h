@@ -38,6 +38,13 @@ int sslab_dgemm_strided_batched(int transA, int transB, int M, int N, int K, dou
                                 const double* A, int lda, long long strideA, const double* B,
                                 int ldb, long long strideB, double beta, double* C, int ldc,
                                 long long strideC, int batch, void* stream);
+/* FP32 version (v_mfma_f32_16x16x4_f32, FP32 accumulation; K > 0): the sketch products
+ * Y = S Omega, Z = S^T Psi of the HBS compression in FP32 (matAssembly/HBS/HBSnew.py:
+ * 147-170 forms them in FP64 with numpy). */
+int sslab_sgemm_strided_batched(int transA, int transB, int M, int N, int K, float alpha,
+                                const float* A, int lda, long long strideA, const float* B,
+                                int ldb, long long strideB, float beta, float* C, int ldc,
+                                long long strideC, int batch, void* stream);
 
 /* In-place LU with partial pivoting (scipy.linalg.lu_factor semantics: ipiv is
  * 0-based, info = i+1 for the first zero pivot).  Replaces lu_factor at

@@ -29,6 +29,9 @@ _SIGS = {
     "sslab_dgemm_strided_batched": [_c_int, _c_int, _c_int, _c_int, _c_int, _c_d,
                                     _vp, _c_int, _c_ll, _vp, _c_int, _c_ll, _c_d,
                                     _vp, _c_int, _c_ll, _c_int, _vp],
+    "sslab_sgemm_strided_batched": [_c_int, _c_int, _c_int, _c_int, _c_int, ctypes.c_float,
+                                    _vp, _c_int, _c_ll, _vp, _c_int, _c_ll, ctypes.c_float,
+                                    _vp, _c_int, _c_ll, _c_int, _vp],
     "sslab_gemm_set_policy": [_vp, _c_int],
     "sslab_gemm_get_policy": [_vp, _c_int],
     "sslab_dgetrf_batched": [_c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int, _vp],

@@ -57,6 +57,21 @@ int sslab_dgemm_strided_batched(int transA, int transB, int M, int N, int K, dou
                              beta, C, ldc, strideC, batch, as_stream(stream));
 }
 
+int sslab_sgemm_strided_batched(int transA, int transB, int M, int N, int K, float alpha,
+                                const float* A, int lda, long long strideA, const float* B,
+                                int ldb, long long strideB, float beta, float* C, int ldc,
+                                long long strideC, int batch, void* stream) {
+  SSLAB_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "sgemm: negative size");
+  SSLAB_REQUIRE(C != nullptr, "sgemm: null C");
+  if (M == 0 || N == 0 || batch == 0) return 0;
+  SSLAB_REQUIRE(K > 0, "sgemm: K must be positive");
+  SSLAB_REQUIRE(A != nullptr && B != nullptr, "sgemm: null operand");
+  SSLAB_REQUIRE(ldc >= N, "sgemm: ldc < N");
+  SSLAB_REQUIRE(lda >= (transA ? M : K) && ldb >= (transB ? K : N), "sgemm: bad lda/ldb");
+  return sslab::gemm_strided<float>(transA, transB, M, N, K, alpha, A, lda, strideA, B, ldb,
+                                    strideB, beta, C, ldc, strideC, batch, as_stream(stream));
+}
+
 int sslab_gemm_set_policy(const int* p, int n) {
   SSLAB_REQUIRE(p != nullptr && n == 7, "gemm_set_policy: expected 7 integers");
   sslab::GemmPolicy q;

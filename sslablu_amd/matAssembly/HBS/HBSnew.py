@@ -45,8 +45,13 @@ def _dense_apply(Ad, X, transpose):
 class HBSMAT:
     """HBS approximation of a square operator on a cluster tree (HBSnew.py:65-416)."""
 
-    def __init__(self, A=None, tree=None, quad=True, device=None, seed=0):
+    def __init__(self, A=None, tree=None, quad=True, device=None, seed=0, sketch_fp32=False):
+        """sketch_fp32: for a dense A, form the sketches Y = A Omega, Z = A^T Psi with the
+        FP32 MFMA GEMM (A and the test matrices rounded to float); the compression that
+        follows (QR, interpolative bases, null-space projections) stays FP64.  Linear
+        operators are always applied in FP64."""
         _lib.require_gpu()
+        self.sketch_fp32 = bool(sketch_fp32)
         self.device = torch.device(device) if device is not None else torch.device(
             "cuda", torch.cuda.current_device())
         self.mode = "N"
@@ -133,8 +138,16 @@ class HBSMAT:
         Psp = np.zeros_like(Psi)
         Psp[perm] = Psi
         pt = torch.as_tensor(perm).to(dev)
-        Y = _apply_op(self.A, fromnumpy(Omp, dev), False, dev)[pt].contiguous()
-        Z = _apply_op(self.A, fromnumpy(Psp, dev), True, dev)[pt].contiguous()
+        if self.sketch_fp32 and isinstance(self.A, torch.Tensor):
+            from ...ops import sgemm
+            f32 = torch.float32
+            A32 = self.A.to(device=dev, dtype=f32)
+            Y = sgemm(A32, fromnumpy(Omp, dev).to(f32)).to(F64)[pt].contiguous()
+            Z = sgemm(A32.t(), fromnumpy(Psp, dev).to(f32)).to(F64)[pt].contiguous()
+            del A32
+        else:
+            Y = _apply_op(self.A, fromnumpy(Omp, dev), False, dev)[pt].contiguous()
+            Z = _apply_op(self.A, fromnumpy(Psp, dev), True, dev)[pt].contiguous()
         torch.cuda.synchronize(dev)
         self.tSample += time.time() - t0
         t0 = time.time()

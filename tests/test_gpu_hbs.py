@@ -89,3 +89,48 @@ def test_ulv_solve(T, k, nl, L, fac):
     assert rel(H.solve(bt), np.linalg.solve(A, b)) < 1e-10
     assert rel(H.solve(bt, "T"), np.linalg.solve(A.T, b)) < 1e-10
     assert rel(H.solve(bt[:, 0]), np.linalg.solve(A, b[:, 0])) < 1e-10
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(300, 130, 500, 0, 0), (1024, 90, 1024, 1, 0),
+                                         (77, 64, 33, 0, 1), (1, 1, 1, 0, 0)])
+def test_sgemm(T, M, N, K, ta, tb):
+    """FP32 GEMM (the sketch products of the FP32-sketch compression): FP32 accumulation,
+    so relative error about sqrt(K) * 2^-24 against the FP64 product."""
+    from sslablu_amd.ops import sgemm
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    At = T.tensor(A, device="cuda")
+    Bt = T.tensor(B, device="cuda")
+    C = sgemm(At.t() if ta else At, Bt.t() if tb else Bt)
+    ref = (A.T if ta else A).astype(np.float64) @ (B.T if tb else B).astype(np.float64)
+    assert C.dtype == T.float32
+    assert rel(C.double(), ref) < 1e-5
+
+
+def test_fp32_sketch_compression(T):
+    """Sketches formed in FP32 (HBSMAT(sketch_fp32=True)), compression in FP64: on an
+    exactly-HBS operator the compressed operator is as accurate as FP32 sketches allow
+    (relative error near 1e-6, where the FP64 sketches give 1e-9) -- the rounding of the
+    sketches, not the FP64 pivoting, bounds it."""
+    from sslablu_amd.matAssembly.HBS import HBSnew
+
+    class _Tree:  # the oracle generator's identity-ordered binary tree
+        def __init__(self, N, nl, L):
+            self.perm_leaf = np.arange(N)
+            self.nleaves = N // nl
+            self.nlevels = L
+
+    rng = np.random.default_rng(3)
+    k, nl, L, fac = 6, 16, 4, 2
+    U, D, V, Nbvec = oh.gen_random_hbs(k, nl, L, fac, rng)
+    A = oh.hbs_dense(U, D, V)
+    At = T.tensor(A, device="cuda")
+    errs = {}
+    for f32 in (False, True):
+        h = HBSnew.HBSMAT(At, _Tree(A.shape[0], nl, L), quad=False, seed=4, sketch_fp32=f32)
+        h.construct(k)
+        x = rng.standard_normal((A.shape[0], 5))
+        errs[f32] = rel(h.matmat(T.tensor(x, device="cuda")), A @ x)
+    assert errs[False] < 1e-9
+    assert errs[True] < 1e-5
