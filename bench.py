@@ -501,6 +501,9 @@ def main():
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-interior", action="store_true")
+    ap.add_argument("--no-split-tune", action="store_true",
+                    help="one joint tile policy for the whole step (no separate tuning of the "
+                         "boundary-map launch and the backward sweep)")
     ap.add_argument("--graph", action="store_true",
                     help="also time the step replayed from a captured HIP graph")
     ap.add_argument("--policy", default=None,
@@ -569,6 +572,32 @@ def main():
     else:
         pol, _ = fac.autotune(NRHS, fn=step)
     tuning.set_policy(pol)  # the boundary-map launch runs under it as well
+    tune_split = None
+    if not args.policy and not args.no_split_tune:
+        # the boundary-map launch and the backward sweep tuned apart (their GEMM shapes
+        # differ); kept only when the whole step is faster than under the joint policy
+        def step_time(n=30):
+            for _ in range(3):
+                step()
+            torch.cuda.synchronize()
+            a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(n):
+                step()
+            b_.record(stream)
+            torch.cuda.synchronize()
+            return a.elapsed_time(b_) * 1e3 / n
+        t_joint = step_time()
+        pol_f, _ = fold.autotune(NRHS)
+        pol_b, _ = fac.autotune(NRHS, fn=lambda: fac.solve_(None, X, phase="backward"))
+        t_split = step_time()
+        tune_split = {"joint_us": round(t_joint, 2), "split_us": round(t_split, 2),
+                      "fold_policy": list(dataclasses.astuple(pol_f)),
+                      "backward_policy": list(dataclasses.astuple(pol_b)),
+                      "kept": "split" if t_split < t_joint else "joint"}
+        if t_split >= t_joint:
+            tuning.remember(fac.tune_key(NRHS), pol)
+            tuning.remember(fold.tune_key(NRHS), pol)
     t_tune = time.perf_counter() - t0
 
     for _ in range(args.warmup):
@@ -740,6 +769,7 @@ def main():
                                "boundary_fold": round(t_fold, 4)},
         "fold_maps_mb": round(fold.nbytes / 2 ** 20, 1),
         "gemm_policy": list(dataclasses.astuple(pol)), "autotune_s": round(t_tune, 3),
+        "split_tuning": tune_split,
         "rel_err_vs_fp64_thomas": rel_err,
         "residual": resid,
         "roofline": roofline,

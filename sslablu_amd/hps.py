@@ -571,16 +571,41 @@ class BoundaryFold:
     def groups(self) -> int:
         return int(_lib.load().sslab_oms_fold_groups(self.handle)) if self.handle else 0
 
+    def tune_key(self, r: int) -> str:
+        return f"oms_fold:{self.fac.nb}:{self.fac.m}:{self.nF}:{r}"
+
+    def autotune(self, r: int, reps: int = 10, cands=None):
+        """Pick the GEMM tile policy of the boundary-map launch alone (its shapes differ
+        from the backward sweep's) by timing every candidate; later apply_ / solve_ calls
+        of width r use it.  Returns (policy, microseconds per call)."""
+        from . import tuning
+        nbm = self.fac.nb * self.fac.m
+        F = torch.randn(self.nF, r, dtype=F64, device=self.op.device)
+        X = torch.empty(nbm, r, dtype=F64, device=self.op.device)
+        key = self.tune_key(r)
+        with tuning.policy(tuning.DEFAULT):
+            tuning.lookup(key)
+            saved = tuning._cache.pop(key, None)
+            try:
+                return tuning.autotune(lambda: self.apply_(F, X), key, cands=cands, reps=reps)
+            except Exception:
+                if saved is not None:
+                    tuning._cache[key] = saved
+                raise
+
     def apply_(self, F: torch.Tensor, X: torch.Tensor, stream=None):
-        """X = M F: the backward sweep's entry state for boundary data F (nF x r)."""
+        """X = M F: the backward sweep's entry state for boundary data F (nF x r), under the
+        policy tuned for this width if there is one (else the caller's)."""
+        from . import tuning
         r = F.shape[1]
         nbm = self.fac.nb * self.fac.m
         if F.shape[0] != self.nF or F.dtype != F64 or X.shape != (nbm, r) or X.dtype != F64:
             raise ValueError(f"BoundaryFold: F must be float64 ({self.nF}, r) and X ({nbm}, r)")
         if r > 0 and (F.stride(1) != 1 or X.stride(1) != 1):
             raise ValueError("BoundaryFold: unit column stride required")
-        call("sslab_oms_fold_apply", self.handle, ptr(F), max(F.stride(0), r), r, ptr(X),
-             max(X.stride(0), r), stream_handle(stream))
+        with tuning.policy(tuning.lookup(self.tune_key(r))):
+            call("sslab_oms_fold_apply", self.handle, ptr(F), max(F.stride(0), r), r, ptr(X),
+                 max(X.stride(0), r), stream_handle(stream))
         return X
 
     def solve_(self, F: torch.Tensor, X: torch.Tensor, stream=None):
