@@ -40,7 +40,7 @@ CALIB_MIB = 256
 # the step's 4 GEMM dispatches, each counter scaled by its reading for the calibration
 # copy (summary and commands: profiles/rb_fold_solve_c2_helmholtz_hbm_traffic.md).  A
 # measured constant of the current kernels: re-measure after changing them.
-TRAFFIC = 54869200
+TRAFFIC = 56775689
 
 
 class PDO2d:
@@ -501,6 +501,9 @@ def main():
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-interior", action="store_true")
+    ap.add_argument("--fold-policy", default=None,
+                    help="with --policy: the boundary-map launch's own tile policy (7 ints, "
+                         "as reported in split_tuning.fold_policy)")
     ap.add_argument("--no-split-tune", action="store_true",
                     help="one joint tile policy for the whole step (no separate tuning of the "
                          "boundary-map launch and the backward sweep)")
@@ -569,6 +572,9 @@ def main():
     if args.policy:
         pol = tuning.Policy(*[int(v) for v in args.policy.split(",")])
         tuning.remember(fac.tune_key(NRHS), pol)
+        if args.fold_policy:
+            tuning.remember(fold.tune_key(NRHS),
+                            tuning.Policy(*[int(v) for v in args.fold_policy.split(",")]))
     else:
         pol, _ = fac.autotune(NRHS, fn=step)
     tuning.set_policy(pol)  # the boundary-map launch runs under it as well
