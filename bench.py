@@ -504,6 +504,9 @@ def main():
     ap.add_argument("--fold-policy", default=None,
                     help="with --policy: the boundary-map launch's own tile policy (7 ints, "
                          "as reported in split_tuning.fold_policy)")
+    ap.add_argument("--bwd-policies", default=None,
+                    help="with --policy: one tile policy per backward level, ';'-separated "
+                         "(as reported in split_tuning.backward_level_policies)")
     ap.add_argument("--no-split-tune", action="store_true",
                     help="one joint tile policy for the whole step (no separate tuning of the "
                          "boundary-map launch and the backward sweep)")
@@ -604,6 +607,18 @@ def main():
         if t_split >= t_joint:
             tuning.remember(fac.tune_key(NRHS), pol)
             tuning.remember(fold.tune_key(NRHS), pol)
+        # then one policy per backward level (the levels multiply 3, 7 and 15 blocks here)
+        t_before = min(t_joint, t_split)
+        table, _ = fac.autotune_backward_steps(NRHS)
+        t_levels = step_time()
+        tune_split["per_level_us"] = round(t_levels, 2)
+        tune_split["backward_level_policies"] = [list(dataclasses.astuple(q)) for q in table]
+        tune_split["per_level_kept"] = t_levels < t_before
+        if t_levels >= t_before:
+            fac._bwd_policies.pop(NRHS, None)
+    if args.policy and args.bwd_policies:
+        fac.__dict__.setdefault("_bwd_policies", {})[NRHS] = [
+            tuning.Policy(*[int(v) for v in q.split(",")]) for q in args.bwd_policies.split(";")]
     t_tune = time.perf_counter() - t0
 
     for _ in range(args.warmup):

@@ -139,6 +139,11 @@ int sslab_rb_solve_multi(int nsys, const int* nb, const int* m, const int* nrhs,
  * chosen automatically for launches with at most 16 / 32 columns).  Results do not depend on the
  * policy beyond floating-point summation order. */
 int sslab_gemm_set_policy(const int* p, int n);
+/* Per-step tile policies of the red-black backward sweep for the calling thread's later
+ * solves: step u (level L-1-u) runs under p[7u .. 7u+6] (same layout as above); steps past
+ * nsteps use the thread's policy.  nsteps = 0 clears the table; nsteps <= 16.  No
+ * reference counterpart (autotuning hook, like sslab_gemm_set_policy). */
+int sslab_rb_set_backward_policies(const int* p, int nsteps);
 int sslab_gemm_get_policy(int* p, int n);
 
 /* ---- mixed precision (FP32 red-black preconditioner + FP64 refinement) -------

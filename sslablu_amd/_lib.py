@@ -33,6 +33,7 @@ _SIGS = {
                                     _vp, _c_int, _c_ll, _vp, _c_int, _c_ll, ctypes.c_float,
                                     _vp, _c_int, _c_ll, _c_int, _vp],
     "sslab_gemm_set_policy": [_vp, _c_int],
+    "sslab_rb_set_backward_policies": [_vp, _c_int],
     "sslab_gemm_get_policy": [_vp, _c_int],
     "sslab_dgetrf_batched": [_c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int, _vp],
     "sslab_dgetrs_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int,

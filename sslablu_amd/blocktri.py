@@ -164,7 +164,8 @@ class BlockTriFactor:
         s = stream_handle(stream)
         flags = ((1 if self.ident0 else 0) | (2 if level0_reduced else 0) |
                  {"all": 0, "forward": 4, "backward": 8}[phase])
-        with tuning.policy(tuning.lookup(self.tune_key(r))):
+        bwd = getattr(self, "_bwd_policies", {}).get(r) if self.method == "rb" else None
+        with tuning.policy(tuning.lookup(self.tune_key(r))), tuning.backward_policies(bwd):
             if self.method == "rb":
                 call("sslab_rb_solve_ex", nb, m, r, ptr(self.factor),
                      0 if phase == "backward" else ptr(V), V.stride(0),
@@ -303,6 +304,51 @@ class BlockTriFactor:
                 if _saved is not None:
                     tuning._cache[key] = _saved
                 raise
+
+    def autotune_backward_steps(self, r: int, reps: int = 10, cands=None, fn=None):
+        """Pick a tile policy per step of the red-black backward sweep (its levels differ in
+        how many blocks they multiply) by timing each step's candidates with the other steps
+        fixed, starting from the policy tuned for the whole solve; later solve_ calls of
+        width r use the table.  `fn` (optional) is the caller's step to time (default: the
+        backward sweep alone).  Returns (list of policies, microseconds per call)."""
+        if self.method != "rb":
+            raise ValueError("autotune_backward_steps: red-black factors only")
+        _lib.require_gpu()
+        levels, _ = rb_levels(self.nb, self.m)
+        if levels == 0:
+            return [], 0.0
+        X = torch.randn(self.nb * self.m, r, dtype=F64, device=self.device)
+        if fn is None:
+            fn = lambda: self.solve_(None, X, phase="backward")  # noqa: E731
+        cands = list(cands) if cands is not None else tuning.candidates()
+        base = tuning.lookup(self.tune_key(r)) or tuning.get_policy()
+        table = [base] * levels
+        store = self.__dict__.setdefault("_bwd_policies", {})
+        store.pop(r, None)
+        s = torch.cuda.current_stream()
+
+        def timed():
+            fn()
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(reps):
+                fn()
+            e1.record(s)
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) * 1e3 / reps
+
+        best_t = None
+        for u in range(levels):
+            for p in [table[u]] + cands:
+                trial = table[:u] + [p] + table[u + 1:]
+                with tuning.backward_policies(trial):
+                    t = timed()
+                if best_t is None or t < best_t:
+                    best_t, table = t, trial
+        store[r] = table
+        return table, best_t
 
     def residual(self, B: torch.Tensor, X: torch.Tensor, stream=None) -> torch.Tensor:
         """R = B - S X in FP64 (requires enable_mixed for the stored blocks)."""

@@ -359,6 +359,32 @@ struct RbSolveSys {
   int top_max;  // the plan's coarsest-level limit (0: the default, rb_top_max(m))
 };
 
+// Tile policies of the backward sweep's steps (step u runs level L-1-u), per calling
+// thread; steps past the table use the thread's GEMM policy.  Empty by default.
+constexpr int kMaxBwdPolicies = 16;
+static thread_local GemmPolicy g_bwd_policy[kMaxBwdPolicies];
+static thread_local int g_bwd_npolicy = 0;
+
+int rb_set_backward_policies(const GemmPolicy* p, int n) {
+  SSLAB_REQUIRE(n >= 0 && n <= kMaxBwdPolicies, "rb_set_backward_policies: 0..%d steps",
+                kMaxBwdPolicies);
+  SSLAB_REQUIRE(n == 0 || p != nullptr, "rb_set_backward_policies: null table");
+  const GemmPolicy saved = gemm_get_policy();
+  for (int u = 0; u < n; ++u) {  // validate every entry through the policy setter
+    const int rc = gemm_set_policy(p[u]);
+    if (rc != kOk) {
+      gemm_set_policy(saved);
+      return rc;
+    }
+  }
+  gemm_set_policy(saved);
+  for (int u = 0; u < n; ++u) g_bwd_policy[u] = p[u];
+  g_bwd_npolicy = n;
+  return kOk;
+}
+
+int rb_get_backward_policy_count() { return g_bwd_npolicy; }
+
 // Append a descriptor, launching the pending ones first when the launch is full.  The
 // descriptors of one level write disjoint outputs from inputs no descriptor of that
 // level writes, so a level may be split over several launches.
@@ -525,14 +551,19 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
     SSLAB_TRY(copy_block_batched<T>(S.m, S.r, S.V + bv, S.ldv, 2 * bv, S.X + bx, S.ldx, 2 * bx,
                                     S.p.lv[0].nO, st));
   }
+  const GemmPolicy saved = gemm_get_policy();
   for (int u = 0; u < Lmax; ++u) {
-    for (int s = 0; s < nsys; ++s) {
+    if (u < g_bwd_npolicy) gemm_set_policy(g_bwd_policy[u]);
+    int rc = kOk;
+    for (int s = 0; s < nsys && rc == kOk; ++s) {
       const RbSolveSys<T>& S = sys[s];
       const int l = S.p.L - 1 - u;
       if (S.r == 0 || l < 0 || S.phase == 1) continue;
-      SSLAB_TRY(rb_bwd_level(G, S, l, st));
+      rc = rb_bwd_level(G, S, l, st);
     }
-    SSLAB_TRY(rb_flush(G, st));
+    if (rc == kOk) rc = rb_flush(G, st);
+    if (u < g_bwd_npolicy) gemm_set_policy(saved);
+    if (rc != kOk) return rc;
   }
   return kOk;
 }

@@ -1,6 +1,7 @@
 // Block-tridiagonal S-system factor / solve (FP64, gfx950).  See blocktri_f64.hip.
 #pragma once
 
+#include "gemm.hpp"
 #include "sslab_common.hpp"
 
 namespace sslab {
@@ -18,6 +19,12 @@ int rb_factor_f32(int nb, int m, const float* T, const float* SiM, const float* 
                   float* factor, float* work, int* iwork, int* info, hipStream_t st);
 int rb_solve(int nb, int m, int r, const double* factor, double* V, int ldv, double* X, int ldx,
              hipStream_t st);
+// Per-step tile policies of the red-black backward sweep (step u runs level L-1-u) for
+// the calling thread's later solves; n = 0 clears them.  Each entry is validated as by
+// gemm_set_policy.
+int rb_set_backward_policies(const GemmPolicy* p, int n);
+int rb_get_backward_policy_count();
+
 int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, double* X,
                 int ldx, int flags, hipStream_t st);
 int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,

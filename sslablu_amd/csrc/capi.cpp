@@ -80,6 +80,18 @@ int sslab_gemm_set_policy(const int* p, int n) {
   return sslab::gemm_set_policy(q);
 }
 
+int sslab_rb_set_backward_policies(const int* p, int nsteps) {
+  SSLAB_REQUIRE(nsteps >= 0 && nsteps <= 16, "rb_set_backward_policies: 0..16 steps");
+  SSLAB_REQUIRE(nsteps == 0 || p != nullptr, "rb_set_backward_policies: null table");
+  sslab::GemmPolicy q[16];
+  for (int u = 0; u < nsteps; ++u) {
+    const int* e = p + 7 * u;
+    q[u].target_wgs = e[0]; q[u].split_wgs = e[1]; q[u].max_ksplit = e[2];
+    q[u].min_split_k = e[3]; q[u].cfg_mid = e[4]; q[u].cfg_split = e[5]; q[u].max_tile = e[6];
+  }
+  return sslab::rb_set_backward_policies(q, nsteps);
+}
+
 int sslab_gemm_get_policy(int* p, int n) {
   SSLAB_REQUIRE(p != nullptr && n == 7, "gemm_get_policy: expected 7 integers");
   const sslab::GemmPolicy q = sslab::gemm_get_policy();

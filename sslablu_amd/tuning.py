@@ -52,6 +52,29 @@ def set_policy(p: Policy) -> None:
     call("sslab_gemm_set_policy", ctypes.addressof(arr), 7)
 
 
+def set_backward_policies(pols) -> None:
+    """Per-step tile policies of the red-black backward sweep for the calling thread's
+    later solves (step u runs the sweep's level L-1-u); an empty list clears them."""
+    pols = list(pols or [])
+    arr = (ctypes.c_int * (7 * max(1, len(pols))))()
+    for u, p in enumerate(pols):
+        arr[7 * u:7 * u + 7] = list(astuple(p))
+    call("sslab_rb_set_backward_policies", ctypes.addressof(arr), len(pols))
+
+
+@contextmanager
+def backward_policies(pols):
+    """Run the body with per-step backward-sweep policies `pols` (None: leave them)."""
+    if pols is None:
+        yield
+        return
+    set_backward_policies(pols)
+    try:
+        yield
+    finally:
+        set_backward_policies([])
+
+
 def get_policy() -> Policy:
     arr = (ctypes.c_int * 7)()
     call("sslab_gemm_get_policy", ctypes.addressof(arr), 7)

@@ -76,3 +76,32 @@ def test_register_streamed_tiles(T, cfg):
     with tuning.policy(pol):
         x = fac.solve(Bv)
     assert (T.linalg.norm(x - ref) / T.linalg.norm(ref)).item() < 1e-12
+
+
+def test_backward_level_policies(T):
+    """Per-level tile policies of the red-black backward sweep (sslab_rb_set_backward_policies)
+    change speed only: the same solution under a mixed table, invalid entries are rejected,
+    and the per-level autotuner's table is used by later solves of that width."""
+    from sslablu_amd import tuning
+    from sslablu_amd._lib import SslabError
+    from sslablu_amd.blocktri import BlockTriFactor, rb_levels
+    rng = np.random.default_rng(5)
+    nb, m, r = 31, 64, 32
+    SiM = 0.3 * rng.standard_normal((nb, m, m)) / np.sqrt(m)
+    SiP = 0.3 * rng.standard_normal((nb, m, m)) / np.sqrt(m)
+    fac = BlockTriFactor("rb").factorize(None, SiM, SiP)
+    B = T.tensor(rng.standard_normal((nb * m, r)), device="cuda")
+    ref = fac.solve(B)
+    levels, _ = rb_levels(nb, m)
+    assert levels >= 2
+    table = [tuning.Policy(100000, 4096, 64, 64, 5, 5, 0), tuning.Policy(1, 512, 64, 1 << 30, 4, 4, 2),
+             tuning.DEFAULT] * levels
+    with tuning.backward_policies(table[:levels]):
+        x = fac.solve(B)
+    assert (T.linalg.norm(x - ref) / T.linalg.norm(ref)).item() < 1e-12
+    with pytest.raises(SslabError):
+        tuning.set_backward_policies([tuning.Policy(cfg_split=99)])
+    best, us = fac.autotune_backward_steps(r, reps=2, cands=tuning.candidates()[::40])
+    assert len(best) == levels and us > 0 and fac._bwd_policies[r] == best
+    x = fac.solve(B)
+    assert (T.linalg.norm(x - ref) / T.linalg.norm(ref)).item() < 1e-12
