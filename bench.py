@@ -211,17 +211,30 @@ def c4_bench(dev, seed=0, reps=5):
                 itertools.product((512, 1024), (512, 1024), (2, 3), (2, 3, 5), (0, 1))]
     pol_c4, _ = fac.autotune(r, reps=2, cands=c4_cands)
     out["autotuned_policy_solve"] = list(dataclasses.astuple(pol_c4))
-    fac.solve_(V, X)
-    ts = []
-    for _ in range(reps):
-        V.copy_(B)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
+
+    def time_solve():
         fac.solve_(V, X)
-        e1.record()
-        torch.cuda.synchronize()
-        ts.append(e0.elapsed_time(e1))
-    t_solve = float(np.median(ts)) / 1e3
+        ts = []
+        for _ in range(reps):
+            V.copy_(B)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fac.solve_(V, X)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts)) / 1e3
+
+    t_solve = time_solve()
+    # one policy per backward level (the levels multiply 32, 16, ..., 1 blocks of n = 2048)
+    table, _ = fac.autotune_backward_steps(r, reps=2, cands=c4_cands)
+    t_lv = time_solve()
+    out["per_level_backward"] = {"solve_ms": round(t_lv * 1e3, 3), "kept": t_lv < t_solve,
+                                 "policies": [list(dataclasses.astuple(q)) for q in table]}
+    if t_lv < t_solve:
+        t_solve = t_lv
+    else:
+        fac._bwd_policies.pop(r, None)  # X already holds the solution (either table solves)
     fac.enable_mixed(T, M, P)
     res = (fac.residual(B, X).norm() / B.norm()).item()
     # FP32 factorization + FP64 refinement on the same system
@@ -492,8 +505,8 @@ def c3_bench(dev, seed=0, rk=40, nrhs=16):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-mixed", action="store_true")
@@ -620,6 +633,11 @@ def main():
         fac.__dict__.setdefault("_bwd_policies", {})[NRHS] = [
             tuning.Policy(*[int(v) for v in q.split(",")]) for q in args.bwd_policies.split(";")]
     t_tune = time.perf_counter() - t0
+    # the backward sweep's policy and level table stand for the thread, so the solver's
+    # per-call policy scopes are no-ops in the timed loop (only the boundary-map launch
+    # switches to its own policy and back)
+    tuning.set_policy(tuning.lookup(fac.tune_key(NRHS)) or pol)
+    tuning.set_backward_policies(getattr(fac, "_bwd_policies", {}).get(NRHS, ()))
 
     for _ in range(args.warmup):
         step()
@@ -669,6 +687,7 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
 
     tuning.set_policy(tuning.DEFAULT)
+    tuning.set_backward_policies(())
 
     # ---- accuracy: independent FP64 Thomas path and the S-system residual --------
     fac_th = BlockTriFactor("thomas", dev).factorize(

@@ -46,20 +46,40 @@ _lock = threading.Lock()
 _loaded = False
 
 
+# The calling thread's policy and backward-level table as last set through this module
+# (the library keeps them per thread too): equal settings skip the library call, so the
+# per-call policy scopes of the solvers cost no host time once a policy is standing.
+_tls = threading.local()
+
+
 def set_policy(p: Policy) -> None:
     """Set the calling thread's GEMM policy in the library."""
+    if getattr(_tls, "policy", None) == p:
+        return
     arr = (ctypes.c_int * 7)(*astuple(p))
-    call("sslab_gemm_set_policy", ctypes.addressof(arr), 7)
+    try:
+        call("sslab_gemm_set_policy", ctypes.addressof(arr), 7)
+    except Exception:
+        _tls.policy = None
+        raise
+    _tls.policy = p
 
 
 def set_backward_policies(pols) -> None:
     """Per-step tile policies of the red-black backward sweep for the calling thread's
     later solves (step u runs the sweep's level L-1-u); an empty list clears them."""
-    pols = list(pols or [])
+    pols = tuple(pols or ())
+    if getattr(_tls, "bwd", None) == pols:
+        return
     arr = (ctypes.c_int * (7 * max(1, len(pols))))()
     for u, p in enumerate(pols):
         arr[7 * u:7 * u + 7] = list(astuple(p))
-    call("sslab_rb_set_backward_policies", ctypes.addressof(arr), len(pols))
+    try:
+        call("sslab_rb_set_backward_policies", ctypes.addressof(arr), len(pols))
+    except Exception:
+        _tls.bwd = None
+        raise
+    _tls.bwd = pols
 
 
 @contextmanager
@@ -68,17 +88,22 @@ def backward_policies(pols):
     if pols is None:
         yield
         return
+    old = getattr(_tls, "bwd", None)
     set_backward_policies(pols)
     try:
         yield
     finally:
-        set_backward_policies([])
+        set_backward_policies(old if old is not None else ())
 
 
 def get_policy() -> Policy:
+    p = getattr(_tls, "policy", None)
+    if p is not None:
+        return p
     arr = (ctypes.c_int * 7)()
     call("sslab_gemm_get_policy", ctypes.addressof(arr), 7)
-    return Policy(*arr)
+    _tls.policy = Policy(*arr)
+    return _tls.policy
 
 
 @contextmanager
