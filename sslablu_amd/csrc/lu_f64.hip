@@ -195,21 +195,23 @@ __device__ __forceinline__ int wave_min(int v) {
   return min(a, b);
 }
 
-// Base panel with its rows in registers: thread t holds panel row t (rows <= NT) over the
-// w <= W panel columns, so the rank-1 updates are register FMAs.  The column loop is fully
-// unrolled so that every register index is static.  Rows never move between lanes during
-// the factorization: each lane carries the row's current position (`pos`), an interchange
-// swaps two positions, and every row is stored at its final position at the end.
+// Base panel with its rows in registers: thread t holds panel rows t + k NT (k < R;
+// rows <= R NT) over the w <= W panel columns, so the rank-1 updates are register FMAs.  The
+// column loop is fully unrolled so that every register index is static.  Rows never move
+// between lanes during the factorization: each row carries its current position (`pos`),
+// an interchange swaps two positions, and every row is stored at its final position at the
+// end.
 //
-// Per column c: each wave finds its best candidate (largest |a| among positions >= c,
-// smallest position on ties) with DPP reductions and publishes (value, position) and the
-// candidate's U row (columns c..W-1) in LDS; after the column's single barrier every thread
-// picks the winning wave and reads the pivot row from its slot.  The LDS slots alternate
-// between two buffers, so a column's writes never meet the previous column's reads.  Same
-// pivot choice as the LDS panel (first index on ties; NaN entries lose to every number, so
-// an all-NaN column keeps its diagonal).  (A rolled loop over a row that shifts one column
-// per step measured slower: it does the full row width of work every column.)
-template <typename E, int NT, int W>
+// Per column c: each lane's candidate is its best row (largest |a| among positions >= c,
+// smallest position on ties), each wave finds its best candidate with DPP reductions and
+// publishes (value, position) and the candidate's U row (columns c..W-1) in LDS; after the
+// column's single barrier every thread picks the winning wave and reads the pivot row from
+// its slot.  The LDS slots alternate between two buffers, so a column's writes never meet
+// the previous column's reads.  Same pivot choice as the LDS panel (first index on ties;
+// NaN entries lose to every number, so an all-NaN column keeps its diagonal).  (A rolled
+// loop over a row that shifts one column per step measured slower: it does the full row
+// width of work every column.)
+template <typename E, int NT, int W, int R = 1>
 __global__ void __launch_bounds__(NT)
 k_getrf_panel_reg(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
                   int* __restrict__ ipiv, long long sP, int* __restrict__ info) {
@@ -225,30 +227,50 @@ k_getrf_panel_reg(int n, int j0, int w, E* __restrict__ A, int lda, long long sA
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wv = tid >> 6;
-  const bool live = tid < rows;
-  int pos = tid;
+  bool live[R];
+  int pos[R];
+  E P[R][W];
+#pragma clang loop unroll(full)
+  for (int k = 0; k < R; ++k) {
+    const int i = tid + k * NT;
+    live[k] = i < rows;
+    pos[k] = i;
+#pragma clang loop unroll(full)
+    for (int j = 0; j < W; ++j) P[k][j] = (live[k] && j < w) ? Ab[(long long)i * lda + j] : E(0);
+  }
   int my_piv = 0;
-  E P[W];
-#pragma unroll
-  for (int j = 0; j < W; ++j) P[j] = (live && j < w) ? Ab[(long long)tid * lda + j] : E(0);
 #pragma clang loop unroll(full)
   for (int c = 0; c < W; ++c) {
     if (c >= w) continue;  // uniform; a full unroll keeps every P index static
     const int buf = c & 1;
     // candidates: live rows at positions >= c; NaN ranks below every number (-0.5),
     // non-candidates below that (-1)
-    const E a = abs_e(P[c]);
-    const E v = (live && pos >= c) ? (a == a ? a : E(-0.5)) : E(-1);
+    E v = E(-1);
+    int vp = INT_MAX;
+#pragma clang loop unroll(full)
+    for (int k = 0; k < R; ++k) {
+      const E a = abs_e(P[k][c]);
+      const E vk = (live[k] && pos[k] >= c) ? (a == a ? a : E(-0.5)) : E(-1);
+      if (vk > v || (vk == v && pos[k] < vp)) {
+        v = vk;
+        vp = pos[k];
+      }
+    }
     const E m = wave_max(v);
-    const int lp = wave_min(v == m ? pos : INT_MAX);
+    const int lp = wave_min(v == m ? vp : INT_MAX);
     if (lane == 0) {
       s_val[buf][wv] = m;
       s_pos[buf][wv] = m < E(-0.75) ? INT_MAX : lp;
     }
-    if (m >= E(-0.75) && pos == lp) {
-      // the U row (columns c..W-1), from an even column so that the stores pair up
-#pragma unroll
-      for (int j = c & ~1; j < W; ++j) s_row[buf][wv][j] = P[j];
+    if (m >= E(-0.75)) {
+#pragma clang loop unroll(full)
+      for (int k = 0; k < R; ++k) {
+        if (pos[k] == lp) {
+          // the U row (columns c..W-1), from an even column so that the stores pair up
+#pragma clang loop unroll(full)
+          for (int j = c & ~1; j < W; ++j) s_row[buf][wv][j] = P[k][j];
+        }
+      }
     }
     __syncthreads();
     // winning wave: lanes 0..NW-1 read the published candidates and the same DPP reductions
@@ -260,26 +282,35 @@ k_getrf_panel_reg(int n, int j0, int w, E* __restrict__ A, int lda, long long sA
     const int q = __ffsll((unsigned long long)__ballot(lane < NW && cv == cm && cp == p)) - 1;
     // the row at position c is always a candidate, so some wave published a row
     const E* prow = s_row[buf][q];
-    if (pos == p) pos = c;
-    else if (pos == c) pos = p;
+#pragma clang loop unroll(full)
+    for (int k = 0; k < R; ++k) {
+      if (pos[k] == p) pos[k] = c;
+      else if (pos[k] == c) pos[k] = p;
+    }
     if (tid == c) my_piv = p;
     const E piv = prow[c];
     if (piv == E(0) && tid == 0 && info != nullptr && info[b] == 0) info[b] = j0 + c + 1;
-    if (live && pos > c) {
-      E l = P[c];
-      if (piv != E(0)) {
-        l = l / piv;
-        P[c] = l;
+#pragma clang loop unroll(full)
+    for (int k = 0; k < R; ++k) {
+      if (live[k] && pos[k] > c) {
+        E l = P[k][c];
+        if (piv != E(0)) {
+          l = l / piv;
+          P[k][c] = l;
+        }
+#pragma clang loop unroll(full)
+        for (int j = c + 1; j < W; ++j) P[k][j] -= l * prow[j];
       }
-#pragma unroll
-      for (int j = c + 1; j < W; ++j) P[j] -= l * prow[j];
     }
   }
   if (tid < w) pv[j0 + tid] = j0 + my_piv;
-  if (live) {
-#pragma unroll
-    for (int j = 0; j < W; ++j)
-      if (j < w) Ab[(long long)pos * lda + j] = P[j];
+#pragma clang loop unroll(full)
+  for (int k = 0; k < R; ++k) {
+    if (live[k]) {
+#pragma clang loop unroll(full)
+      for (int j = 0; j < W; ++j)
+        if (j < w) Ab[(long long)pos[k] * lda + j] = P[k][j];
+    }
   }
 }
 
@@ -542,7 +573,9 @@ int panel_rec(const PanelCtx<E>& c, int j0, int w) {
     const char* e = getenv("SSLAB_PANEL_REG");
     return !(e && e[0] == '0');
   }();
-  const int base = (reg64 && rows <= 512) ? 64 : (bw > 0 ? bw : 32);
+  const int base = (reg64 && rows <= 512) ? 64
+                   : (reg64 && rows > 1024 && rows <= 2048) ? 16
+                                                            : (bw > 0 ? bw : 32);
   if (w <= base) {
     static const bool reg = [] {
       const char* e = getenv("SSLAB_PANEL_REG");
@@ -554,6 +587,12 @@ int panel_rec(const PanelCtx<E>& c, int j0, int w) {
     }
     if (reg && w <= 32 && rows <= 1024) {
       SSLAB_TRY((launch_panel_reg<E, 32>(c, j0, w, rows)));
+      return kOk;
+    }
+    if (reg && w <= 16 && rows <= 2048) {  // two rows per lane, 1024 threads
+      hipLaunchKernelGGL((k_getrf_panel_reg<E, 1024, 16, 2>), dim3(c.batch), dim3(1024), 0, c.st,
+                         c.n, j0, w, c.A, c.lda, c.sA, c.ipiv, c.sP, c.info);
+      SSLAB_HIP_CHECK(hipGetLastError());
       return kOk;
     }
     if (bw > 0) {

@@ -112,6 +112,7 @@ def test_getrf_getrs(torch_dev, n, batch):
     (200, (7, 30, 150, 199)),    # ties inside a wave and across waves
     (450, (70, 71, 300, 449)),   # 512-thread panel
     (900, (100, 640, 899)),      # 32-column register panel, 1024 threads
+    (1500, (6, 1030, 1499)),     # two rows per lane (rows 6 and 1030 share lane 6)
 ])
 def test_getrf_pivot_ties_first_index(torch_dev, n, tie_rows):
     """Exact ties for the largest |a| in the first column go to the first row (LAPACK's
