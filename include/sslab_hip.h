@@ -137,8 +137,8 @@ int sslab_rb_solve_multi(int nsys, const int* nb, const int* m, const int* nrhs,
  * cfg_mid, cfg_split, max_tile}, n = 7.  Tile ids: 0 128x128, 1 128x64, 2 64x64/BK32,
  * 3 64x64/BK16, 4 64x32, 5 32x32, 6 32x64, 7 64x128 (8 32x16 and 9 32x32/BK32 are
  * chosen automatically for launches with at most 16 / 32 columns), 10 / 11 128x64 with A
- * streamed to registers, 12 32x32/BK32.  Results do not depend on the policy beyond
- * floating-point summation order. */
+ * streamed to registers, 12 32x32/BK32, 13 32x32/BK64, 14 64x32/BK32.  Results do not
+ * depend on the policy beyond floating-point summation order. */
 int sslab_gemm_set_policy(const int* p, int n);
 /* Per-step tile policies of the red-black backward sweep for the calling thread's later
  * solves: step u (level L-1-u) runs under p[7u .. 7u+6] (same layout as above); steps past

@@ -881,6 +881,7 @@ static int fast_kind(const GemmLaunchT<T>& P) {
 //  10: 128x64 BK32, 11: 128x64 BK64 -- A streamed to registers (k_gemm_adirect), aligned
 //      launches with at most 64 columns; otherwise the same tile on k_gemm_fast
 //  12: 32x32 BK32 (4 waves): twice the MFMA work of tile 5 per LDS stage and barrier
+//  13: 32x32 BK64 (4 waves), 14: 64x32 BK32 (4 waves)
 using Cfg128 = GemmCfg<128, 128, 16, 2, 2>;
 using Cfg128x64 = GemmCfg<128, 64, 16, 2, 2>;
 using Cfg64 = GemmCfg<64, 64, 32, 2, 2>;
@@ -892,11 +893,13 @@ using Cfg64x128 = GemmCfg<64, 128, 16, 2, 2>;
 using Cfg32x16 = GemmCfg<32, 16, 32, 2, 1>;
 using Cfg32w = GemmCfg<32, 32, 32, 2, 1>;
 using Cfg32k = GemmCfg<32, 32, 32, 2, 2>;
+using Cfg32k64 = GemmCfg<32, 32, 64, 2, 2>;
+using Cfg64x32k = GemmCfg<64, 32, 32, 2, 2>;
 using CfgAD32 = GemmCfg<128, 64, 32, 4, 1>;  // A in registers (k_gemm_adirect)
 using CfgAD64 = GemmCfg<128, 64, 64, 4, 1>;
-constexpr int kNumCfg = 13;
-constexpr int kCfgBM[kNumCfg] = {128, 128, 64, 64, 64, 32, 32, 64, 32, 32, 128, 128, 32};
-constexpr int kCfgBN[kNumCfg] = {128, 64, 64, 64, 32, 32, 64, 128, 16, 32, 64, 64, 32};
+constexpr int kNumCfg = 15;
+constexpr int kCfgBM[kNumCfg] = {128, 128, 64, 64, 64, 32, 32, 64, 32, 32, 128, 128, 32, 32, 64};
+constexpr int kCfgBN[kNumCfg] = {128, 64, 64, 64, 32, 32, 64, 128, 16, 32, 64, 64, 32, 32, 32};
 
 // Workgroups a launch would create with BM x BN tiles.
 template <typename T>
@@ -1074,7 +1077,9 @@ static int launch_id(int id, GemmLaunchT<T>& L, hipStream_t stream, int ks, int 
     case 9: return launch_cfg<Cfg32w, T>(L, stream, ks, maxparts);
     case 10: return launch_cfg<CfgAD32, T, true>(L, stream, ks, maxparts);
     case 11: return launch_cfg<CfgAD64, T, true>(L, stream, ks, maxparts);
-    default: return launch_cfg<Cfg32k, T>(L, stream, ks, maxparts);
+    case 12: return launch_cfg<Cfg32k, T>(L, stream, ks, maxparts);
+    case 13: return launch_cfg<Cfg32k64, T>(L, stream, ks, maxparts);
+    default: return launch_cfg<Cfg64x32k, T>(L, stream, ks, maxparts);
   }
 }
 

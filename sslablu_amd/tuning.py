@@ -126,7 +126,7 @@ def candidates() -> list[Policy]:
     workgroups of a narrow launch are latency-bound, so more of them in flight help)."""
     out = []
     for target, split, cmid, csplit, mt in itertools.product(
-            (512, 1024), (512, 1024, 2048, 4096), (2, 3, 10), (2, 3, 4, 5, 6, 9, 10, 12),
+            (512, 1024), (512, 1024, 2048, 4096), (2, 3, 10), (2, 3, 4, 5, 6, 9, 10, 12, 13, 14),
             (0, 1)):
         out.append(Policy(target, split, 64, 256, cmid, csplit, mt))
     return out

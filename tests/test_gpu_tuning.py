@@ -48,10 +48,10 @@ def test_solve_same_under_all_policies(T, r):
     assert (T.linalg.norm(x - ref) / T.linalg.norm(ref)).item() < 1e-12
 
 
-@pytest.mark.parametrize("cfg", [10, 11, 12])
+@pytest.mark.parametrize("cfg", [10, 11, 12, 13, 14])
 def test_register_streamed_tiles(T, cfg):
     """Tiles 10 / 11 stream A from HBM into MFMA registers (aligned launches with at most
-    64 columns), tile 12 is 32x32 with 32-deep K slabs: GEMMs with ragged M, K tails and
+    64 columns), tiles 12-14 take 32- or 64-deep K slabs: GEMMs with ragged M, K tails and
     split-K, and a red-black solve, match."""
     from sslablu_amd import tuning
     from sslablu_amd._lib import call, ptr, stream_handle
