@@ -40,7 +40,7 @@ CALIB_MIB = 256
 # the step's 4 GEMM dispatches, each counter scaled by its reading for the calibration
 # copy (summary and commands: profiles/rb_fold_solve_c2_helmholtz_hbm_traffic.md).  A
 # measured constant of the current kernels: re-measure after changing them.
-TRAFFIC = 56775689
+TRAFFIC = 56812966
 
 
 class PDO2d:
