@@ -219,7 +219,7 @@ def c4_bench(dev, seed=0, reps=5):
     V, X = B.clone(), torch.empty_like(B)
     # tile policy of this solve shape (a short candidate list: each try is a full solve)
     c4_cands = [tuning.Policy(t, sp, 64, 256, cm, cs, mt) for t, sp, cm, cs, mt in
-                itertools.product((512, 1024), (512, 1024), (2, 3), (2, 3, 5), (0, 1))]
+                itertools.product((512, 1024), (512, 1024), (2, 3), (2, 3, 5, 12, 14), (0, 1))]
     pol_c4, _ = fac.autotune(r, reps=2, cands=c4_cands)
     out["autotuned_policy_solve"] = list(dataclasses.astuple(pol_c4))
 
