@@ -141,10 +141,18 @@ def require_gpu():
         raise SslabError("no ROCm GPU visible: the slab solver runs only on the GPU")
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_cur_device = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def stream_handle(stream=None) -> int:
-    """hipStream_t of `stream` (or torch's current stream) as an int."""
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return int(s.cuda_stream)
+    """hipStream_t of `stream` (or torch's current stream) as an int.  The current stream
+    is read without building a torch Stream object (every small launch pays this)."""
+    if stream is not None:
+        return int(stream.cuda_stream)
+    if _raw_stream is not None and _cur_device is not None:
+        return int(_raw_stream(_cur_device()))
+    return int(torch.cuda.current_stream().cuda_stream)
 
 
 def ptr(t: torch.Tensor) -> int:
