@@ -38,6 +38,34 @@ int sslab_dgemm_strided_batched(int transA, int transB, int M, int N, int K, dou
                                 const double* A, int lda, long long strideA, const double* B,
                                 int ldb, long long strideB, double beta, double* C, int ldc,
                                 long long strideC, int batch, void* stream);
+/* Grouped products: one launch computes every descriptor
+ *   C_b = beta * Cin_b + sum_t alpha_t * op(A_t,b) op(B_t,b),   b < batch,
+ * with up to 3 summed terms per descriptor (terms share M and N; K may differ; X_b =
+ * X + b*strideX, stride 0 broadcasts one matrix over the batch).  Cin may equal C (in-place
+ * accumulation); Cin = NULL requires beta = 0.  The descriptors of one call run in any
+ * order and concurrently, so none may read another's output; more than 12 descriptors
+ * are split over several launches.  Replaces chains of the numpy products of one
+ * tree level in matAssembly/HBS/ULVsparse.py:158-226 and HBSnew.py:306-416 (ULV solve and
+ * HBS apply) with one launch each. */
+typedef struct {
+  const double* A;
+  const double* B;
+  long long strideA, strideB;
+  int lda, ldb, K, transA, transB;
+  double alpha;
+} sslab_gemm_term;
+typedef struct {
+  sslab_gemm_term term[3];
+  int nterms, M, N, batch;
+  const double* Cin;
+  long long strideCin;
+  int ldcin;
+  double* C;
+  long long strideC;
+  int ldc;
+  double beta;
+} sslab_gemm_desc;
+int sslab_dgemm_grouped(const sslab_gemm_desc* descs, int ndesc, void* stream);
 /* FP32 version (v_mfma_f32_16x16x4_f32, FP32 accumulation; K > 0): the sketch products
  * Y = S Omega, Z = S^T Psi of the HBS compression in FP32 (matAssembly/HBS/HBSnew.py:
  * 147-170 forms them in FP64 with numpy). */

@@ -29,6 +29,7 @@ _SIGS = {
     "sslab_dgemm_strided_batched": [_c_int, _c_int, _c_int, _c_int, _c_int, _c_d,
                                     _vp, _c_int, _c_ll, _vp, _c_int, _c_ll, _c_d,
                                     _vp, _c_int, _c_ll, _c_int, _vp],
+    "sslab_dgemm_grouped": [_vp, _c_int, _vp],
     "sslab_sgemm_strided_batched": [_c_int, _c_int, _c_int, _c_int, _c_int, ctypes.c_float,
                                     _vp, _c_int, _c_ll, _vp, _c_int, _c_ll, ctypes.c_float,
                                     _vp, _c_int, _c_ll, _c_int, _vp],

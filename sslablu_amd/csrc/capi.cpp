@@ -57,6 +57,41 @@ int sslab_dgemm_strided_batched(int transA, int transB, int M, int N, int K, dou
                              beta, C, ldc, strideC, batch, as_stream(stream));
 }
 
+int sslab_dgemm_grouped(const sslab_gemm_desc* descs, int ndesc, void* stream) {
+  SSLAB_REQUIRE(ndesc >= 0 && (ndesc == 0 || descs != nullptr), "dgemm_grouped: bad table");
+  sslab::GemmLaunch L{};
+  L.nd = 0;
+  for (int i = 0; i < ndesc; ++i) {
+    const sslab_gemm_desc& e = descs[i];
+    SSLAB_REQUIRE(e.nterms >= 1 && e.nterms <= 3, "dgemm_grouped: descriptor %d: 1..3 terms", i);
+    SSLAB_REQUIRE(e.M >= 0 && e.N >= 0 && e.batch >= 0, "dgemm_grouped: descriptor %d: negative size", i);
+    if (e.M == 0 || e.N == 0 || e.batch == 0) continue;
+    SSLAB_REQUIRE(e.C != nullptr && e.ldc >= e.N, "dgemm_grouped: descriptor %d: bad C", i);
+    SSLAB_REQUIRE(e.beta == 0.0 || (e.Cin != nullptr && e.ldcin >= e.N),
+                  "dgemm_grouped: descriptor %d: beta != 0 needs Cin", i);
+    sslab::GemmDesc d = sslab::gemm_desc(e.M, e.N, e.batch, e.C, e.ldc, e.strideC,
+                                         e.beta != 0.0 ? e.Cin : nullptr, e.ldcin, e.strideCin,
+                                         e.beta);
+    for (int t = 0; t < e.nterms; ++t) {
+      const sslab_gemm_term& m = e.term[t];
+      SSLAB_REQUIRE(m.K >= 1 && m.A != nullptr && m.B != nullptr,
+                    "dgemm_grouped: descriptor %d term %d: K >= 1 and operands required", i, t);
+      SSLAB_REQUIRE(m.lda >= (m.transA ? e.M : m.K) && m.ldb >= (m.transB ? m.K : e.N),
+                    "dgemm_grouped: descriptor %d term %d: bad lda/ldb", i, t);
+      sslab::gemm_add_term(d, sslab::gemm_term(m.A, m.lda, m.strideA, m.transA ? 1 : 0, m.B,
+                                               m.ldb, m.strideB, m.transB ? 1 : 0, m.K,
+                                               m.alpha));
+    }
+    if (L.nd == sslab::kMaxDesc) {
+      SSLAB_TRY(sslab::gemm_launch(L, as_stream(stream)));
+      L.nd = 0;
+    }
+    L.d[L.nd++] = d;
+  }
+  if (L.nd > 0) SSLAB_TRY(sslab::gemm_launch(L, as_stream(stream)));
+  return 0;
+}
+
 int sslab_sgemm_strided_batched(int transA, int transB, int M, int N, int K, float alpha,
                                 const float* A, int lda, long long strideA, const float* B,
                                 int ldb, long long strideB, float beta, float* C, int ldc,

@@ -21,6 +21,8 @@ Differences from the reference, on purpose:
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -61,6 +63,67 @@ def bgemm(A, B, out=None, alpha=1.0, beta=0.0, ta=False, tb=False):
          A.stride(1), _bstride(A), ptr(B), B.stride(1), _bstride(B), float(beta), ptr(out),
          out.stride(1), out.stride(0), nb, stream_handle())
     return out
+
+
+class _GTerm(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("strideA", ctypes.c_longlong),
+                ("strideB", ctypes.c_longlong), ("lda", ctypes.c_int), ("ldb", ctypes.c_int),
+                ("K", ctypes.c_int), ("transA", ctypes.c_int), ("transB", ctypes.c_int),
+                ("alpha", ctypes.c_double)]
+
+
+class _GDesc(ctypes.Structure):
+    _fields_ = [("term", _GTerm * 3), ("nterms", ctypes.c_int), ("M", ctypes.c_int),
+                ("N", ctypes.c_int), ("batch", ctypes.c_int), ("Cin", ctypes.c_void_p),
+                ("strideCin", ctypes.c_longlong), ("ldcin", ctypes.c_int),
+                ("C", ctypes.c_void_p), ("strideC", ctypes.c_longlong), ("ldc", ctypes.c_int),
+                ("beta", ctypes.c_double)]
+
+
+def gdesc(out, terms, cin=None, beta=0.0):
+    """Descriptor of out_b = beta cin_b + sum_t alpha_t op(A_t,b) op(B_t,b) for
+    sslab_dgemm_grouped; terms = [(A, B, alpha[, ta, tb]), ...] (up to 3, (nb, ., .)
+    tensors, a batch of 1 broadcasts).  Returns (descriptor, tensors to keep alive)."""
+    if not 1 <= len(terms) <= 3:
+        raise ValueError("gdesc: 1..3 terms")
+    if out.stride(2) != 1:
+        raise ValueError("gdesc: out needs unit column stride")
+    d = _GDesc()
+    keep = [out]
+    nb, M, N = out.shape
+    for i, t in enumerate(terms):
+        A, B, alpha = t[0], t[1], t[2]
+        ta = bool(t[3]) if len(t) > 3 else False
+        tb = bool(t[4]) if len(t) > 4 else False
+        if A.stride(2) != 1:
+            A = A.contiguous()
+        if B.stride(2) != 1:
+            B = B.contiguous()
+        Ma = A.shape[2] if ta else A.shape[1]
+        K = A.shape[1] if ta else A.shape[2]
+        K2 = B.shape[2] if tb else B.shape[1]
+        Nb = B.shape[1] if tb else B.shape[2]
+        if Ma != M or Nb != N or K != K2 or A.shape[0] not in (1, nb) or B.shape[0] not in (1, nb):
+            raise ValueError(f"gdesc: term {i} does not match the output")
+        keep += [A, B]
+        d.term[i] = _GTerm(ptr(A), ptr(B), _bstride(A), _bstride(B), A.stride(1), B.stride(1),
+                           K, int(ta), int(tb), float(alpha))
+    d.nterms, d.M, d.N, d.batch = len(terms), M, N, nb
+    d.C, d.strideC, d.ldc = ptr(out), out.stride(0), out.stride(1)
+    if cin is not None and beta != 0.0:
+        if tuple(cin.shape) != (nb, M, N) or cin.stride(2) != 1:
+            raise ValueError("gdesc: cin must match the output")
+        keep.append(cin)
+        d.Cin, d.strideCin, d.ldcin, d.beta = ptr(cin), cin.stride(0), cin.stride(1), float(beta)
+    else:
+        d.Cin, d.strideCin, d.ldcin, d.beta = None, 0, N, 0.0
+    return d, keep
+
+
+def ggemm(*descs, stream=None):
+    """Launch descriptors made by gdesc (independent outputs) as one grouped product."""
+    arr = (_GDesc * len(descs))(*[d for d, _ in descs])
+    call("sslab_dgemm_grouped", ctypes.addressof(arr), len(descs), stream_handle(stream))
 
 
 def bqr(A, want_q=True, pivot=False):
@@ -169,8 +232,9 @@ class DeviceHBS:
         for lvl in range(len(Out) - 1, -1, -1):
             M = Out[lvl]
             nb, n, k = M.shape
-            out = bgemm(M, u.view(nb, k, r))
-            bgemm(self.Dmats[lvl], VV[lvl].view(nb, n, r), out, 1.0, 1.0, ta=tD)
+            out = torch.empty(nb, n, r, dtype=F64, device=self.device)
+            ggemm(gdesc(out, [(M, u.view(nb, k, r), 1.0),
+                              (self.Dmats[lvl], VV[lvl].view(nb, n, r), 1.0, tD)]))
             u = out.reshape(nb * n, r)
         if self.perm is not None:
             w = torch.empty_like(u)
@@ -252,8 +316,16 @@ class DeviceHBS:
         for lev in levels[:-1]:
             nb, n, k, Q = lev["nb"], lev["n"], lev["k"], lev["Q"]
             c3 = cur.view(nb, n, r)
-            tops.append(bgemm(Q[:, :, :n - k], c3, ta=True) if n > k else None)
-            cur = bgemm(Q[:, :, n - k:], c3, ta=True).reshape(nb * k, r)
+            nxt = torch.empty(nb, k, r, dtype=F64, device=self.device)
+            if n > k:  # both halves of Q^T c in one launch
+                top = torch.empty(nb, n - k, r, dtype=F64, device=self.device)
+                ggemm(gdesc(top, [(Q[:, :, :n - k], c3, 1.0, True)]),
+                      gdesc(nxt, [(Q[:, :, n - k:], c3, 1.0, True)]))
+                tops.append(top)
+            else:
+                bgemm(Q[:, :, n - k:], c3, nxt, ta=True)
+                tops.append(None)
+            cur = nxt.reshape(nb * k, r)
         root = levels[-1]
         y = bgemm(root["G"], cur.view(1, -1, r))
         v = bgemm(root["D"], y)
@@ -265,17 +337,16 @@ class DeviceHBS:
             y2 = y.view(nb, k, r)
             vb = v.view(nb, k, r)
             x = torch.empty(nb, n, r, dtype=F64, device=self.device)
-            if n > k:
-                rhs0 = tops[lvl].clone()
-                bgemm(lev["Uu"], vb, rhs0, -1.0, 1.0)
-                bgemm(lev["R12"], y2, rhs0, -1.0, 1.0)
+            if n > k:  # summed terms: one launch per dependent product
+                rhs0 = torch.empty_like(tops[lvl])
+                ggemm(gdesc(rhs0, [(lev["Uu"], vb, -1.0), (lev["R12"], y2, -1.0)],
+                            cin=tops[lvl], beta=1.0))
                 y1 = bgemm(lev["R11inv"], rhs0)
-                bgemm(lev["W"][:, :, :n - k], y1, x)
-                bgemm(lev["W"][:, :, n - k:], y2, x, 1.0, 1.0)
+                ggemm(gdesc(x, [(lev["W"][:, :, :n - k], y1, 1.0), (lev["W"][:, :, n - k:], y2, 1.0)]))
             else:
                 bgemm(lev["W"], y2, x)
-            vn = bgemm(lev["U"], vb)
-            bgemm(lev["D"], x, vn, 1.0, 1.0)
+            vn = torch.empty(nb, n, r, dtype=F64, device=self.device)
+            ggemm(gdesc(vn, [(lev["U"], vb, 1.0), (lev["D"], x, 1.0)]))
             v = vn.reshape(nb * n, r)
             y = x.reshape(nb * n, r)
         if self.perm is not None:

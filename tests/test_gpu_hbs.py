@@ -134,3 +134,30 @@ def test_fp32_sketch_compression(T):
         errs[f32] = rel(h.matmat(T.tensor(x, device="cuda")), A @ x)
     assert errs[False] < 1e-9
     assert errs[True] < 1e-5
+
+
+def test_grouped_gemm(T):
+    """sslab_dgemm_grouped: independent descriptors with summed terms (transposed operands,
+    a broadcast batch-1 operand, an input matrix with beta), more than 12 descriptors in
+    one call (split over launches), against numpy."""
+    from sslablu_amd.hbs import gdesc, ggemm
+    rng = np.random.default_rng(9)
+    dev = "cuda"
+    specs = []
+    for i in range(14):
+        nb, M, N = 3, 20 + i, 5 + (i % 4)
+        K1, K2 = 7 + i, 33
+        A1 = rng.standard_normal((nb, K1, M))          # used transposed
+        B1 = rng.standard_normal((nb, K1, N))
+        A2 = rng.standard_normal((1, M, K2))            # broadcast over the batch
+        B2 = rng.standard_normal((nb, N, K2))           # used transposed
+        C0 = rng.standard_normal((nb, M, N))
+        ref = 0.5 * C0 + 2.0 * np.swapaxes(A1, 1, 2) @ B1 - A2 @ np.swapaxes(B2, 1, 2)
+        t = [T.tensor(x, device=dev) for x in (A1, B1, A2, B2, C0)]
+        out = T.empty(nb, M, N, dtype=T.float64, device=dev)
+        d = gdesc(out, [(t[0], t[1], 2.0, True, False), (t[2], t[3], -1.0, False, True)],
+                  cin=t[4], beta=0.5)
+        specs.append((d, out, ref, t))
+    ggemm(*[d for d, _, _, _ in specs])
+    for _, out, ref, _ in specs:
+        assert rel(out, ref) < 1e-13

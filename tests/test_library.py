@@ -75,3 +75,31 @@ def test_python_plan_mirror_matches_library():
             assert n == n_top
             assert _lib.sizes("rb", nb, m)[0] == words * m * m + (n_top * m) ** 2
             assert rb_backward_blocks(nb, m) <= 2 * sum(1 for _ in range(L)) * nb
+
+
+def test_grouped_gemm_struct_layout_matches_header(tmp_path):
+    """The ctypes mirrors of sslab_gemm_term / sslab_gemm_desc (sslablu_amd/hbs.py) have
+    the C layout of include/sslab_hip.h (offsets and sizes from a gcc-compiled probe)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    from sslablu_amd.hbs import _GDesc, _GTerm
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "sslab_hip.h"\n'
+        "int main(void) {\n"
+        '  printf("%zu %zu\\n", sizeof(sslab_gemm_term), sizeof(sslab_gemm_desc));\n'
+        '  printf("%zu %zu %zu %zu %zu\\n", offsetof(sslab_gemm_term, K), offsetof(sslab_gemm_term, alpha),'
+        " offsetof(sslab_gemm_desc, nterms), offsetof(sslab_gemm_desc, Cin), offsetof(sslab_gemm_desc, ldcin));\n"
+        '  printf("%zu %zu %zu\\n", offsetof(sslab_gemm_desc, C), offsetof(sslab_gemm_desc, ldc),'
+        " offsetof(sslab_gemm_desc, beta));\n  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                           text=True).stdout.split()]
+    import ctypes as ct
+    assert vals == [ct.sizeof(_GTerm), ct.sizeof(_GDesc), _GTerm.K.offset, _GTerm.alpha.offset,
+                    _GDesc.nterms.offset, _GDesc.Cin.offset, _GDesc.ldcin.offset,
+                    _GDesc.C.offset, _GDesc.ldc.offset, _GDesc.beta.offset]
