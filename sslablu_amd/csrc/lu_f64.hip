@@ -16,6 +16,7 @@
 #include <cstdlib>
 
 #include "gemm.hpp"
+#include "wave_ops.hpp"
 
 namespace sslab {
 
@@ -138,62 +139,8 @@ k_getrf_panel(int n, int j0, int w, E* __restrict__ A, int lda, long long sA,
   }
 }
 
-// Cross-lane helpers of the register panel.  The wave-wide reductions run on DPP row
-// permutations (xor 1, xor 2, half-row and row mirrors reduce each row of 16 lanes) and four
-// readlanes combine the rows; every lane of the wave must be active.
-template <int kCtrl>
-__device__ __forceinline__ int dpp_i32(int x) {
-  // every lane has a valid source lane for the permutations used here
-  return __builtin_amdgcn_mov_dpp(x, kCtrl, 0xF, 0xF, false);
-}
-template <int kCtrl>
-__device__ __forceinline__ double dpp_e(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = dpp_i32<kCtrl>((int)b);
-  const int hi = dpp_i32<kCtrl>((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-template <int kCtrl>
-__device__ __forceinline__ float dpp_e(float x) {
-  return __int_as_float(dpp_i32<kCtrl>(__float_as_int(x)));
-}
-__device__ __forceinline__ double readlane_e(double x, int l) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)b, l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ float readlane_e(float x, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
-}
-constexpr int kDppXor1 = 0xB1;       // quad_perm [1, 0, 3, 2]
-constexpr int kDppXor2 = 0x4E;       // quad_perm [2, 3, 0, 1]
-constexpr int kDppHalfMirror = 0x141;
-constexpr int kDppMirror = 0x140;
-
-template <typename E>
-__device__ __forceinline__ E max_e(E a, E b) { return a > b ? a : b; }
-
-// Wave maximum of a value that is never NaN (same result in every lane).
-template <typename E>
-__device__ __forceinline__ E wave_max(E v) {
-  v = max_e(v, dpp_e<kDppXor1>(v));
-  v = max_e(v, dpp_e<kDppXor2>(v));
-  v = max_e(v, dpp_e<kDppHalfMirror>(v));
-  v = max_e(v, dpp_e<kDppMirror>(v));
-  const E a = max_e(readlane_e(v, 0), readlane_e(v, 16));
-  const E b = max_e(readlane_e(v, 32), readlane_e(v, 48));
-  return max_e(a, b);
-}
-__device__ __forceinline__ int wave_min(int v) {
-  v = min(v, dpp_i32<kDppXor1>(v));
-  v = min(v, dpp_i32<kDppXor2>(v));
-  v = min(v, dpp_i32<kDppHalfMirror>(v));
-  v = min(v, dpp_i32<kDppMirror>(v));
-  const int a = min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16));
-  const int b = min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48));
-  return min(a, b);
-}
+using wave::wave_max;
+using wave::wave_min;
 
 // Base panel with its rows in registers: thread t holds panel rows t + k NT (k < R;
 // rows <= R NT) over the w <= W panel columns, so the rank-1 updates are register FMAs.  The

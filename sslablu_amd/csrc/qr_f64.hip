@@ -13,6 +13,8 @@
 
 #include <cstdlib>
 
+#include "wave_ops.hpp"
+
 namespace sslab {
 
 namespace {
@@ -259,6 +261,148 @@ k_geqrf_lds(int m, int n, double* __restrict__ A, int lda, long long sA, double*
   }
 }
 
+// The LDS-staged factorization with the complete Q held in registers instead of global
+// memory: wave w holds rows w + 16 k (k < RW) of Q and lane l its columns l + 64 t (t < CI),
+// so applying a reflector to Q needs no memory traffic (each row's product with v is a
+// DPP wave sum), and Q is stored once at the end.  The trailing-column products use the
+// same DPP sums.  For m <= min(16 RW, 64 CI) and matrices whose LDS panel fits; same
+// results as k_geqrf_lds up to the summation order.
+template <int RW, int CI>
+__global__ void __launch_bounds__(kQrLdsThreads)
+k_geqrf_lds_qreg(int m, int n, double* __restrict__ A, int lda, long long sA,
+                 double* __restrict__ Q, int ldq, long long sQ, int* __restrict__ perm,
+                 long long sPerm) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char qr_smem[];
+  double* L = reinterpret_cast<double*>(qr_smem);  // L[c * m + i] = A[i][c]
+  double* v = L + (long long)m * n;
+  double* cn = v + m;
+  __shared__ double s_tau, s_beta, s_scale;
+  __shared__ int s_piv;
+  const int b = blockIdx.x;
+  double* Ab = A + (long long)b * sA;
+  double* Qb = Q + (long long)b * sQ;
+  int* pb = perm ? perm + (long long)b * sPerm : nullptr;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  for (int e = tid; e < m * n; e += kQrLdsThreads) {
+    const int i = e / n, c = e - (e / n) * n;
+    L[(long long)c * m + i] = Ab[(long long)i * lda + c];
+  }
+  if (pb)
+    for (int c = tid; c < n; c += kQrLdsThreads) pb[c] = c;
+  double q[RW][CI];
+#pragma unroll
+  for (int k = 0; k < RW; ++k)
+#pragma unroll
+    for (int t = 0; t < CI; ++t) q[k][t] = (wave + 16 * k == lane + 64 * t) ? 1.0 : 0.0;
+  __syncthreads();
+
+  const int kmax = m < n ? m : n;
+  for (int j = 0; j < kmax; ++j) {
+    if (pb) {
+      for (int c = j + wave; c < n; c += kQrLdsWaves) {
+        const double* col = L + (long long)c * m;
+        double s = 0.0;
+        for (int i = j + lane; i < m; i += 64) s += col[i] * col[i];
+        s = wave::wave_sum(s);
+        if (lane == 0) cn[c] = s;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int p = j;
+        double best = cn[j];
+        for (int c = j + 1; c < n; ++c)
+          if (cn[c] > best) {
+            best = cn[c];
+            p = c;
+          }
+        s_piv = p;
+        if (p != j) {
+          const int t = pb[j];
+          pb[j] = pb[p];
+          pb[p] = t;
+        }
+      }
+      __syncthreads();
+      const int p = s_piv;
+      if (p != j) {
+        double* cj = L + (long long)j * m;
+        double* cp = L + (long long)p * m;
+        for (int i = tid; i < m; i += kQrLdsThreads) {
+          const double t = cj[i];
+          cj[i] = cp[i];
+          cp[i] = t;
+        }
+      }
+      __syncthreads();
+    }
+    double* colj = L + (long long)j * m;
+    if (wave == 0) {
+      double s = 0.0;
+      for (int i = j + 1 + lane; i < m; i += 64) s += colj[i] * colj[i];
+      s = wave::wave_sum(s);
+      if (lane == 0) {
+        const double alpha = colj[j];
+        if (s == 0.0) {
+          s_tau = 0.0;
+          s_beta = alpha;
+          s_scale = 0.0;
+        } else {
+          const double nrm = sqrt(alpha * alpha + s);
+          const double beta = alpha >= 0.0 ? -nrm : nrm;
+          s_tau = (beta - alpha) / beta;
+          s_beta = beta;
+          s_scale = 1.0 / (alpha - beta);
+        }
+      }
+    }
+    __syncthreads();
+    const double tau = s_tau;
+    for (int i = j + tid; i < m; i += kQrLdsThreads) v[i] = (i == j) ? 1.0 : colj[i] * s_scale;
+    __syncthreads();
+    for (int i = j + tid; i < m; i += kQrLdsThreads) colj[i] = (i == j) ? s_beta : 0.0;
+    if (tau != 0.0) {
+      for (int c = j + 1 + wave; c < n; c += kQrLdsWaves) {
+        double* col = L + (long long)c * m;
+        double w = 0.0;
+        for (int i = j + lane; i < m; i += 64) w += v[i] * col[i];
+        w = wave::wave_sum(w) * tau;
+        for (int i = j + lane; i < m; i += 64) col[i] -= w * v[i];
+      }
+      // Q <- Q (I - tau v v^T) on the register tile (v is zero above row j)
+      double vv[CI];
+#pragma unroll
+      for (int t = 0; t < CI; ++t) {
+        const int i = lane + 64 * t;
+        vv[t] = (i >= j && i < m) ? v[i] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < RW; ++k) {
+        double part = 0.0;
+#pragma unroll
+        for (int t = 0; t < CI; ++t) part += q[k][t] * vv[t];
+        const double w = wave::wave_sum(part) * tau;
+#pragma unroll
+        for (int t = 0; t < CI; ++t) q[k][t] -= w * vv[t];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int r = wave + 16 * k;
+#pragma unroll
+    for (int t = 0; t < CI; ++t) {
+      const int i = lane + 64 * t;
+      if (r < m && i < m) Qb[(long long)r * ldq + i] = q[k][t];
+    }
+  }
+  for (int e = tid; e < m * n; e += kQrLdsThreads) {
+    const int i = e / n, c = e - (e / n) * n;
+    Ab[(long long)i * lda + c] = L[(long long)c * m + i];
+  }
+}
+
 // Y[b][blk*bs + i][blk*bs + j] += X[b*nblk + blk][i][j]: add nblk diagonal bs x bs blocks.
 __global__ void k_add_diag_blocks(int bs, int nblk, const double* __restrict__ X,
                                   double* __restrict__ Y, int ldy, long long sY) {
@@ -292,6 +436,30 @@ int geqrf_batched(int m, int n, double* A, int lda, long long sA, double* Q, int
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_geqrf_lds),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
       attr = true;
+    }
+    static const bool qreg = [] {
+      const char* e = getenv("SSLAB_QR_QREG");
+      return !(e && e[0] == '0');
+    }();
+    if (qreg && Q != nullptr && m <= 192) {
+      // the complete Q in registers: ceil(m / 16) rows per wave, ceil(m / 64) columns per lane
+#define SSLAB_QR_QREG_LAUNCH(RW, CI)                                                          \
+  do {                                                                                          \
+    static bool attr_q = false;                                                                 \
+    if (!attr_q) {                                                                              \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_geqrf_lds_qreg<RW, CI>),      \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);       \
+      attr_q = true;                                                                            \
+    }                                                                                           \
+    hipLaunchKernelGGL((k_geqrf_lds_qreg<RW, CI>), dim3(batch), dim3(kQrLdsThreads), lds, st, m, \
+                       n, A, lda, sA, Q, ldq, sQ, perm, sPerm);                                 \
+  } while (0)
+      if (m <= 64) SSLAB_QR_QREG_LAUNCH(4, 1);
+      else if (m <= 128) SSLAB_QR_QREG_LAUNCH(8, 2);
+      else SSLAB_QR_QREG_LAUNCH(12, 3);
+#undef SSLAB_QR_QREG_LAUNCH
+      SSLAB_HIP_CHECK(hipGetLastError());
+      return kOk;
     }
     hipLaunchKernelGGL(k_geqrf_lds, dim3(batch), dim3(kQrLdsThreads), lds, st, m, n, A, lda, sA,
                        Q, ldq, sQ, perm, sPerm);
