@@ -429,12 +429,18 @@ def c3_bench(dev, seed=0, rk=40, nrhs=16):
                    for sd in ("left", "right"))
 
     # sketch products Y = S Omega, Z = S^T Psi in FP32 (MFMA), QR / bases in FP64; the
-    # FP64-sketch compression of the same blocks is reported beside it
-    H, t_c = compress(True)
+    # FP64-sketch compression of the same blocks is reported beside it.  Both are warmed
+    # once (first-use allocations and code loads), then the best of 3 is reported.
+    def best_compress(fp32):
+        compress(fp32)
+        runs = [compress(fp32) for _ in range(3)]
+        return runs[-1][0], min(t for _, t in runs)
+
+    H, t_c = best_compress(True)
     out["sketch"] = "fp32 sketch GEMMs (v_mfma_f32), fp64 QR and interpolative bases"
     out["compress_s"] = round(t_c, 3)
     out["compression_rel_err"] = comp_err(H)
-    H64, t_c64 = compress(False)
+    H64, t_c64 = best_compress(False)
     out["compress_fp64_sketch_s"] = round(t_c64, 3)
     out["compression_rel_err_fp64_sketch"] = comp_err(H64)
     del H64

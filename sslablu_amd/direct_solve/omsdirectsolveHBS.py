@@ -293,7 +293,9 @@ class ThomasSolverHBS(DirectSolver):
         for i in range(1, n + 1):
             op = Sprime_Linop(Sl[i - 1], Sprime[i - 1], Sr[i - 1], id=(i == 1))
             h = self._hbsmat(op, tree, quad)
-            h.construct(rk, compute_ULV=True)
+            h.construct(rk)
+            # S'_i^{-1} and S'_i^{-T} both sample S'_{i+1}: factor the two modes at once
+            h.compute_ULV(both=i < n)
             Sprime.append(h)
         self.A, self.B, self.C = Sl, Sprime, Sr
 
@@ -306,7 +308,8 @@ class ThomasSolverHBS(DirectSolver):
         B = [D_list[0] if hasattr(D_list[0], "solve") else dense_to_linop(D_list[0])]
         for i in range(1, n + 1):
             h = self._hbsmat(Dprime_Linop(D_list[i], A[i - 1], C[i - 1], B[-1]), tree, quad)
-            h.construct(rk, compute_ULV=True)
+            h.construct(rk)
+            h.compute_ULV(both=i < n)
             B.append(h)
         self.A, self.B, self.C = A, B, C
 

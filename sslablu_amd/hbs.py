@@ -32,6 +32,26 @@ from ._lib import call, ptr, stream_handle
 F64 = torch.float64
 I32 = torch.int32
 
+_PERMS: dict = {}
+
+
+def device_perm(perm, device):
+    """Leaf permutation as an int64 device tensor, or None for the identity (no gathers).
+    Host arrays are uploaded once per distinct permutation and device: every compression
+    of a slab elimination reuses its tree's copy instead of a blocking pageable upload."""
+    if perm is None:
+        return None
+    if isinstance(perm, torch.Tensor) and perm.is_cuda:
+        return perm.to(device=device, dtype=torch.int64)
+    p = np.ascontiguousarray(np.asarray(perm), dtype=np.int64)
+    key = (str(torch.device(device)), p.tobytes())
+    if key not in _PERMS:
+        if len(_PERMS) >= 16:
+            _PERMS.clear()
+        ident = p.ndim == 1 and np.array_equal(p, np.arange(p.size))
+        _PERMS[key] = None if ident else torch.as_tensor(p).to(device)
+    return _PERMS[key]
+
 
 # ---------------------------------------------------------------------------
 # batched primitives on (nb, rows, cols) device tensors with unit last stride
@@ -166,7 +186,7 @@ class DeviceHBS:
         self.device = self.Dmats[0].device
         N = self.Dmats[0].shape[0] * self.Dmats[0].shape[1]
         self.N = N
-        self.perm = None if perm is None else torch.as_tensor(np.asarray(perm)).to(self.device).long()
+        self.perm = device_perm(perm, self.device)
         self.Nbvec = [D.shape[0] for D in self.Dmats]
         self._ulv = {}
 
@@ -199,8 +219,7 @@ class DeviceHBS:
             Y3 = Yl.reshape(nb, n, s)
             Z3 = Zl.reshape(nb, n, s)
             if lvl > 0:
-                U, pinvO = _basis_and_pinv(O3, Y3, rkm)
-                V, pinvP = _basis_and_pinv(P3, Z3, rkm)
+                U, V, pinvO, pinvP = _basis_and_pinv_pair(O3, P3, Y3, Z3, rkm)
                 T1 = bgemm(Y3, pinvO)
                 bgemm(U, bgemm(U, T1, ta=True), T1, -1.0, 1.0)
                 T2 = bgemm(Z3, pinvP)
@@ -251,7 +270,7 @@ class DeviceHBS:
 
     def transpose(self):
         return DeviceHBS(self.Vmats, [_t(D).contiguous() for D in self.Dmats], self.Umats,
-                         self.fac, None if self.perm is None else self.perm.cpu().numpy())
+                         self.fac, self.perm)
 
     # -- ULV (ULVsparse.py:231-358) -----------------------------------------------------
     def ulv_factor(self, mode="N"):
@@ -302,6 +321,37 @@ class DeviceHBS:
             Uhat = Ud
         self._ulv[mode] = levels
         return levels
+
+    def ulv_factor_pair(self):
+        """The ULV factors of A and A^T from one stacked factorization: the transposed
+        HBS (V, D^T, U) is appended to this one along every level's batch (block p of
+        the second half has its children at 2p, 2p+1 of the next level's second half,
+        as in the first), so each level's QRs and products run once at twice the batch
+        instead of twice in sequence.  Caches both modes; returns the mode-N levels."""
+        if "N" in self._ulv and "T" in self._ulv:
+            return self._ulv["N"]
+        if len(self.Dmats) < 2:
+            self.ulv_factor("T")
+            return self.ulv_factor("N")
+        st = DeviceHBS([torch.cat([U, V]) for U, V in zip(self.Umats, self.Vmats)],
+                       [torch.cat([D, _t(D)]) for D in self.Dmats],
+                       [torch.cat([V, U]) for U, V in zip(self.Umats, self.Vmats)], self.fac)
+        both = st.ulv_factor("N")
+        halves = ([], [])
+        for lev in both:
+            for h, out in enumerate(halves):
+                d = {}
+                for key, val in lev.items():
+                    if isinstance(val, torch.Tensor):
+                        nb2 = val.shape[0]
+                        d[key] = val[: nb2 // 2] if h == 0 else val[nb2 // 2:]
+                    elif key == "nb":
+                        d[key] = val // 2
+                    else:
+                        d[key] = val
+                out.append(d)
+        self._ulv["N"], self._ulv["T"] = halves
+        return self._ulv["N"]
 
     def solve(self, b: torch.Tensor, mode="N") -> torch.Tensor:
         """x with A x = b (mode N) or A^T x = b (mode T), b of shape (N,) or (N, r)."""
@@ -354,6 +404,27 @@ class DeviceHBS:
             w[self.perm] = y
             y = w
         return y[:, 0] if one_d else y
+
+
+def _basis_and_pinv_pair(O3, P3, Y3, Z3, rk):
+    """_basis_and_pinv of the column side (O3, Y3) and the row side (P3, Z3) of one level
+    in stacked launches of twice the batch: the two sides are independent, and each
+    batched QR runs one workgroup per box, so the stacked launch costs about what one
+    side's does."""
+    nb, n, s = O3.shape
+    if s <= n:
+        raise ValueError(f"HBS compression needs more samples than box rows ({s} <= {n})")
+    OP = torch.empty(2 * nb, s, n, dtype=F64, device=O3.device)
+    OP[:nb].copy_(O3.transpose(1, 2))
+    OP[nb:].copy_(P3.transpose(1, 2))
+    Q, R, _ = bqr(OP)                             # (2nb, s, s), R (2nb, s, n)
+    Rinv = btri_inv(R[:, :n, :n].contiguous())
+    pinv = bgemm(Q[:, :, :n], Rinv, tb=True)      # (2nb, s, n)
+    YZ = torch.cat([Y3, Z3])
+    M = bgemm(YZ, Q[:, :, n:])                    # (2nb, n, s - n)
+    QM, _, _ = bqr(M, pivot=True)
+    B = QM[:, :, :rk]
+    return B[:nb].contiguous(), B[nb:].contiguous(), pinv[:nb], pinv[nb:]
 
 
 def _basis_and_pinv(Om3, Y3, rk):

@@ -10,20 +10,61 @@ dtype, nSamples, tSample, tCompress.  The numerics run in sslablu_amd.hbs.Device
 The operator A may be a float64 device tensor (sketched with MFMA GEMMs), an object
 whose matmat/rmatmat take and return device tensors (attribute ``device_op = True``),
 or any scipy-style LinearOperator / numpy array (sketched on the host, then uploaded).
-Sketches use numpy.random.default_rng(seed) (seed attribute, default 0) where the
-reference draws unseeded samples (HBSnew.py:218-219).
+Gaussian test matrices are drawn on the device from a torch.Generator seeded with the
+seed attribute (default 0) where the reference draws unseeded numpy samples
+(HBSnew.py:218-219); nothing crosses PCIe for a device operator.
 """
 from __future__ import annotations
 
-import time
 
 import numpy as np
 import torch
 
 from ... import _lib
-from ...hbs import DeviceHBS, F64
+from ...hbs import DeviceHBS, F64, device_perm
 
 fromnumpy = lambda X, dev: torch.as_tensor(np.ascontiguousarray(X, dtype=np.float64)).to(dev)
+
+
+class _StageClock:
+    """Seconds spent in a construction stage, measured with HIP events on the current
+    stream: no host synchronisation while the stages run (the slab eliminations queue
+    the next compression behind this one); the events are read, and waited for, only
+    when the time is asked for."""
+
+    def __init__(self):
+        self.base = 0.0
+        self.open = []
+
+    def start(self):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    def stop(self, e0):
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record()
+        self.open.append((e0, e1))
+
+    def value(self):
+        for e0, e1 in self.open:
+            e1.synchronize()
+            self.base += e0.elapsed_time(e1) / 1e3
+        self.open = []
+        return self.base
+
+    def set(self, v):
+        self.base = float(v)
+        self.open = []
+
+
+def _stage_time(name):
+    def get(self):
+        return self._clocks[name].value()
+
+    def put(self, v):
+        self._clocks[name].set(v)
+    return property(get, put)
 
 
 def _apply_op(A, X, transpose, device):
@@ -44,6 +85,9 @@ def _dense_apply(Ad, X, transpose):
 
 class HBSMAT:
     """HBS approximation of a square operator on a cluster tree (HBSnew.py:65-416)."""
+
+    tSample = _stage_time("sample")
+    tCompress = _stage_time("compress")
 
     def __init__(self, A=None, tree=None, quad=True, device=None, seed=0, sketch_fp32=False):
         """sketch_fp32: for a dense A, form the sketches Y = A Omega, Z = A^T Psi with the
@@ -73,9 +117,8 @@ class HBSMAT:
         self.quad = quad
         self.fac = 4 if quad else 2
         self.blockSolveTime = self.nullTime = self.setupTime = self.DTime = 0.0
-        self.tSample = 0.0
+        self._clocks = {"sample": _StageClock(), "compress": _StageClock()}
         self.tConstruct = 0.0
-        self.tCompress = 0.0
         self.Nbvec = []
 
     # -- set-up --------------------------------------------------------------------
@@ -128,47 +171,54 @@ class HBSMAT:
         s = self._samples(rk)
         self.nSamples = s
         dev = self.device
-        t0 = time.time()
-        rng = np.random.default_rng(self.seed)
-        Om = rng.standard_normal((N, s))
-        Psi = rng.standard_normal((N, s))
-        perm = np.asarray(self.perm)
-        Omp = np.zeros_like(Om)
-        Omp[perm] = Om
-        Psp = np.zeros_like(Psi)
-        Psp[perm] = Psi
-        pt = torch.as_tensor(perm).to(dev)
+        clk = self._clocks
+        ev = clk["sample"].start()
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(int(self.seed))
+        Om = torch.randn(N, s, generator=gen, device=dev, dtype=F64)
+        Psi = torch.randn(N, s, generator=gen, device=dev, dtype=F64)
+        pt = device_perm(self.perm, dev)      # None: identity leaf order, no gathers
+        if pt is None:
+            Omp, Psp = Om, Psi
+            rows = lambda X: X.contiguous()
+        else:
+            Omp = torch.empty_like(Om)
+            Omp[pt] = Om
+            Psp = torch.empty_like(Psi)
+            Psp[pt] = Psi
+            rows = lambda X: X[pt].contiguous()
         if self.sketch_fp32 and isinstance(self.A, torch.Tensor):
             from ...ops import sgemm
             f32 = torch.float32
             A32 = self.A.to(device=dev, dtype=f32)
-            Y = sgemm(A32, fromnumpy(Omp, dev).to(f32)).to(F64)[pt].contiguous()
-            Z = sgemm(A32.t(), fromnumpy(Psp, dev).to(f32)).to(F64)[pt].contiguous()
+            Y = rows(sgemm(A32, Omp.to(f32)).to(F64))
+            Z = rows(sgemm(A32.t(), Psp.to(f32)).to(F64))
             del A32
         else:
-            Y = _apply_op(self.A, fromnumpy(Omp, dev), False, dev)[pt].contiguous()
-            Z = _apply_op(self.A, fromnumpy(Psp, dev), True, dev)[pt].contiguous()
-        torch.cuda.synchronize(dev)
-        self.tSample += time.time() - t0
-        t0 = time.time()
-        self._h = DeviceHBS.from_sketches(fromnumpy(Om, dev), fromnumpy(Psi, dev), Y, Z,
-                                          self.Nb, self.L, rk, self.fac, perm)
+            Y = rows(_apply_op(self.A, Omp, False, dev))
+            Z = rows(_apply_op(self.A, Psp, True, dev))
+        del Omp, Psp
+        clk["sample"].stop(ev)
+        clk["compress"].set(0.0)
+        ev = clk["compress"].start()
+        self._h = DeviceHBS.from_sketches(Om, Psi, Y, Z, self.Nb, self.L, rk, self.fac, pt)
         self.Nbvec = self._h.Nbvec
-        torch.cuda.synchronize(dev)
-        self.tCompress = time.time() - t0
+        clk["compress"].stop(ev)
 
     def constructHBS_ULV(self, rk):
         self.constructHBS(rk)
         self.compute_ULV()
 
-    def compute_ULV(self):
-        t0 = time.time()
-        lev = self._h.ulv_factor("N")
+    def compute_ULV(self, both=False):
+        """ULV factors for solve(mode="N"); with both=True those of the transpose too
+        (mode "T", otherwise factored on its first use), from one stacked factorization."""
+        ev = self._clocks["compress"].start()
+        lev = self._h.ulv_factor_pair() if both else self._h.ulv_factor("N")
         self.Qlist = [l["Q"].reshape(-1, l["Q"].shape[2]) if not l["root"] else None for l in lev]
         self.Wlist = [l["W"].reshape(-1, l["W"].shape[2]) for l in lev if not l["root"]]
         self.Uulist = [l["Uu"] for l in lev if not l["root"]]
         self.Rlist = [l["R11inv"] for l in lev if not l["root"]]
-        self.tCompress += time.time() - t0
+        self._clocks["compress"].stop(ev)
 
     # -- attributes in the reference's stacked 2-D form --------------------------------
     @property

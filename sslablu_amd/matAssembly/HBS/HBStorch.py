@@ -15,12 +15,9 @@ reference pins them in host memory, :236-239); 288 GB of HBM holds them.
 """
 from __future__ import annotations
 
-import time
-
-import numpy as np
 import torch
 
-from ...hbs import DeviceHBS
+from ...hbs import DeviceHBS, device_perm
 from .HBSnew import HBSMAT as _HBSnewMAT
 from .HBSnew import fromnumpy
 
@@ -39,23 +36,22 @@ class HBSMAT(_HBSnewMAT):
 
     def constructHBS(self, rk, Om0, Psi0, Y0, Z0, fast=False):
         dev = self.device
-        t0 = time.time()
-        pt = torch.as_tensor(np.asarray(self.perm, dtype=np.int64)).to(dev)
+        clk = self._clocks
+        ev = clk["sample"].start()
+        pt = device_perm(self.perm, dev)
 
         def perm_rows(X):
             X = fromnumpy(X, dev) if not isinstance(X, torch.Tensor) else X.to(dev, torch.float64)
-            return X[pt].contiguous()
+            return X.contiguous() if pt is None else X[pt].contiguous()
 
         Om, Psi, Y, Z = (perm_rows(X) for X in (Om0, Psi0, Y0, Z0))
         self.nSamples = Om.shape[1]
-        torch.cuda.synchronize(dev)
-        self.tSample += time.time() - t0
-        t0 = time.time()
-        self._h = DeviceHBS.from_sketches(Om, Psi, Y, Z, self.Nb, self.L, rk, self.fac,
-                                          np.asarray(self.perm))
+        clk["sample"].stop(ev)
+        clk["compress"].set(0.0)
+        ev = clk["compress"].start()
+        self._h = DeviceHBS.from_sketches(Om, Psi, Y, Z, self.Nb, self.L, rk, self.fac, pt)
         self.Nbvec = self._h.Nbvec
-        torch.cuda.synchronize(dev)
-        self.tCompress = time.time() - t0
+        clk["compress"].stop(ev)
 
     def constructHBS_ULV(self, rk, fast=False):
         _HBSnewMAT.constructHBS(self, rk)

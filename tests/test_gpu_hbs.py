@@ -89,6 +89,13 @@ def test_ulv_solve(T, k, nl, L, fac):
     assert rel(H.solve(bt), np.linalg.solve(A, b)) < 1e-10
     assert rel(H.solve(bt, "T"), np.linalg.solve(A.T, b)) < 1e-10
     assert rel(H.solve(bt[:, 0]), np.linalg.solve(A, b[:, 0])) < 1e-10
+    # both modes from one stacked factorization (ulv_factor_pair) solve the same systems
+    H2 = _device_hbs(T, U, D, V, fac)
+    H2.ulv_factor_pair()
+    assert set(H2._ulv) == {"N", "T"}
+    assert rel(H2.solve(bt), np.linalg.solve(A, b)) < 1e-10
+    assert rel(H2.solve(bt, "T"), np.linalg.solve(A.T, b)) < 1e-10
+    assert rel(H2.solve(bt, "T"), H.solve(bt, "T").cpu().numpy()) < 1e-12
 
 
 @pytest.mark.parametrize("M,N,K,ta,tb", [(300, 130, 500, 0, 0), (1024, 90, 1024, 1, 0),
@@ -132,6 +139,15 @@ def test_fp32_sketch_compression(T):
         h.construct(k)
         x = rng.standard_normal((A.shape[0], 5))
         errs[f32] = rel(h.matmat(T.tensor(x, device="cuda")), A @ x)
+        # stage times come from HIP events, resolved when read; compute_ULV adds to tCompress
+        ts, tc = h.tSample, h.tCompress
+        assert ts > 0 and tc > 0
+        h.compute_ULV()
+        assert h.tCompress > tc and h.tSample == ts
+        # the seeded device generator repeats its draws (split-K atomics may reorder sums)
+        h2 = HBSnew.HBSMAT(At, _Tree(A.shape[0], nl, L), quad=False, seed=4, sketch_fp32=f32)
+        h2.construct(k)
+        assert rel(h2._h.Dmats[-1], h._h.Dmats[-1].cpu().numpy()) < 1e-12
     assert errs[False] < 1e-9
     assert errs[True] < 1e-5
 
