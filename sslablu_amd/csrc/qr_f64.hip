@@ -63,19 +63,28 @@ k_geqrf(int m, int n, double* __restrict__ A, int lda, long long sA, double* __r
         if (lane == 0) cn[c] = s;
       }
       __syncthreads();
-      if (tid == 0) {
-        int p = j;
-        double best = cn[j];
-        for (int c = j + 1; c < n; ++c)
-          if (cn[c] > best) {
-            best = cn[c];
-            p = c;
+      if (wave == 0) {
+        // first column of largest norm (the serial scan's choice): each lane scans its
+        // columns in order, then the wave takes the maximum and the smallest index at it
+        double best = -1.0;
+        int bi = 0x7fffffff;
+        for (int c = j + lane; c < n; c += 64) {
+          const double x = cn[c];
+          if (x > best) {
+            best = x;
+            bi = c;
           }
-        s_piv = p;
-        if (p != j) {
-          const int t = pb[j];
-          pb[j] = pb[p];
-          pb[p] = t;
+        }
+        const double mx = wave::wave_max(best);
+        int p = wave::wave_min(best == mx ? bi : 0x7fffffff);
+        if (p >= n) p = j;  // no finite norm: keep the column
+        if (lane == 0) {
+          s_piv = p;
+          if (p != j) {
+            const int t = pb[j];
+            pb[j] = pb[p];
+            pb[p] = t;
+          }
         }
       }
       __syncthreads();
@@ -184,19 +193,28 @@ k_geqrf_lds(int m, int n, double* __restrict__ A, int lda, long long sA, double*
         if (lane == 0) cn[c] = s;
       }
       __syncthreads();
-      if (tid == 0) {
-        int p = j;
-        double best = cn[j];
-        for (int c = j + 1; c < n; ++c)
-          if (cn[c] > best) {
-            best = cn[c];
-            p = c;
+      if (wave == 0) {
+        // first column of largest norm (the serial scan's choice): each lane scans its
+        // columns in order, then the wave takes the maximum and the smallest index at it
+        double best = -1.0;
+        int bi = 0x7fffffff;
+        for (int c = j + lane; c < n; c += 64) {
+          const double x = cn[c];
+          if (x > best) {
+            best = x;
+            bi = c;
           }
-        s_piv = p;
-        if (p != j) {
-          const int t = pb[j];
-          pb[j] = pb[p];
-          pb[p] = t;
+        }
+        const double mx = wave::wave_max(best);
+        int p = wave::wave_min(best == mx ? bi : 0x7fffffff);
+        if (p >= n) p = j;  // no finite norm: keep the column
+        if (lane == 0) {
+          s_piv = p;
+          if (p != j) {
+            const int t = pb[j];
+            pb[j] = pb[p];
+            pb[p] = t;
+          }
         }
       }
       __syncthreads();
@@ -308,19 +326,28 @@ k_geqrf_lds_qreg(int m, int n, double* __restrict__ A, int lda, long long sA,
         if (lane == 0) cn[c] = s;
       }
       __syncthreads();
-      if (tid == 0) {
-        int p = j;
-        double best = cn[j];
-        for (int c = j + 1; c < n; ++c)
-          if (cn[c] > best) {
-            best = cn[c];
-            p = c;
+      if (wave == 0) {
+        // first column of largest norm (the serial scan's choice): each lane scans its
+        // columns in order, then the wave takes the maximum and the smallest index at it
+        double best = -1.0;
+        int bi = 0x7fffffff;
+        for (int c = j + lane; c < n; c += 64) {
+          const double x = cn[c];
+          if (x > best) {
+            best = x;
+            bi = c;
           }
-        s_piv = p;
-        if (p != j) {
-          const int t = pb[j];
-          pb[j] = pb[p];
-          pb[p] = t;
+        }
+        const double mx = wave::wave_max(best);
+        int p = wave::wave_min(best == mx ? bi : 0x7fffffff);
+        if (p >= n) p = j;  // no finite norm: keep the column
+        if (lane == 0) {
+          s_piv = p;
+          if (p != j) {
+            const int t = pb[j];
+            pb[j] = pb[p];
+            pb[p] = t;
+          }
         }
       }
       __syncthreads();

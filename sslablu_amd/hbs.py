@@ -284,6 +284,19 @@ class DeviceHBS:
         levels = []
         Uhat = R22 = None
         nlev = len(self.Dmats)
+        # Q of every level's V depends on V alone: one QR launch per distinct box shape
+        # (leaf level and upper levels) instead of one per level in the sequential sweep
+        QVs = [None] * (nlev - 1)
+        groups: dict = {}
+        for lvl in range(nlev - 1):
+            groups.setdefault(tuple(self.Vmats[lvl].shape[1:]), []).append(lvl)
+        for lvls in groups.values():
+            Qs, _, _ = bqr(torch.cat([self.Vmats[lvl] for lvl in lvls]))
+            off = 0
+            for lvl in lvls:
+                nbl = self.Vmats[lvl].shape[0]
+                QVs[lvl] = Qs[off:off + nbl]
+                off += nbl
         for lvl in range(nlev):
             D = self.Dmats[lvl]
             if lvl == 0:
@@ -303,7 +316,7 @@ class DeviceHBS:
             V = self.Vmats[lvl]
             U = self.Umats[lvl]
             nb, n, k = V.shape
-            QV, _, _ = bqr(V)
+            QV = QVs[lvl]
             W = torch.cat([QV[:, :, k:], V], dim=2).contiguous()
             Q, R, _ = bqr(bgemm(Rhat, W))
             if lvl == 0:
