@@ -150,7 +150,8 @@ int sslab_rb_solve_ex(int nb, int m, int nrhs, const double* factor, double* V, 
                       double* X, int ldx, int flags, void* stream);
 
 /* Several independent systems (any mix of nb, m, nrhs) solved together: each
- * elimination level of all systems shares one grouped GEMM launch.  Arrays of length
+ * elimination level of all systems with nrhs <= 32 shares one grouped GEMM launch;
+ * systems with more right-hand sides keep their own launches.  Arrays of length
  * nsys (host memory) give each system's sizes, factor (from sslab_rb_factor), V and X
  * exactly as for sslab_rb_solve.  No reference counterpart (the reference solves one
  * system per call, direct_solve/omsdirectsolve.py:221-279); nsys <= 64. */

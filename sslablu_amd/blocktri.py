@@ -396,9 +396,10 @@ def rb_backward_blocks(nb: int, m: int) -> int:
 
 
 def solve_multi(factors, Vs, Xs, stream=None):
-    """Solve several red-black-factored systems together: every elimination level of all
-    systems shares one grouped GEMM launch (sslab_rb_solve_multi), so systems of
-    different block sizes and right-hand-side counts go through the sweeps at once.
+    """Solve several red-black-factored systems together (sslab_rb_solve_multi): every
+    elimination level of the systems with at most 32 right-hand sides shares one grouped
+    GEMM launch, so many small systems of different block sizes go through the sweeps at
+    once; wider systems keep their own launches, which fill the chip by themselves.
     Vs[i] (overwritten) / Xs[i]: float64 device tensors (nb_i*m_i, r_i), unit column
     stride, as for BlockTriFactor.solve_."""
     import ctypes
