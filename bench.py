@@ -771,11 +771,13 @@ def main():
             torch.cuda.synchronize()
             wall_m = time.perf_counter() - t0
             _, lat_m = timed(step_mixed)
+            step_mixed()  # the error of the timed configuration (after the autotuned policy)
+            err_timed = (torch.linalg.norm(Xm - X) / torch.linalg.norm(X)).item()
             mixed = {"value": round(NRHS * args.steps / wall_m, 1), "unit": "RHS solves/s",
                      "precision": "fp32 factorization (LU + MFMA Schur updates) and fp32 MFMA "
                                   "solve, fp64 residual refinement",
                      "rb_factor_fp64_s": round(t_rb64, 4), "rb_factor_fp32_s": round(t_rb32, 4),
-                     "iters": iters, "rel_err_vs_fp64": errs[-1], "err_by_iters": errs,
+                     "iters": iters, "rel_err_vs_fp64": err_timed, "err_by_iters": errs,
                      "p50_ms": round(float(np.percentile(lat_m, 50)), 4),
                      "p99_ms": round(float(np.percentile(lat_m, 99)), 4),
                      "speedup_vs_fp64": round(float(np.mean(lat_ms) / np.mean(lat_m)), 3)}
