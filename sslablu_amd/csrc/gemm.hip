@@ -5,7 +5,7 @@
 // fragments.  K is consumed in BK-wide slabs staged through LDS, k-major with a
 // pad, and the next slab is prefetched into registers while the current one is
 // multiplied.  FP64 launches with few output tiles but long K split K over several
-// workgroups that accumulate with native global FP atomics (FP32 launches keep K whole).
+// workgroups that accumulate with native global FP atomics (FP32 launches at most 4 parts).
 //
 // Operand lane layouts (one element per lane):
 //   A (16x4):  lane l holds A[l % 16][l / 16];   B (4x16): lane l holds B[l / 16][l % 16]
@@ -1131,15 +1131,17 @@ int gemm_launch(GemmLaunchT<T>& L, hipStream_t stream) {
   const int spl = nmax <= 16 ? 8 : (nmax <= 32 ? 9 : P.cfg_split);
   if (count_tiles(L, kCfgBM[mid], kCfgBN[mid]) >= target) return launch_id<T>(mid, L, stream, 1);
   if (kmax >= P.min_split_k) {
-    // FP32 products keep K whole: every split-K part is rounded to FP32 before the atomic
-    // add, and the FP32 red-black solve is the preconditioner of an FP64 refinement whose
-    // convergence follows its accuracy (C2: 9e-5 with splits, 4e-5 without, so one
-    // refinement step reaches 2e-9 instead of 1.2e-8).  The split tile is kept.
-    if constexpr (std::is_same<T, float>::value) return launch_id<T>(spl, L, stream, 1);
+    // FP32 products split K into at most 4 parts: every part is rounded to FP32 before
+    // the atomic add, and the FP32 red-black solve is the preconditioner of an FP64
+    // refinement whose convergence follows its accuracy (C2 FP32 solve: 9e-5 with up to
+    // 64 parts, 5.6e-5 with 4, 4e-5 unsplit; one refinement step then reaches 1.2e-8,
+    // 4.4e-9 and 2e-9, and the 4-part cap keeps most of the split's speed).
     const long long tn = count_tiles(L, kCfgBM[spl], kCfgBN[spl]);
     const long long want = (P.split_wgs + tn - 1) / (tn > 0 ? tn : 1);
-    const int ks = (int)(want < P.max_ksplit ? want : P.max_ksplit);
-    return launch_id<T>(spl, L, stream, ks, P.max_ksplit);
+    const int cap = std::is_same<T, float>::value ? (P.max_ksplit < 4 ? P.max_ksplit : 4)
+                                                  : P.max_ksplit;
+    const int ks = (int)(want < cap ? want : cap);
+    return launch_id<T>(spl, L, stream, ks, cap);
   }
   if (nmax <= 32) return launch_id<T>(mid, L, stream, 1);
   if (count_tiles(L, 64, 32) >= target) return launch_id<T>(4, L, stream, 1);
