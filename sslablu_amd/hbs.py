@@ -22,6 +22,8 @@ Differences from the reference, on purpose:
 from __future__ import annotations
 
 import ctypes
+import os
+import threading
 
 import numpy as np
 import torch
@@ -33,6 +35,23 @@ F64 = torch.float64
 I32 = torch.int32
 
 _PERMS: dict = {}
+_PLANS = os.environ.get("SSLAB_HBS_PLANS", "1") != "0"
+
+
+class _Recorder(threading.local):
+    """Per-thread record of the GEMM launches of a DeviceHBS solve / apply being planned:
+    (entry point, raw arguments, objects the arguments point into).  `bad` is set when a
+    launch needed a copy of an operand (GPU work that a replay would not redo)."""
+    ops = None
+    bad = False
+
+
+_rec = _Recorder()
+
+
+def _recording_copy():
+    if _rec.ops is not None:
+        _rec.bad = True
 
 
 def device_perm(perm, device):
@@ -64,8 +83,10 @@ def bgemm(A, B, out=None, alpha=1.0, beta=0.0, ta=False, tb=False):
     """out_b = alpha op(A_b) op(B_b) + beta out_b for (nb, ., .) tensors (a batch of 1
     broadcasts)."""
     if A.stride(2) != 1:
+        _recording_copy()
         A = A.contiguous()
     if B.stride(2) != 1:
+        _recording_copy()
         B = B.contiguous()
     nb = max(A.shape[0], B.shape[0])
     M = A.shape[2] if ta else A.shape[1]
@@ -79,9 +100,12 @@ def bgemm(A, B, out=None, alpha=1.0, beta=0.0, ta=False, tb=False):
         beta = 0.0
     if out.stride(2) != 1 or tuple(out.shape) != (nb, M, N):
         raise ValueError("bgemm: bad output")
-    call("sslab_dgemm_strided_batched", int(ta), int(tb), M, N, K, float(alpha), ptr(A),
-         A.stride(1), _bstride(A), ptr(B), B.stride(1), _bstride(B), float(beta), ptr(out),
-         out.stride(1), out.stride(0), nb, stream_handle())
+    args = (int(ta), int(tb), M, N, K, float(alpha), ptr(A), A.stride(1), _bstride(A), ptr(B),
+            B.stride(1), _bstride(B), float(beta), ptr(out), out.stride(1), out.stride(0), nb,
+            stream_handle())
+    call("sslab_dgemm_strided_batched", *args)
+    if _rec.ops is not None:
+        _rec.ops.append(("sslab_dgemm_strided_batched", args, (A, B, out)))
     return out
 
 
@@ -116,8 +140,10 @@ def gdesc(out, terms, cin=None, beta=0.0):
         ta = bool(t[3]) if len(t) > 3 else False
         tb = bool(t[4]) if len(t) > 4 else False
         if A.stride(2) != 1:
+            _recording_copy()
             A = A.contiguous()
         if B.stride(2) != 1:
+            _recording_copy()
             B = B.contiguous()
         Ma = A.shape[2] if ta else A.shape[1]
         K = A.shape[1] if ta else A.shape[2]
@@ -143,7 +169,10 @@ def gdesc(out, terms, cin=None, beta=0.0):
 def ggemm(*descs, stream=None):
     """Launch descriptors made by gdesc (independent outputs) as one grouped product."""
     arr = (_GDesc * len(descs))(*[d for d, _ in descs])
-    call("sslab_dgemm_grouped", ctypes.addressof(arr), len(descs), stream_handle(stream))
+    args = (ctypes.addressof(arr), len(descs), stream_handle(stream))
+    call("sslab_dgemm_grouped", *args)
+    if _rec.ops is not None:
+        _rec.ops.append(("sslab_dgemm_grouped", args, (arr, [k for _, k in descs])))
 
 
 def bqr(A, want_q=True, pivot=False):
@@ -189,6 +218,7 @@ class DeviceHBS:
         self.perm = device_perm(perm, self.device)
         self.Nbvec = [D.shape[0] for D in self.Dmats]
         self._ulv = {}
+        self._plans = {}
 
     # -- construction from sketches (HBSnew.py:147-210) --------------------------------
     @staticmethod
@@ -237,8 +267,55 @@ class DeviceHBS:
     def apply(self, x: torch.Tensor, mode="N") -> torch.Tensor:
         one_d = x.dim() == 1
         X = x.reshape(x.shape[0], -1).to(self.device, F64)
+        u = self._planned("A", mode, X, lambda inp: self._apply_core(inp, mode))
+        return u[:, 0] if one_d else u
+
+    # -- launch plans -------------------------------------------------------------------
+    def _planned(self, kind, mode, B, core):
+        """core(inp) on B (N x r, leaf order restored on return) through a launch plan per
+        (kind, mode, r, stream): the first call runs core on plan-owned buffers and records
+        its GEMM launches (raw arguments); later calls gather B into the plan's input and
+        replay them, so the per-call host work is the launches alone.  The result is always
+        a new tensor.  Plans are not built or used while the stream is being captured, and
+        a core whose launches needed operand copies is never replayed."""
+        r = B.shape[1]
+        if not _PLANS or r == 0 or torch.cuda.is_current_stream_capturing():
+            inp = B[self.perm] if self.perm is not None else B.contiguous()
+            return self._unpermute(core(inp), fresh=False)
+        key = (kind, mode, r, stream_handle())
+        plan = self._plans.get(key)
+        if plan is not None:
+            inp, out, ops = plan
+            self._gather_into(inp, B)
+            for name, args, _ in ops:
+                call(name, *args)
+            return self._unpermute(out, fresh=True)
+        inp = torch.empty(self.N, r, dtype=F64, device=self.device)
+        self._gather_into(inp, B)
+        _rec.ops, _rec.bad = [], False
+        try:
+            out = core(inp)
+            ops, bad = _rec.ops, _rec.bad
+        finally:
+            _rec.ops, _rec.bad = None, False
+        if not bad:
+            self._plans[key] = (inp, out, ops)
+        return self._unpermute(out, fresh=True)
+
+    def _gather_into(self, inp, B):
         if self.perm is not None:
-            X = X[self.perm]
+            torch.index_select(B, 0, self.perm, out=inp)
+        else:
+            inp.copy_(B)
+
+    def _unpermute(self, y, fresh):
+        if self.perm is not None:
+            w = torch.empty_like(y)
+            w[self.perm] = y
+            return w
+        return y.clone() if fresh else y
+
+    def _apply_core(self, X, mode):
         r = X.shape[1]
         In, Out = (self.Vmats, self.Umats) if mode == "N" else (self.Umats, self.Vmats)
         tD = mode != "N"
@@ -255,11 +332,7 @@ class DeviceHBS:
             ggemm(gdesc(out, [(M, u.view(nb, k, r), 1.0),
                               (self.Dmats[lvl], VV[lvl].view(nb, n, r), 1.0, tD)]))
             u = out.reshape(nb * n, r)
-        if self.perm is not None:
-            w = torch.empty_like(u)
-            w[self.perm] = u
-            u = w
-        return u[:, 0] if one_d else u
+        return u
 
     def dense(self):
         return self.apply(torch.eye(self.N, dtype=F64, device=self.device))
@@ -368,11 +441,13 @@ class DeviceHBS:
 
     def solve(self, b: torch.Tensor, mode="N") -> torch.Tensor:
         """x with A x = b (mode N) or A^T x = b (mode T), b of shape (N,) or (N, r)."""
-        levels = self.ulv_factor(mode)
+        levels = self.ulv_factor(mode)  # before any plan records: the factor is not replayed
         one_d = b.dim() == 1
         B = b.reshape(b.shape[0], -1).to(self.device, F64)
-        if self.perm is not None:
-            B = B[self.perm]
+        y = self._planned("S", mode, B, lambda inp: self._solve_core(levels, inp))
+        return y[:, 0] if one_d else y
+
+    def _solve_core(self, levels, B):
         r = B.shape[1]
         cur = B.contiguous()
         tops = []
@@ -412,11 +487,7 @@ class DeviceHBS:
             ggemm(gdesc(vn, [(lev["U"], vb, 1.0), (lev["D"], x, 1.0)]))
             v = vn.reshape(nb * n, r)
             y = x.reshape(nb * n, r)
-        if self.perm is not None:
-            w = torch.empty_like(y)
-            w[self.perm] = y
-            y = w
-        return y[:, 0] if one_d else y
+        return y
 
 
 def _basis_and_pinv_pair(O3, P3, Y3, Z3, rk):

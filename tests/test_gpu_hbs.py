@@ -98,6 +98,51 @@ def test_ulv_solve(T, k, nl, L, fac):
     assert rel(H2.solve(bt, "T"), H.solve(bt, "T").cpu().numpy()) < 1e-12
 
 
+@pytest.mark.parametrize("permuted", [False, True])
+def test_planned_solve_and_apply_replay(T, permuted, monkeypatch):
+    """DeviceHBS solve / apply through launch plans: after the first call of a (mode,
+    width) the GEMM launches are replayed on plan buffers; every later call with new data
+    matches the dense operator, results are fresh tensors (an earlier result is not
+    overwritten), and the unplanned path gives the same numbers."""
+    from sslablu_amd import hbs
+    rng = np.random.default_rng(21)
+    k, nl, L, fac = 6, 10, 4, 2
+    U, D, V, Nbvec = oh.gen_random_hbs(k, nl, L, fac, rng)
+    D[0] = D[0] + 10 * np.eye(D[0].shape[1])
+    A = oh.hbs_dense(U, D, V)
+    N = A.shape[0]
+    t = lambda Xs: [T.tensor(X, device="cuda") for X in Xs]
+    perm = rng.permutation(N) if permuted else None
+    H = hbs.DeviceHBS(t(U), t(D), t(V), fac, perm)
+    # with a leaf permutation p the device operator acts on x as A[inv, :][:, inv]
+    if permuted:
+        inv = np.argsort(perm)
+        A = A[np.ix_(inv, inv)]
+    firsts = []
+    for r in (3, 5):
+        for it in range(3):
+            b = rng.standard_normal((N, r))
+            bt = T.tensor(b, device="cuda")
+            x = H.solve(bt)
+            xt = H.solve(bt, "T")
+            y = H.apply(bt)
+            yt = H.apply(bt, "T")
+            assert rel(x, np.linalg.solve(A, b)) < 1e-10
+            assert rel(xt, np.linalg.solve(A.T, b)) < 1e-10
+            assert rel(y, A @ b) < 1e-12
+            assert rel(yt, A.T @ b) < 1e-12
+            if it == 0:
+                firsts.append((x, x.clone()))
+    for x, keep in firsts:  # later replays did not write into earlier results
+        assert T.equal(x, keep)
+    assert len(H._plans) == 8
+    monkeypatch.setattr(hbs, "_PLANS", False)
+    b = T.tensor(rng.standard_normal((N, 3)), device="cuda")
+    x0 = H.solve(b)
+    monkeypatch.setattr(hbs, "_PLANS", True)
+    assert rel(H.solve(b), x0.cpu().numpy()) < 1e-13
+
+
 @pytest.mark.parametrize("M,N,K,ta,tb", [(300, 130, 500, 0, 0), (1024, 90, 1024, 1, 0),
                                          (77, 64, 33, 0, 1), (1, 1, 1, 0, 0)])
 def test_sgemm(T, M, N, K, ta, tb):
