@@ -299,6 +299,8 @@ class DeviceHBS:
         finally:
             _rec.ops, _rec.bad = None, False
         if not bad:
+            if len(self._plans) >= 16:  # many widths or streams: drop the oldest plan
+                self._plans.pop(next(iter(self._plans)))
             self._plans[key] = (inp, out, ops)
         return self._unpermute(out, fresh=True)
 
