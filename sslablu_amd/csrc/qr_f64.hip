@@ -389,12 +389,18 @@ k_geqrf_lds_qreg(int m, int n, double* __restrict__ A, int lda, long long sA,
     __syncthreads();
     for (int i = j + tid; i < m; i += kQrLdsThreads) colj[i] = (i == j) ? s_beta : 0.0;
     if (tau != 0.0) {
-      for (int c = j + 1 + wave; c < n; c += kQrLdsWaves) {
-        double* col = L + (long long)c * m;
+      // four trailing columns per wave at a time, 16 lanes per column: the dot products
+      // end in one 16-lane DPP row reduction each (no cross-row readlanes)
+      const int sub = lane >> 4, sl = lane & 15;
+      for (int c0 = j + 1 + 4 * wave; c0 < n; c0 += 4 * kQrLdsWaves) {  // wave-uniform
+        const int c = c0 + sub;
+        double* col = L + (long long)(c < n ? c : j) * m;
         double w = 0.0;
-        for (int i = j + lane; i < m; i += 64) w += v[i] * col[i];
-        w = wave::wave_sum(w) * tau;
-        for (int i = j + lane; i < m; i += 64) col[i] -= w * v[i];
+        if (c < n)
+          for (int i = j + sl; i < m; i += 16) w += v[i] * col[i];
+        w = wave::row16_sum(w) * tau;
+        if (c < n)
+          for (int i = j + sl; i < m; i += 16) col[i] -= w * v[i];
       }
       // Q <- Q (I - tau v v^T) on the register tile (v is zero above row j)
       double vv[CI];

@@ -61,6 +61,15 @@ __device__ __forceinline__ int wave_min(int v) {
   const int b = min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48));
   return min(a, b);
 }
+// Sum over each row of 16 lanes (lanes 16 q .. 16 q + 15), the same in the row's lanes.
+template <typename E>
+__device__ __forceinline__ E row16_sum(E v) {
+  v += dpp_e<kDppXor1>(v);
+  v += dpp_e<kDppXor2>(v);
+  v += dpp_e<kDppHalfMirror>(v);
+  v += dpp_e<kDppMirror>(v);
+  return v;
+}
 // Wave sum (the same in every lane).
 template <typename E>
 __device__ __forceinline__ E wave_sum(E v) {
