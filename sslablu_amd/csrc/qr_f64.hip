@@ -280,14 +280,17 @@ k_geqrf_lds(int m, int n, double* __restrict__ A, int lda, long long sA, double*
 }
 
 // The LDS-staged factorization with the complete Q held in registers instead of global
-// memory: wave w holds rows w + 16 k (k < RW) of Q and lane l its columns l + 64 t (t < CI),
-// so applying a reflector to Q needs no memory traffic (each row's product with v is a
-// DPP wave sum), and Q is stored once at the end.  The trailing-column products use the
-// same DPP sums.  For m <= min(16 RW, 64 CI) and matrices whose LDS panel fits; same
-// results as k_geqrf_lds up to the summation order.
+// memory, so applying a reflector to Q needs no memory traffic and Q is stored once at the
+// end.  The Q tile is grouped by 16-lane rows: a wave holds four Q rows side by side (one
+// per DPP row of 16 lanes), so the row dot products of the per-column update
+// Q <- Q (I - tau v v^T) end in one 16-lane reduction each.  For m <= min(16 RW, 64 CI)
+// and matrices whose LDS panel fits; same results as k_geqrf_lds up to the summation order.
+// RW must be a multiple of 4.  Up to 128 columns a lane keeps its 4 CI values of v in
+// registers; beyond, it reads them from LDS twice per column (the 128-register budget of a
+// 1024-thread workgroup holds the Q tile but not that slice as well).
 template <int RW, int CI>
 __global__ void __launch_bounds__(kQrLdsThreads)
-k_geqrf_lds_qreg(int m, int n, double* __restrict__ A, int lda, long long sA,
+k_geqrf_lds_qrow(int m, int n, double* __restrict__ A, int lda, long long sA,
                  double* __restrict__ Q, int ldq, long long sQ, int* __restrict__ perm,
                  long long sPerm) {
   extern __shared__ __attribute__((aligned(16))) unsigned char qr_smem[];
@@ -308,11 +311,15 @@ k_geqrf_lds_qreg(int m, int n, double* __restrict__ A, int lda, long long sA,
   }
   if (pb)
     for (int c = tid; c < n; c += kQrLdsThreads) pb[c] = c;
-  double q[RW][CI];
+  // Q tile: lane row s = lane / 16 of wave w holds Q rows w + 16 (4 g + s) (g < RW / 4) at
+  // columns (lane % 16) + 16 t (t < 4 CI)
+  constexpr int G = RW / 4, C4 = 4 * CI;
+  const int s16 = lane >> 4, l16 = lane & 15;
+  double q[G][C4];
 #pragma unroll
-  for (int k = 0; k < RW; ++k)
+  for (int g = 0; g < G; ++g)
 #pragma unroll
-    for (int t = 0; t < CI; ++t) q[k][t] = (wave + 16 * k == lane + 64 * t) ? 1.0 : 0.0;
+    for (int t = 0; t < C4; ++t) q[g][t] = (wave + 16 * (4 * g + s16) == l16 + 16 * t) ? 1.0 : 0.0;
   __syncthreads();
 
   const int kmax = m < n ? m : n;
@@ -403,31 +410,55 @@ k_geqrf_lds_qreg(int m, int n, double* __restrict__ A, int lda, long long sA,
           for (int i = j + sl; i < m; i += 16) col[i] -= w * v[i];
       }
       // Q <- Q (I - tau v v^T) on the register tile (v is zero above row j)
-      double vv[CI];
+      if constexpr (C4 <= 8) {  // the lane's v slice stays in registers
+        double vv[C4];
 #pragma unroll
-      for (int t = 0; t < CI; ++t) {
-        const int i = lane + 64 * t;
-        vv[t] = (i >= j && i < m) ? v[i] : 0.0;
-      }
+        for (int t = 0; t < C4; ++t) {
+          const int i = l16 + 16 * t;
+          vv[t] = (i >= j && i < m) ? v[i] : 0.0;
+        }
 #pragma unroll
-      for (int k = 0; k < RW; ++k) {
-        double part = 0.0;
+        for (int g = 0; g < G; ++g) {
+          double part = 0.0;
 #pragma unroll
-        for (int t = 0; t < CI; ++t) part += q[k][t] * vv[t];
-        const double w = wave::wave_sum(part) * tau;
+          for (int t = 0; t < C4; ++t) part += q[g][t] * vv[t];
+          const double w = wave::row16_sum(part) * tau;  // one Q row per 16-lane DPP row
 #pragma unroll
-        for (int t = 0; t < CI; ++t) q[k][t] -= w * vv[t];
+          for (int t = 0; t < C4; ++t) q[g][t] -= w * vv[t];
+        }
+      } else {  // v read from LDS once for the dots and once for the update (register budget)
+        double part[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) part[g] = 0.0;
+#pragma unroll
+        for (int t = 0; t < C4; ++t) {
+          const int i = l16 + 16 * t;
+          const double vt = (i >= j && i < m) ? v[i] : 0.0;
+#pragma unroll
+          for (int g = 0; g < G; ++g) part[g] += q[g][t] * vt;
+        }
+        double w[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) w[g] = wave::row16_sum(part[g]) * tau;
+        __asm__ volatile("" ::: "memory");  // reload v below instead of keeping 4 CI values live
+#pragma unroll
+        for (int t = 0; t < C4; ++t) {
+          const int i = l16 + 16 * t;
+          const double vt = (i >= j && i < m) ? v[i] : 0.0;
+#pragma unroll
+          for (int g = 0; g < G; ++g) q[g][t] -= w[g] * vt;
+        }
       }
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int k = 0; k < RW; ++k) {
-    const int r = wave + 16 * k;
+  for (int g = 0; g < G; ++g) {
+    const int r = wave + 16 * (4 * g + s16);
 #pragma unroll
-    for (int t = 0; t < CI; ++t) {
-      const int i = lane + 64 * t;
-      if (r < m && i < m) Qb[(long long)r * ldq + i] = q[k][t];
+    for (int t = 0; t < C4; ++t) {
+      const int i = l16 + 16 * t;
+      if (r < m && i < m) Qb[(long long)r * ldq + i] = q[g][t];
     }
   }
   for (int e = tid; e < m * n; e += kQrLdsThreads) {
@@ -476,20 +507,20 @@ int geqrf_batched(int m, int n, double* A, int lda, long long sA, double* Q, int
     }();
     if (qreg && Q != nullptr && m <= 192) {
       // the complete Q in registers: ceil(m / 16) rows per wave, ceil(m / 64) columns per lane
-#define SSLAB_QR_QREG_LAUNCH(RW, CI)                                                          \
+#define SSLAB_QR_QREG_LAUNCH(KERNEL, RW, CI)                                                  \
   do {                                                                                          \
     static bool attr_q = false;                                                                 \
     if (!attr_q) {                                                                              \
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_geqrf_lds_qreg<RW, CI>),      \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&KERNEL<RW, CI>),                \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);       \
       attr_q = true;                                                                            \
     }                                                                                           \
-    hipLaunchKernelGGL((k_geqrf_lds_qreg<RW, CI>), dim3(batch), dim3(kQrLdsThreads), lds, st, m, \
-                       n, A, lda, sA, Q, ldq, sQ, perm, sPerm);                                 \
+    hipLaunchKernelGGL((KERNEL<RW, CI>), dim3(batch), dim3(kQrLdsThreads), lds, st, m, n, A,    \
+                       lda, sA, Q, ldq, sQ, perm, sPerm);                                       \
   } while (0)
-      if (m <= 64) SSLAB_QR_QREG_LAUNCH(4, 1);
-      else if (m <= 128) SSLAB_QR_QREG_LAUNCH(8, 2);
-      else SSLAB_QR_QREG_LAUNCH(12, 3);
+      if (m <= 64) SSLAB_QR_QREG_LAUNCH(k_geqrf_lds_qrow, 4, 1);
+      else if (m <= 128) SSLAB_QR_QREG_LAUNCH(k_geqrf_lds_qrow, 8, 2);
+      else SSLAB_QR_QREG_LAUNCH(k_geqrf_lds_qrow, 12, 3);
 #undef SSLAB_QR_QREG_LAUNCH
       SSLAB_HIP_CHECK(hipGetLastError());
       return kOk;
