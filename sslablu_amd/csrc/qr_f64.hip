@@ -325,12 +325,15 @@ k_geqrf_lds_qrow(int m, int n, double* __restrict__ A, int lda, long long sA,
   const int kmax = m < n ? m : n;
   for (int j = 0; j < kmax; ++j) {
     if (pb) {
-      for (int c = j + wave; c < n; c += kQrLdsWaves) {
-        const double* col = L + (long long)c * m;
+      // squared norms of the remaining columns, four per wave at a time (16 lanes each)
+      for (int c0 = j + 4 * wave; c0 < n; c0 += 4 * kQrLdsWaves) {  // wave-uniform
+        const int c = c0 + s16;
+        const double* col = L + (long long)(c < n ? c : j) * m;
         double s = 0.0;
-        for (int i = j + lane; i < m; i += 64) s += col[i] * col[i];
-        s = wave::wave_sum(s);
-        if (lane == 0) cn[c] = s;
+        if (c < n)
+          for (int i = j + l16; i < m; i += 16) s += col[i] * col[i];
+        s = wave::row16_sum(s);
+        if (c < n && l16 == 0) cn[c] = s;
       }
       __syncthreads();
       if (wave == 0) {
