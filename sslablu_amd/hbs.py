@@ -232,15 +232,23 @@ class DeviceHBS:
         Umats, Vmats, Dmats = [], [], []
         U = V = D = None
         for lvl in range(L - 1, -1, -1):
-            if lvl < L - 1:
+            if lvl < L - 1:  # sketches of the next level: two grouped launches
                 O3 = Oml.view(nb, n, s)
                 P3 = Psl.view(nb, n, s)
-                Ynew = bgemm(D, O3, Yl.view(nb, n, s).clone(), -1.0, 1.0)
-                Yl = bgemm(U, Ynew, ta=True).reshape(-1, s)
-                Znew = bgemm(D, P3, Zl.view(nb, n, s).clone(), -1.0, 1.0, ta=True)
-                Zl = bgemm(V, Znew, ta=True).reshape(-1, s)
-                Oml = bgemm(V, O3, ta=True).reshape(-1, s)
-                Psl = bgemm(U, P3, ta=True).reshape(-1, s)
+                kU, kV = U.shape[2], V.shape[2]
+                dev = O3.device
+                Ynew = torch.empty(nb, n, s, dtype=F64, device=dev)
+                Znew = torch.empty(nb, n, s, dtype=F64, device=dev)
+                On = torch.empty(nb, kV, s, dtype=F64, device=dev)
+                Pn = torch.empty(nb, kU, s, dtype=F64, device=dev)
+                ggemm(gdesc(Ynew, [(D, O3, -1.0)], cin=Yl.view(nb, n, s), beta=1.0),
+                      gdesc(Znew, [(D, P3, -1.0, True)], cin=Zl.view(nb, n, s), beta=1.0),
+                      gdesc(On, [(V, O3, 1.0, True)]), gdesc(Pn, [(U, P3, 1.0, True)]))
+                Yn = torch.empty(nb, kU, s, dtype=F64, device=dev)
+                Zn = torch.empty(nb, kV, s, dtype=F64, device=dev)
+                ggemm(gdesc(Yn, [(U, Ynew, 1.0, True)]), gdesc(Zn, [(V, Znew, 1.0, True)]))
+                Yl, Zl = Yn.reshape(-1, s), Zn.reshape(-1, s)
+                Oml, Psl = On.reshape(-1, s), Pn.reshape(-1, s)
                 nb //= fac
                 n = Oml.shape[0] // nb
             rkm = min(rk, nl * fac ** (L - 1 - lvl))
@@ -250,10 +258,17 @@ class DeviceHBS:
             Z3 = Zl.reshape(nb, n, s)
             if lvl > 0:
                 U, V, pinvO, pinvP = _basis_and_pinv_pair(O3, P3, Y3, Z3, rkm)
-                T1 = bgemm(Y3, pinvO)
-                bgemm(U, bgemm(U, T1, ta=True), T1, -1.0, 1.0)
-                T2 = bgemm(Z3, pinvP)
-                bgemm(V, bgemm(V, T2, ta=True), T2, -1.0, 1.0)
+                # T1 = (I - U U^T) Y pinv(Omega), T2 = (I - V V^T) Z pinv(Psi): each step of
+                # the two sides in one grouped launch
+                dev = Y3.device
+                T1 = torch.empty(nb, n, n, dtype=F64, device=dev)
+                T2 = torch.empty(nb, n, n, dtype=F64, device=dev)
+                ggemm(gdesc(T1, [(Y3, pinvO, 1.0)]), gdesc(T2, [(Z3, pinvP, 1.0)]))
+                W1 = torch.empty(nb, U.shape[2], n, dtype=F64, device=dev)
+                W2 = torch.empty(nb, V.shape[2], n, dtype=F64, device=dev)
+                ggemm(gdesc(W1, [(U, T1, 1.0, True)]), gdesc(W2, [(V, T2, 1.0, True)]))
+                ggemm(gdesc(T1, [(U, W1, -1.0)], cin=T1, beta=1.0),
+                      gdesc(T2, [(V, W2, -1.0)], cin=T2, beta=1.0))
                 D = bgemm(U, bgemm(U, T2, ta=True, tb=True), T1, 1.0, 1.0)
                 Umats.append(U)
                 Vmats.append(V)
