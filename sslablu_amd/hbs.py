@@ -414,8 +414,13 @@ class DeviceHBS:
             else:  # blkdiag(Ud of the fac children) U: one product per child row block
                 kc = Uhat.shape[2]
                 Uc = bgemm(Uhat, U.reshape(nb * fac, kc, k)).reshape(nb, n, k)
-            Uu = bgemm(Q[:, :, :n - k], Uc, ta=True) if n > k else None
-            Ud = bgemm(Q[:, :, n - k:], Uc, ta=True)
+            if n > k:  # Uu = Q_1^T Uc and Ud = Q_2^T Uc in one grouped launch
+                Uu = torch.empty(nb, n - k, Uc.shape[2], dtype=F64, device=Uc.device)
+                Ud = torch.empty(nb, k, Uc.shape[2], dtype=F64, device=Uc.device)
+                ggemm(gdesc(Uu, [(Q[:, :, :n - k], Uc, 1.0, True)]),
+                      gdesc(Ud, [(Q[:, :, n - k:], Uc, 1.0, True)]))
+            else:
+                Uu, Ud = None, bgemm(Q[:, :, n - k:], Uc, ta=True)
             R22 = R[:, n - k:, n - k:].contiguous()
             lev = dict(root=False, nb=nb, n=n, k=k, Q=Q, W=W, Uu=Uu,
                        R12=R[:, :n - k, n - k:].contiguous(),
