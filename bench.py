@@ -70,12 +70,26 @@ def algorithmic_work(nb, m, g_sum, r, w=8):
     return 2.0 * r * ent, w * ent + 8.0 * r * (g_sum + 2 * nb * m)
 
 
-def cpu_baseline(op, F_host, budget_s=20.0):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(op, F_host, budget_s=24.0):
     """The oracle's restatement of the reference CPU path (scipy lu_factor/lu_solve
     Thomas, direct_solve/omsdirectsolve.py:48-141, plus the rhs maps of
-    multislab/oms.py:180-186), timed on the host cores on the same S-system."""
+    multislab/oms.py:180-186), timed on the host cores on the same S-system with the
+    BLAS/OpenMP pools limited to 1, 4 and all the threads this process may use; the
+    fastest is the baseline, all are reported."""
+    from threadpoolctl import threadpool_info, threadpool_limits
     from oracle import blocktri as ob
-    import torch
     g = op.g
     m, nb = g.m, g.nb
     SiM = op.SiM.cpu().numpy()
@@ -101,21 +115,39 @@ def cpu_baseline(op, F_host, budget_s=20.0):
             out[i * m:(i + 1) * m] = -v
         return out
 
-    t0 = time.perf_counter()
-    fac = ob.thomas_factor(S, T)
-    t_fac = time.perf_counter() - t0
-    times = []
-    x = None
-    t_start = time.perf_counter()
-    while True:
+    def run(F, budget, min_reps):
+        times, x = [], None
+        t_start = time.perf_counter()
+        while True:
+            t0 = time.perf_counter()
+            x = ob.thomas_solve(*fac, rhs_maps(F))
+            times.append(time.perf_counter() - t0)
+            if len(times) >= 50 or (time.perf_counter() - t_start > budget
+                                    and len(times) >= min_reps):
+                break
+        t_med = float(np.median(times[1:] if len(times) > 2 else times))
+        return F.shape[1] / t_med, t_med, len(times), x
+
+    # BLAS/OpenMP thread counts tried on the same sample; the baseline is the fastest
+    # (small lu_solve / GEMM calls of a 512-block system lose to oversubscribed pools)
+    allowed = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    counts = sorted({1, min(4, allowed), allowed})
+    with threadpool_limits(limits=allowed):
         t0 = time.perf_counter()
-        x = ob.thomas_solve(*fac, rhs_maps(F_host))
-        times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_start > budget_s or len(times) >= 50:
-            break
-    t_med = float(np.median(times[1:] if len(times) > 2 else times))
-    return dict(value=F_host.shape[1] / t_med, t_solve=t_med, t_factor=t_fac, reps=len(times),
-                x=x)
+        fac = ob.thomas_factor(S, T)
+        t_fac = time.perf_counter() - t0
+    sweep = {}
+    x = None
+    for n in counts:
+        with threadpool_limits(limits=n):
+            used = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
+            v, t_med, reps, x = run(F_host, budget_s / len(counts), 3)
+        sweep[used] = dict(value=v, t_solve=t_med, reps=reps)
+    best = max(sweep, key=lambda k: sweep[k]["value"])
+    return dict(value=sweep[best]["value"], t_solve=sweep[best]["t_solve"], t_factor=t_fac,
+                reps=sweep[best]["reps"], x=x, threads=best,
+                by_threads={k: round(v["value"], 1) for k, v in sweep.items()},
+                cpu=_cpu_model(), logical_cpus=os.cpu_count())
 
 
 def serving_bench(dev, n_req=2000, seed=0):
@@ -415,8 +447,8 @@ def c3_bench(dev, seed=0, rk=40, nrhs=16):
         t0_ = time.perf_counter()
         H_ = {}
         for side in ("left", "right"):
-            h = HBSnew.HBSMAT(s.S[side], tree, quad=False, seed=seed, sketch_fp32=fp32)
-            h._samples = lambda rk_, s_=ns: s_
+            h = HBSnew.HBSMAT(s.S[side], tree, quad=False, seed=seed, sketch_fp32=fp32,
+                              samples=ns)
             h.construct(rk)
             H_[side] = h
         torch.cuda.synchronize()
@@ -450,8 +482,8 @@ def c3_bench(dev, seed=0, rk=40, nrhs=16):
     sweep = {}
     for rk2, leaf2 in ((72, 144), (120, 288)):
         tr2 = FaceTree(ny - 2, ny - 2, leaf2)
-        h2 = HBSnew.HBSMAT(s.S["left"], tr2, quad=False, seed=seed)
-        h2._samples = lambda rk_, s_=max(2 * rk2, leaf2) + rk2 + 10: s_
+        h2 = HBSnew.HBSMAT(s.S["left"], tr2, quad=False, seed=seed,
+                           samples=max(2 * rk2, leaf2) + rk2 + 10)
         h2.construct(rk2)
         sweep[f"rank {rk2}, leaf {leaf2}"] = {
             "rel_err": ((h2.matmat(X) - s.S["left"] @ X).norm() / (s.S["left"] @ X).norm()).item(),
@@ -650,6 +682,16 @@ def main():
         fac.__dict__.setdefault("_bwd_policies", {})[NRHS] = [
             tuning.Policy(*[int(v) for v in q.split(",")]) for q in args.bwd_policies.split(";")]
     t_tune = time.perf_counter() - t0
+    # the exact tile policies the timed step runs under, as the flags that fix them again
+    # (profiler runs pass these back so that they trace the configuration timed here)
+    def _pstr(q):
+        return ",".join(str(v) for v in dataclasses.astuple(q))
+    pol_bwd = tuning.lookup(fac.tune_key(NRHS)) or pol
+    pol_fold = tuning.lookup(fold.tune_key(NRHS)) or pol_bwd
+    table = getattr(fac, "_bwd_policies", {}).get(NRHS, ())
+    timed_flags = f"--policy {_pstr(pol_bwd)} --fold-policy {_pstr(pol_fold)}"
+    if table:
+        timed_flags += " --bwd-policies '" + ";".join(_pstr(q) for q in table) + "'"
     # the backward sweep's policy and level table stand for the thread, so the solver's
     # per-call policy scopes are no-ops in the timed loop (only the boundary-map launch
     # switches to its own policy and back)
@@ -786,30 +828,36 @@ def main():
                      "note": "refinement did not reach 1e-8 within 3 steps"}
 
     # ---- roofline: the step is GEMM launches of the FP64 MFMA kernel (k_gemm_fast) plus
-    # one fill kernel (X = 0): the boundary-map launch (the folded forward sweep) and the L
-    # backward levels.  Work is what those launches execute: every compact map element
-    # and every backward-sweep block is multiplied into all NRHS columns once.  The
-    # per-launch average is the HIP-event step time divided by the GEMM launches, so it
-    # slightly overstates a launch (the fill kernel is included).
+    # one fill kernel: the boundary-map launch (the folded forward sweep and the dense
+    # coarsest level) and the L backward levels.  `achieved` counts the algorithmic work of
+    # one solve (the Thomas minimum F = 2 r [(3nb-2) m^2 + m sum g], method-independent)
+    # over the timed step (ms_per_step: K steps back to back), per launch; the flops the
+    # launches execute (every compact map element and backward-sweep block times NRHS
+    # columns) are reported beside it.
     levels, n_top = rb_levels(nb, m)
     n_launch = 1 + levels
     map_elems = fold.nbytes // 8
     bwd_elems = rb_backward_blocks(nb, m) * m * m
-    flops = 2.0 * NRHS * (map_elems + bwd_elems)
+    exec_flops = 2.0 * NRHS * (map_elems + bwd_elems)
     nbytes = 8.0 * (map_elems + bwd_elems) + 8.0 * NRHS * (fold.nF + 4 * nb * m)
     g_sum = nb * 4 * g.nx * g.p + 2 * m
-    thomas_flops, _ = algorithmic_work(nb, m, g_sum, NRHS)
-    step_s = float(np.mean(lat_ms)) / 1e3
-    achieved_tf = flops / step_s / 1e12
+    thomas_flops, thomas_bytes = algorithmic_work(nb, m, g_sum, NRHS)
+    step_s = ms_per_step / 1e3
+    achieved_tf = thomas_flops / step_s / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": 78.6,
                 "unit": "TFLOP/s", "frac": round(achieved_tf / 78.6, 4), "traffic": TRAFFIC,
                 "kernel": f"k_gemm_fast<f64> ({n_launch} launches per step: boundary maps "
                           f"(folded forward sweep and dense {n_top}-node coarsest level), "
                           f"{levels} backward levels)",
-                "launches_per_step": n_launch, "avg_launch_us": round(step_s / n_launch * 1e6, 2),
-                "algorithmic_flops_per_launch": flops / n_launch,
-                "algorithmic_flops_per_step": flops, "algorithmic_bytes_per_step": nbytes,
-                "thomas_flops_per_step": thomas_flops}
+                "launches_per_step": n_launch,
+                "avg_launch_us": round(step_s / n_launch * 1e6, 2),
+                "algorithmic_flops_per_launch": thomas_flops / n_launch,
+                "algorithmic_flops_per_step": thomas_flops,
+                "algorithmic_bytes_per_step": thomas_bytes,
+                "executed_flops_per_step": exec_flops,
+                "executed_bytes_per_step": nbytes,
+                "executed_tflops": round(exec_flops / step_s / 1e12, 3),
+                "event_step_us": round(float(np.mean(lat_ms)) * 1e3, 2)}
 
     out = {
         "metric": "RHS solves/sec + p50/p99 latency at <=1e-8 rel err vs FP64; factor time",
@@ -836,6 +884,7 @@ def main():
         "fold_maps_mb": round(fold.nbytes / 2 ** 20, 1),
         "gemm_policy": list(dataclasses.astuple(pol)), "autotune_s": round(t_tune, 3),
         "split_tuning": tune_split,
+        "timed_policy_flags": timed_flags,
         "rel_err_vs_fp64_thomas": rel_err,
         "residual": resid,
         "roofline": roofline,
@@ -853,12 +902,15 @@ def main():
         cb = cpu_baseline(op, F_host, args.cpu_budget)
         out["cpu_baseline"] = {
             "value": round(cb["value"], 1), "unit": "RHS solves/s",
-            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "cores": cb["threads"],
             "kind": "port",
             "sample": f"oracle Thomas (scipy lu_factor/lu_solve) + rhs maps on the same "
-                      f"S-system, {NRHS} RHS per solve, median of {cb['reps']} solves; "
-                      f"CPU factor {cb['t_factor']:.2f} s (S assembly not included)",
+                      f"S-system, {NRHS} RHS per solve, median of {cb['reps']} solves, BLAS "
+                      f"pools limited to {cb['threads']} thread(s), the fastest of the counts "
+                      f"tried; CPU factor {cb['t_factor']:.2f} s (S assembly not included)",
             "factor_s": round(cb["t_factor"], 3),
+            "rhs_solves_per_s_by_blas_threads": cb["by_threads"],
+            "cpu_model": cb["cpu"], "logical_cpus": cb["logical_cpus"],
         }
         out["cpu_rel_diff"] = float(np.linalg.norm(X.cpu().numpy() - cb["x"]) /
                                     np.linalg.norm(cb["x"]))

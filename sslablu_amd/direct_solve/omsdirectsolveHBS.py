@@ -267,11 +267,7 @@ class ThomasSolverHBS(DirectSolver):
         self.solve_method = None
 
     def _hbsmat(self, op, tree, quad):
-        h = HBSnew.HBSMAT(op, tree, quad)
-        if self.samples is not None:
-            sm = self.samples
-            h._samples = sm if callable(sm) else (lambda rk_, s_=int(sm): s_)
-        return h
+        return HBSnew.HBSMAT(op, tree, quad, samples=self.samples)
 
     def factorize_helper(self, S_rk_list, diagList=None):
         if diagList is None:

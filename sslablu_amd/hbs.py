@@ -79,9 +79,24 @@ def _bstride(X):
     return 0 if X.shape[0] == 1 else X.stride(0)
 
 
+def _check_f64(where, *tensors):
+    """Every operand handed to the FP64 kernels must be a float64 GPU tensor: the
+    kernels take raw pointers and would read past a float32 or host allocation."""
+    for t in tensors:
+        if t is None:
+            continue
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{where}: expected a torch.Tensor, got {type(t).__name__}")
+        if t.dtype != F64:
+            raise TypeError(f"{where}: float64 operands required, got {t.dtype}")
+        if t.device.type != "cuda":
+            raise TypeError(f"{where}: GPU operands required, got a tensor on {t.device}")
+
+
 def bgemm(A, B, out=None, alpha=1.0, beta=0.0, ta=False, tb=False):
     """out_b = alpha op(A_b) op(B_b) + beta out_b for (nb, ., .) tensors (a batch of 1
     broadcasts)."""
+    _check_f64("bgemm", A, B, out)
     if A.stride(2) != 1:
         _recording_copy()
         A = A.contiguous()
@@ -130,6 +145,7 @@ def gdesc(out, terms, cin=None, beta=0.0):
     tensors, a batch of 1 broadcasts).  Returns (descriptor, tensors to keep alive)."""
     if not 1 <= len(terms) <= 3:
         raise ValueError("gdesc: 1..3 terms")
+    _check_f64("gdesc", out, cin, *[x for t in terms for x in t[:2]])
     if out.stride(2) != 1:
         raise ValueError("gdesc: out needs unit column stride")
     d = _GDesc()
@@ -178,6 +194,7 @@ def ggemm(*descs, stream=None):
 def bqr(A, want_q=True, pivot=False):
     """Batched Householder QR of (nb, m, n): returns (Q (nb, m, m) or None, R (nb, m, n),
     perm (nb, n) or None) with A[:, :, perm] = Q R."""
+    _check_f64("bqr", A)
     R = A.contiguous().clone()
     nb, m, n = R.shape
     Q = torch.empty(nb, m, m, dtype=F64, device=A.device) if want_q else None
@@ -189,6 +206,7 @@ def bqr(A, want_q=True, pivot=False):
 
 def btri_inv(R):
     """Inverse of a batch of upper-triangular (nb, n, n) blocks."""
+    _check_f64("btri_inv", R)
     nb, n, _ = R.shape
     R = R.contiguous()
     X = torch.eye(n, dtype=F64, device=R.device).expand(nb, n, n).contiguous()
@@ -208,6 +226,7 @@ class DeviceHBS:
     """HBS matrix with device-resident factors (lists ordered leaf level first)."""
 
     def __init__(self, Umats, Dmats, Vmats, fac=2, perm=None):
+        _check_f64("DeviceHBS", *Umats, *Dmats, *Vmats)
         self.Umats = list(Umats)
         self.Dmats = list(Dmats)
         self.Vmats = list(Vmats)
@@ -225,6 +244,7 @@ class DeviceHBS:
     def from_sketches(Om, Psi, Y, Z, Nb, L, rk, fac=2, perm=None):
         """Om, Psi (N x s) test matrices and Y = A Om, Z = A^T Psi, all in leaf-permuted
         order, on the device."""
+        _check_f64("DeviceHBS.from_sketches", Om, Psi, Y, Z)
         N, s = Om.shape
         nl = N // Nb
         nb, n = Nb, nl

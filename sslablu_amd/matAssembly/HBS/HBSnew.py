@@ -89,11 +89,15 @@ class HBSMAT:
     tSample = _stage_time("sample")
     tCompress = _stage_time("compress")
 
-    def __init__(self, A=None, tree=None, quad=True, device=None, seed=0, sketch_fp32=False):
+    def __init__(self, A=None, tree=None, quad=True, device=None, seed=0, sketch_fp32=False,
+                 samples=None):
         """sketch_fp32: for a dense A, form the sketches Y = A Omega, Z = A^T Psi with the
         FP32 MFMA GEMM (A and the test matrices rounded to float); the compression that
         follows (QR, interpolative bases, null-space projections) stays FP64.  Linear
-        operators are always applied in FP64."""
+        operators are always applied in FP64.
+        samples: sketch columns per side, an int or a callable rk -> int; None keeps the
+        reference's rule (HBSnew.py:147-150: 4 rk + 2 leaf + 5 binary, 6 rk + 4 leaf + 5
+        quad)."""
         _lib.require_gpu()
         self.sketch_fp32 = bool(sketch_fp32)
         self.device = torch.device(device) if device is not None else torch.device(
@@ -102,6 +106,9 @@ class HBSMAT:
         self._h = None
         self.Qlist, self.Rlist, self.Wlist, self.Uulist = [], [], [], []
         self.seed = seed
+        if samples is not None and not callable(samples) and int(samples) < 1:
+            raise ValueError("samples must be a positive int or a callable rk -> int")
+        self.samples = samples
         if A is not None:
             self.A = A
             self.shape = tuple(A.shape)
@@ -164,6 +171,8 @@ class HBSMAT:
             self.constructHBS(rk)
 
     def _samples(self, rk):
+        if self.samples is not None:
+            return int(self.samples(rk)) if callable(self.samples) else int(self.samples)
         return 6 * rk + 4 * self.nl + 5 if self.fac == 4 else 4 * rk + 2 * self.nl + 5
 
     def constructHBS(self, rk):

@@ -6,6 +6,10 @@ solve(Umats, Dmats, Qlist, Wlist, Uulist, Rlist, NNvec, Nbvec, rhs, mode='N')
 (ULVsparse.py:231-358).  The factor lists are device tensors of this implementation
 (Rlist holds the inverted triangular blocks R11^{-1}, the root entry holds
 R^{-1} Q^T); pass them back to solve() unchanged.
+
+The device factorization travels with the returned lists (an attribute of the tensor
+Qlist[0]), so it lives exactly as long as the caller keeps the factors, and solve()
+refuses lists that compute_ULV did not return.
 """
 from __future__ import annotations
 
@@ -14,7 +18,7 @@ import torch
 
 from ...hbs import DeviceHBS, F64
 
-_cache = {}
+_ATTR = "_sslab_ulv_factor"
 
 
 def _blocks(X, nb, dev):
@@ -41,12 +45,12 @@ def compute_ULV(Umats, Dmats, Vmats, Nbvec, fac=None):
     Rlist = [l["R11inv"] for l in lev if not l["root"]] + [lev[-1]["G"][0]]
     NNvec = np.concatenate([[0], np.cumsum([(l["n"] - l["k"]) * l["nb"] for l in lev if not l["root"]]
                                            + [lev[-1]["G"].shape[1]])]).astype(np.int64)
-    _cache[id(Qlist[0])] = h
+    setattr(Qlist[0], _ATTR, h)
     return Qlist, Wlist, Uulist, Rlist, NNvec
 
 
 def solve(Umats, Dmats, Qlist, Wlist, Uulist, Rlist, NNvec, Nbvec, rhs, mode="N"):
-    h = _cache.get(id(Qlist[0]))
+    h = getattr(Qlist[0], _ATTR, None) if len(Qlist) else None
     if h is None:
         raise ValueError("ULV factors must come from this module's compute_ULV")
     was_np = not isinstance(rhs, torch.Tensor)

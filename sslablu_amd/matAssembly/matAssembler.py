@@ -71,9 +71,8 @@ class matAssembler:
             self.matOpts.tree = tree
             if not tree.uniform():
                 raise ValueError("rkHBS needs a cluster tree with equally sized leaves")
-            H = HBSnew.HBSMAT(linOp, tree, quad=False, seed=self.seed)
             s = max(2 * k, self.matOpts.leaf_size) + k + 10
-            H._samples = lambda rk, s=s: s
+            H = HBSnew.HBSMAT(linOp, tree, quad=False, seed=self.seed, samples=s)
             H.construct(k)
             self.stats.timeSample = H.tSample
             self.stats.timeCompress = H.tCompress

@@ -85,7 +85,8 @@ def make_pdo(spec: ProblemSpec):
 
 class CachedOperator:
     """A built operator: rhs maps (op.Y), red-black factor, the boundary-folded forward
-    sweep (FP64 batches: one launch + the backward sweep), optional FP32 factor."""
+    sweep (FP64 batches: one launch + the backward sweep; built in every cache, so FP64
+    batches on a mixed cache are folded too), optional FP32 factor."""
 
     def __init__(self, spec: ProblemSpec, device, mixed: bool):
         t0 = time.perf_counter()
@@ -96,7 +97,7 @@ class CachedOperator:
         fac = BlockTriFactor("rb", device).factorize(None, op.SiM, op.SiP)  # T = I
         if mixed:
             fac.enable_mixed(None, op.SiM, op.SiP)
-        self.fold = None if mixed else op.fold(fac)
+        self.fold = op.fold(fac)
         torch.cuda.synchronize(device)
         self.spec, self.op, self.fac = spec, op, fac
         self.N = nb * m
@@ -109,9 +110,9 @@ class CachedOperator:
 
     @property
     def nbytes(self):
+        # fac.nbytes covers the FP64 factor and, in mixed caches, its FP32 copy; the FP64
+        # blocks the refinement residual reads are op.SiM / op.SiP themselves (no copy)
         n = self.fac.nbytes + (self.op.Y.numel() + self.op.SiM.numel() + self.op.SiP.numel()) * 8
-        if getattr(self.fac, "factor32", None) is not None:
-            n += self.fac.factor32.numel() * 4
         if self.fold is not None:
             n += self.fold.nbytes
         return n + self.graph_bytes
@@ -138,22 +139,24 @@ class FactorCache:
         return sum(e.nbytes for e in self.ops.values())
 
     def estimate_bytes(self, spec: ProblemSpec) -> int:
-        """Resident bytes of a built operator (FP64 factor + maps), before graphs; for FP64
-        caches an upper bound of the folded boundary maps (an odd node of level l reads
-        at most 2^(l+2) strip windows of bottom and top data plus both boundary segments,
-        a coarsest-level node all nF boundary values)."""
+        """Resident bytes of a built operator before graphs: FP64 factor, its FP32 copy in
+        mixed caches, the maps Y and coupling blocks, and an upper bound of the folded
+        boundary maps (an odd node of level l reads at most 2^(l+2) strip windows of
+        bottom and top data plus both boundary segments, a coarsest-level node all nF
+        boundary values)."""
         nb, m, p = spec.nb, spec.m, spec.p
         ncols = 2 * m + 4 * p
         rb, _, _ = _lib.sizes("rb", nb, m)
         n = 8 * (rb + nb * m * ncols + 2 * nb * m * m)
-        if not self.mixed:
-            L, n_top = rb_levels(nb, m)
-            nF = 2 * spec.slabs * p + 2 * m
-            cols, k = n_top * nF, nb
-            for lvl in range(L):
-                cols += (k // 2) * min(nF, (4 << lvl) * 2 * p + 2 * m)
-                k = (k + 1) // 2
-            n += 8 * m * cols
+        if self.mixed:
+            n += 4 * rb
+        L, n_top = rb_levels(nb, m)
+        nF = 2 * spec.slabs * p + 2 * m
+        cols, k = n_top * nF, nb
+        for lvl in range(L):
+            cols += (k // 2) * min(nF, (4 << lvl) * 2 * p + 2 * m)
+            k = (k + 1) // 2
+        n += 8 * m * cols
         return n
 
     def _evict_for(self, need: int) -> bool:

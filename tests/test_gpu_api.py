@@ -167,3 +167,32 @@ def test_ulvsparse_torch_tensor_form(gpu):
     b = rng.standard_normal((A.shape[0], 2))
     x = ULVsparse_torch.solve(U, D, Q, W, Uu, R, NN, b, "cuda")
     assert np.linalg.norm(x - np.linalg.solve(A, b)) / np.linalg.norm(b) < 1e-10
+
+
+def test_ulvsparse_factor_lifetime(gpu):
+    """ULVsparse.compute_ULV's device factorization lives exactly as long as the returned
+    lists: dropping them releases it, and solve() refuses lists it did not return."""
+    import gc
+    import weakref
+
+    import pytest
+    import torch
+    from oracle import hbs as oh
+    from sslablu_amd.matAssembly.HBS import ULVsparse
+    rng = np.random.default_rng(4)
+    U, D, V, Nbvec = oh.gen_random_hbs(4, 8, 3, 2, rng)
+    D[0] = D[0] + 10 * np.eye(D[0].shape[1])
+    A = oh.hbs_dense(U, D, V)
+    Um, Dm, Vm = ([x.reshape(-1, x.shape[-1]) for x in X] for X in (U, D, V))
+    Q, W, Uu, R, NN = ULVsparse.compute_ULV(Um, Dm, Vm, Nbvec)
+    b = rng.standard_normal(A.shape[0])
+    x = ULVsparse.solve(Um, Dm, Q, W, Uu, R, NN, Nbvec, b)
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) / np.linalg.norm(b) < 1e-10
+    foreign = [q.clone() for q in Q]
+    with pytest.raises(ValueError):
+        ULVsparse.solve(Um, Dm, foreign, W, Uu, R, NN, Nbvec, b)
+    ref = weakref.ref(getattr(Q[0], ULVsparse._ATTR))
+    del Q, W, Uu, R, NN, x
+    gc.collect()
+    assert ref() is None
+    torch.cuda.synchronize()

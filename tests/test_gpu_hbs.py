@@ -28,7 +28,8 @@ def T(gpu):
 @pytest.mark.parametrize("m,n,pivot", [(7, 5, False), (40, 40, False), (293, 72, False),
                                        (72, 72, True), (50, 30, True), (30, 50, True),
                                        (130, 72, False), (72, 58, True), (226, 40, False),
-                                       (300, 200, True)])
+                                       (300, 200, True), (160, 100, True), (192, 90, True),
+                                       (150, 150, True), (180, 120, False)])
 def test_batched_qr(T, m, n, pivot):
     from sslablu_amd.hbs import bqr
     rng = np.random.default_rng(m + n)
@@ -45,6 +46,25 @@ def test_batched_qr(T, m, n, pivot):
         if pivot:
             d = np.abs(np.diag(R[b]))
             assert np.all(d[:-1] + 1e-9 * d[0] >= d[1:])  # non-increasing |R_jj|
+
+
+@pytest.mark.parametrize("m,n", [(48, 40), (160, 100), (192, 90)])
+def test_pivoted_qr_ties_take_first_index(T, m, n):
+    """Duplicated columns have equal norms at every step until one of a pair is chosen
+    (then the other's residual vanishes): the pivot takes the first index on ties, so
+    the first n/2 pivots are exactly the first half of the columns."""
+    from sslablu_amd.hbs import bqr
+    rng = np.random.default_rng(m)
+    h = n // 2
+    B = rng.standard_normal((2, m, h)) * (1.0 + np.arange(h) % 3)  # some equal scales too
+    A = np.concatenate([B, B], axis=2)
+    Q, R, perm = bqr(T.tensor(A, device="cuda"), pivot=True)
+    perm = perm.cpu().numpy()
+    for b in range(2):
+        assert sorted(perm[b, :h].tolist()) == list(range(h)), perm[b]
+        assert perm[b, 0] == int(np.argmax(np.linalg.norm(B[b], axis=0)))
+        Ab = A[b][:, perm[b]]
+        assert np.linalg.norm(Q.cpu().numpy()[b] @ R.cpu().numpy()[b] - Ab) / np.linalg.norm(Ab) < 1e-13
 
 
 def _device_hbs(T, U, D, V, fac):

@@ -77,8 +77,8 @@ def test_gmres_hbs_preconditioned_3d(T):
     rk = 8
     H = {}
     for side in ("left", "right"):
-        h = HBSnew.HBSMAT(s.S[side], tree, quad=False, seed=1)
-        h._samples = lambda rk_, s_=max(2 * rk, 64) + rk + 10: s_
+        h = HBSnew.HBSMAT(s.S[side], tree, quad=False, seed=1,
+                          samples=max(2 * rk, 64) + rk + 10)
         h.construct(rk)
         H[side] = h
     nb = s.nb

@@ -67,8 +67,7 @@ def test_hbs_compressed_thomas_solve(T):
     rk = 30
     H = {}
     for side in ("left", "right"):
-        h = HBSnew.HBSMAT(s.S[side], tree, quad=False)
-        h._samples = lambda rk_, s_=max(2 * rk, 64) + rk + 10: s_
+        h = HBSnew.HBSMAT(s.S[side], tree, quad=False, samples=max(2 * rk, 64) + rk + 10)
         h.construct(rk)
         H[side] = h
     I = torch.eye(s.m, dtype=torch.float64, device="cuda")
@@ -98,8 +97,8 @@ def test_hbs_solve_graph_replay(T):
     rk = 20
     H = {}
     for side in ("left", "right"):
-        h = HBSnew.HBSMAT(s.S[side], tree, quad=False, seed=2)
-        h._samples = lambda rk_, s_=max(2 * rk, 64) + rk + 10: s_
+        h = HBSnew.HBSMAT(s.S[side], tree, quad=False, seed=2,
+                          samples=max(2 * rk, 64) + rk + 10)
         h.construct(rk)
         H[side] = h
     nb = s.nb
