@@ -41,6 +41,12 @@ CALIB_MIB = 256
 # copy (summary and commands: profiles/rb_fold_solve_c2_helmholtz_hbm_traffic.md).  A
 # measured constant of the current kernels: re-measure after changing them.
 TRAFFIC = 56812966
+# Tile policies of the timed step (backward sweep, boundary-map launch, one per backward
+# level 2, 1, 0), as the autotuner picks them on this workload (bench.py --autotune prints
+# them in timed_policy_flags); pinned so that a profiler run of the same flags traces the
+# configuration the bench line times.
+STEP_POLICIES = ("512,512,64,256,3,14,0", "1024,2048,64,256,3,5,1",
+                 "512,1024,64,256,3,12,0;512,1024,64,256,10,12,1;1024,512,64,256,3,14,0")
 
 
 class PDO2d:
@@ -577,7 +583,12 @@ def main():
     ap.add_argument("--policy", default=None,
                     help="fixed GEMM tile policy (7 comma-separated ints, as reported in "
                          "gemm_policy) instead of autotuning; for profiler runs")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the step's tile policies by timing candidates on this box "
+                         "instead of the pinned set (STEP_POLICIES)")
     args = ap.parse_args()
+    if not args.policy and not args.autotune:
+        args.policy, args.fold_policy, args.bwd_policies = STEP_POLICIES
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box here has a single GPU and the "
               "path is benchmarked at N=1 (multi-GPU sharding is design-only).", file=sys.stderr)
