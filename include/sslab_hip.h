@@ -87,6 +87,16 @@ int sslab_dgetrs_batched(int n, int nrhs, const double* LU, int lda, long long s
                          long long strideB, double* X, int ldx, long long strideX,
                          int* perm_work, int batch, void* stream);
 
+/* Solve A_b^T X_b = B_b with the factors of sslab_dgetrf_batched (P A = L U, so
+ * U^T L^T (P X) = B).  B is read, X (distinct from B) written; work: batch x n x nrhs
+ * doubles (stride n*nrhs), perm_work: batch x n ints.  Replaces the transposed sparse
+ * solves behind the S maps' rmatvec / rmatmat (multislab/oms.py:111-129:
+ * solver_ii.rmatvec = A_ii^{-T}, used by the rkHBS sketches Z = S^T Psi,
+ * matAssembly/matAssembler.py:103-133). */
+int sslab_dgetrs_trans_batched(int n, int nrhs, const double* LU, int lda, long long strideA,
+                               const int* ipiv, long long strideP, const double* B, int ldb,
+                               long long strideB, double* X, int ldx, long long strideX,
+                               double* work, int* perm_work, int batch, void* stream);
 /* Householder QR, optionally column-pivoted (m, n <= 1024), one matrix per
  * workgroup: A_b (m x n) is overwritten by R (zeros below the diagonal); Q (m x m,
  * optional) receives the complete orthogonal factor; perm (n ints, optional)
@@ -236,6 +246,27 @@ int sslab_hps_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx,
                        const double* Ni, const double* c11, const double* c22,
                        const double* c1, const double* c2, const double* c0, double* T,
                        double* X, double* work, int* iwork, int* info, void* stream);
+
+/* ---- HPS slab assembly, 3-D Chebyshev leaves (q^3 points, p = q - 2) ----------
+ * The 3-D discretization of the reference's cube drivers (test/3D/cubeTest.py:114-126:
+ * solverOptions('hpsalt', [p_disc]*3, a), hpsalt's Domain_Driver, solver/solver.py:
+ * 120-144; absent upstream).  Leaf DtN maps T_l = N_b - N_i A_ii^{-1} A_ib (6p^2 x 6p^2;
+ * faces x-lo, x-hi, y-lo, y-hi, z-lo, z-hi, each p^2 nodes in lexicographic order of
+ * its two tangential grid indices, edges and corners excluded).  Dx..Dzz: q x q
+ * derivative matrices; Nb (6p^2 x 6p^2), Ni (6p^2 x p^3): outward-normal derivative
+ * rows (shared by all leaves); c11, c22, c33, c1, c2, c3, c0: nleaves*p^3 values at the
+ * interior nodes (row ((a-1) p + (b-1)) p + (c-1), a along x), any may be NULL (= 0);
+ * X (optional, nleaves x p^3 x 6p^2) receives A_ii^{-1} A_ib; info: nleaves ints.
+ * Box merges and pair factors of 3-D slabs use the generic sslab_hps_merge,
+ * sslab_gather2, sslab_dgetrf_batched and sslab_dgetrs(_trans)_batched. */
+int sslab_hps3_leaf_sizes(int nleaves, int q, long long* work_doubles, long long* work_ints);
+int sslab_hps3_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx,
+                        const double* Dy, const double* Dyy, const double* Dz,
+                        const double* Dzz, const double* Nb, const double* Ni,
+                        const double* c11, const double* c22, const double* c33,
+                        const double* c1, const double* c2, const double* c3,
+                        const double* c0, double* T, double* X, double* work, int* iwork,
+                        int* info, void* stream);
 
 /* Merge nmerge box pairs (alpha_b = Ta + b*sTa, beta_b = Tb + b*sTb) that share c
  * edge nodes (positions ca in alpha, cb in beta) into boxes with n_new boundary

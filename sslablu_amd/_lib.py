@@ -39,6 +39,8 @@ _SIGS = {
     "sslab_dgetrf_batched": [_c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int, _vp],
     "sslab_dgetrs_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int,
                              _c_ll, _vp, _c_int, _c_ll, _vp, _c_int, _vp],
+    "sslab_dgetrs_trans_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp,
+                                   _c_int, _c_ll, _vp, _c_int, _c_ll, _vp, _vp, _c_int, _vp],
     "sslab_dgeqrf_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_int, _c_ll, _vp,
                              _c_ll, _c_int, _vp],
     "sslab_dtrsm_upper_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_int, _c_ll,
@@ -62,6 +64,8 @@ _SIGS = {
     "sslab_thomas_solve": [_c_int, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _vp],
     "sslab_hps_leaf_sizes": [_c_int, _c_int, _vp, _vp],
     "sslab_hps_leaf_dtn": [_c_int, _c_int] + [_vp] * 16 + [_vp],
+    "sslab_hps3_leaf_sizes": [_c_int, _c_int, _vp, _vp],
+    "sslab_hps3_leaf_dtn": [_c_int, _c_int] + [_vp] * 19 + [_vp],
     "sslab_hps_merge_sizes": [_c_int, _c_int, _c_int, _vp, _vp],
     "sslab_hps_merge": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_ll, _vp, _c_ll,
                         _vp, _vp, _vp, _vp, _vp, _c_ll, _vp, _vp, _vp, _vp],

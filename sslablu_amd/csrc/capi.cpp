@@ -148,6 +148,14 @@ int sslab_dgetrs_batched(int n, int nrhs, const double* LU, int lda, long long s
                               strideX, perm_work, batch, as_stream(stream));
 }
 
+int sslab_dgetrs_trans_batched(int n, int nrhs, const double* LU, int lda, long long strideA,
+                               const int* ipiv, long long strideP, const double* B, int ldb,
+                               long long strideB, double* X, int ldx, long long strideX,
+                               double* work, int* perm_work, int batch, void* stream) {
+  return sslab::getrs_trans_batched(n, nrhs, LU, lda, strideA, ipiv, strideP, B, ldb, strideB, X,
+                                    ldx, strideX, work, perm_work, batch, as_stream(stream));
+}
+
 int sslab_dgeqrf_batched(int m, int n, double* A, int lda, long long strideA, double* Q, int ldq,
                          long long strideQ, int* perm, long long stridePerm, int batch,
                          void* stream) {
@@ -247,6 +255,21 @@ int sslab_hps_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx, 
                        double* T, double* X, double* work, int* iwork, int* info, void* stream) {
   return sslab::hps_leaf_dtn(nleaves, q, Dx, Dxx, Dy, Dyy, Nb, Ni, c11, c22, c1, c2, c0, T, X,
                              work, iwork, info, as_stream(stream));
+}
+
+int sslab_hps3_leaf_sizes(int nleaves, int q, long long* work_doubles, long long* work_ints) {
+  return sslab::hps3_leaf_sizes(nleaves, q, work_doubles, work_ints);
+}
+
+int sslab_hps3_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx, const double* Dy,
+                        const double* Dyy, const double* Dz, const double* Dzz, const double* Nb,
+                        const double* Ni, const double* c11, const double* c22, const double* c33,
+                        const double* c1, const double* c2, const double* c3, const double* c0,
+                        double* T, double* X, double* work, int* iwork, int* info, void* stream) {
+  const double* D[6] = {Dx, Dxx, Dy, Dyy, Dz, Dzz};
+  const double* coef[7] = {c11, c22, c33, c1, c2, c3, c0};
+  return sslab::hps3_leaf_dtn(nleaves, q, D, Nb, Ni, coef, T, X, work, iwork, info,
+                              as_stream(stream));
 }
 
 int sslab_hps_merge_sizes(int nmerge, int n_new, int c, long long* work_doubles,

@@ -1,4 +1,5 @@
-// HPS slab assembly on gfx950 (FP64): leaf Dirichlet-to-Neumann maps and box merges.
+// HPS slab assembly on gfx950 (FP64): leaf Dirichlet-to-Neumann maps (2-D and 3-D leaves)
+// and box merges (any dimension: the caller's index maps describe the geometry).
 //
 // Leaf (q x q Chebyshev grid, p = q-2):  interior collocation rows
 //   A[u] = -c11 u_xx - c22 u_yy + c1 u_x + c2 u_y + c0 u      (pdo/pdo.py:19-39)
@@ -64,6 +65,59 @@ __global__ void k_leaf_assemble(int q, const double* __restrict__ Dx, const doub
     Aii[leaf * p2 * p2 + (long long)r * p2 + col] = v;
   else
     Aib[leaf * p2 * 4 * p + (long long)r * 4 * p + (col - p2)] = v;
+}
+
+// 3-D leaf (q x q x q Chebyshev grid, p = q-2): one thread per (row r, column col) of
+// [A_ii | A_ib] for one leaf (blockIdx.z).  Interior node (a, b, c), 1-based grid
+// indices with a along x, is row r = ((a-1) p + (b-1)) p + (c-1).  Boundary columns are
+// the six faces [x-lo, x-hi, y-lo, y-hi, z-lo, z-hi], p^2 nodes each (edges and corners
+// excluded), each face in lexicographic order of its two tangential grid indices.
+//   A[u] = -c11 u_xx - c22 u_yy - c33 u_zz + c1 u_x + c2 u_y + c3 u_z + c0 u
+__global__ void k_leaf_assemble3(int q, const double* __restrict__ Dx,
+                                 const double* __restrict__ Dxx, const double* __restrict__ Dy,
+                                 const double* __restrict__ Dyy, const double* __restrict__ Dz,
+                                 const double* __restrict__ Dzz, const double* __restrict__ c11,
+                                 const double* __restrict__ c22, const double* __restrict__ c33,
+                                 const double* __restrict__ c1, const double* __restrict__ c2,
+                                 const double* __restrict__ c3, const double* __restrict__ c0,
+                                 double* __restrict__ Aii, double* __restrict__ Aib) {
+  const int p = q - 2;
+  const int p2 = p * p, p3 = p2 * p, nb6 = 6 * p2;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  const long long leaf = blockIdx.z;
+  if (col >= p3 + nb6) return;
+  const int a = r / p2 + 1, b = (r / p) % p + 1, c = r % p + 1;
+  const long long ci = leaf * p3 + r;
+  const double k11 = c11 ? c11[ci] : 0.0;
+  const double k22 = c22 ? c22[ci] : 0.0;
+  const double k33 = c33 ? c33[ci] : 0.0;
+  const double k1 = c1 ? c1[ci] : 0.0;
+  const double k2 = c2 ? c2[ci] : 0.0;
+  const double k3 = c3 ? c3[ci] : 0.0;
+  const double k0 = c0 ? c0[ci] : 0.0;
+  int a2, b2, c2i;
+  if (col < p3) {
+    a2 = col / p2 + 1;
+    b2 = (col / p) % p + 1;
+    c2i = col % p + 1;
+  } else {
+    const int e = col - p3;
+    const int face = e / p2, u = (e % p2) / p + 1, v = e % p + 1;
+    const int end = (face & 1) ? q - 1 : 0;
+    if (face < 2) { a2 = end; b2 = u; c2i = v; }
+    else if (face < 4) { a2 = u; b2 = end; c2i = v; }
+    else { a2 = u; b2 = v; c2i = end; }
+  }
+  double val = 0.0;
+  if (b2 == b && c2i == c) val += -k11 * Dxx[a * q + a2] + k1 * Dx[a * q + a2];
+  if (a2 == a && c2i == c) val += -k22 * Dyy[b * q + b2] + k2 * Dy[b * q + b2];
+  if (a2 == a && b2 == b) val += -k33 * Dzz[c * q + c2i] + k3 * Dz[c * q + c2i];
+  if (a2 == a && b2 == b && c2i == c) val += k0;
+  if (col < p3)
+    Aii[leaf * p3 * p3 + (long long)r * p3 + col] = val;
+  else
+    Aib[leaf * p3 * nb6 + (long long)r * nb6 + (col - p3)] = val;
 }
 
 // out[b][i][j] = sum_s [rid_s[i] >= 0 && cid_s[j] >= 0] * src_s[b][rid_s[i]][cid_s[j]]
@@ -134,6 +188,45 @@ int hps_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx, const 
   G.nd = 1;
   G.d[0] = gemm_desc(nb4, nb4, nleaves, T, nb4, (long long)nb4 * nb4, Nb, nb4, 0, 1.0);
   gemm_add_term(G.d[0], gemm_term(Ni, p2, 0, 0, Xs, nb4, (long long)p2 * nb4, 0, p2, -1.0));
+  SSLAB_TRY(gemm_launch(G, st));
+  return kOk;
+}
+
+int hps3_leaf_sizes(int nleaves, int q, long long* work_doubles, long long* work_ints) {
+  SSLAB_REQUIRE(nleaves >= 1 && q >= 3, "hps3_leaf_sizes: bad sizes");
+  const long long p = q - 2, p3 = p * p * p, nb6 = 6 * p * p;
+  if (work_doubles) *work_doubles = (long long)nleaves * (p3 * p3 + 2 * p3 * nb6);
+  if (work_ints) *work_ints = (long long)nleaves * 2 * p3 + nleaves;
+  return kOk;
+}
+
+int hps3_leaf_dtn(int nleaves, int q, const double* const* D, const double* Nb, const double* Ni,
+                  const double* const* coef, double* T, double* X, double* work, int* iwork,
+                  int* info, hipStream_t st) {
+  SSLAB_REQUIRE(nleaves >= 1 && q >= 3, "hps3_leaf_dtn: bad sizes");
+  SSLAB_REQUIRE(nleaves <= 65535, "hps3_leaf_dtn: at most 65535 leaves per call");
+  SSLAB_REQUIRE(D && coef && Nb && Ni && T && work && iwork, "hps3_leaf_dtn: null pointer");
+  for (int i = 0; i < 6; ++i) SSLAB_REQUIRE(D[i] != nullptr, "hps3_leaf_dtn: null derivative");
+  const int p = q - 2, p3 = p * p * p, nb6 = 6 * p * p;
+  SSLAB_REQUIRE(p3 <= 65535, "hps3_leaf_dtn: leaf order too large");
+  double* Aii = work;
+  double* Aib = Aii + (long long)nleaves * p3 * p3;
+  double* Xs = X ? X : Aib + (long long)nleaves * p3 * nb6;
+  int* ipiv = iwork;
+  int* perm = iwork + (long long)nleaves * p3;
+  int* linfo = info ? info : perm + (long long)nleaves * p3;
+  hipLaunchKernelGGL(k_leaf_assemble3, dim3(ceil_div(p3 + nb6, 256), p3, nleaves), dim3(256), 0,
+                     st, q, D[0], D[1], D[2], D[3], D[4], D[5], coef[0], coef[1], coef[2], coef[3],
+                     coef[4], coef[5], coef[6], Aii, Aib);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  SSLAB_TRY(getrf_batched(p3, Aii, p3, (long long)p3 * p3, ipiv, p3, linfo, nleaves, st));
+  SSLAB_TRY(getrs_batched(p3, nb6, Aii, p3, (long long)p3 * p3, ipiv, p3, Aib, nb6,
+                          (long long)p3 * nb6, Xs, nb6, (long long)p3 * nb6, perm, nleaves, st));
+  // T = N_b - N_i X   (N shared by all leaves: batch stride 0)
+  GemmLaunch G{};
+  G.nd = 1;
+  G.d[0] = gemm_desc(nb6, nb6, nleaves, T, nb6, (long long)nb6 * nb6, Nb, nb6, 0, 1.0);
+  gemm_add_term(G.d[0], gemm_term(Ni, p3, 0, 0, Xs, nb6, (long long)p3 * nb6, 0, p3, -1.0));
   SSLAB_TRY(gemm_launch(G, st));
   return kOk;
 }

@@ -16,6 +16,15 @@ int hps_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx, const 
                  const double* c22, const double* c1, const double* c2, const double* c0,
                  double* T, double* X, double* work, int* iwork, int* info, hipStream_t st);
 
+// 3-D leaves (q^3 Chebyshev points): D = {Dx, Dxx, Dy, Dyy, Dz, Dzz} (q x q each),
+// Nb (6p^2 x 6p^2) and Ni (6p^2 x p^3) the outward-normal flux rows, coef = {c11, c22,
+// c33, c1, c2, c3, c0} per leaf-interior node (entries may be null).  T: nleaves x
+// 6p^2 x 6p^2; X (optional): nleaves x p^3 x 6p^2 = A_ii^{-1} A_ib.
+int hps3_leaf_sizes(int nleaves, int q, long long* work_doubles, long long* work_ints);
+int hps3_leaf_dtn(int nleaves, int q, const double* const* D, const double* Nb, const double* Ni,
+                  const double* const* coef, double* T, double* X, double* work, int* iwork,
+                  int* info, hipStream_t st);
+
 int hps_merge_sizes(int nmerge, int n_new, int c, long long* work_doubles, long long* work_ints);
 int hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta, long long sTa,
               const double* Tb, long long sTb, const int* ca, const int* cb, const int* na,

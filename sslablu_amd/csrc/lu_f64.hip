@@ -291,7 +291,10 @@ __global__ void k_laswp(int r0, int w, int a0, int a1, int b0, int b1, E* __rest
 // coordinates (J L J is upper triangular for the reversal J), because the compiler
 // schedules the unrolled backward loop without hoisting the whole triangle into
 // registers, while the unrolled forward loop spills.
-template <typename E, int NB, bool kLower>
+// The solved matrix M is T (kTrans = false) or T^T read from the same storage: kRev
+// selects the reversed coordinates (M lower triangular), kUnit a unit diagonal (the
+// packed LU's L, whose stored diagonal is U's).
+template <typename E, int NB, bool kRev, bool kUnit, bool kTrans>
 __global__ void __launch_bounds__(256)
 k_trsm_cols(int nr, int ncols, const E* __restrict__ T, int ldt, long long sT,
             E* __restrict__ B, int ldb, long long sB) {
@@ -300,10 +303,9 @@ k_trsm_cols(int nr, int ncols, const E* __restrict__ T, int ldt, long long sT,
   E* Bb = B + (long long)blockIdx.y * sB;
   for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
     const int i = e / NB, j = e % NB;
-    const int ri = kLower ? NB - 1 - i : i, rj = kLower ? NB - 1 - j : j;
-    // the unit lower triangle's diagonal holds U's in the packed LU: use 1
-    Ts[e] = (ri < nr && rj < nr && !(kLower && ri == rj)) ? Tb[(long long)ri * ldt + rj]
-                                                          : (i == j ? 1.0 : 0.0);
+    const int ri = kRev ? NB - 1 - i : i, rj = kRev ? NB - 1 - j : j;
+    const long long off = kTrans ? (long long)rj * ldt + ri : (long long)ri * ldt + rj;
+    Ts[e] = (ri < nr && rj < nr && !(kUnit && ri == rj)) ? Tb[off] : (i == j ? 1.0 : 0.0);
   }
   __syncthreads();
   const int col0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -313,7 +315,7 @@ k_trsm_cols(int nr, int ncols, const E* __restrict__ T, int ldt, long long sT,
   E x[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
-    const int ri = kLower ? NB - 1 - i : i;
+    const int ri = kRev ? NB - 1 - i : i;
     const E v = bcol[(long long)(ri < nr ? ri : nr - 1) * ldb];
     x[i] = (ri < nr) ? v : 0.0;
   }
@@ -327,9 +329,22 @@ k_trsm_cols(int nr, int ncols, const E* __restrict__ T, int ldt, long long sT,
   E* ocol = Bb + col0;
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
-    const int ri = kLower ? NB - 1 - i : i;
+    const int ri = kRev ? NB - 1 - i : i;
     if (ri < nr) ocol[(long long)ri * ldb] = x[i];
   }
+}
+
+// X[perm[i], :] = W[i, :]  (the inverse of k_gather_rows)
+template <typename E>
+__global__ void k_scatter_rows(int n, int ncols, const int* __restrict__ perm,
+                               const E* __restrict__ W, int ldw, long long sW,
+                               E* __restrict__ X, int ldx, long long sX) {
+  const int b = blockIdx.z;
+  const int i = blockIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncols) return;
+  const int dst = perm[(long long)b * n + i];
+  X[(long long)b * sX + (long long)dst * ldx + c] = W[(long long)b * sW + (long long)i * ldw + c];
 }
 
 // perm = the row order produced by applying ipiv[0..n) to the identity.
@@ -433,11 +448,13 @@ int trsm_2level(bool lower, int n, int nrhs, const E* T, int ldt, long long sT, 
   const dim3 grid(ceil_div(nrhs, 256), batch);
   auto diag = [&](int ib, int ni) -> int {
     if (lower)
-      hipLaunchKernelGGL((k_trsm_cols<E, NB, true>), grid, dim3(256), 0, st, ni, nrhs,
-                         T + (long long)ib * ldt + ib, ldt, sT, X + (long long)ib * ldx, ldx, sX);
+      hipLaunchKernelGGL((k_trsm_cols<E, NB, true, true, false>), grid, dim3(256), 0, st, ni,
+                         nrhs, T + (long long)ib * ldt + ib, ldt, sT, X + (long long)ib * ldx, ldx,
+                         sX);
     else
-      hipLaunchKernelGGL((k_trsm_cols<E, NB, false>), grid, dim3(256), 0, st, ni, nrhs,
-                         T + (long long)ib * ldt + ib, ldt, sT, X + (long long)ib * ldx, ldx, sX);
+      hipLaunchKernelGGL((k_trsm_cols<E, NB, false, false, false>), grid, dim3(256), 0, st, ni,
+                         nrhs, T + (long long)ib * ldt + ib, ldt, sT, X + (long long)ib * ldx, ldx,
+                         sX);
     SSLAB_HIP_CHECK(hipGetLastError());
     return kOk;
   };
@@ -449,6 +466,62 @@ int trsm_2level(bool lower, int n, int nrhs, const E* T, int ldt, long long sT, 
                                 ldx, sX, batch, st);
   };
   if (lower) {
+    for (int kb = 0; kb < n; kb += kOuterNB) {
+      const int nr = n - kb < kOuterNB ? n - kb : kOuterNB;
+      for (int ib = kb; ib < kb + nr; ib += NB) {
+        const int ni = kb + nr - ib < NB ? kb + nr - ib : NB;
+        SSLAB_TRY(diag(ib, ni));
+        SSLAB_TRY(upd(ib + ni, kb + nr - ib - ni, ib, ni));
+      }
+      SSLAB_TRY(upd(kb + nr, n - kb - nr, kb, nr));
+    }
+  } else {
+    const int nob = ceil_div(n, kOuterNB);
+    for (int ob = nob - 1; ob >= 0; --ob) {
+      const int kb = ob * kOuterNB;
+      const int nr = n - kb < kOuterNB ? n - kb : kOuterNB;
+      const int nib = ceil_div(nr, NB);
+      for (int q = nib - 1; q >= 0; --q) {
+        const int ib = kb + q * NB;
+        const int ni = kb + nr - ib < NB ? kb + nr - ib : NB;
+        SSLAB_TRY(diag(ib, ni));
+        SSLAB_TRY(upd(kb, ib - kb, ib, ni));
+      }
+      SSLAB_TRY(upd(0, kb, kb, nr));
+    }
+  }
+  return kOk;
+}
+
+// X (n x nrhs) <- (T^T)^{-1} X, T read from the packed LU: U^T (lower, non-unit: a
+// forward substitution, ut = true) or L^T (unit upper: a backward substitution).
+// Off-diagonal blocks are transposed-operand GEMMs on the same storage.
+template <typename E>
+int trsm_2level_trans(bool ut, int n, int nrhs, const E* T, int ldt, long long sT, E* X,
+                      int ldx, long long sX, int batch, hipStream_t st) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return kOk;
+  constexpr int NB = kTrsmNB;
+  const dim3 grid(ceil_div(nrhs, 256), batch);
+  auto diag = [&](int ib, int ni) -> int {
+    if (ut)
+      hipLaunchKernelGGL((k_trsm_cols<E, NB, true, false, true>), grid, dim3(256), 0, st, ni,
+                         nrhs, T + (long long)ib * ldt + ib, ldt, sT, X + (long long)ib * ldx, ldx,
+                         sX);
+    else
+      hipLaunchKernelGGL((k_trsm_cols<E, NB, false, true, true>), grid, dim3(256), 0, st, ni,
+                         nrhs, T + (long long)ib * ldt + ib, ldt, sT, X + (long long)ib * ldx, ldx,
+                         sX);
+    SSLAB_HIP_CHECK(hipGetLastError());
+    return kOk;
+  };
+  // X[r0:r0+nr) -= (T^T)[r0:r0+nr, c0:c0+nc) X[c0:c0+nc) = T[c0:c0+nc, r0:r0+nr)^T X[c0:..)
+  auto upd = [&](int r0, int nr, int c0, int nc) -> int {
+    if (nr <= 0 || nc <= 0) return kOk;
+    return gemm_strided<E>(1, 0, nr, nrhs, nc, -1.0, T + (long long)c0 * ldt + r0, ldt, sT,
+                           X + (long long)c0 * ldx, ldx, sX, 1.0, X + (long long)r0 * ldx, ldx,
+                           sX, batch, st);
+  };
+  if (ut) {
     for (int kb = 0; kb < n; kb += kOuterNB) {
       const int nr = n - kb < kOuterNB ? n - kb : kOuterNB;
       for (int ib = kb; ib < kb + nr; ib += NB) {
@@ -650,6 +723,33 @@ int getrs_batched(int n, int nrhs, const E* LU, int lda, long long sA, const int
 }
 
 template <typename E>
+int getrs_trans_batched(int n, int nrhs, const E* LU, int lda, long long sA, const int* ipiv,
+                        long long sP, const E* B, int ldb, long long sB, E* X, int ldx,
+                        long long sX, E* W, int* perm, int batch, hipStream_t stream) {
+  SSLAB_REQUIRE(n >= 0 && nrhs >= 0 && batch >= 0, "getrs_trans: negative size");
+  SSLAB_REQUIRE(lda >= n && ldb >= nrhs && ldx >= nrhs, "getrs_trans: bad leading dimension");
+  SSLAB_REQUIRE(W != nullptr && W != B && W != X, "getrs_trans: W must be a distinct workspace");
+  if (n == 0 || nrhs == 0 || batch == 0) return kOk;
+  SSLAB_REQUIRE(batch <= 65535 && n <= 65535, "getrs_trans: batch or n too large");
+  SSLAB_REQUIRE((size_t)n * sizeof(int) <= (size_t)max_lds_bytes(), "getrs_trans: n too large");
+  // P A = L U, so A^T X = B reads U^T L^T (P X) = B: W = B, W <- U^-T W, W <- L^-T W,
+  // X[perm[i]] = W[i]
+  const long long sW = (long long)n * nrhs;
+  hipLaunchKernelGGL(k_ipiv_to_perm, dim3(batch), dim3(256), (size_t)n * sizeof(int), stream, n,
+                     ipiv, sP, perm);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_copy_block<E>, dim3(ceil_div(nrhs, 256), n, batch), dim3(256), 0, stream,
+                     nrhs, B, ldb, sB, W, nrhs, sW);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  SSLAB_TRY(trsm_2level_trans<E>(true, n, nrhs, LU, lda, sA, W, nrhs, sW, batch, stream));
+  SSLAB_TRY(trsm_2level_trans<E>(false, n, nrhs, LU, lda, sA, W, nrhs, sW, batch, stream));
+  hipLaunchKernelGGL(k_scatter_rows<E>, dim3(ceil_div(nrhs, 256), n, batch), dim3(256), 0, stream,
+                     n, nrhs, perm, W, nrhs, sW, X, ldx, sX);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+template <typename E>
 int trsm_upper_batched(int n, int nrhs, const E* U, int ldu, long long sU, E* X, int ldx,
                        long long sX, int batch, hipStream_t stream) {
   SSLAB_REQUIRE(n >= 0 && nrhs >= 0 && batch >= 0, "trsm: negative size");
@@ -697,6 +797,9 @@ int zero_block_batched(int m, int n, E* Y, int ldy, long long sY, int batch,
   template int getrs_batched<E>(int, int, const E*, int, long long, const int*, long long,       \
                                 const E*, int, long long, E*, int, long long, int*, int,       \
                                 hipStream_t);                                                  \
+  template int getrs_trans_batched<E>(int, int, const E*, int, long long, const int*, long long, \
+                                      const E*, int, long long, E*, int, long long, E*, int*,  \
+                                      int, hipStream_t);                                       \
   template int trsm_upper_batched<E>(int, int, const E*, int, long long, E*, int, long long,    \
                                      int, hipStream_t);                                        \
   template int set_identity_batched<E>(int, E*, int, long long, int, hipStream_t);             \

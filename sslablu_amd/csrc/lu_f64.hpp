@@ -25,6 +25,13 @@ int getrs_batched(int n, int nrhs, const E* LU, int lda, long long sA, const int
                   long long sP, const E* B, int ldb, long long sB, E* X, int ldx,
                   long long sX, int* perm, int batch, hipStream_t stream);
 
+// Solve A_b^T X_b = B_b with the factors from getrf_batched (B read, X written, distinct
+// from B).  W: batch x n x nrhs workspace (stride n*nrhs); perm as in getrs_batched.
+template <typename E>
+int getrs_trans_batched(int n, int nrhs, const E* LU, int lda, long long sA, const int* ipiv,
+                        long long sP, const E* B, int ldb, long long sB, E* X, int ldx,
+                        long long sX, E* W, int* perm, int batch, hipStream_t stream);
+
 // In place X_b <- U_b^{-1} X_b for upper-triangular U_b (n x n, non-unit diagonal).
 template <typename E>
 int trsm_upper_batched(int n, int nrhs, const E* U, int ldu, long long sU, E* X, int ldx,

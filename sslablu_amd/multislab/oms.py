@@ -9,9 +9,9 @@ rhs = -(A_ii^{-1} A_ib[:, Igb] f)[Ic]; S_rk_list = [[S_r], [S_l, S_r], ..., [S_l
 contiguous global dof ranges per interface.
 
 The reference discretizes every double slab on its own (oms.py:160-205); here the
-double slabs of a dSlabs(N) decomposition of the unit square with the 'hps'
-discretization are assembled together on the GPU (every single slab once, see
-sslablu_amd.hps).  S blocks come back as DeviceMatrix (dense assembler) or HBSMAT
+double slabs of a dSlabs(N) decomposition of the unit square or cube with the 'hps' /
+'hpsalt' discretization are assembled together on the GPU (every single slab once, see
+sslablu_amd.hps and sslablu_amd.hps3d).  S blocks come back as DeviceMatrix (dense assembler) or HBSMAT
 (rkHBS assembler) objects that live in HBM; rhs_list entries are numpy arrays.
 """
 from __future__ import annotations
@@ -23,6 +23,7 @@ import torch
 from scipy.sparse.linalg import LinearOperator
 
 from ..hps import HPSSlabOperator, SlabGrid2D
+from ..hps3d import HPSSlabOperator3D, SlabGrid3D
 from ..linop import DeviceMatrix
 from ..solver.solver import hps_grid_params, stMap
 
@@ -101,37 +102,58 @@ class oms:
     # -- discretization ------------------------------------------------------------------
     def _grid(self):
         slabs = [np.asarray(s, dtype=np.float64) for s in self.slabList]
+        ndim = slabs[0].shape[1]
         N = len(slabs) + 1
         H = (slabs[0][1][0] - slabs[0][0][0]) / 2.0
         x0 = slabs[0][0][0]
-        y0, y1 = slabs[0][0][1], slabs[0][1][1]
         for k, s in enumerate(slabs):
-            if abs(s[0][0] - (x0 + k * H)) > 1e-12 or abs(s[1][0] - (x0 + (k + 2) * H)) > 1e-12:
+            if abs(s[0][0] - (x0 + k * H)) > 1e-12 or abs(s[1][0] - (x0 + (k + 2) * H)) > 1e-12 \
+                    or np.any(np.abs(s[:, 1:] - slabs[0][:, 1:]) > 1e-12):
                 raise NotImplementedError("slabs must be the consecutive double slabs of a "
-                                          "rectangle (geometry.geom_2D.square.dSlabs)")
+                                          "box (geometry.geom_2D.square / geom_3D.cube dSlabs)")
         for k, c in enumerate(self.connectivity):
             want = [k - 1 if k > 0 else -1, k + 1 if k < len(slabs) - 1 else -1]
             if list(c) != want:
                 raise NotImplementedError("only the non-periodic chain connectivity of dSlabs "
                                           "is assembled on the GPU")
-        q, hx, hy = hps_grid_params(self.opts, 0, 0)
+        lo, hi = slabs[0][0], slabs[0][1]
+        if ndim == 3:
+            q, hx, hy, hz = hps_grid_params(self.opts, ndim=3)
+            nl = (int(round(H / hx)), int(round((hi[1] - lo[1]) / hy)),
+                  int(round((hi[2] - lo[2]) / hz)))
+            return SlabGrid3D(N, *nl, q, origin=(x0, lo[1], lo[2]),
+                              size=(N * H, hi[1] - lo[1], hi[2] - lo[2]))
+        if ndim != 2:
+            raise NotImplementedError("the GPU HPS discretization is 2-D or 3-D")
+        q, hx, hy = hps_grid_params(self.opts)
         nx = int(round(H / hx))
-        ny = int(round((y1 - y0) / hy))
-        return SlabGrid2D(N, nx, ny, q, x0=x0, width=N * H, y0=y0, height=y1 - y0)
+        ny = int(round((hi[1] - lo[1]) / hy))
+        return SlabGrid2D(N, nx, ny, q, x0=x0, width=N * H, y0=lo[1], height=hi[1] - lo[1])
 
     def discretize(self):
-        """Assemble (once) the S maps of every double slab on the GPU."""
+        """Assemble (once) the S maps of every double slab on the GPU (2-D: dense maps;
+        3-D: the pair factors behind the S-map operators)."""
         if self.op is None:
             t0 = time.time()
-            self.op = HPSSlabOperator(self.pdo, self._grid()).build()
+            g = self._grid()
+            cls = HPSSlabOperator3D if isinstance(g, SlabGrid3D) else HPSSlabOperator
+            self.op = cls(self.pdo, g).build()
             torch.cuda.synchronize()
             self.stats.discr_timing = (time.time() - t0) / max(len(self.slabList), 1)
         return self.op
 
     def compute_stmaps(self, k):
-        """stMap objects of the left / right S maps of double slab k (oms.py:99-129)."""
+        """stMap objects of the left / right S maps of double slab k (oms.py:99-129):
+        dense device blocks in 2-D, device operators (sparse-solve-like applies through
+        the pair factors) in 3-D, as the reference's LinearOperators."""
         op = self.discretize()
         g = op.g
+        if isinstance(g, SlabGrid3D):
+            m = g.m
+            XXl = g.slab_boundary_points(k)[:m]
+            XXc = g.slab_boundary_points(k)[m:2 * m]
+            XXr = g.slab_boundary_points(k + 1)[m:2 * m]
+            return stMap(op.smap(k, "l"), XXl, XXc), stMap(op.smap(k, "r"), XXr, XXc)
         XXc = g.interface_points()[k * g.m:(k + 1) * g.m]
         ys = XXc[:, 1]
         xl = g.x0 + k * g.H

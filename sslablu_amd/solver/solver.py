@@ -1,10 +1,11 @@
 """Local slab solver wrapper and the source-target map container.
 
 Mirror of solver/solver.py (SimonDirckx/SslabLU): stMap (:30-36), solverOptions
-(:39-50) and solverWrapper (:65-163).  The discretization type 'hps' is this project's
-2-D Chebyshev HPS on the GPU (the reference's HPS submodules are not part of it):
-ord = [q, q] Chebyshev points per leaf side (q = p + 2) and a = [hx/2, hy/2] the leaf
-half-sizes, as in the reference drivers (test/3D/cubeTest.py:114-118).
+(:39-50) and solverWrapper (:65-163).  The discretization types 'hps' and 'hpsalt' are
+this project's Chebyshev HPS on the GPU, 2-D (sslablu_amd.hps) and 3-D
+(sslablu_amd.hps3d) (the reference's HPS submodules are not part of it): ord = [q]*d
+Chebyshev points per leaf side (q = p + 2, the reference drivers' p_disc) and a the
+leaf half-sizes, as in the reference drivers (test/3D/cubeTest.py:114-118).
 
 solverWrapper.construct(geom, PDE) discretizes one double slab and exposes the
 reference's attributes on its skeleton: XX (node coordinates), Ib (the slab boundary,
@@ -28,7 +29,9 @@ class stMap:
 
 
 class solverOptions:
-    """type: 'hps' (GPU Chebyshev HPS, 2-D); ord: points per leaf side; a: leaf half-sizes."""
+    """type: 'hps' (GPU Chebyshev HPS; 'hpsalt', the name the reference's 3-D drivers use,
+    selects the same discretization, with ord = p_disc points per leaf side); ord: points
+    per leaf side, one per dimension; a: leaf half-sizes."""
 
     def __init__(self, type: str, ord, a=None):
         self.type = type
@@ -36,18 +39,21 @@ class solverOptions:
         self.a = a
 
 
-def hps_grid_params(opts, xl, xr, yl=0.0, yr=1.0):
-    """(q, hx, hy) of an 'hps' solverOptions."""
-    if opts.type != "hps":
+def hps_grid_params(opts, *_, ndim=2):
+    """(q, hx, hy) of an 'hps' / 'hpsalt' solverOptions, or (q, hx, hy, hz) for ndim 3."""
+    if opts.type not in ("hps", "hpsalt"):
         raise NotImplementedError(f"discretization {opts.type!r} is not provided on the GPU "
-                                  "(available: 'hps')")
-    q = int(opts.ord[0])
-    if len(opts.ord) > 1 and int(opts.ord[1]) != q:
-        raise NotImplementedError("equal leaf orders in x and y are required")
+                                  "(available: 'hps', 'hpsalt')")
+    ordv = np.atleast_1d(np.asarray(opts.ord, dtype=np.int64))
+    q = int(ordv[0])
+    if np.any(ordv != q):
+        raise NotImplementedError("equal leaf orders along every axis are required")
     a = np.asarray(opts.a, dtype=np.float64).ravel()
     if a.size == 1:
-        a = np.array([a[0], a[0]])
-    return q, 2 * a[0], 2 * a[1]
+        a = np.full(ndim, a[0])
+    if a.size < ndim:
+        raise ValueError(f"solverOptions.a needs {ndim} leaf half-sizes")
+    return (q, *(2.0 * a[:ndim]))
 
 
 class solverWrapper:
@@ -59,13 +65,15 @@ class solverWrapper:
         self.constructed = False
 
     def construct(self, geom, PDE, verbose=False, compute_inverse=True):
-        from ..hps import HPSSlabOperator, SlabGrid2D
         geom = np.asarray(geom, dtype=np.float64)
         self.ndim = geom.shape[1]
+        if self.ndim == 3:
+            return self._construct3(geom, PDE)
         if self.ndim != 2:
-            raise NotImplementedError("the GPU HPS discretization is 2-D")
+            raise NotImplementedError("the GPU HPS discretization is 2-D or 3-D")
+        from ..hps import HPSSlabOperator, SlabGrid2D
         (xl, yl), (xr, yr) = geom
-        q, hx, hy = hps_grid_params(self.opts, xl, xr, yl, yr)
+        q, hx, hy = hps_grid_params(self.opts)
         H = (xr - xl) / 2.0
         nx = int(round(H / hx))
         ny = int(round((yr - yl) / hy))
@@ -78,7 +86,29 @@ class solverWrapper:
         left, right = pts_b[2 * nxp:2 * nxp + g.m], pts_b[2 * nxp + g.m:]
         centre = g.interface_points()
         self.XX = np.concatenate([left, right, bottom, top, centre])
-        nb = self.XX.shape[0] - centre.shape[0]
+        return self._finish(centre.shape[0])
+
+    def _construct3(self, geom, PDE):
+        """One 3-D double slab: two single slabs of HPS leaves (sslablu_amd.hps3d); the
+        boundary is ordered [left face | right face | side faces of the first single slab
+        | side faces of the second], the interior skeleton is the centre plane."""
+        from ..hps3d import HPSSlabOperator3D, SlabGrid3D
+        lo, hi = geom
+        q, hx, hy, hz = hps_grid_params(self.opts, ndim=3)
+        H = (hi[0] - lo[0]) / 2.0
+        nl = [int(round(H / hx)), int(round((hi[1] - lo[1]) / hy)),
+              int(round((hi[2] - lo[2]) / hz))]
+        grid = SlabGrid3D(2, *nl, q, origin=lo, size=hi - lo)
+        self.op = HPSSlabOperator3D(PDE, grid).build()
+        g = grid
+        pts = g.gb_points()  # [sides of slab 0 | sides of slab 1 | left | right]
+        ns, m = g.n_side, g.m
+        centre = g.interface_points()
+        self.XX = np.concatenate([pts[2 * ns:2 * ns + m], pts[2 * ns + m:], pts[:2 * ns], centre])
+        return self._finish(centre.shape[0])
+
+    def _finish(self, ncentre):
+        nb = self.XX.shape[0] - ncentre
         self.Ib = np.arange(nb)
         self.Ii = np.arange(nb, self.XX.shape[0])
         self.XXb = self.XX[self.Ib]
@@ -88,17 +118,21 @@ class solverWrapper:
         return self
 
     def smap(self, which):
-        """'l', 'r' or 'gb': the S map from the left edge, the right edge, or the
-        top/bottom edges (the columns of A_ii^{-1} A_ib restricted to the centre line)."""
+        """'l', 'r' or 'gb': the S map from the left face, the right face, or the rest of
+        the boundary (the columns of A_ii^{-1} A_ib restricted to the centre skeleton).
+        2-D: dense device tensors; 3-D: device operators (matmat / rmatmat)."""
+        if which not in ("l", "r", "gb"):
+            raise ValueError(which)
+        if self.ndim == 3:
+            from ..hps3d import PairSMap, SideMap
+            return SideMap(self.op) if which == "gb" else PairSMap(self.op, 0, which)
         Y = self.op.Y[0]
         m = self.op.g.m
         if which == "l":
             return Y[:, :m]
         if which == "r":
             return Y[:, m:2 * m]
-        if which == "gb":
-            return Y[:, 2 * m:]
-        raise ValueError(which)
+        return Y[:, 2 * m:]
 
     @property
     def solver_ii(self):
