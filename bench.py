@@ -40,7 +40,7 @@ CALIB_MIB = 256
 # the step's 4 GEMM dispatches, each counter scaled by its reading for the calibration
 # copy (summary and commands: profiles/rb_fold_solve_c2_helmholtz_hbm_traffic.md).  A
 # measured constant of the current kernels: re-measure after changing them.
-TRAFFIC = 56812966
+TRAFFIC = 56811078
 # Tile policies of the timed step (backward sweep, boundary-map launch, one per backward
 # level 2, 1, 0), as the autotuner picks them on this workload (bench.py --autotune prints
 # them in timed_policy_flags); pinned so that a profiler run of the same flags traces the
@@ -416,6 +416,102 @@ def interior_bench(dev, reps=10):
             "plane_wave_max_err": err}
 
 
+def c3_hps_bench(dev, seed=0, rk=40, nrhs=16, leaf=72):
+    """3-D Poisson on [0,1]^3 on HPS leaves: 16 slabs (15 interfaces), every single slab
+    1 x 16 x 16 leaves of q = 8 Chebyshev points per side (p_disc = 8: 6 x 6 face nodes per
+    leaf face), so each interface carries m = 96^2 = 9216 nodes.  Factor: leaf DtN maps,
+    merge trees and the interfaces' pair LU factors on the GPU; S blocks HBS-compressed
+    (128-leaf face tree, 72 points per leaf, rank <= 40, s = max(2k, leaf) + k + 10 seeded
+    Gaussian sketch columns with FP32 sketch GEMMs and FP64 LU solves / QR / bases);
+    ULV-factored HBS Thomas elimination (each S'_i sketched the same way); 16 right-hand
+    sides of Dirichlet data sum_j a_j sin(pi k_j . x), k_j in {1,2,3}^3 (8 modes, seeded);
+    then GMRES on the exact S-system (applied through the pair LU factors) with the HBS
+    solve as preconditioner to a relative residual of 1e-10 (test/3D/cubeTest.py:163-186),
+    checked against a 1e-13 GMRES solve of the same system."""
+    import torch
+    from sslablu_amd.direct_solve.omsdirectsolveHBS import ThomasSolverHBS
+    from sslablu_amd.hps3d import (HPSSlabOperator3D, SlabGrid3D, compress_s_blocks,
+                                   hbs_S_rk_list)
+    from sslablu_amd.krylov import gmres
+    from sslablu_amd.spectral3d import FaceTree
+    N, ny, q = 16, 16, 8
+    rng = np.random.default_rng(seed)
+    modes = rng.integers(1, 4, size=(nrhs, 8, 3))
+    amps = rng.standard_normal((nrhs, 8))
+
+    class Poisson:
+        c11 = c22 = c33 = staticmethod(lambda P: np.ones(P.shape[0]))
+        c1 = c2 = c3 = c = None
+
+    def gdata(P):
+        return np.stack([(amps[c][None, :] * np.sin(np.pi * P @ modes[c].T)).sum(1)
+                         for c in range(nrhs)], axis=1)
+
+    g = SlabGrid3D(N, 1, ny, ny, q)
+    m = g.m
+    out = {"workload": f"3-D Poisson, {N} slabs, HPS leaves q={q} (6x6 face nodes), "
+                       f"{ny}x{ny} leaves per interface, m = {m}, HBS rank {rk}, leaf {leaf}, "
+                       f"{nrhs} RHS"}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op = HPSSlabOperator3D(Poisson(), g, device=dev, slab_chunk=4).build()
+    torch.cuda.synchronize()
+    out["hps_factor_s"] = round(time.perf_counter() - t0, 3)
+    out["hps_factor_gb"] = round(op.nbytes / 1e9, 2)
+    out["peak_hbm_gb"] = round(torch.cuda.max_memory_allocated() / 1e9, 1)
+    F = torch.tensor(gdata(g.gb_points()), device=dev)
+    B = op.rhs_from_boundary(F)
+    side = ny * (q - 2)
+    tree = FaceTree(side, side, leaf)
+    ns = max(2 * rk, leaf) + rk + 10
+    compress_s_blocks(op, tree, rk, ns, seed, sketch_fp32=True)   # warm-up (code loads)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    H_l, H_r = compress_s_blocks(op, tree, rk, ns, seed, sketch_fp32=True)
+    torch.cuda.synchronize()
+    out["compress_s (30 blocks, fp32 sketch GEMMs)"] = round(time.perf_counter() - t0, 3)
+    X = torch.randn(m, 16, dtype=torch.float64, device=dev)
+    i = g.nb // 2
+    ex = op.smap(i, "l").matmat(X)
+    out["compression_rel_err"] = ((H_l[i].matmat(X) - ex).norm() / ex.norm()).item()
+    out["hbs_bytes_per_block"] = int(H_l[i].nbytes)
+    out["dense_bytes_per_block"] = int(m * m * 8)
+    t0 = time.perf_counter()
+    solver = ThomasSolverHBS(m, rk, samples=ns)
+    solver.factorize(hbs_S_rk_list(H_l, H_r))
+    torch.cuda.synchronize()
+    out["hbs_factor_s"] = round(time.perf_counter() - t0, 3)
+    solver.solve(B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x_hbs = solver.solve(B)
+    torch.cuda.synchronize()
+    t_hbs = time.perf_counter() - t0
+    out["hbs_solve_ms"] = round(t_hbs * 1e3, 2)
+    out["hbs_rhs_solves_per_s"] = round(nrhs / t_hbs, 1)
+    A = op.tridiag_apply()
+    M = lambda v: torch.as_tensor(solver.solve(v), device=dev).contiguous()  # noqa: E731
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Xg, info = gmres(A, B, apply_M=M, tol=1e-10, restart=60, maxiter=120)
+    torch.cuda.synchronize()
+    t_g = time.perf_counter() - t0
+    Xr, info_r = gmres(A, B, apply_M=M, tol=1e-13, restart=80, maxiter=200)
+    res = ((A(Xg) - B).norm(dim=0) / B.norm(dim=0)).max().item()
+    out["gmres_hbs_preconditioned"] = {
+        "iterations": info.iterations, "converged": bool(info.converged),
+        "time_s": round(t_g, 3), "rhs_solves_per_s": round(nrhs / t_g, 1),
+        "max_rel_residual": res,
+        "rel_err_vs_fp64_reference": ((Xg - Xr).norm() / Xr.norm()).item(),
+        "fp64_reference": f"GMRES to 1e-13 ({info_r.iterations} iterations) on the same exact "
+                          "operator",
+        "hbs_only_rel_err_vs_reference": ((x_hbs - Xr).norm() / Xr.norm()).item(),
+        "residual_history": [float(f"{h:.2e}") for h in info.history]}
+    del op, H_l, H_r, solver
+    torch.cuda.empty_cache()
+    return out
+
+
 def c3_bench(dev, seed=0, rk=40, nrhs=16):
     """3-D Poisson on [0,1]^3: 16 slabs, every double slab one 9 x 98 x 98 Chebyshev grid,
     so each interface carries m = 96^2 = 9216 nodes; S blocks HBS-compressed on a 128-leaf
@@ -568,6 +664,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-spectral", action="store_true",
+                    help="also run the C3 sizes on one Chebyshev tensor grid per double slab "
+                         "(fast diagonalization) with a dense FP64 red-black reference solve")
     ap.add_argument("--no-interior", action="store_true")
     ap.add_argument("--fold-policy", default=None,
                     help="with --policy: the boundary-map launch's own tile policy (7 ints, "
@@ -906,6 +1005,8 @@ def main():
     if not args.no_c4:
         out["c4_blocktri"] = c4_bench(dev)
     if not args.no_c3:
+        out["c3_hps3d_hbs"] = c3_hps_bench(dev)
+    if args.c3_spectral:
         out["c3_spectral3d_hbs"] = c3_bench(dev)
     if not args.no_interior:
         out["interior_reconstruction"] = interior_bench(dev)

@@ -420,15 +420,37 @@ class HPSSlabOperator3D:
                  r, m * r, ptr(X), r, m * r, ptr(perm), nb, s)
         return X
 
-    def smaps_apply(self, side, X, transpose=False, stream=None):
+    def smaps_apply(self, side, X, transpose=False, stream=None, fp32=False):
         """S_l(i) X_i (side 'l') or S_r(i) X_i ('r') for every interface at once; X
-        (nb, m, r).  transpose: S^T X = B^T Tcc^{-T} X."""
+        (nb, m, r).  transpose: S^T X = B^T Tcc^{-T} X.  fp32: the coupling-block product
+        on the FP32 MFMA GEMM (B and its operand rounded to float, FP32 accumulation);
+        the LU solves stay FP64."""
         from .hbs import bgemm
         B = self.Bl if side == "l" else self.Br
         X = X.to(self.device, F64).contiguous()
+        prod = self._bgemm32 if fp32 else (lambda A, Y, ta: bgemm(A, Y, ta=ta))
         if transpose:
-            return bgemm(B, self._solve(X, True, stream), ta=True)
-        return self._solve(bgemm(B, X), False, stream)
+            return prod(B, self._solve(X, True, stream), True)
+        return self._solve(prod(B, X, False), False, stream)
+
+    def _bgemm32(self, B, X, ta):
+        """op(B_i) X_i in FP32 for (nb, m, m) B and (nb, m, r) X; returns float64."""
+        nb, m, _ = B.shape
+        r = X.shape[2]
+        s = stream_handle()
+        f32 = torch.float32
+
+        def cast(t):
+            y = torch.empty(t.shape, dtype=f32, device=self.device)
+            call("sslab_cast_f64_to_f32", t.numel(), ptr(t), ptr(y), s)
+            return y
+
+        B32, X32 = cast(B.contiguous()), cast(X)
+        Y32 = torch.empty(nb, m, r, dtype=f32, device=self.device)
+        call("sslab_sgemm_strided_batched", int(ta), 0, m, r, m, 1.0, ptr(B32), m, m * m,
+             ptr(X32), r, m * r, 0.0, ptr(Y32), r, m * r, nb, s)
+        del B32, X32
+        return Y32.to(F64)
 
     def smap(self, i, side):
         """The S map of interface i, side 'l' (from the left neighbour's interface) or 'r',
@@ -604,3 +626,42 @@ class _SubFactor:
             call("sslab_dgetrs_batched", m, r, ptr(lu), m, m * m, ptr(piv), m, ptr(B), r, m * r,
                  ptr(X), r, m * r, ptr(perm), 1, stream_handle())
         return X
+
+
+def compress_s_blocks(op: HPSSlabOperator3D, tree, rk: int, samples: int, seed: int = 0,
+                      sketch_fp32: bool = False):
+    """HBS compressions of every S block of `op` (matAssembly/matAssembler.py:103-133,
+    the rkHBS assembler of multislab/oms.py:160-205) with the sketches of all
+    interfaces formed together: Y = S Omega and Z = S^T Psi for the nb left blocks in
+    one batched product and one batched (transposed) LU solve, likewise the right
+    blocks; every block shares the seeded test matrices of its side.  Returns
+    (H_l, H_r): lists of nb HBStorch.HBSMAT (H_l[0] and H_r[nb-1] are the end blocks,
+    which couple to the end faces' boundary data, not to an interface).  sketch_fp32:
+    the coupling-block products of the sketches on the FP32 MFMA GEMM (the LU solves,
+    QRs and interpolative bases stay FP64)."""
+    from .matAssembly.HBS import HBStorch
+    g = op.g
+    nb, m = g.nb, g.m
+    dev = op.device
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    out = []
+    for side in ("l", "r"):
+        Om = torch.randn(m, samples, generator=gen, device=dev, dtype=F64)
+        Psi = torch.randn(m, samples, generator=gen, device=dev, dtype=F64)
+        Y = op.smaps_apply(side, Om.expand(nb, m, samples), fp32=sketch_fp32)
+        Z = op.smaps_apply(side, Psi.expand(nb, m, samples), transpose=True, fp32=sketch_fp32)
+        hs = []
+        for i in range(nb):
+            h = HBStorch.HBSMAT(op.smap(i, side), dev, tree, quad=False)
+            h.construct(rk, Om, Psi, Y[i], Z[i])
+            hs.append(h)
+        del Y, Z
+        out.append(hs)
+    return out[0], out[1]
+
+
+def hbs_S_rk_list(H_l, H_r):
+    """S_rk_list ([[S_r], [S_l, S_r], ..., [S_l]]) of compressed blocks."""
+    nb = len(H_l)
+    return [[H_r[0]]] + [[H_l[i], H_r[i]] for i in range(1, nb - 1)] + [[H_l[nb - 1]]]
