@@ -211,6 +211,17 @@ int sslab_rb_residual(int nb, int m, int nrhs, const double* T, const double* Si
 int sslab_rb_solve_mixed(int nb, int m, int nrhs, const float* factor32, const double* T,
                          const double* SiM, const double* SiP, const double* B, double* X,
                          double* work64, float* work32, int iters, void* stream);
+/* sslab_rb_solve_mixed that allocates nothing and takes no workspace: no counterpart in
+ * the reference.  It is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The FP64
+ * residual and the two FP32 panels live in a static device array of 64 MiB that every GPU
+ * the library runs on holds until the process ends, so 16 * nb*m*nrhs bytes (plus up to
+ * 768 bytes of alignment) must fit in 64 MiB; larger calls fail with SSLAB_BAD_ARGUMENT
+ * and a message.  All calls on one GPU share that array: calls that overlap on one GPU
+ * (different streams or threads) race, so serialise them. */
+int sslab_rb_solve_mixed_noalloc(int nb, int m, int nrhs, const float* factor32,
+                                 const double* T, const double* SiM, const double* SiP,
+                                 const double* B, double* X, int iters, void* stream);
 
 /* ---- Thomas block-tridiagonal solver ---------------------------------------
  * Same system; sequential block elimination with explicit block inverses.

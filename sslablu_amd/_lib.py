@@ -59,6 +59,8 @@ _SIGS = {
                           _vp, _c_int, _vp],
     "sslab_rb_solve_mixed": [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                              _c_int, _vp],
+    "sslab_rb_solve_mixed_noalloc": [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _c_int, _vp],
     "sslab_thomas_sizes": [_c_int, _c_int, _vp, _vp, _vp],
     "sslab_thomas_factor": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "sslab_thomas_solve": [_c_int, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _vp],

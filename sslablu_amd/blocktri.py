@@ -271,6 +271,22 @@ class BlockTriFactor:
                  stream_handle(stream))
         return X
 
+    def solve_mixed_noalloc_(self, B: torch.Tensor, X: torch.Tensor, iters: int = 2,
+                             stream=None):
+        """solve_mixed_ without a workspace (sslab_rb_solve_mixed_noalloc): the library's
+        static per-GPU scratch holds the FP64 residual and the FP32 panels, so the call
+        allocates nothing (usable under HIP graph capture); 16 * nb*m*r bytes must fit its
+        64 MiB.  Calls that overlap on one GPU race on that scratch."""
+        nb, m = self.nb, self.m
+        N, r = B.shape
+        if N != nb * m or tuple(X.shape) != (N, r) or not (B.is_contiguous() and X.is_contiguous()):
+            raise ValueError("solve_mixed_noalloc_: contiguous (nb*m, r) float64 B and X required")
+        with tuning.policy(tuning.lookup(self.tune_key(r, "mixed"))):
+            call("sslab_rb_solve_mixed_noalloc", nb, m, r, ptr(self.factor32),
+                 ptr(self._T64) if self._T64 is not None else None, ptr(self._M64),
+                 ptr(self._P64), ptr(B), ptr(X), int(iters), stream_handle(stream))
+        return X
+
     # -- autotuning ---------------------------------------------------------------------
     def tune_key(self, r: int, precision: str = "fp64") -> str:
         return f"blocktri:{self.method}:{self.nb}:{self.m}:{r}:{precision}"

@@ -31,6 +31,7 @@
 // sequential variant: A_i = SiM_i B_{i-1}^{-1}, B_i = T_i - A_i SiP_{i-1}.
 #include "blocktri_f64.hpp"
 
+#include <mutex>
 #include <vector>
 
 #include "gemm.hpp"
@@ -713,6 +714,66 @@ int rb_solve_mixed(int nb, int m, int r, const float* factor32, const double* T,
     SSLAB_TRY(acc_f32_into_f64(n, X32, X, st));
   }
   return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// rb_solve_mixed without a caller workspace: the FP64 residual and the two FP32 panels
+// live in a file-scope device array (one per GPU, held until the process ends).  Host
+// code never passes the array's name to a kernel: its device address is looked up once
+// per device with hipGetSymbolAddress and cached here.
+
+namespace {
+
+constexpr long long kMixedScratchBytes = 64LL << 20;
+__device__ __align__(256) unsigned char scratch[kMixedScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+long long round256(long long b) { return (b + 255) / 256 * 256; }
+
+// Device address of this GPU's scratch array (looked up on the first call per device).
+int mixed_scratch(void** out) {
+  int dev = 0;
+  SSLAB_HIP_CHECK(hipGetDevice(&dev));
+  SSLAB_REQUIRE(dev >= 0 && dev < kMaxDevices, "rb_solve_mixed_noalloc: device %d out of range",
+                dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    SSLAB_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SSLAB_REQUIRE(p != nullptr, "rb_solve_mixed_noalloc: no device address for the scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
+}  // namespace
+
+int rb_solve_mixed_noalloc(int nb, int m, int r, const float* factor32, const double* T,
+                           const double* SiM, const double* SiP, const double* B, double* X,
+                           int iters, hipStream_t st) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1 && r >= 0 && iters >= 0,
+                "rb_solve_mixed_noalloc: bad sizes");
+  const long long n = (long long)nb * m * r;
+  const long long need = round256(8 * n) + 2 * round256(4 * n);
+  SSLAB_REQUIRE(need <= kMixedScratchBytes,
+                "rb_solve_mixed_noalloc: nb*m*nrhs = %lld needs %lld bytes of scratch, more "
+                "than the %lld this library holds per GPU (call sslab_rb_solve_mixed with a "
+                "workspace instead)",
+                n, need, kMixedScratchBytes);
+  if (r == 0) return kOk;
+  void* base = nullptr;
+  SSLAB_TRY(mixed_scratch(&base));
+  unsigned char* p = static_cast<unsigned char*>(base);
+  double* R64 = reinterpret_cast<double*>(p);
+  float* V32 = reinterpret_cast<float*>(p + round256(8 * n));
+  float* X32 = reinterpret_cast<float*>(p + round256(8 * n) + round256(4 * n));
+  // every panel is written before it is read (cast, solve, residual), so nothing of an
+  // earlier call's contents is used
+  return rb_solve_mixed(nb, m, r, factor32, T, SiM, SiP, B, X, R64, V32, X32, iters, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -39,6 +39,12 @@ int rb_solve_mixed(int nb, int m, int r, const float* factor32, const double* T,
                    const double* SiM, const double* SiP, const double* B, double* X, double* R64,
                    float* V32, float* X32, int iters, hipStream_t st);
 
+// rb_solve_mixed with its workspace in a static per-GPU device array (64 MiB: 16 bytes
+// per element of the nb*m x r panel, each of its three parts rounded up to 256 bytes).
+int rb_solve_mixed_noalloc(int nb, int m, int r, const float* factor32, const double* T,
+                           const double* SiM, const double* SiP, const double* B, double* X,
+                           int iters, hipStream_t st);
+
 int thomas_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
                  long long* work_ints);
 int thomas_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,

@@ -230,6 +230,13 @@ int sslab_rb_solve_mixed(int nb, int m, int nrhs, const float* factor32, const d
                                work32 ? work32 + n : nullptr, iters, as_stream(stream));
 }
 
+int sslab_rb_solve_mixed_noalloc(int nb, int m, int nrhs, const float* factor32, const double* T,
+                                 const double* SiM, const double* SiP, const double* B,
+                                 double* X, int iters, void* stream) {
+  return sslab::rb_solve_mixed_noalloc(nb, m, nrhs, factor32, T, SiM, SiP, B, X, iters,
+                                       as_stream(stream));
+}
+
 int sslab_thomas_sizes(int nb, int m, long long* factor_doubles, long long* work_doubles,
                        long long* work_ints) {
   return sslab::thomas_sizes(nb, m, factor_doubles, work_doubles, work_ints);
