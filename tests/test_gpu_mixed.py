@@ -97,3 +97,34 @@ def test_fp32_factorization_refined(T, identity, nb, m, r):
     assert errs[2] < 1e-10
     with pytest.raises(ValueError):
         f32.solve(B)
+
+
+def test_c2_size_mixed_vs_oracle(T):
+    """The C2 workload at full size (variable-coefficient Helmholtz kappa = 20, 32 slabs,
+    p = 16, 32 leaves per interface: m = 512, 31 interfaces, 64 RHS of N(0,1) boundary
+    data): the FP32 factorization refined with one FP64 residual step, and the FP64
+    folded solve, against the oracle's FP64 Thomas solve (scipy lu_factor / lu_solve,
+    oracle/blocktri.py) of the same S-system: mixed <= 1e-8, FP64 <= 1e-10."""
+    from sslablu_amd.blocktri import BlockTriFactor
+    from sslablu_amd.hps import HPSSlabOperator, SlabGrid2D
+    from sslablu_amd.serving import ProblemSpec, make_pdo
+    spec = ProblemSpec("helmholtz", 32, 16, 32)
+    g = SlabGrid2D(spec.slabs, 1, spec.leaves, spec.p + 2)
+    op = HPSSlabOperator(make_pdo(spec), g).build()
+    nb, m = g.nb, g.m
+    assert (nb, m) == (31, 512)
+    rng = np.random.default_rng(0)
+    F = T.tensor(rng.standard_normal((g.gb_points().shape[0], 64)), device="cuda")
+    B = op.rhs_from_boundary(F)
+    f32 = BlockTriFactor("rb").factorize_mixed(None, op.SiM, op.SiP)
+    Xm = T.empty_like(B)
+    f32.solve_mixed_(B.contiguous(), Xm, iters=1)
+    f64 = BlockTriFactor("rb").factorize(None, op.SiM, op.SiP)
+    X64 = op.fold(f64).solve(F)
+    SiM, SiP = op.SiM.cpu().numpy(), op.SiP.cpu().numpy()
+    S = [(np.zeros((m, m)) if i == 0 else SiM[i], np.zeros((m, m)) if i == nb - 1 else SiP[i])
+         for i in range(nb)]
+    x_ref = ob.thomas_direct(S, [np.eye(m)] * nb, B.cpu().numpy())
+    nrm = np.linalg.norm(x_ref)
+    assert np.linalg.norm(Xm.cpu().numpy() - x_ref) / nrm < 1e-8
+    assert np.linalg.norm(X64.cpu().numpy() - x_ref) / nrm < 1e-10
