@@ -9,8 +9,9 @@ compressor samples run as chains of MFMA GEMMs and ULV solves in HBM.
 
 Differences, on purpose: right-hand sides may be (N,) or (N, r) for both solvers; the
 factorize_with_diag path of ThomasSolverHBS takes (D_list) as documented (the reference
-passes an extra argument to Dprime_Linop, :267); RedBlackSolverHBS needs a power-of-two
-block count like the reference (:401-402).
+passes an extra argument to Dprime_Linop, :267); RedBlackSolverHBS takes any block count
+(the reference needs a power of two, :401-402) and compresses each level's blocks in
+stacked launches.
 """
 from __future__ import annotations
 
@@ -94,6 +95,8 @@ def _mm(op, X, transpose=False):
     """Apply any supported operator to a device tensor X."""
     if isinstance(op, DeviceLinOp):
         return op._mm(X, transpose)
+    if getattr(op, "device_op", False) and not isinstance(op, DeviceMatrix):
+        return op.rmatmat(X) if transpose else op.matmat(X)
     if isinstance(op, DeviceMatrix):
         return op.rmatmat(X) if transpose else op.matmat(X)
     if isinstance(op, HBSnew.HBSMAT):
@@ -348,58 +351,103 @@ class ThomasSolverHBS(DirectSolver):
         return self.solve_helper(rhs, glob_target_dofs)
 
 
+class _ZeroOp(DeviceLinOp):
+    """The zero block (the missing neighbour couplings of a non-periodic chain)."""
+
+    def _mm(self, X, transpose):
+        rows = self.shape[1] if transpose else self.shape[0]
+        return torch.zeros(rows, X.shape[1], dtype=F64, device=X.device)
+
+
+def _is_zero(op):
+    return isinstance(op, _ZeroOp)
+
+
 class RedBlackSolverHBS(DirectSolver):
     """Red-black reduction where every new diagonal, left and right block is an HBS
-    compression of the lazily composed operator (omsdirectsolveHBS.py:359-558)."""
+    compression of the lazily composed operator (omsdirectsolveHBS.py:359-558).
 
-    def __init__(self, m, rk, tree, quad, cyclic=False):
+    Any block count: each level eliminates the odd-indexed nodes and keeps the
+    ceil(n/2) even ones (the reference needs a power of two, :401-402; periodic chains
+    still do).  The compressions of one level are stacked: all new diagonal blocks in
+    one compression + ULV factorization, all nonzero new off-diagonal blocks in another
+    (HBSnew.construct_many), instead of one compression per node."""
+
+    def __init__(self, m, rk, tree, quad, cyclic=False, samples=None):
         super().__init__(m, cyclic)
         self.rk = rk
         self.tree = tree
         self.quad = quad
+        self.samples = samples
 
     def _hbs(self, linop, rk=None):
-        h = HBSnew.HBSMAT(linop, self.tree, self.quad)
+        h = HBSnew.HBSMAT(linop, self.tree, self.quad, samples=self.samples)
         h.construct(self.rk if rk is None else rk, compute_ULV=True)
         return h
+
+    def _hbs_many(self, ops, ulv):
+        mats = [HBSnew.HBSMAT(op, self.tree, self.quad, samples=self.samples) for op in ops]
+        return HBSnew.construct_many(mats, self.rk, compute_ULV=ulv)
 
     def factorize(self, S_rk_list, T=None):
         m = S_rk_list[0][0].shape[0]
         nSlabs = len(S_rk_list)
-        if not ((nSlabs & (nSlabs - 1) == 0) and nSlabs != 0):
-            raise ValueError("Number of slabs must be a power of 2.")
+        if nSlabs == 0:
+            raise ValueError("empty S_rk_list")
+        if self.cyclic and nSlabs & (nSlabs - 1):
+            raise ValueError("Number of slabs must be a power of 2 for a periodic chain.")
         SiM = [s[0] for s in S_rk_list]
         SiP = [s[-1] for s in S_rk_list]
         if not self.cyclic:
-            SiM[0] = _linop_from_mat(torch.zeros(m, m, dtype=F64, device=_dev()))
-            SiP[-1] = _linop_from_mat(torch.zeros(m, m, dtype=F64, device=_dev()))
+            SiM[0] = _ZeroOp((m, m))
+            SiP[-1] = _ZeroOp((m, m))
         if T is None:
             T = [id_op(m) for _ in range(nSlabs)]
         RB = [(SiM, T, T, SiP)]
         n = nSlabs
         while n > 1:
-            RB.append(self._build_level(m, n, RB[-1], self.rk))
-            n //= 2
+            RB.append(self._build_level(m, n, RB[-1]))
+            n = len(RB[-1][0])
+        if nSlabs == 1:  # a single block: its own compression is the whole factorization
+            RB.append(([_ZeroOp((m, m))], T, self._hbs_many([T[0]], True),
+                       [_ZeroOp((m, m))]))
         self.nSlabs = nSlabs
         self.RB = RB
 
-    def _build_level(self, m, nSlabs, RB_level, rk):
+    def _build_level(self, m, n, RB_level):
+        """Eliminate the odd nodes of an n-node level: for every even i,
+            B_i = T_i - SiM_i T_{i-1}^{-1} SiP_{i-1} - SiP_i T_{i+1}^{-1} SiM_{i+1},
+            A_i = -SiM_i T_{i-1}^{-1} SiM_{i-1},   C_i = -SiP_i T_{i+1}^{-1} SiP_{i+1},
+        (absent neighbours drop their terms; A_0 and the last C are zero unless periodic)."""
         SiM, T, T_hbs, SiP = RB_level
         cyc = self.cyclic
-        B_i, T_new = [], []
-        for i in range(0, nSlabs, 2):
-            spm = SiP[(i - 1) % nSlabs] if (i > 0 or cyc) else None
-            smp = SiM[(i + 1) % nSlabs] if (i < nSlabs - 1 or cyc) else None
-            tm = T_hbs[(i - 1) % nSlabs] if spm is not None else None
-            tp = T_hbs[(i + 1) % nSlabs] if smp is not None else None
-            Bl = RB_linop(T[i], tm, tp, SiP[i], SiM[i], smp, spm)
-            B_i.append(Bl)
-            T_new.append(self._hbs(Bl, rk))
-        A_i = [self._hbs(STS_linop(SiM[i], T_hbs[(i - 1) % nSlabs], SiM[(i - 1) % nSlabs]), rk)
-               for i in range(0, nSlabs, 2)]
-        C_i = [self._hbs(STS_linop(SiP[i], T_hbs[(i + 1) % nSlabs], SiP[(i + 1) % nSlabs]), rk)
-               for i in range(0, nSlabs, 2)]
-        return (A_i, B_i, T_new, C_i)
+        evens = list(range(0, n, 2))
+        B_ops = []
+        for i in evens:
+            spm = SiP[(i - 1) % n] if (i > 0 or cyc) else None
+            smp = SiM[(i + 1) % n] if (i < n - 1 or cyc) else None
+            if spm is not None and (_is_zero(spm) or _is_zero(SiM[i])):
+                spm = None  # a term with a zero factor drops out
+            if smp is not None and (_is_zero(smp) or _is_zero(SiP[i])):
+                smp = None
+            tm = T_hbs[(i - 1) % n] if spm is not None else None
+            tp = T_hbs[(i + 1) % n] if smp is not None else None
+            B_ops.append(RB_linop(T[i], tm, tp, SiP[i], SiM[i], smp, spm))
+        T_new = self._hbs_many(B_ops, True)
+        A_ops, C_ops = [], []
+        for i in evens:
+            a = c = None
+            if (i > 0 or cyc) and not (_is_zero(SiM[i]) or _is_zero(SiM[(i - 1) % n])):
+                a = STS_linop(SiM[i], T_hbs[(i - 1) % n], SiM[(i - 1) % n])
+            if (i + 1 < n or cyc) and not (_is_zero(SiP[i]) or _is_zero(SiP[(i + 1) % n])):
+                c = STS_linop(SiP[i], T_hbs[(i + 1) % n], SiP[(i + 1) % n])
+            A_ops.append(a)
+            C_ops.append(c)
+        live = [op for op in A_ops + C_ops if op is not None]
+        comp = iter(self._hbs_many(live, False))
+        A_i = [next(comp) if op is not None else _ZeroOp((m, m)) for op in A_ops]
+        C_i = [next(comp) if op is not None else _ZeroOp((m, m)) for op in C_ops]
+        return (A_i, B_ops, T_new, C_i)
 
     def solve(self, rhs):
         m = self.m
@@ -409,33 +457,37 @@ class RedBlackSolverHBS(DirectSolver):
         for lv in range(len(RB) - 1):
             SiM, _, T_hbs, SiP = RB[lv]
             n = len(SiM)
-            nr = n // 2
+            nr = len(RB[lv + 1][0])
+            if self.nSlabs == 1:
+                break
             prev_v = vs[-1]
             new_v = torch.empty(m * nr, prev_v.shape[1], dtype=F64, device=prev_v.device)
             for j in range(nr):
                 i = 2 * j
                 c = prev_v[i * m:(i + 1) * m].clone()
-                if self.cyclic or i > 0:
+                if (self.cyclic or i > 0) and not _is_zero(SiM[i]):
                     p = (i - 1) % n
                     c -= _mm(SiM[i], _sv(T_hbs[p], prev_v[p * m:(p + 1) * m]))
-                if self.cyclic or i < n - 1:
+                if (self.cyclic or i < n - 1) and not _is_zero(SiP[i]):
                     q = (i + 1) % n
                     c -= _mm(SiP[i], _sv(T_hbs[q], prev_v[q * m:(q + 1) * m]))
                 new_v[j * m:(j + 1) * m] = c
             vs.append(new_v)
         vs[-1] = _sv(RB[-1][2][0], vs[-1])
+        if self.nSlabs == 1:
+            return _back(vs[-1], was_np, one_d)
         for lv in range(len(RB) - 1, 0, -1):
             SiM, _, T_hbs, SiP = RB[lv - 1]
-            nr = len(SiM) // 2
+            n = len(SiM)
+            nr = len(RB[lv][0])
             cur, coarse = vs[lv - 1], vs[lv]
             for j in range(nr):
-                i = 2 * j
-                cur[i * m:(i + 1) * m] = coarse[j * m:(j + 1) * m]
-            for j in range(nr):
-                i = 2 * j
-                nj = (j + 1) % nr
-                c = _mm(SiM[i + 1], coarse[j * m:(j + 1) * m])
-                if self.cyclic or j + 1 < nr:
-                    c = c + _mm(SiP[i + 1], coarse[nj * m:(nj + 1) * m])
-                cur[(i + 1) * m:(i + 2) * m] = _sv(T_hbs[i + 1], cur[(i + 1) * m:(i + 2) * m] - c)
+                cur[2 * j * m:(2 * j + 1) * m] = coarse[j * m:(j + 1) * m]
+            for i in range(1, n, 2):  # odd nodes: both neighbours are even nodes
+                j = i // 2
+                c = _mm(SiM[i], coarse[j * m:(j + 1) * m])
+                if (self.cyclic or i + 1 < n) and not _is_zero(SiP[i]):
+                    nj = ((i + 1) % n) // 2
+                    c = c + _mm(SiP[i], coarse[nj * m:(nj + 1) * m])
+                cur[i * m:(i + 1) * m] = _sv(T_hbs[i], cur[i * m:(i + 1) * m] - c)
         return _back(vs[0], was_np, one_d)

@@ -382,6 +382,25 @@ class DeviceHBS:
         return DeviceHBS(self.Vmats, [_t(D).contiguous() for D in self.Dmats], self.Umats,
                          self.fac, self.perm)
 
+    def split(self, K, perm=None):
+        """The K matrices of a stacked DeviceHBS (built from K operators' sketches stacked
+        along every level's batch, so matrix k owns boxes [k nb_l / K, (k+1) nb_l / K) of
+        each level, root k included), with their ULV factors if any were computed."""
+        def part(M, k):
+            n = M.shape[0] // K
+            return M[k * n:(k + 1) * n]
+
+        out = []
+        for k in range(K):
+            h = DeviceHBS([part(U, k) for U in self.Umats], [part(D, k) for D in self.Dmats],
+                          [part(V, k) for V in self.Vmats], self.fac, perm)
+            for mode, levels in self._ulv.items():
+                h._ulv[mode] = [{key: (part(v, k) if isinstance(v, torch.Tensor)
+                                       else (v // K if key == "nb" else v))
+                                 for key, v in lev.items()} for lev in levels]
+            out.append(h)
+        return out
+
     # -- ULV (ULVsparse.py:231-358) -----------------------------------------------------
     def ulv_factor(self, mode="N"):
         if mode in self._ulv:

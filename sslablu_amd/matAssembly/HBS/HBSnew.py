@@ -176,6 +176,17 @@ class HBSMAT:
         return 6 * rk + 4 * self.nl + 5 if self.fac == 4 else 4 * rk + 2 * self.nl + 5
 
     def constructHBS(self, rk):
+        Om, Psi, Y, Z, pt = self._sketch(rk)
+        clk = self._clocks
+        clk["compress"].set(0.0)
+        ev = clk["compress"].start()
+        self._h = DeviceHBS.from_sketches(Om, Psi, Y, Z, self.Nb, self.L, rk, self.fac, pt)
+        self.Nbvec = self._h.Nbvec
+        clk["compress"].stop(ev)
+
+    def _sketch(self, rk):
+        """Seeded test matrices and the sketches Y = A Om, Z = A^T Psi in leaf order
+        (HBSnew.py:147-150, 218-225); returns (Om, Psi, Y, Z, device permutation)."""
         N = self.shape[0]
         s = self._samples(rk)
         self.nSamples = s
@@ -208,11 +219,7 @@ class HBSMAT:
             Z = rows(_apply_op(self.A, Psp, True, dev))
         del Omp, Psp
         clk["sample"].stop(ev)
-        clk["compress"].set(0.0)
-        ev = clk["compress"].start()
-        self._h = DeviceHBS.from_sketches(Om, Psi, Y, Z, self.Nb, self.L, rk, self.fac, pt)
-        self.Nbvec = self._h.Nbvec
-        clk["compress"].stop(ev)
+        return Om, Psi, Y, Z, pt
 
     def constructHBS_ULV(self, rk):
         self.constructHBS(rk)
@@ -279,3 +286,35 @@ class HBSMAT:
         if self.mode == "T":
             mode = "T" if mode == "N" else "N"
         return self._wrap(b, lambda x: self._h.solve(x, mode))
+
+
+def construct_many(mats, rk, compute_ULV=False):
+    """Compress several HBSMATs over the same tree (same leaf order, sample count and
+    rank) together: each operator is sketched on its own, then the sketches are stacked
+    along every tree level's batch, so the compression's batched QRs and GEMMs (and the
+    ULV factorization, compute_ULV: both modes) run once for all of them -- a batched
+    launch costs about what one operator's costs at the upper tree levels, where each
+    operator has only a few boxes.  Equivalent to construct(rk, compute_ULV) on each."""
+    mats = list(mats)
+    if not mats:
+        return mats
+    h0 = mats[0]
+    sk = [h._sketch(rk) for h in mats]
+    K = len(mats)
+    for h in mats[1:]:
+        if (h.Nb, h.L, h.fac, h.shape) != (h0.Nb, h0.L, h0.fac, h0.shape):
+            raise ValueError("construct_many: every matrix needs the same tree and shape")
+    ev = h0._clocks["compress"].start()
+    big = DeviceHBS.from_sketches(*(torch.cat([s_[j] for s_ in sk]) for j in range(4)),
+                                  h0.Nb * K, h0.L, rk, h0.fac, None)
+    if compute_ULV:
+        big.ulv_factor_pair()
+    parts = big.split(K, perm=sk[0][4])
+    for h, part in zip(mats, parts):
+        h._h = part
+        h.Nbvec = part.Nbvec
+        h._clocks["compress"].set(0.0)
+        if compute_ULV:
+            h.compute_ULV()
+    h0._clocks["compress"].stop(ev)
+    return mats
