@@ -489,25 +489,42 @@ def c3_hps_bench(dev, seed=0, rk=40, nrhs=16, leaf=72):
     t_hbs = time.perf_counter() - t0
     out["hbs_solve_ms"] = round(t_hbs * 1e3, 2)
     out["hbs_rhs_solves_per_s"] = round(nrhs / t_hbs, 1)
-    A = op.tridiag_apply()
-    M = lambda v: torch.as_tensor(solver.solve(v), device=dev).contiguous()  # noqa: E731
+    # the configured elimination: HBS red-black (15 interfaces -> 8 -> 4 -> 2 -> 1), each
+    # level's compressions and ULV factors in stacked launches
+    from sslablu_amd.direct_solve.omsdirectsolveHBS import RedBlackSolverHBS
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    Xg, info = gmres(A, B, apply_M=M, tol=1e-10, restart=60, maxiter=120)
+    rb = RedBlackSolverHBS(m, rk, tree, False, samples=ns)
+    rb.factorize(hbs_S_rk_list(H_l, H_r))
     torch.cuda.synchronize()
-    t_g = time.perf_counter() - t0
-    Xr, info_r = gmres(A, B, apply_M=M, tol=1e-13, restart=80, maxiter=200)
-    res = ((A(Xg) - B).norm(dim=0) / B.norm(dim=0)).max().item()
-    out["gmres_hbs_preconditioned"] = {
-        "iterations": info.iterations, "converged": bool(info.converged),
-        "time_s": round(t_g, 3), "rhs_solves_per_s": round(nrhs / t_g, 1),
-        "max_rel_residual": res,
-        "rel_err_vs_fp64_reference": ((Xg - Xr).norm() / Xr.norm()).item(),
-        "fp64_reference": f"GMRES to 1e-13 ({info_r.iterations} iterations) on the same exact "
-                          "operator",
-        "hbs_only_rel_err_vs_reference": ((x_hbs - Xr).norm() / Xr.norm()).item(),
-        "residual_history": [float(f"{h:.2e}") for h in info.history]}
-    del op, H_l, H_r, solver
+    out["hbs_rb_factor_s"] = round(time.perf_counter() - t0, 3)
+    rb.solve(B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x_rb = rb.solve(B)
+    torch.cuda.synchronize()
+    out["hbs_rb_solve_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    A = op.tridiag_apply()
+    Xr, info_r = gmres(A, B, apply_M=lambda v: solver.solve(v).contiguous(), tol=1e-13,
+                       restart=80, maxiter=200)
+    for name, slv, x0 in (("thomas", solver, x_hbs), ("red_black", rb, x_rb)):
+        M = lambda v, s_=slv: s_.solve(v).contiguous()  # noqa: E731
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Xg, info = gmres(A, B, apply_M=M, tol=1e-10, restart=60, maxiter=120)
+        torch.cuda.synchronize()
+        t_g = time.perf_counter() - t0
+        res = ((A(Xg) - B).norm(dim=0) / B.norm(dim=0)).max().item()
+        out[f"gmres_hbs_{name}_preconditioned"] = {
+            "iterations": info.iterations, "converged": bool(info.converged),
+            "time_s": round(t_g, 3), "rhs_solves_per_s": round(nrhs / t_g, 1),
+            "max_rel_residual": res,
+            "rel_err_vs_fp64_reference": ((Xg - Xr).norm() / Xr.norm()).item(),
+            "hbs_only_rel_err_vs_reference": ((x0 - Xr).norm() / Xr.norm()).item(),
+            "residual_history": [float(f"{h:.2e}") for h in info.history]}
+    out["fp64_reference"] = (f"GMRES to 1e-13 ({info_r.iterations} iterations) on the same "
+                             "exact operator")
+    del op, H_l, H_r, solver, rb
     torch.cuda.empty_cache()
     return out
 

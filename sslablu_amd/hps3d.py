@@ -575,6 +575,32 @@ class PairSMap:
     def __matmul__(self, X):
         return self._run(X, False)
 
+    @property
+    def T(self):
+        """The transposed map (the reference's LinearOperator .T)."""
+        outer = self
+
+        class _Tr:
+            device_op = True
+            dtype = np.dtype(np.float64)
+            shape = (outer.shape[1], outer.shape[0])
+            tree = quad = None
+
+            def matmat(self, X):
+                return outer._run(X, True)
+
+            def rmatmat(self, X):
+                return outer._run(X, False)
+
+            matvec = matmat
+            rmatvec = rmatmat
+            __matmul__ = matmat
+
+            @property
+            def T(self):
+                return outer
+        return _Tr()
+
 
 class SideMap:
     """The boundary-data map of one 3-D double slab (interface 0 of its operator):

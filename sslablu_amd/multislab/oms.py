@@ -3,7 +3,8 @@
 Mirror of multislab/oms.py (SimonDirckx/SslabLU): slab (:17-52), omsStats (:53-59) and
 oms (:61-320) with compute_global_dofs, compute_stmaps, construct_Stot_helper,
 construct_Stot_and_rhstot_linearOperator, construct_Stot_and_rhstot and
-construct_rhstot.  Conventions kept: the S-system u_i + S_{i,i-1} u_{i-1} +
+construct_rhstot, and oms_lu (:321-525, the S maps left as factor-backed
+operators).  Conventions kept: the S-system u_i + S_{i,i-1} u_{i-1} +
 S_{i,i+1} u_{i+1} = rhs_i with S_J = (A_ii^{-1} A_ib[:, J])[Ic] and
 rhs = -(A_ii^{-1} A_ib[:, Igb] f)[Ic]; S_rk_list = [[S_r], [S_l, S_r], ..., [S_l]];
 contiguous global dof ranges per interface.
@@ -246,3 +247,38 @@ class oms:
         """Stacked S-system right-hand side for boundary data bc (numpy); the reference
         leaves this as a TODO (oms.py:316-320)."""
         return self.rhs_device(bc)[:, 0].cpu().numpy()
+
+
+class oms_lu(oms):
+    """Overlapping multislab S-system without interface compression (oms.py:321-525):
+    the S maps stay operators backed by the local factorizations -- in 3-D the pair LU
+    factors of HPSSlabOperator3D (PairSMap), in 2-D the dense maps the HPS pair solve
+    produces (DeviceMatrix).  construct_Stot_helper(bc, dbg) takes no assembler."""
+
+    def construct_Stot_helper(self, bc, dbg=0):
+        op = self.discretize()
+        g = op.g
+        nc = g.m
+        S_lu_list, glob_target_dofs = [], []
+        for k in range(len(self.slabList)):
+            st_l, st_r = self.compute_stmaps(k)
+            left, right = self.connectivity[k]
+            blocks = [st.A for st, present in ((st_l, left >= 0), (st_r, right >= 0)) if present]
+            S_lu_list.append([DeviceMatrix(b) if isinstance(b, torch.Tensor) else b
+                              for b in blocks])
+            glob_target_dofs.append(range(k * nc, (k + 1) * nc))
+            if dbg > 0:
+                print("overlapping slab ", k + 1, " of ", len(self.connectivity), " done")
+        rhs_list = []
+        if bc is not None:
+            rhs = self.rhs_device(bc)
+            rhs_list = [rhs[k * nc:(k + 1) * nc, 0].cpu().numpy() for k in range(len(self.slabList))]
+        self.glob_target_dofs = glob_target_dofs
+        self.glob_source_dofs = []
+        self.compute_global_dofs()
+        self.nc = nc
+        return S_lu_list, rhs_list, nc * len(self.slabList), nc
+
+    def construct_Stot_and_rhstot(self, bc, assembler=None, dbg=0):
+        S_lu_list, rhs_list, Ntot, nc = self.construct_Stot_helper(bc, dbg)
+        return self.construct_Stot_and_rhstot_linearOperator(S_lu_list, rhs_list, Ntot, nc, dbg)
