@@ -258,6 +258,24 @@ int sslab_hps_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx,
                        const double* c1, const double* c2, const double* c0, double* T,
                        double* X, double* work, int* iwork, int* info, void* stream);
 
+/* ---- Spectral double slabs, 2-D, variable coefficients -----------------------
+ * The reference's 'spectral' slab discretization (solver/spectral/spectralSolver.py:
+ * 161-173 constructPDO2D, 198-235 spectralSolver; solver/solver.py:145-157): every double
+ * slab one qx x qy tensor Chebyshev grid (qx = ordx + 1 odd, qy = ordy + 1).  For nslab
+ * slabs: Y_s = (A_ii^{-1} A_ib)[centre line] (m x nbdry, m = qy - 2, nbdry = 2m + 2(qx-2),
+ * columns [left | right | bottom | top], corners dropped), the S maps and boundary-data
+ * maps of multislab/oms.py:99-129, 180-186.  Dx..Dyy: qx x qx / qy x qy derivative
+ * matrices of the slab grid (shared); c11, c22, c1, c2, c0: nslab*(qx-2)*(qy-2) values at
+ * the interior nodes (row (a-1)(qy-2) + (b-1), a along x), any may be NULL (= 0); info:
+ * nslab ints. */
+int sslab_spectral2d_sizes(int nslab, int qx, int qy, long long* work_doubles,
+                           long long* work_ints);
+int sslab_spectral2d_slab_maps(int nslab, int qx, int qy, const double* Dx, const double* Dxx,
+                               const double* Dy, const double* Dyy, const double* c11,
+                               const double* c22, const double* c1, const double* c2,
+                               const double* c0, double* Y, double* work, int* iwork, int* info,
+                               void* stream);
+
 /* ---- HPS slab assembly, 3-D Chebyshev leaves (q^3 points, p = q - 2) ----------
  * The 3-D discretization of the reference's cube drivers (test/3D/cubeTest.py:114-126:
  * solverOptions('hpsalt', [p_disc]*3, a), hpsalt's Domain_Driver, solver/solver.py:

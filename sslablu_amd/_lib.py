@@ -66,6 +66,8 @@ _SIGS = {
     "sslab_thomas_solve": [_c_int, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _vp],
     "sslab_hps_leaf_sizes": [_c_int, _c_int, _vp, _vp],
     "sslab_hps_leaf_dtn": [_c_int, _c_int] + [_vp] * 16 + [_vp],
+    "sslab_spectral2d_sizes": [_c_int, _c_int, _c_int, _vp, _vp],
+    "sslab_spectral2d_slab_maps": [_c_int, _c_int, _c_int] + [_vp] * 13 + [_vp],
     "sslab_hps3_leaf_sizes": [_c_int, _c_int, _vp, _vp],
     "sslab_hps3_leaf_dtn": [_c_int, _c_int] + [_vp] * 19 + [_vp],
     "sslab_hps_merge_sizes": [_c_int, _c_int, _c_int, _vp, _vp],

@@ -5,6 +5,7 @@
 #include "blocktri_f64.hpp"
 #include "gemm.hpp"
 #include "hps_f64.hpp"
+#include "spectral_f64.hpp"
 #include "krylov_f64.hpp"
 #include "lu_f64.hpp"
 #include "oms_f64.hpp"
@@ -277,6 +278,20 @@ int sslab_hps3_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx,
   const double* coef[7] = {c11, c22, c33, c1, c2, c3, c0};
   return sslab::hps3_leaf_dtn(nleaves, q, D, Nb, Ni, coef, T, X, work, iwork, info,
                               as_stream(stream));
+}
+
+int sslab_spectral2d_sizes(int nslab, int qx, int qy, long long* work_doubles,
+                           long long* work_ints) {
+  return sslab::spectral2d_sizes(nslab, qx, qy, work_doubles, work_ints);
+}
+
+int sslab_spectral2d_slab_maps(int nslab, int qx, int qy, const double* Dx, const double* Dxx,
+                               const double* Dy, const double* Dyy, const double* c11,
+                               const double* c22, const double* c1, const double* c2,
+                               const double* c0, double* Y, double* work, int* iwork, int* info,
+                               void* stream) {
+  return sslab::spectral2d_slab_maps(nslab, qx, qy, Dx, Dxx, Dy, Dyy, c11, c22, c1, c2, c0, Y,
+                                     work, iwork, info, as_stream(stream));
 }
 
 int sslab_hps_merge_sizes(int nmerge, int n_new, int c, long long* work_doubles,

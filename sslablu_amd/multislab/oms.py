@@ -107,16 +107,7 @@ class oms:
         N = len(slabs) + 1
         H = (slabs[0][1][0] - slabs[0][0][0]) / 2.0
         x0 = slabs[0][0][0]
-        for k, s in enumerate(slabs):
-            if abs(s[0][0] - (x0 + k * H)) > 1e-12 or abs(s[1][0] - (x0 + (k + 2) * H)) > 1e-12 \
-                    or np.any(np.abs(s[:, 1:] - slabs[0][:, 1:]) > 1e-12):
-                raise NotImplementedError("slabs must be the consecutive double slabs of a "
-                                          "box (geometry.geom_2D.square / geom_3D.cube dSlabs)")
-        for k, c in enumerate(self.connectivity):
-            want = [k - 1 if k > 0 else -1, k + 1 if k < len(slabs) - 1 else -1]
-            if list(c) != want:
-                raise NotImplementedError("only the non-periodic chain connectivity of dSlabs "
-                                          "is assembled on the GPU")
+        self._check_chain(slabs)
         lo, hi = slabs[0][0], slabs[0][1]
         if ndim == 3:
             q, hx, hy, hz = hps_grid_params(self.opts, ndim=3)
@@ -131,14 +122,65 @@ class oms:
         ny = int(round((hi[1] - lo[1]) / hy))
         return SlabGrid2D(N, nx, ny, q, x0=x0, width=N * H, y0=lo[1], height=hi[1] - lo[1])
 
+    def _spectral(self):
+        """The 'spectral' discretization (solver/spectral/spectralSolver.py:198-235): one
+        Chebyshev tensor grid of ord + 1 points per axis on every double slab; 2-D with
+        variable coefficients (SpectralSlabs2D), 3-D with constant ones (SpectralSlabs3D,
+        fast diagonalization)."""
+        slabs = [np.asarray(s, dtype=np.float64) for s in self.slabList]
+        self._check_chain(slabs)
+        N = len(slabs) + 1
+        H = (slabs[0][1][0] - slabs[0][0][0]) / 2.0
+        lo, hi = slabs[0][0], slabs[0][1]
+        ordv = [int(o) for o in np.atleast_1d(self.opts.ord)]
+        if slabs[0].shape[1] == 2:
+            from ..spectral2d import SpectralSlabs2D
+            return SpectralSlabs2D(self.pdo, N, ordv[0], ordv[-1], x0=lo[0], width=N * H,
+                                   y0=lo[1], height=hi[1] - lo[1]).build()
+        from ..spectral3d import SpectralSlabs3D
+        if np.any(np.abs(lo - np.array([0.0, 0.0, 0.0])) > 1e-12) or abs(N * H - 1) > 1e-12 \
+                or np.any(np.abs(hi[1:] - 1.0) > 1e-12):
+            raise NotImplementedError("3-D spectral slabs are assembled on the unit cube")
+        P = np.random.default_rng(0).random((16, 3))
+        coef = []
+        for name, default in (("c11", 1.0), ("c22", 1.0), ("c33", 1.0), ("c", 0.0)):
+            f = getattr(self.pdo, name, None)
+            v = np.full(16, default) if f is None else np.asarray(f(P), dtype=np.float64)
+            if np.ptp(v) > 1e-14 * max(1.0, np.abs(v).max()):
+                raise NotImplementedError("3-D spectral slabs need constant coefficients")
+            coef.append(float(v[0]))
+        for name in ("c1", "c2", "c3", "c12", "c13", "c23"):
+            if getattr(self.pdo, name, None) is not None:
+                raise NotImplementedError("3-D spectral slabs: no first-order or mixed terms")
+        op = SpectralSlabs3D(N, ordv[0] + 1, ordv[1] + 1, ordv[-1] + 1, *coef).build()
+        op.SiM, op.SiP = op.blocks()
+        return op
+
+    def _check_chain(self, slabs):
+        x0 = slabs[0][0][0]
+        H = (slabs[0][1][0] - slabs[0][0][0]) / 2.0
+        for k, s in enumerate(slabs):
+            if abs(s[0][0] - (x0 + k * H)) > 1e-12 or abs(s[1][0] - (x0 + (k + 2) * H)) > 1e-12 \
+                    or np.any(np.abs(s[:, 1:] - slabs[0][:, 1:]) > 1e-12):
+                raise NotImplementedError("slabs must be the consecutive double slabs of a "
+                                          "box (geometry.geom_2D.square / geom_3D.cube dSlabs)")
+        for k, c in enumerate(self.connectivity):
+            want = [k - 1 if k > 0 else -1, k + 1 if k < len(slabs) - 1 else -1]
+            if list(c) != want:
+                raise NotImplementedError("only the non-periodic chain connectivity of dSlabs "
+                                          "is assembled on the GPU")
+
     def discretize(self):
         """Assemble (once) the S maps of every double slab on the GPU (2-D: dense maps;
-        3-D: the pair factors behind the S-map operators)."""
+        3-D HPS: the pair factors behind the S-map operators)."""
         if self.op is None:
             t0 = time.time()
-            g = self._grid()
-            cls = HPSSlabOperator3D if isinstance(g, SlabGrid3D) else HPSSlabOperator
-            self.op = cls(self.pdo, g).build()
+            if self.opts.type == "spectral":
+                self.op = self._spectral()
+            else:
+                g = self._grid()
+                cls = HPSSlabOperator3D if isinstance(g, SlabGrid3D) else HPSSlabOperator
+                self.op = cls(self.pdo, g).build()
             torch.cuda.synchronize()
             self.stats.discr_timing = (time.time() - t0) / max(len(self.slabList), 1)
         return self.op
@@ -148,6 +190,13 @@ class oms:
         dense device blocks in 2-D, device operators (sparse-solve-like applies through
         the pair factors) in 3-D, as the reference's LinearOperators."""
         op = self.discretize()
+        if not hasattr(op, "g"):  # spectral slabs: centre line / plane of double slab k
+            m = op.m
+            XXc = op.interface_points()[k * m:(k + 1) * m]
+            XXl, XXr = XXc.copy(), XXc.copy()
+            XXl[:, 0] -= op.H
+            XXr[:, 0] += op.H
+            return stMap(op.SiM[k], XXl, XXc), stMap(op.SiP[k], XXr, XXc)
         g = op.g
         if isinstance(g, SlabGrid3D):
             m = g.m
@@ -167,8 +216,7 @@ class oms:
     # -- S-system (oms.py:131-264) ------------------------------------------------------
     def construct_Stot_helper(self, bc, assembler, dbg=0):
         op = self.discretize()
-        g = op.g
-        nc = g.m
+        nc = op.g.m if hasattr(op, "g") else op.m
         S_rk_list, glob_target_dofs = [], []
         compress_t = sample_t = 0.0
         for k in range(len(self.slabList)):
@@ -209,6 +257,10 @@ class oms:
         """S-system right-hand sides (N_tot x r device tensor) for boundary data bc:
         a callable of points (one column) or an array of values at boundary_points()."""
         op = self.discretize()
+        if not hasattr(op, "g"):  # spectral slabs: every double slab has its own grid
+            if not callable(bc):
+                raise ValueError("spectral slabs take boundary data as a callable of points")
+            return op.rhs_from_function(bc)
         F = bc(self.boundary_points()) if callable(bc) else bc
         if not isinstance(F, torch.Tensor):
             F = torch.as_tensor(np.asarray(F, dtype=np.float64))
@@ -257,8 +309,7 @@ class oms_lu(oms):
 
     def construct_Stot_helper(self, bc, dbg=0):
         op = self.discretize()
-        g = op.g
-        nc = g.m
+        nc = op.g.m if hasattr(op, "g") else op.m
         S_lu_list, glob_target_dofs = [], []
         for k in range(len(self.slabList)):
             st_l, st_r = self.compute_stmaps(k)

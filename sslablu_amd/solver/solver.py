@@ -30,8 +30,9 @@ class stMap:
 
 class solverOptions:
     """type: 'hps' (GPU Chebyshev HPS; 'hpsalt', the name the reference's 3-D drivers use,
-    selects the same discretization, with ord = p_disc points per leaf side); ord: points
-    per leaf side, one per dimension; a: leaf half-sizes."""
+    selects the same discretization, with ord = p_disc points per leaf side; a: leaf
+    half-sizes) or 'spectral' (one Chebyshev grid per double slab, ord = the polynomial
+    degree per axis: ord + 1 points)."""
 
     def __init__(self, type: str, ord, a=None):
         self.type = type
@@ -67,6 +68,8 @@ class solverWrapper:
     def construct(self, geom, PDE, verbose=False, compute_inverse=True):
         geom = np.asarray(geom, dtype=np.float64)
         self.ndim = geom.shape[1]
+        if self.type == "spectral":
+            return self._construct_spectral(geom, PDE)
         if self.ndim == 3:
             return self._construct3(geom, PDE)
         if self.ndim != 2:
@@ -107,6 +110,21 @@ class solverWrapper:
         self.XX = np.concatenate([pts[2 * ns:2 * ns + m], pts[2 * ns + m:], pts[:2 * ns], centre])
         return self._finish(centre.shape[0])
 
+    def _construct_spectral(self, geom, PDE):
+        """One 2-D double slab on a spectral grid of (ord[0]+1) x (ord[1]+1) Chebyshev
+        points (solver/spectral/spectralSolver.py:198-235); the boundary is ordered
+        [left | right | bottom | top] (corners dropped), the skeleton is the centre line."""
+        if self.ndim != 2:
+            raise NotImplementedError("solverWrapper('spectral') on one double slab is 2-D "
+                                      "(3-D spectral slabs are assembled through oms)")
+        from ..spectral2d import SpectralSlabs2D
+        (xl, yl), (xr, yr) = geom
+        ordv = [int(o) for o in np.atleast_1d(self.opts.ord)]
+        self.op = SpectralSlabs2D(PDE, 2, ordv[0], ordv[-1], x0=xl, width=xr - xl, y0=yl,
+                                  height=yr - yl).build()
+        self.XX = np.concatenate([self.op.boundary_points(0), self.op.interface_points()])
+        return self._finish(self.op.m)
+
     def _finish(self, ncentre):
         nb = self.XX.shape[0] - ncentre
         self.Ib = np.arange(nb)
@@ -127,7 +145,7 @@ class solverWrapper:
             from ..hps3d import PairSMap, SideMap
             return SideMap(self.op) if which == "gb" else PairSMap(self.op, 0, which)
         Y = self.op.Y[0]
-        m = self.op.g.m
+        m = self.op.m if self.type == "spectral" else self.op.g.m
         if which == "l":
             return Y[:, :m]
         if which == "r":
