@@ -94,21 +94,23 @@ class HPSRect:
     -- then leaf interiors (leaf-major, x-major inside a leaf).
     Equations: PDE at leaf interiors, flux continuity at internal edges."""
 
-    def __init__(self, pdo, x0, y0, hx, hy, nxl, nyl, q):
+    def __init__(self, pdo, x0, y0, hx, hy, nxl, nyl, q, periodic_x=False):
         self.pdo, self.x0, self.y0, self.hx, self.hy = pdo, x0, y0, hx, hy
         self.nxl, self.nyl, self.q = nxl, nyl, q
+        self.periodic_x = periodic_x
         p = q - 2
         self.p = p
         self.ref = LeafRef.make(q)
         t = self.ref.t
-        nv = (nxl + 1) * nyl * p
+        # periodic in x: the vertical line k = nxl is line 0 (one set of nodes)
+        nv = (nxl + (0 if periodic_x else 1)) * nyl * p
         nh = (nyl + 1) * nxl * p
         self.nv, self.nh = nv, nh
         self.nedge = nv + nh
         self.nint = nxl * nyl * p * p
         self.n = self.nedge + self.nint
         XX = np.zeros((self.n, 2))
-        for k in range(nxl + 1):
+        for k in range(nxl + (0 if periodic_x else 1)):
             for j in range(nyl):
                 s = self.vidx(k, j)
                 XX[s:s + p, 0] = x0 + k * hx
@@ -128,6 +130,8 @@ class HPSRect:
 
     # index helpers -------------------------------------------------------------
     def vidx(self, k, j):
+        if self.periodic_x:
+            k %= self.nxl
         return (k * self.nyl + j) * self.p
 
     def hidx(self, j, i):
@@ -215,6 +219,9 @@ class HPSRect:
 
     def outer_edge_nodes(self):
         p, nyl, nxl = self.p, self.nyl, self.nxl
+        if self.periodic_x:  # only the bottom and top lines bound the cylinder
+            return np.concatenate([self.hidx(0, 0) + np.arange(nxl * p),
+                                   self.hidx(nyl, 0) + np.arange(nxl * p)])
         left = self.vidx(0, 0) + np.arange(nyl * p)
         right = self.vidx(nxl, 0) + np.arange(nyl * p)
         bottom = self.hidx(0, 0) + np.arange(nxl * p)
@@ -254,6 +261,20 @@ def gb_unit_square(P):
             | (np.abs(P[:, 1]) < 1e-14) | (np.abs(P[:, 1] - 1) < 1e-14))
 
 
+def dSlabs_periodic(N):
+    """Double slabs of [0,1]^2 periodic in x and their connectivity, the convention of
+    geometry/geom_3D/squareTorus.py:268-277: double slab n is centred on x = n H
+    (n = 0..N-1, the first straddling x = 0 = 1), connected to (n-1) % N and (n+1) % N."""
+    H = 1.0 / N
+    slabs = [[[n * H - H, 0.0], [n * H + H, 1.0]] for n in range(N)]
+    return slabs, [[(n - 1) % N, (n + 1) % N] for n in range(N)], H
+
+
+def gb_bottom_top(P):
+    """Global boundary of the x-periodic unit square: y = 0 and y = 1."""
+    return (np.abs(P[:, 1]) < 1e-14) | (np.abs(P[:, 1] - 1) < 1e-14)
+
+
 def dSlabs(N):
     """Double slabs of [0,1]^2 and their connectivity (geometry/geom_2D/square.py:33-47)."""
     H = 1.0 / N
@@ -285,13 +306,16 @@ def slab_index_sets(solver, bounds, gb=gb_unit_square):
     return Il, Ir, Ic, Igb, XXi, XXb
 
 
-def oms_stot(pdo, N, leaf_h, q, bc=None, gb=gb_unit_square):
+def oms_stot(pdo, N, leaf_h, q, bc=None, gb=gb_unit_square, periodic=False):
     """S_rk_list, rhs maps and rhs of multislab/oms.py:131-264 with the dense assembler
     (matAssembly/matAssembler.py:93-98: M = linOp @ I):
     per double slab S_l = (A_ii^{-1} A_ib[:, Il])[Ic], S_r likewise,
     R = (A_ii^{-1} A_ib[:, Igb])[Ic], rhs = -R bc(XXb[Igb]).
+    periodic: the x-periodic square (dSlabs_periodic; every S_rk_list entry [S_l, S_r]).
     Returns dict(S_rk_list, R_list, XXgb_list, Ic_pts, rhs_list, nc)."""
-    slabs, conn, H = dSlabs(N)
+    slabs, conn, H = dSlabs_periodic(N) if periodic else dSlabs(N)
+    if periodic:
+        gb = gb_bottom_top
     S_rk_list, R_list, XXgb_list, Ic_pts, rhs_list = [], [], [], [], []
     for k, bnd in enumerate(slabs):
         sol = SlabSolver(pdo, bnd, leaf_h, q)
@@ -317,6 +341,18 @@ def oms_stot(pdo, N, leaf_h, q, bc=None, gb=gb_unit_square):
                 rhs_list=rhs_list, nc=len(Ic_pts[0]), conn=conn, H=H)
 
 
+def stot_dense_cyclic(S_rk_list):
+    """Dense periodic S = I + S_l (block column i-1 mod n) + S_r (block column i+1 mod n)."""
+    nb = len(S_rk_list)
+    m = S_rk_list[0][0].shape[0]
+    S = np.eye(nb * m)
+    for i, (Sl, Sr) in enumerate(S_rk_list):
+        jl, jr = (i - 1) % nb, (i + 1) % nb
+        S[i * m:(i + 1) * m, jl * m:(jl + 1) * m] += Sl
+        S[i * m:(i + 1) * m, jr * m:(jr + 1) * m] += Sr
+    return S
+
+
 def stot_dense(S_rk_list):
     """Dense S = I + off-diagonal blocks (multislab/oms.py:267-304 with contiguous dofs)."""
     nb = len(S_rk_list)
@@ -333,14 +369,15 @@ def stot_dense(S_rk_list):
     return S
 
 
-def global_solution_on_lines(pdo, N, leaf_h, q, bc):
-    """Brute force: the single HPS system on all of [0,1]^2 with Dirichlet data bc,
-    solved by sparse LU; returns (values, points) on the interface lines x = k/N,
-    k = 1..N-1, ordered like the S-system unknowns."""
+def global_solution_on_lines(pdo, N, leaf_h, q, bc, periodic=False):
+    """Brute force: the single HPS system on all of [0,1]^2 (periodic in x when
+    `periodic`) with Dirichlet data bc, solved by sparse LU; returns (values, points) on
+    the interface lines x = k/N, k = 1..N-1 (periodic: k = 0..N-1), ordered like the
+    S-system unknowns."""
     hx, hy = leaf_h
     nxl = int(round(1.0 / hx))
     nyl = int(round(1.0 / hy))
-    rect = HPSRect(pdo, 0.0, 0.0, hx, hy, nxl, nyl, q)
+    rect = HPSRect(pdo, 0.0, 0.0, hx, hy, nxl, nyl, q, periodic_x=periodic)
     A = rect.assemble().tocsc()
     outer = rect.outer_edge_nodes()
     rhs = np.zeros(rect.n)
@@ -348,7 +385,7 @@ def global_solution_on_lines(pdo, N, leaf_h, q, bc):
     u = spla.spsolve(A, rhs)
     per = int(round((1.0 / N) / hx))
     vals, pts = [], []
-    for k in range(1, N):
+    for k in range(0 if periodic else 1, N):
         s = rect.vidx(k * per, 0)
         sl = slice(s, s + nyl * rect.p)
         vals.append(u[sl])

@@ -65,8 +65,9 @@ class HPSBox3D:
     local indices on a leaf boundary (an edge or corner) is not a node.  Equations: PDE
     at leaf interiors, flux continuity at internal faces, identity at the outer faces."""
 
-    def __init__(self, pdo, origin, h, nl, q):
+    def __init__(self, pdo, origin, h, nl, q, periodic_x=False):
         self.pdo = pdo
+        self.periodic_x = periodic_x
         self.origin = np.asarray(origin, dtype=np.float64)
         self.h = np.asarray(h, dtype=np.float64)
         self.nl = tuple(int(v) for v in nl)
@@ -84,13 +85,20 @@ class HPSBox3D:
         for i in range(self.nl[0]):
             for j in range(self.nl[1]):
                 for l in range(self.nl[2]):
-                    keys.append(loc[self.loc_ok] + np.array([i, j, l]) * Q)
+                    keys.append(self._wrap(loc[self.loc_ok] + np.array([i, j, l]) * Q))
         keys = np.unique(np.concatenate(keys), axis=0)
         self.keys = keys
         self.n = len(keys)
         self._dims = np.array(self.nl) * Q + 1
         self._code = {int(c): k for k, c in enumerate(self._encode(keys))}
         self.XX = self.coords(keys)
+
+    def _wrap(self, K):
+        """Periodic in x: the face x = nl[0] (q-1) is the face x = 0 (one set of nodes)."""
+        if self.periodic_x:
+            K = K.copy()
+            K[:, 0] %= self.nl[0] * (self.q - 1)
+        return K
 
     def _encode(self, K):
         d = self._dims
@@ -108,7 +116,7 @@ class HPSBox3D:
         """Node index of every local grid point (q^3, C order over (a, b, c)); -1 at
         edges and corners."""
         g = -np.ones(len(self.loc), dtype=np.int64)
-        K = self.loc[self.loc_ok] + np.array([i, j, l]) * (self.q - 1)
+        K = self._wrap(self.loc[self.loc_ok] + np.array([i, j, l]) * (self.q - 1))
         g[self.loc_ok] = self.index(K)
         return g.reshape(self.q, self.q, self.q)
 
@@ -116,7 +124,8 @@ class HPSBox3D:
         Q = self.q - 1
         K = self.keys
         lim = np.array(self.nl) * Q
-        return np.where(np.any((K == 0) | (K == lim), axis=1))[0]
+        d0 = 1 if self.periodic_x else 0
+        return np.where(np.any((K[:, d0:] == 0) | (K[:, d0:] == lim[d0:]), axis=1))[0]
 
     def assemble(self, _plant=None):
         """Sparse square matrix with one row per node (see the class docstring).
@@ -229,13 +238,31 @@ def slab_index_sets(solver, bounds, gb=gb_unit_cube):
     return Il, Ir, Ic, Igb, XXi, XXb
 
 
-def oms_stot(pdo, N, leaf_h, q, bc=None, gb=gb_unit_cube, _plant=None):
+def dSlabs_periodic(N):
+    """Double slabs of [0,1]^3 periodic in x (geometry/geom_3D/squareTorus.py:268-277):
+    double slab n centred on x = n H, connected to (n-1) % N and (n+1) % N."""
+    H = 1.0 / N
+    slabs = [[[n * H - H, 0.0, 0.0], [n * H + H, 1.0, 1.0]] for n in range(N)]
+    return slabs, [[(n - 1) % N, (n + 1) % N] for n in range(N)], H
+
+
+def gb_sides(P):
+    """Global boundary of the x-periodic cube: the faces y, z = 0, 1
+    (geometry/geom_3D/squareTorus.py:43-48)."""
+    P = np.asarray(P)
+    return np.any((np.abs(P[:, 1:]) < 1e-14) | (np.abs(P[:, 1:] - 1.0) < 1e-14), axis=1)
+
+
+def oms_stot(pdo, N, leaf_h, q, bc=None, gb=gb_unit_cube, _plant=None, periodic=False):
     """S_rk_list, rhs maps and rhs of multislab/oms.py:131-264 for cube.dSlabs(N) with
     the dense assembler (matAssembly/matAssembler.py:93-98): per double slab
     S_l = (A_ii^{-1} A_ib[:, Il])[Ic], S_r likewise, R = (A_ii^{-1} A_ib[:, Igb])[Ic],
     rhs = -R bc(XXb[Igb]).  Returns dict(S_rk_list, R_list, XXgb_list, Ic_pts, rhs_list,
-    nc, conn, H)."""
-    slabs, conn, H = dSlabs(N)
+    nc, conn, H).  periodic: the x-periodic cube (dSlabs_periodic, every entry
+    [S_l, S_r])."""
+    slabs, conn, H = dSlabs_periodic(N) if periodic else dSlabs(N)
+    if periodic:
+        gb = gb_sides
     S_rk_list, R_list, XXgb_list, Ic_pts, rhs_list = [], [], [], [], []
     for k, bnd in enumerate(slabs):
         sol = SlabSolver3D(pdo, bnd, leaf_h, q, _plant)
@@ -261,20 +288,20 @@ def oms_stot(pdo, N, leaf_h, q, bc=None, gb=gb_unit_cube, _plant=None):
                 rhs_list=rhs_list, nc=len(Ic_pts[0]), conn=conn, H=H)
 
 
-def global_solution_on_planes(pdo, N, leaf_h, q, bc):
+def global_solution_on_planes(pdo, N, leaf_h, q, bc, periodic=False):
     """Brute force: the single HPS system on all of [0,1]^3 with Dirichlet data bc,
     solved by sparse LU; returns (values, points) on the interface planes x = k/N,
     k = 1..N-1, each in (y, z) order (the S-system unknowns)."""
     h = np.asarray(leaf_h, dtype=np.float64)
     nl = np.rint(1.0 / h).astype(int)
-    box = HPSBox3D(pdo, np.zeros(3), h, nl, q)
+    box = HPSBox3D(pdo, np.zeros(3), h, nl, q, periodic_x=periodic)
     A = box.assemble().tocsc()
     outer = box.outer_nodes()
     rhs = np.zeros(box.n)
     rhs[outer] = bc(box.XX[outer])
     u = spla.spsolve(A, rhs)
     vals, pts = [], []
-    for k in range(1, N):
+    for k in range(0 if periodic else 1, N):
         sel = np.where(np.abs(box.XX[:, 0] - k / N) < 1e-14)[0]
         sel = sel[_yz_order(box.XX[sel])]
         vals.append(u[sel])

@@ -138,9 +138,11 @@ class SlabGrid2D:
     square) by N single slabs of nx x ny leaves with q points."""
 
     def __init__(self, N: int, nx: int, ny: int, q: int, x0: float = 0.0,
-                 width: float = 1.0, y0: float = 0.0, height: float = 1.0):
-        if N < 2:
-            raise ValueError("need at least two single slabs (one interface)")
+                 width: float = 1.0, y0: float = 0.0, height: float = 1.0,
+                 periodic: bool = False):
+        if N < 2 or (periodic and N < 3):
+            raise ValueError("need at least two single slabs (one interface), three when "
+                             "periodic")
         if not (_is_pow2(nx) and _is_pow2(ny)):
             raise NotImplementedError("leaves per slab across / up must be powers of two")
         if q < 3:
@@ -152,7 +154,11 @@ class SlabGrid2D:
         self.hx = self.H / nx
         self.hy = height / ny
         self.m = ny * self.p
-        self.nb = N - 1
+        # periodic in x (x0 and x0 + width identified): N interfaces, interface k at
+        # x0 + k H between single slabs k-1 (mod N) and k, the ordering of the reference's
+        # periodic dSlabs (geometry/geom_3D/squareTorus.py:268-277)
+        self.periodic = bool(periodic)
+        self.nb = N if periodic else N - 1
         self.nleaves = N * nx * ny
         self.t = cheb_unit(q)[0]
 
@@ -178,19 +184,24 @@ class SlabGrid2D:
         return (np.arange(n_leaves)[:, None] + ti[None, :]).ravel() * h
 
     def gb_points(self):
-        """Global boundary nodes in the order [bottom | top | left | right]."""
+        """Global boundary nodes in the order [bottom | top | left | right] (periodic:
+        [bottom | top])."""
         xs = self.x0 + self.edge_coords(self.N * self.nx, self.hx)
         ys = self.y0 + self.edge_coords(self.ny, self.hy)
         bottom = np.stack([xs, np.full_like(xs, self.y0)], 1)
         top = np.stack([xs, np.full_like(xs, self.y0 + self.height)], 1)
+        if self.periodic:
+            return np.concatenate([bottom, top])
         left = np.stack([np.full_like(ys, self.x0), ys], 1)
         right = np.stack([np.full_like(ys, self.x0 + self.width), ys], 1)
         return np.concatenate([bottom, top, left, right])
 
     def interface_points(self):
-        """(nb*m, 2) interface nodes, interface i at x = x0 + (i+1) H, bottom to top."""
+        """(nb*m, 2) interface nodes, bottom to top: interface i at x = x0 + (i+1) H
+        (periodic: at x0 + i H)."""
         ys = self.y0 + self.edge_coords(self.ny, self.hy)
-        pts = [np.stack([np.full_like(ys, self.x0 + (i + 1) * self.H), ys], 1)
+        sh = 0 if self.periodic else 1
+        pts = [np.stack([np.full_like(ys, self.x0 + (i + sh) * self.H), ys], 1)
                for i in range(self.nb)]
         return np.concatenate(pts)
 
@@ -314,7 +325,15 @@ class HPSSlabOperator:
 
     def pair_solve(self, Ts, stream=None):
         g = self.g
-        p, m, nb = g.p, g.m, g.nb
+        if g.periodic:  # interface 0 joins the last slab to the first; 1.. as usual
+            Yr = self._pair_solve(Ts, g.N - 1, stream)
+            Yw = self._pair_solve(torch.stack([Ts[g.N - 1], Ts[0]]), 1, stream)
+            return torch.cat([Yw, Yr])
+        return self._pair_solve(Ts, g.nb, stream)
+
+    def _pair_solve(self, Ts, nb, stream=None):
+        g = self.g
+        p, m = g.p, g.m
         ca, cb, cola, colb = pair_maps(g.nx, g.ny, p)
         ncols = len(cola)
         n_s = Ts.shape[1]
@@ -350,8 +369,11 @@ class HPSSlabOperator:
 
     # -------------------------------------------------------------------------
     def S_rk_list(self):
-        """[[S_r], [S_l, S_r], ..., [S_l]] device views (multislab/oms.py:239-244)."""
+        """[[S_r], [S_l, S_r], ..., [S_l]] device views (multislab/oms.py:239-244);
+        periodic: [S_l, S_r] for every interface."""
         nb = self.g.nb
+        if self.g.periodic:
+            return [[self.SiM[i], self.SiP[i]] for i in range(nb)]
         out = []
         for i in range(nb):
             if i == 0:
@@ -371,6 +393,8 @@ class HPSSlabOperator:
             F = F[:, None]
         F = F.to(device=self.device, dtype=F64).contiguous()
         r = F.shape[1]
+        if g.periodic:
+            return self._rhs_periodic(F, out)
         ngb = 2 * g.N * nxp + 2 * m
         if F.shape[0] != ngb:
             raise ValueError(f"boundary data needs {ngb} rows, got {F.shape[0]}")
@@ -380,6 +404,29 @@ class HPSSlabOperator:
         call("sslab_oms_rhs", nb, m, nxp, ncols, ptr(self.Y), ptr(F), F.stride(0), r,
              g.N, ptr(out), out.stride(0), stream_handle(stream))
         return out
+
+    def _rhs_periodic(self, F, out):
+        """rhs of the x-periodic S-system: interface k reads the bottom / top windows of
+        single slabs (k-1) mod N and k, gathered into one batch (the wrapping pair has
+        no contiguous window), then one batched product with the maps."""
+        from .hbs import bgemm
+        g = self.g
+        m, N, nxp = g.m, g.N, g.nx * g.p
+        if F.shape[0] != 2 * N * nxp:
+            raise ValueError(f"boundary data needs {2 * N * nxp} rows, got {F.shape[0]}")
+        k = np.arange(N)
+        a, b = (k - 1) % N, k
+        w = np.arange(nxp)
+        rows = np.concatenate([(a[:, None] * nxp + w), (b[:, None] * nxp + w),
+                               (N * nxp + a[:, None] * nxp + w), (N * nxp + b[:, None] * nxp + w)],
+                              axis=1)                                   # (N, 4 nxp)
+        idx = torch.as_tensor(rows.ravel(), device=self.device)
+        Fp = F.index_select(0, idx).view(N, 4 * nxp, F.shape[1])
+        X = bgemm(self.Y[:, :, 2 * m:], Fp, alpha=-1.0).reshape(N * m, -1)
+        if out is not None:
+            out.copy_(X)
+            return out
+        return X
 
     # -------------------------------------------------------------------------
     def solve_dir(self, x_iface: torch.Tensor, F: torch.Tensor, ff_body=None, stream=None):
@@ -394,6 +441,8 @@ class HPSSlabOperator:
         the leaf and merge operators here are those of the homogeneous equation."""
         if ff_body is not None:
             raise NotImplementedError("solve_dir: body loads are not supported")
+        if self.g.periodic:
+            raise NotImplementedError("solve_dir: periodic slabs are not provided")
         if not self.keep_solution_ops or self.leaf_X is None or (
                 len(self._down) == 0 and (self.g.nx > 1 or self.g.ny > 1)):
             raise ValueError("solve_dir: build the operator with keep_solution_ops=True")
@@ -473,6 +522,8 @@ class HPSSlabOperator:
             return self._rb0
         from .hbs import bgemm
         g = self.g
+        if g.periodic:
+            raise NotImplementedError("rb0_maps: periodic operators are solved unfolded")
         m, nb, nxp = g.m, g.nb, g.nx * g.p
         if nb < 2:
             raise ValueError("rb0_maps: needs at least two interfaces")
@@ -543,6 +594,8 @@ class BoundaryFold:
     def __init__(self, op: HPSSlabOperator, fac, stream=None):
         if fac.method != "rb" or not fac.ident0 or fac.cyclic or fac.factor is None:
             raise ValueError("BoundaryFold: needs the FP64 red-black factor built with T = None")
+        if op.g.periodic:
+            raise ValueError("BoundaryFold: periodic operators are solved unfolded")
         g = op.g
         m, nb, nxp = g.m, g.nb, g.nx * g.p
         if fac.nb != nb or fac.m != m:

@@ -185,9 +185,11 @@ class SlabGrid3D:
     """Leaf tiling of the box origin + [0, W] x [0, Hy] x [0, Hz] by N single slabs
     (along x) of nx x ny x nz leaves with q^3 points each."""
 
-    def __init__(self, N, nx, ny, nz, q, origin=(0.0, 0.0, 0.0), size=(1.0, 1.0, 1.0)):
-        if N < 2:
-            raise ValueError("need at least two single slabs (one interface)")
+    def __init__(self, N, nx, ny, nz, q, origin=(0.0, 0.0, 0.0), size=(1.0, 1.0, 1.0),
+                 periodic=False):
+        if N < 2 or (periodic and N < 3):
+            raise ValueError("need at least two single slabs (one interface), three when "
+                             "periodic")
         if not (_is_pow2(nx) and _is_pow2(ny) and _is_pow2(nz)):
             raise NotImplementedError("leaves per slab along each axis must be powers of two")
         if q < 3:
@@ -199,7 +201,10 @@ class SlabGrid3D:
         self.H = self.size[0] / N
         self.hx, self.hy, self.hz = self.H / nx, self.size[1] / ny, self.size[2] / nz
         self.m = ny * nz * self.p ** 2
-        self.nb = N - 1
+        # periodic in x: N interfaces, interface k on x0 + k H between single slabs
+        # (k-1) mod N and k (geometry/geom_3D/squareTorus.py:268-277)
+        self.periodic = bool(periodic)
+        self.nb = N if periodic else N - 1
         self.sched = merge_schedule(nx, ny, nz)
         self.pos = leaf_positions(self.sched)        # (leaves per slab, 3)
         self.leaves_per_slab = nx * ny * nz
@@ -240,16 +245,21 @@ class SlabGrid3D:
 
     def gb_points(self):
         """Global boundary nodes: [sides of slab 0 | ... | sides of slab N-1 | x = x0
-        face | x = x0 + W face], each slab's sides [y-lo | y-hi | z-lo | z-hi]."""
+        face | x = x0 + W face], each slab's sides [y-lo | y-hi | z-lo | z-hi] (periodic:
+        the sides only)."""
         m = self.m
         out = [self.slab_boundary_points(s)[2 * m:] for s in range(self.N)]
-        out.append(self.slab_boundary_points(0)[:m])
-        out.append(self.slab_boundary_points(self.N - 1)[m:2 * m])
+        if not self.periodic:
+            out.append(self.slab_boundary_points(0)[:m])
+            out.append(self.slab_boundary_points(self.N - 1)[m:2 * m])
         return np.concatenate(out)
 
     def interface_points(self):
-        """(nb m, 3) interface nodes, interface i on x = x0 + (i+1) H, (y, z) order."""
+        """(nb m, 3) interface nodes in (y, z) order: interface i on x = x0 + (i+1) H
+        (periodic: on x0 + i H)."""
         m = self.m
+        if self.periodic:
+            return np.concatenate([self.slab_boundary_points(i)[:m] for i in range(self.nb)])
         return np.concatenate([self.slab_boundary_points(i)[m:2 * m] for i in range(self.nb)])
 
 
@@ -359,7 +369,21 @@ class HPSSlabOperator3D:
     # -- 3. pair factors -------------------------------------------------------------
     def pair_factor(self, Ts, stream=None):
         g = self.g
-        m, nb, n_s, ns = g.m, g.nb, g.n_s, g.n_side
+        if g.periodic:
+            # interface 0 joins the last single slab to the first: factor it from a copy of
+            # those two DtN maps, the others as in the chain, interface 0 first
+            reg, wrap = self._pair_factor(Ts, g.N - 1, stream), None
+            Tw = torch.stack([Ts[g.N - 1], Ts[0]])
+            wrap = self._pair_factor(Tw, 1, stream)
+            del Tw
+            self.lu, self.Bl, self.Br, self.G, self.piv = (
+                torch.cat([w, r]) for w, r in zip(wrap, reg))
+            return
+        self.lu, self.Bl, self.Br, self.G, self.piv = self._pair_factor(Ts, g.nb, stream)
+
+    def _pair_factor(self, Ts, nb, stream=None):
+        g = self.g
+        m, n_s, ns = g.m, g.n_s, g.n_side
         dev = self.device
         s = stream_handle(stream)
         L = np.arange(m)
@@ -368,27 +392,28 @@ class HPSSlabOperator3D:
         neg_m = -np.ones(m, dtype=np.int64)
         dR, dL, dside = (self._dev(x, I32) for x in (R, L, side))
         sT = n_s * n_s
-        self.lu = torch.empty(nb, m, m, dtype=F64, device=dev)
+        lu = torch.empty(nb, m, m, dtype=F64, device=dev)
         # Tcc_i = T^i[R, R] + T^{i+1}[L, L]
         call("sslab_gather2", nb, m, m, ptr(Ts), n_s, sT, ptr(Ts) + 8 * sT, n_s, sT,
-             ptr(dR), ptr(dR), ptr(dL), ptr(dL), ptr(self.lu), m, m * m, s)
-        self.Bl = torch.empty(nb, m, m, dtype=F64, device=dev)
-        self.Br = torch.empty(nb, m, m, dtype=F64, device=dev)
+             ptr(dR), ptr(dR), ptr(dL), ptr(dL), ptr(lu), m, m * m, s)
+        Bl = torch.empty(nb, m, m, dtype=F64, device=dev)
+        Br = torch.empty(nb, m, m, dtype=F64, device=dev)
         call("sslab_gather2", nb, m, m, ptr(Ts), n_s, sT, None, 0, 0, ptr(dR), ptr(dL), None,
-             None, ptr(self.Bl), m, m * m, s)
+             None, ptr(Bl), m, m * m, s)
         call("sslab_gather2", nb, m, m, ptr(Ts) + 8 * sT, n_s, sT, None, 0, 0, ptr(dL), ptr(dR),
-             None, None, ptr(self.Br), m, m * m, s)
+             None, None, ptr(Br), m, m * m, s)
         # G_i = [T^i[R, side] | T^{i+1}[L, side]]
-        self.G = torch.empty(nb, m, 2 * ns, dtype=F64, device=dev)
+        G = torch.empty(nb, m, 2 * ns, dtype=F64, device=dev)
         call("sslab_gather2", nb, m, ns, ptr(Ts), n_s, sT, None, 0, 0, ptr(dR), ptr(dside),
-             None, None, ptr(self.G), 2 * ns, m * 2 * ns, s)
+             None, None, ptr(G), 2 * ns, m * 2 * ns, s)
         call("sslab_gather2", nb, m, ns, ptr(Ts) + 8 * sT, n_s, sT, None, 0, 0, ptr(dL),
-             ptr(dside), None, None, ptr(self.G) + 8 * ns, 2 * ns, m * 2 * ns, s)
-        self.piv = torch.empty(nb, m, dtype=I32, device=dev)
+             ptr(dside), None, None, ptr(G) + 8 * ns, 2 * ns, m * 2 * ns, s)
+        piv = torch.empty(nb, m, dtype=I32, device=dev)
         info = torch.zeros(nb, dtype=I32, device=dev)
-        call("sslab_dgetrf_batched", m, ptr(self.lu), m, m * m, ptr(self.piv), m, ptr(info), nb, s)
+        call("sslab_dgetrf_batched", m, ptr(lu), m, m * m, ptr(piv), m, ptr(info), nb, s)
         self._infos.append(info)
         del neg_m
+        return lu, Bl, Br, G, piv
 
     def build(self, stream=None):
         """Run the whole assembly; returns self."""
@@ -473,17 +498,26 @@ class HPSSlabOperator3D:
             F = F[:, None]
         F = F.to(device=self.device, dtype=F64).contiguous()
         r = F.shape[1]
-        ngb = N * ns + 2 * m
+        ngb = N * ns + (0 if g.periodic else 2 * m)
         if F.shape[0] != ngb:
             raise ValueError(f"boundary data needs {ngb} rows, got {F.shape[0]}")
-        # W_i = G_i F[i ns : (i+2) ns]  (consecutive double slabs overlap by one slab)
-        Fw = F.as_strided((nb, 2 * ns, r), (ns * r, r, 1))
         W = torch.empty(nb, m, r, dtype=F64, device=self.device)
-        ggemm(gdesc(W, [(self.G, Fw, 1.0)]), stream=stream)
-        left = F[N * ns:N * ns + m].reshape(1, m, r)
-        right = F[N * ns + m:].reshape(1, m, r)
-        bgemm(self.Bl[0:1], left, out=W[0:1], beta=1.0)
-        bgemm(self.Br[nb - 1:nb], right, out=W[nb - 1:nb], beta=1.0)
+        if g.periodic:
+            # interface k reads the side faces of slabs (k-1) mod N and k: gathered, since
+            # the wrapping pair's rows are not one window
+            k = np.arange(N)
+            rows = np.concatenate([((k - 1) % N)[:, None] * ns + np.arange(ns),
+                                   k[:, None] * ns + np.arange(ns)], axis=1)
+            Fw = F.index_select(0, torch.as_tensor(rows.ravel(), device=self.device))
+            ggemm(gdesc(W, [(self.G, Fw.view(nb, 2 * ns, r), 1.0)]), stream=stream)
+        else:
+            # W_i = G_i F[i ns : (i+2) ns]  (consecutive double slabs overlap by one slab)
+            Fw = F.as_strided((nb, 2 * ns, r), (ns * r, r, 1))
+            ggemm(gdesc(W, [(self.G, Fw, 1.0)]), stream=stream)
+            left = F[N * ns:N * ns + m].reshape(1, m, r)
+            right = F[N * ns + m:].reshape(1, m, r)
+            bgemm(self.Bl[0:1], left, out=W[0:1], beta=1.0)
+            bgemm(self.Br[nb - 1:nb], right, out=W[nb - 1:nb], beta=1.0)
         X = self._solve(W, False, stream)
         X.neg_()
         X = X.reshape(nb * m, r)
@@ -493,8 +527,11 @@ class HPSSlabOperator3D:
         return X
 
     def S_rk_list(self):
-        """[[S_r], [S_l, S_r], ..., [S_l]] as device operators (multislab/oms.py:239-244)."""
+        """[[S_r], [S_l, S_r], ..., [S_l]] as device operators (multislab/oms.py:239-244);
+        periodic: [S_l, S_r] for every interface."""
         nb = self.g.nb
+        if self.g.periodic:
+            return [[self.smap(i, "l"), self.smap(i, "r")] for i in range(nb)]
         out = []
         for i in range(nb):
             if i == 0:
@@ -511,6 +548,8 @@ class HPSSlabOperator3D:
         GMRES applies, multislab/oms.py:267-304)."""
         nb, m = self.g.nb, self.g.m
 
+        periodic = self.g.periodic
+
         def apply(X):
             r = X.shape[1]
             X3 = X.reshape(nb, m, r)
@@ -518,6 +557,9 @@ class HPSSlabOperator3D:
             Zr = torch.zeros(nb, m, r, dtype=F64, device=self.device)
             Zl[1:] = X3[:-1]
             Zr[:-1] = X3[1:]
+            if periodic:
+                Zl[0] = X3[nb - 1]
+                Zr[nb - 1] = X3[0]
             from .hbs import bgemm
             W = bgemm(self.Bl, Zl)
             bgemm(self.Br, Zr, out=W, beta=1.0)

@@ -104,23 +104,26 @@ class oms:
     def _grid(self):
         slabs = [np.asarray(s, dtype=np.float64) for s in self.slabList]
         ndim = slabs[0].shape[1]
-        N = len(slabs) + 1
+        periodic = self._check_chain(slabs)
         H = (slabs[0][1][0] - slabs[0][0][0]) / 2.0
-        x0 = slabs[0][0][0]
-        self._check_chain(slabs)
+        # the chain of dSlabs(N) has N - 1 double slabs from x0; the periodic one has N,
+        # the first centred on x0 (geometry/geom_3D/squareTorus.py:268-277)
+        N = len(slabs) if periodic else len(slabs) + 1
+        x0 = slabs[0][0][0] + (H if periodic else 0.0)
         lo, hi = slabs[0][0], slabs[0][1]
         if ndim == 3:
             q, hx, hy, hz = hps_grid_params(self.opts, ndim=3)
             nl = (int(round(H / hx)), int(round((hi[1] - lo[1]) / hy)),
                   int(round((hi[2] - lo[2]) / hz)))
             return SlabGrid3D(N, *nl, q, origin=(x0, lo[1], lo[2]),
-                              size=(N * H, hi[1] - lo[1], hi[2] - lo[2]))
+                              size=(N * H, hi[1] - lo[1], hi[2] - lo[2]), periodic=periodic)
         if ndim != 2:
             raise NotImplementedError("the GPU HPS discretization is 2-D or 3-D")
         q, hx, hy = hps_grid_params(self.opts)
         nx = int(round(H / hx))
         ny = int(round((hi[1] - lo[1]) / hy))
-        return SlabGrid2D(N, nx, ny, q, x0=x0, width=N * H, y0=lo[1], height=hi[1] - lo[1])
+        return SlabGrid2D(N, nx, ny, q, x0=x0, width=N * H, y0=lo[1], height=hi[1] - lo[1],
+                          periodic=periodic)
 
     def _spectral(self):
         """The 'spectral' discretization (solver/spectral/spectralSolver.py:198-235): one
@@ -128,7 +131,8 @@ class oms:
         variable coefficients (SpectralSlabs2D), 3-D with constant ones (SpectralSlabs3D,
         fast diagonalization)."""
         slabs = [np.asarray(s, dtype=np.float64) for s in self.slabList]
-        self._check_chain(slabs)
+        if self._check_chain(slabs):
+            raise NotImplementedError("periodic slabs use the 'hps' discretization")
         N = len(slabs) + 1
         H = (slabs[0][1][0] - slabs[0][0][0]) / 2.0
         lo, hi = slabs[0][0], slabs[0][1]
@@ -157,18 +161,25 @@ class oms:
         return op
 
     def _check_chain(self, slabs):
-        x0 = slabs[0][0][0]
+        """The double slabs must be consecutive ([x + kH - H, x + kH + H]-style, equal
+        cross sections) with the chain connectivity of dSlabs, or the periodic one of
+        squareTorus.dSlabs; returns True for the periodic case."""
+        n = len(slabs)
+        first = slabs[0][0][0]
         H = (slabs[0][1][0] - slabs[0][0][0]) / 2.0
         for k, s in enumerate(slabs):
-            if abs(s[0][0] - (x0 + k * H)) > 1e-12 or abs(s[1][0] - (x0 + (k + 2) * H)) > 1e-12 \
+            if abs(s[0][0] - (first + k * H)) > 1e-12 or abs(s[1][0] - (first + (k + 2) * H)) > 1e-12 \
                     or np.any(np.abs(s[:, 1:] - slabs[0][:, 1:]) > 1e-12):
                 raise NotImplementedError("slabs must be the consecutive double slabs of a "
-                                          "box (geometry.geom_2D.square / geom_3D.cube dSlabs)")
-        for k, c in enumerate(self.connectivity):
-            want = [k - 1 if k > 0 else -1, k + 1 if k < len(slabs) - 1 else -1]
-            if list(c) != want:
-                raise NotImplementedError("only the non-periodic chain connectivity of dSlabs "
-                                          "is assembled on the GPU")
+                                          "box (geometry.geom_2D.square / geom_3D.cube / "
+                                          "geom_3D.squareTorus dSlabs)")
+        conn = [list(c) for c in self.connectivity]
+        if n >= 3 and conn == [[(k - 1) % n, (k + 1) % n] for k in range(n)]:
+            return True
+        if conn != [[k - 1 if k > 0 else -1, k + 1 if k < n - 1 else -1] for k in range(n)]:
+            raise NotImplementedError("connectivity must be the chain of dSlabs or the "
+                                      "periodic ring of squareTorus.dSlabs")
+        return False
 
     def discretize(self):
         """Assemble (once) the S maps of every double slab on the GPU (2-D: dense maps;
@@ -198,20 +209,13 @@ class oms:
             XXr[:, 0] += op.H
             return stMap(op.SiM[k], XXl, XXc), stMap(op.SiP[k], XXr, XXc)
         g = op.g
-        if isinstance(g, SlabGrid3D):
-            m = g.m
-            XXl = g.slab_boundary_points(k)[:m]
-            XXc = g.slab_boundary_points(k)[m:2 * m]
-            XXr = g.slab_boundary_points(k + 1)[m:2 * m]
-            return stMap(op.smap(k, "l"), XXl, XXc), stMap(op.smap(k, "r"), XXr, XXc)
         XXc = g.interface_points()[k * g.m:(k + 1) * g.m]
-        ys = XXc[:, 1]
-        xl = g.x0 + k * g.H
-        XXl = np.stack([np.full_like(ys, xl), ys], 1)
-        XXr = np.stack([np.full_like(ys, xl + 2 * g.H), ys], 1)
-        st_l = stMap(op.SiM[k], XXl, XXc)
-        st_r = stMap(op.SiP[k], XXr, XXc)
-        return st_l, st_r
+        XXl, XXr = XXc.copy(), XXc.copy()
+        XXl[:, 0] -= g.H
+        XXr[:, 0] += g.H
+        if isinstance(g, SlabGrid3D):
+            return stMap(op.smap(k, "l"), XXl, XXc), stMap(op.smap(k, "r"), XXr, XXc)
+        return stMap(op.SiM[k], XXl, XXc), stMap(op.SiP[k], XXr, XXc)
 
     # -- S-system (oms.py:131-264) ------------------------------------------------------
     def construct_Stot_helper(self, bc, assembler, dbg=0):

@@ -109,3 +109,33 @@ def test_redblack_on_stot():
     x = ob.redblack_direct(pairs, [np.eye(m)] * nb, rhs)
     u, _ = solve_stot(res)
     assert np.linalg.norm(x - u) / np.linalg.norm(u) < 1e-12
+
+
+def periodic_harmonic(P):
+    return (np.sin(2 * np.pi * P[:, 0]) * np.exp(2 * np.pi * (P[:, 1] - 1))
+            + np.cos(4 * np.pi * P[:, 0]) * np.cosh(4 * np.pi * (P[:, 1] - 0.5)) / np.cosh(2 * np.pi))
+
+
+@pytest.mark.parametrize("pdo_name", ["laplace", "helmholtz"])
+def test_periodic_dd_equals_global_solve(pdo_name):
+    """x-periodic square (squareTorus-style dSlabs): the periodic S-system is an exact
+    domain decomposition of the global HPS system with the lines x = 0 and x = 1
+    identified (the coefficients must be periodic in x too)."""
+    pdo = LAPLACE if pdo_name == "laplace" else PDO2d(
+        ones, ones, c=lambda P: -64.0 * (1 + 0.3 * np.sin(2 * np.pi * P[:, 0])))
+    N, leaf_h, q = 4, (1 / 8, 1 / 4), 8
+    res = oh.oms_stot(pdo, N, leaf_h, q, bc=periodic_harmonic, periodic=True)
+    assert all(len(b) == 2 for b in res["S_rk_list"])
+    u = np.linalg.solve(oh.stot_dense_cyclic(res["S_rk_list"]), np.concatenate(res["rhs_list"]))
+    ug, pts = oh.global_solution_on_lines(pdo, N, leaf_h, q, periodic_harmonic, periodic=True)
+    assert np.allclose(np.concatenate(res["Ic_pts"]), pts, atol=1e-15)
+    assert np.linalg.norm(u - ug) / np.linalg.norm(ug) < 1e-11
+
+
+def test_periodic_spectral_accuracy():
+    errs = []
+    for q in (8, 12):
+        res = oh.oms_stot(LAPLACE, 4, (1 / 8, 1 / 4), q, bc=periodic_harmonic, periodic=True)
+        u = np.linalg.solve(oh.stot_dense_cyclic(res["S_rk_list"]), np.concatenate(res["rhs_list"]))
+        errs.append(np.max(np.abs(u - periodic_harmonic(np.concatenate(res["Ic_pts"])))))
+    assert errs[1] < 1e-3 * errs[0] and errs[1] < 1e-8, errs

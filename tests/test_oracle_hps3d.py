@@ -78,3 +78,17 @@ def test_planted_flux_sign_error_is_caught():
     res = o3.oms_stot(LAP, 3, H3, 5, bc=poly, _plant="flux_sign")
     pts = np.concatenate(res["Ic_pts"])
     assert np.max(np.abs(solve(res) - poly(pts))) > 1e-3
+
+
+def test_periodic_dd_equals_global_solve():
+    """x-periodic cube (geometry/geom_3D/squareTorus.py dSlabs): the periodic S-system
+    equals the global HPS solve with the faces x = 0 and x = 1 identified."""
+    from oracle.hps import stot_dense_cyclic
+    u = lambda P: np.sin(2 * np.pi * P[:, 0]) * np.sin(np.pi * P[:, 2]) * np.exp(  # noqa: E731
+        np.sqrt(5) * np.pi * (P[:, 1] - 1))
+    N, q, h = 4, 5, (1 / 4, 1 / 2, 1 / 2)
+    res = o3.oms_stot(LAP, N, h, q, bc=u, periodic=True)
+    x = np.linalg.solve(stot_dense_cyclic(res["S_rk_list"]), np.concatenate(res["rhs_list"]))
+    ug, pts = o3.global_solution_on_planes(LAP, N, h, q, u, periodic=True)
+    assert np.allclose(np.concatenate(res["Ic_pts"]), pts, atol=1e-15)
+    assert np.linalg.norm(x - ug) / np.linalg.norm(ug) < 1e-11
