@@ -4,9 +4,14 @@ Mirror of matAssembly/matAssembler.py (SimonDirckx/SslabLU): matAssemblerOptions
 matAssemblerStats, matAssembler.assemble(stMap, dbg) with the methods 'dense'
 (M = A I, matAssembler.py:93-98) and 'rkHBS' (binary slabTree on the target points,
 s = max(2k, leaf) + k + 10 Gaussian samples, HBS compression, :103-133), and the
-default assemblers denseMatAssembler / rkHMatAssembler (:213-219).  The strong-
-admissibility and legacy methods ('rkHBS_strong', 'rkHBSold', 'epsHBS') are not
-provided.
+default assemblers denseMatAssembler / rkHMatAssembler (:213-219), and the strong-
+admissibility method 'rkHBS_strong' with rkHMatAssembler_strong (:134-160, :220-222):
+binary slabTree with its edge adjacency ('partial'), s = kmax max(2k, leaf) + k + 20
+Gaussian samples (kmax the largest near-field size; the count of
+test/rkprop/thinSlab3D.py:400-408), strong HBS compression (the HBStorch_strong mirror).
+(The reference's 'rkHBS_strong' branch calls HBSnew_strong.HBSMAT_STRONG, which that
+module does not define; the mirror runs the HBStorch_strong compressor the reference's
+3-D tests use.)  The legacy methods 'rkHBSold' and 'epsHBS' are not provided.
 
 The map stMap.A may be a float64 device tensor, a device operator (device_op = True)
 or a scipy LinearOperator; the result is a device tensor (dense) or an HBSMAT.
@@ -18,13 +23,14 @@ import time
 import numpy as np
 import torch
 
-from .HBS import HBSnew
+from .HBS import HBSnew, HBStorch_strong
 from .HBS.HBSnew import _apply_op
 from .HBS.slabTree import slabTree
 
 
 class matAssemblerOptions:
-    """method: 'dense' (default) or 'rkHBS'; leaf_size; maxRank; tree; ndim; quad."""
+    """method: 'dense' (default), 'rkHBS' or 'rkHBS_strong'; leaf_size; maxRank; tree;
+    ndim; quad."""
 
     def __init__(self, method: str = "dense", tol: float = 1e-5, leaf_size: int = 8,
                  maxRank: int = 8, tree=None, ndim=3, quad=False, reduced=False):
@@ -81,6 +87,34 @@ class matAssembler:
                 print("\t HBS: %d samples, sample %.2e s, compress %.2e s, %.2f MB"
                       % (s, H.tSample, H.tCompress, H.nbytes / 1e6))
             return H
+        if method == "rkHBS_strong":
+            from ..hbs_strong import strong_sample_count
+            k = self.matOpts.maxRank
+            tree = slabTree(stMap.XXI, self.matOpts.quad, self.matOpts.leaf_size)
+            self.matOpts.tree = tree
+            if not tree.uniform():
+                raise ValueError("rkHBS_strong needs a cluster tree with equally sized leaves")
+            t0 = time.time()
+            s = strong_sample_count(tree.level_adj_list, len(tree.perm_leaf) // tree.nleaves, k)
+            N = linOp.shape[0]
+            gen = torch.Generator(device=dev)
+            gen.manual_seed(int(self.seed))
+            Om = torch.randn(N, s, generator=gen, device=dev, dtype=torch.float64)
+            Psi = torch.randn(N, s, generator=gen, device=dev, dtype=torch.float64)
+            Y = _apply_op(linOp, Om, False, dev)
+            Z = _apply_op(linOp, Psi, True, dev)
+            torch.cuda.synchronize(dev)
+            self.stats.timeSample = time.time() - t0
+            H = HBStorch_strong.HBSMAT(device=dev, tree=tree, quad=self.matOpts.quad)
+            H.construct(k, Om, Psi, Y, Z)
+            H.shape = tuple(linOp.shape)
+            H.tSample = self.stats.timeSample
+            self.stats.timeCompress = H.tCompress
+            self.stats.nbytes = H.nbytes
+            if dbg > 0:
+                print("\t strong HBS: %d samples, sample %.2e s, compress %.2e s, %.2f MB"
+                      % (s, H.tSample, H.tCompress, H.nbytes / 1e6))
+            return H
         raise NotImplementedError(f"assembly method {method!r} is not provided")
 
 
@@ -92,3 +126,8 @@ class denseMatAssembler(matAssembler):
 class rkHMatAssembler(matAssembler):
     def __init__(self, leaf_size, rk, tree=None, ndim=3, quad=False):
         super().__init__(matAssemblerOptions("rkHBS", 0, leaf_size, rk, tree, ndim, quad))
+
+
+class rkHMatAssembler_strong(matAssembler):
+    def __init__(self, leaf_size, rk, tree=None, ndim=3, quad=False):
+        super().__init__(matAssemblerOptions("rkHBS_strong", 0, leaf_size, rk, tree, ndim, quad))
