@@ -507,6 +507,7 @@ def c3_hps_bench(dev, seed=0, rk=40, nrhs=16, leaf=72):
     A = op.tridiag_apply()
     Xr, info_r = gmres(A, B, apply_M=lambda v: solver.solve(v).contiguous(), tol=1e-13,
                        restart=80, maxiter=200)
+    out["strong_admissibility"] = c3_strong_part(op, g, B, Xr, rb, X, ex, i, rk, leaf, seed)
     for name, slv, x0 in (("thomas", solver, x_hbs), ("red_black", rb, x_rb)):
         M = lambda v, s_=slv: s_.solve(v).contiguous()  # noqa: E731
         torch.cuda.synchronize()
@@ -527,6 +528,61 @@ def c3_hps_bench(dev, seed=0, rk=40, nrhs=16, leaf=72):
     del op, H_l, H_r, solver, rb
     torch.cuda.empty_cache()
     return out
+
+
+def c3_strong_part(op, g, B, Xr, rb, X, ex, i, rk, leaf, seed):
+    """Strong-admissibility compression of the same 30 S blocks (binary face tree of the
+    interface nodes, edge neighbours, s = 5 max(2k, leaf) + k + 20 samples), its error on
+    the same block, and GMRES on the strong-compressed S-system (the operator the
+    reference's thin-slab test applies, test/rkprop/thinSlab3D.py:371-420) preconditioned
+    by the weak-HBS red-black solve, against the FP64 reference solve."""
+    import torch
+    from sslablu_amd.hbs_strong import strong_sample_count
+    from sslablu_amd.hps3d import compress_s_blocks
+    from sslablu_amd.krylov import gmres
+    from sslablu_amd.matAssembly.HBS.slabTree import slabTree
+    m, nb = g.m, g.nb
+    tree = slabTree(g.interface_points()[:m], False, leaf, adjacency="partial")
+    ns = strong_sample_count(tree.level_adj_list, leaf, rk)
+    res = {"tree": f"binary face tree, {tree.nleaves} leaves of {leaf}, edge neighbours, "
+                   f"{ns} samples"}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S_l, S_r = compress_s_blocks(op, tree, rk, ns, seed + 1, sketch_fp32=True, strong=True)
+    torch.cuda.synchronize()
+    res["compress_s (30 blocks, fp32 sketch GEMMs)"] = round(time.perf_counter() - t0, 3)
+    res["compression_rel_err"] = ((S_l[i].matmat(X) - ex).norm() / ex.norm()).item()
+    res["bytes_per_block"] = int(S_l[i].nbytes)
+    S_l[i].matmat(X)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        S_l[i].matmat(X)
+    torch.cuda.synchronize()
+    res["apply_ms (one block, 16 columns)"] = round((time.perf_counter() - t0) * 100, 3)
+
+    def A_strong(V):
+        V3 = V.reshape(nb, m, -1)
+        out = V3.clone()
+        for k in range(nb):
+            if k > 0:
+                out[k] += S_l[k].matmat(V3[k - 1])
+            if k < nb - 1:
+                out[k] += S_r[k].matmat(V3[k + 1])
+        return out.reshape(nb * m, -1)
+
+    M = lambda v: rb.solve(v).contiguous()  # noqa: E731
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Xg, info = gmres(A_strong, B, apply_M=M, tol=1e-10, restart=60, maxiter=120)
+    torch.cuda.synchronize()
+    res["gmres_on_strong_system"] = {
+        "preconditioner": "weak-HBS red-black solve", "iterations": info.iterations,
+        "converged": bool(info.converged), "time_s": round(time.perf_counter() - t0, 3),
+        "rel_err_vs_fp64_reference": ((Xg - Xr).norm() / Xr.norm()).item()}
+    del S_l, S_r
+    torch.cuda.empty_cache()
+    return res
 
 
 def c3_bench(dev, seed=0, rk=40, nrhs=16):

@@ -21,8 +21,9 @@ Restates matAssembly/HBS/HBStorch_strong.py (SimonDirckx/SslabLU):
     boxes of a level, from the box bounds.
 
 basis='svd' is the reference's choice of far-field basis; basis='pqr' takes the first rk
-columns of a column-pivoted QR of the same projected sketch instead (the device engine's
-choice); both span the far field exactly when it has rank <= rk.
+columns of a column-pivoted QR of the Gram matrix M M^T of the same projected sketch M
+instead (the device engine's choice); both span the far field exactly when it has rank
+<= rk.
 
 Parity pins (tests/test_oracle_hbs_strong.py): compression of operators that are exactly
 strong-admissible at rank rk (nested bases, random dense near field; the construction of
@@ -127,7 +128,7 @@ def _stack_pinv(S):
 def _far_basis(M, rk, basis):
     if basis == "svd":
         return np.linalg.svd(M, full_matrices=False)[0][:, :rk]
-    Qm, _, _ = sla.qr(M, pivoting=True, mode="economic")
+    Qm, _, _ = sla.qr(M @ M.T, pivoting=True)
     return Qm[:, :rk]
 
 

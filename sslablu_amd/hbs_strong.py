@@ -13,9 +13,10 @@ that share a near-field size k (one launch per group and step):
   * G_i = S_i S_i^T and W_i = Y_i S_i^T (MFMA GEMMs), G_i factored by the batched LU;
     P_i = W_i G_i^{-1} = Y_i pinv(S_i) (S_i has full row rank: Gaussian blocks, s >= k b);
   * the far-field part M_i = Y_i - P_i S_i = Y_i (I - pi_i), pi_i the projector on the
-    row space of S_i; U_i = the first r columns of a column-pivoted QR of M_i (the
-    reference takes the leading left singular vectors; both span the far field when it
-    has rank <= r);
+    row space of S_i; U_i = the first r columns of a column-pivoted QR of the b x b Gram
+    matrix M_i M_i^T (the reference takes the leading left singular vectors of M_i; all
+    span the far field when it has rank <= r, and the Gram form keeps the pivoted QR at
+    b x b, inside one workgroup's LDS, instead of b x s);
   * term1 tiles (I - U_i U_i^T) P_i = (I - U_i U_i^T) A_ij (j in NF(i)); the same on the
     Psi / Z side gives V_j and (I - V_j V_j^T) A_ij^T (i in NF(j)); E_ij = term1 +
     U_i U_i^T A_ij (I - V_j V_j^T);
@@ -165,7 +166,7 @@ class DeviceHBSStrong:
                     Wt = bgemm(Snf, Kg, tb=True)                        # (G, kb, b)
                     Pt, _ = _lu_solve(Gm, Wt)                           # P^T = G^{-1} W^T
                     M = bgemm(Pt, Snf, out=Kg, alpha=-1.0, beta=1.0, ta=True)
-                    Qm, _, _ = bqr(M, pivot=True)
+                    Qm, _, _ = bqr(bgemm(M, M, tb=True), pivot=True)     # b x b Gram
                     Bb = Qm[:, :, :rkm].contiguous()                    # (G, b, r)
                     C = bgemm(Bb, Pt, ta=True, tb=True)                 # (G, r, kb)
                     P = Pt.transpose(1, 2).contiguous()

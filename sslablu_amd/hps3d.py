@@ -697,7 +697,7 @@ class _SubFactor:
 
 
 def compress_s_blocks(op: HPSSlabOperator3D, tree, rk: int, samples: int, seed: int = 0,
-                      sketch_fp32: bool = False):
+                      sketch_fp32: bool = False, strong: bool = False):
     """HBS compressions of every S block of `op` (matAssembly/matAssembler.py:103-133,
     the rkHBS assembler of multislab/oms.py:160-205) with the sketches of all
     interfaces formed together: Y = S Omega and Z = S^T Psi for the nb left blocks in
@@ -706,8 +706,10 @@ def compress_s_blocks(op: HPSSlabOperator3D, tree, rk: int, samples: int, seed: 
     (H_l, H_r): lists of nb HBStorch.HBSMAT (H_l[0] and H_r[nb-1] are the end blocks,
     which couple to the end faces' boundary data, not to an interface).  sketch_fp32:
     the coupling-block products of the sketches on the FP32 MFMA GEMM (the LU solves,
-    QRs and interpolative bases stay FP64)."""
-    from .matAssembly.HBS import HBStorch
+    QRs and interpolative bases stay FP64).  strong: strong-admissibility compressions
+    (the HBStorch_strong mirror; `tree` a slabTree over one interface's points, with its
+    adjacency lists)."""
+    from .matAssembly.HBS import HBStorch, HBStorch_strong
     g = op.g
     nb, m = g.nb, g.m
     dev = op.device
@@ -721,7 +723,10 @@ def compress_s_blocks(op: HPSSlabOperator3D, tree, rk: int, samples: int, seed: 
         Z = op.smaps_apply(side, Psi.expand(nb, m, samples), transpose=True, fp32=sketch_fp32)
         hs = []
         for i in range(nb):
-            h = HBStorch.HBSMAT(op.smap(i, side), dev, tree, quad=False)
+            if strong:
+                h = HBStorch_strong.HBSMAT(device=dev, tree=tree)
+            else:
+                h = HBStorch.HBSMAT(op.smap(i, side), dev, tree, quad=False)
             h.construct(rk, Om, Psi, Y[i], Z[i])
             hs.append(h)
         del Y, Z
