@@ -47,6 +47,13 @@ TRAFFIC = 56811078
 # configuration the bench line times.
 STEP_POLICIES = ("512,512,64,256,3,14,0", "1024,2048,64,256,3,5,1",
                  "512,1024,64,256,3,12,0;512,1024,64,256,10,12,1;1024,512,64,256,3,14,0")
+# Tile policy of the backward sweep run as one dependency-ordered launch, the fastest of
+# a sweep over split tiles, split budgets and part caps (--chain-tune sweeps again).  The
+# default run times it beside the level-by-level sweep and keeps the faster; on MI355X
+# the chained sweep measured about twice as slow (its per-tile device-scope release and
+# acquire fences write back and invalidate the XCD's L2), so the level-by-level launches
+# of STEP_POLICIES stay the timed step.  --no-chain skips the comparison.
+CHAIN_POLICY = "512,256,16,256,3,3,0"
 
 
 class PDO2d:
@@ -530,6 +537,60 @@ def c3_hps_bench(dev, seed=0, rk=40, nrhs=16, leaf=72):
     return out
 
 
+def chain_tune(args, fac, step, stream, r):
+    """Time the step with the backward sweep as one dependency-ordered launch under the
+    pinned (or swept) chain policy against the level-by-level sweep; keep the faster."""
+    import torch
+    from sslablu_amd import tuning
+    from sslablu_amd.blocktri import chain_status
+
+    def step_time(n=40):
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        a, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(n):
+            step()
+        b_.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b_) * 1e3 / n
+
+    table = list(fac.__dict__.setdefault("_bwd_policies", {}).get(r, ()))
+    tuning.set_backward_policies(table)
+    t_levels = step_time()
+    fac.chained_backward = True
+    if args.chain_policy:
+        cands = [tuning.Policy(*[int(v) for v in args.chain_policy.split(",")])]
+    elif CHAIN_POLICY and not args.chain_tune:
+        cands = [tuning.Policy(*[int(v) for v in CHAIN_POLICY.split(",")])]
+    else:
+        cands = [tuning.Policy(512, sw, mk, 256, 3, cfg, 0) for cfg in (3, 4, 5, 9, 12, 13, 14)
+                 for sw in (256, 512, 1024, 2048) for mk in (2, 4, 8, 16)]
+    times = []
+    for q in cands:
+        fac._bwd_policies[r] = [q]
+        tuning.set_backward_policies([q])
+        times.append((step_time(20 if len(cands) > 1 else 40), q))
+    aborted = chain_status()
+    t_chain, best = min(times, key=lambda x: x[0])
+    if len(cands) > 1:
+        fac._bwd_policies[r] = [best]
+        tuning.set_backward_policies([best])
+        t_chain = step_time()
+    keep = t_chain < t_levels and not aborted
+    info = {"level_by_level_us": round(t_levels, 2), "chained_us": round(t_chain, 2),
+            "policy": list(dataclasses.astuple(best)), "candidates": len(cands),
+            "wait_aborted": aborted, "kept": keep}
+    if keep:
+        fac._bwd_policies[r] = [best]
+    else:
+        fac.chained_backward = False
+        fac._bwd_policies[r] = table
+    tuning.set_backward_policies(fac._bwd_policies[r])
+    return info
+
+
 def c3_strong_part(op, g, B, Xr, rb, X, ex, i, rk, leaf, seed):
     """Strong-admissibility compression of the same 30 S blocks (binary face tree of the
     interface nodes, edge neighbours, s = 5 max(2k, leaf) + k + 20 samples), its error on
@@ -741,6 +802,13 @@ def main():
                     help="also run the C3 sizes on one Chebyshev tensor grid per double slab "
                          "(fast diagonalization) with a dense FP64 red-black reference solve")
     ap.add_argument("--no-interior", action="store_true")
+    ap.add_argument("--no-chain", action="store_true",
+                    help="time the backward sweep level by level only (no dependency-ordered "
+                         "launch)")
+    ap.add_argument("--chain-policy", default=None,
+                    help="tile policy of the dependency-ordered backward sweep (7 ints)")
+    ap.add_argument("--chain-tune", action="store_true",
+                    help="sweep the dependency-ordered sweep's tile policy")
     ap.add_argument("--fold-policy", default=None,
                     help="with --policy: the boundary-map launch's own tile policy (7 ints, "
                          "as reported in split_tuning.fold_policy)")
@@ -864,6 +932,11 @@ def main():
     if args.policy and args.bwd_policies:
         fac.__dict__.setdefault("_bwd_policies", {})[NRHS] = [
             tuning.Policy(*[int(v) for v in q.split(",")]) for q in args.bwd_policies.split(";")]
+    # ---- the backward sweep as one dependency-ordered launch (k_gemm_chain), kept when
+    # the whole step is faster than with one launch per level ----
+    chain_info = None
+    if not args.no_chain:
+        chain_info = chain_tune(args, fac, step, stream, NRHS)
     t_tune = time.perf_counter() - t0
     # the exact tile policies the timed step runs under, as the flags that fix them again
     # (profiler runs pass these back so that they trace the configuration timed here)
@@ -875,6 +948,10 @@ def main():
     timed_flags = f"--policy {_pstr(pol_bwd)} --fold-policy {_pstr(pol_fold)}"
     if table:
         timed_flags += " --bwd-policies '" + ";".join(_pstr(q) for q in table) + "'"
+    if fac.chained_backward:
+        timed_flags += " --chain-policy " + _pstr(table[0])
+    else:
+        timed_flags += " --no-chain"
     # the backward sweep's policy and level table stand for the thread, so the solver's
     # per-call policy scopes are no-ops in the timed loop (only the boundary-map launch
     # switches to its own policy and back)
@@ -1067,6 +1144,7 @@ def main():
         "fold_maps_mb": round(fold.nbytes / 2 ** 20, 1),
         "gemm_policy": list(dataclasses.astuple(pol)), "autotune_s": round(t_tune, 3),
         "split_tuning": tune_split,
+        "chained_backward": chain_info,
         "timed_policy_flags": timed_flags,
         "rel_err_vs_fp64_thomas": rel_err,
         "residual": resid,

@@ -156,8 +156,20 @@ int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int
  * counterpart (the reference solve is one piece, direct_solve/omsdirectsolve.py:221-279). */
 #define SSLAB_RB_FORWARD_ONLY 4
 #define SSLAB_RB_BACKWARD_ONLY 8
+/* bit 4 (SSLAB_RB_CHAINED_BACKWARD): run the backward sweep as one dependency-ordered
+ * launch (every level's tiles in one grid; a tile starts once the blocks it reads are
+ * final, tracked by per-GPU device counters) when the system has at most 128 blocks,
+ * else level by level as without the flag.  Chained solves on one GPU must not overlap
+ * in time (they share the counters).  No reference counterpart (a launch-structure
+ * choice for the sweep of direct_solve/omsdirectsolve.py:221-279). */
+#define SSLAB_RB_CHAINED_BACKWARD 16
 int sslab_rb_solve_ex(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
                       double* X, int ldx, int flags, void* stream);
+
+/* Whether a chained backward sweep on the current GPU gave up waiting since the last
+ * call (*aborted = 1: a tile's bounded wait ran out and that solve's result is not
+ * valid).  Synchronises the device and clears the flag.  No reference counterpart. */
+int sslab_rb_chain_status(int* aborted);
 
 /* Several independent systems (any mix of nb, m, nrhs) solved together: each
  * elimination level of all systems with nrhs <= 32 shares one grouped GEMM launch;

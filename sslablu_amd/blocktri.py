@@ -17,6 +17,7 @@ x = y - Z K [y_{nb-1}; y_0].
 """
 from __future__ import annotations
 
+import ctypes
 import warnings
 
 import torch
@@ -55,6 +56,7 @@ class BlockTriFactor:
         self.info = None
         self.cyclic = False
         self.ident0 = False  # red-black factor of identity diagonal blocks (T = None)
+        self.chained_backward = False  # backward sweep as one dependency-ordered launch
         self.Z = None
         self.K = None
 
@@ -135,7 +137,9 @@ class BlockTriFactor:
         right-hand sides, X's even rows zero); red-black factors built with T = None
         only.  phase "forward": the forward sweep and coarsest level only, leaving the
         backward sweep's entry state in X; "backward": X holds that state (V unused, may be
-        None) -- red-black factors only."""
+        None) -- red-black factors only.  With the attribute chained_backward set, a
+        red-black backward sweep runs as one dependency-ordered launch
+        (SSLAB_RB_CHAINED_BACKWARD); chained solves on one GPU must not overlap in time."""
         nb, m = self.nb, self.m
         N = nb * m
         if phase not in ("all", "forward", "backward"):
@@ -163,7 +167,8 @@ class BlockTriFactor:
             return X
         s = stream_handle(stream)
         flags = ((1 if self.ident0 else 0) | (2 if level0_reduced else 0) |
-                 {"all": 0, "forward": 4, "backward": 8}[phase])
+                 {"all": 0, "forward": 4, "backward": 8}[phase] |
+                 (16 if getattr(self, "chained_backward", False) else 0))
         bwd = getattr(self, "_bwd_policies", {}).get(r) if self.method == "rb" else None
         with tuning.policy(tuning.lookup(self.tune_key(r))), tuning.backward_policies(bwd):
             if self.method == "rb":
@@ -446,3 +451,11 @@ def solve_multi(factors, Vs, Xs, stream=None):
          stream_handle(stream))
     return Xs
 
+
+def chain_status() -> bool:
+    """True when a chained backward sweep on the current GPU gave up waiting since the
+    last call (its solve is then invalid); synchronises the device and clears the flag
+    (sslab_rb_chain_status)."""
+    v = ctypes.c_int(0)
+    call("sslab_rb_chain_status", ctypes.addressof(v))
+    return bool(v.value)

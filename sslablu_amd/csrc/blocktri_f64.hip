@@ -32,6 +32,7 @@
 #include "blocktri_f64.hpp"
 
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "gemm.hpp"
@@ -358,6 +359,8 @@ struct RbSolveSys {
                 // the backward sweep's entry state (y_j at every level's odd nodes, x at
                 // the coarsest level's nodes); 2: backward sweep only, from that state
   int top_max;  // the plan's coarsest-level limit (0: the default, rb_top_max(m))
+  bool chain;   // run the backward sweep as one dependency-ordered launch (FP64, one
+                // system, at most kChainBlocks blocks); otherwise one launch per level
 };
 
 // Tile policies of the backward sweep's steps (step u runs level L-1-u), per calling
@@ -491,9 +494,20 @@ static int rb_top(GemmLaunchT<T>& G, const RbSolveSys<T>& S, hipStream_t st) {
   return kOk;
 }
 
-// Backward sweep of level l:  x_j = y_j - (Tinv SiM)_j x_{j-1} - (Tinv SiP)_j x_{j+1}.
+// A backward-sweep descriptor with the row blocks of X it writes and reads (block i =
+// rows [i m, (i + 1) m)), in the encoding of GemmChainT (index + 1, 0: none).
 template <typename T>
-static int rb_bwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStream_t st) {
+struct BwdDesc {
+  GemmDescT<T> d;
+  int sig1, sig_stride;
+  int dep1[kMaxTerms], dep_stride[kMaxTerms];
+};
+
+// Backward sweep of level l:  x_j = y_j - (Tinv SiM)_j x_{j-1} - (Tinv SiP)_j x_{j+1}.
+// Odd node c of level l is block 2^l + c 2^(l+1); its neighbours are blocks c 2^(l+1)
+// and (c + 1) 2^(l+1).  Returns the number of descriptors written to out (at most 2).
+template <typename T>
+static int rb_bwd_descs(const RbSolveSys<T>& S, int l, BwdDesc<T>* out) {
   const int m = S.m, r = S.r, ldx = S.ldx;
   T* X = S.X;
   const long long m2 = (long long)m * m;
@@ -503,28 +517,128 @@ static int rb_bwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStr
   const T* TG = S.factor + v.tg_off;
   const long long sx = (2LL << l) * bx;
   const long long ox = (1LL << l) * bx;
-  const int nboth = (n % 2 == 0) ? nO - 1 : nO;  // odd nodes with both neighbours
-  // y_j: in X (written by the forward sweep or, for rb0 / a backward-only phase, by the
-  // caller), or v_j itself in V when Tinv_j = I.
+  const int nboth = (n % 2 == 0) ? nO - 1 : nO;
   const bool from_v = (l == 0 && S.ident0 && !S.rb0 && S.phase == 0);
   const T* Y = from_v ? (const T*)S.V : (const T*)X;
   const int ldy = from_v ? S.ldv : ldx;
   const long long by = (long long)m * ldy;
   const long long sy = (2LL << l) * by, oy = (1LL << l) * by;
+  const int half = 1 << l, step = 2 << l;
+  int k = 0;
   if (nboth > 0) {
-    GemmDescT<T> d = gemm_desc<T>(m, r, nboth, X + ox, ldx, sx, Y + oy, ldy, sy, 1.0);
-    gemm_add_term<T>(d, gemm_term<T>(TG + m, 3 * m, 3 * m2, 0, X, ldx, sx, 0, m, -1.0));
-    gemm_add_term<T>(d, gemm_term<T>(TG + 2 * m, 3 * m, 3 * m2, 0, X + sx, ldx, sx, 0, m, -1.0));
-    SSLAB_TRY(rb_push(G, d, st));
+    BwdDesc<T>& e = out[k++];
+    e = BwdDesc<T>{};
+    e.d = gemm_desc<T>(m, r, nboth, X + ox, ldx, sx, Y + oy, ldy, sy, 1.0);
+    gemm_add_term<T>(e.d, gemm_term<T>(TG + m, 3 * m, 3 * m2, 0, X, ldx, sx, 0, m, -1.0));
+    gemm_add_term<T>(e.d, gemm_term<T>(TG + 2 * m, 3 * m, 3 * m2, 0, X + sx, ldx, sx, 0, m, -1.0));
+    e.sig1 = half + 1;
+    e.sig_stride = step;
+    e.dep1[0] = 1;
+    e.dep_stride[0] = step;
+    e.dep1[1] = step + 1;
+    e.dep_stride[1] = step;
   }
   if (nboth < nO) {  // last odd node: left neighbour only
     const long long c = nO - 1;
-    GemmDescT<T> d = gemm_desc<T>(m, r, 1, X + ox + c * sx, ldx, sx, Y + oy + c * sy, ldy, sy,
-                                  1.0);
-    gemm_add_term<T>(d, gemm_term<T>(TG + c * 3 * m2 + m, 3 * m, 3 * m2, 0, X + c * sx, ldx, sx, 0,
-                                     m, -1.0));
-    SSLAB_TRY(rb_push(G, d, st));
+    BwdDesc<T>& e = out[k++];
+    e = BwdDesc<T>{};
+    e.d = gemm_desc<T>(m, r, 1, X + ox + c * sx, ldx, sx, Y + oy + c * sy, ldy, sy, 1.0);
+    gemm_add_term<T>(e.d, gemm_term<T>(TG + c * 3 * m2 + m, 3 * m, 3 * m2, 0, X + c * sx, ldx,
+                                       sx, 0, m, -1.0));
+    e.sig1 = half + (int)c * step + 1;
+    e.dep1[0] = (int)c * step + 1;
   }
+  return k;
+}
+
+template <typename T>
+static int rb_bwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStream_t st) {
+  BwdDesc<T> e[2];
+  const int k = rb_bwd_descs(S, l, e);
+  for (int i = 0; i < k; ++i) SSLAB_TRY(rb_push(G, e[i].d, st));
+  return kOk;
+}
+
+// Device counters of the dependency-ordered backward sweep, one set per GPU (allocated
+// and cleared on first use; the kernel leaves them at zero).  Chained sweeps on one GPU
+// must not overlap in time.
+constexpr int kChainDevices = 64;
+static std::mutex g_chain_mu;
+static int* g_chain_mem[kChainDevices] = {};
+
+static int chain_counters(int** out) {
+  int dev = 0;
+  SSLAB_HIP_CHECK(hipGetDevice(&dev));
+  SSLAB_REQUIRE(dev >= 0 && dev < kChainDevices, "rb chain: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_chain_mu);
+  if (g_chain_mem[dev] == nullptr) {
+    int* p = nullptr;
+    SSLAB_HIP_CHECK(hipMalloc(&p, sizeof(int) * (kChainBlocks + 4)));
+    const hipError_t e = hipMemset(p, 0, sizeof(int) * (kChainBlocks + 4));
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      SSLAB_HIP_CHECK(e);
+    }
+    g_chain_mem[dev] = p;
+  }
+  *out = g_chain_mem[dev];
+  return kOk;
+}
+
+// Backward sweep of one system as a single dependency-ordered launch (k_gemm_chain):
+// the levels' descriptors coarsest first, each tile waiting for the blocks it reads.
+// *done stays false when the system does not fit a chain (more than kChainBlocks blocks
+// or kChainDesc descriptors); the caller then runs the per-level launches.  The tile
+// policy is the first backward-step policy when a table is set, else the thread's.
+static int rb_bwd_chain(const RbSolveSys<double>& S, hipStream_t st, bool* done) {
+  *done = false;
+  if (S.nb > kChainBlocks || S.p.L < 1) return kOk;
+  GemmChainT<double> C{};
+  C.nd = 0;
+  for (int u = 0; u < S.p.L; ++u) {
+    BwdDesc<double> e[2];
+    const int k = rb_bwd_descs(S, S.p.L - 1 - u, e);
+    if (C.nd + k > kChainDesc) return kOk;
+    for (int i = 0; i < k; ++i) {
+      C.d[C.nd] = e[i].d;
+      C.sig1[C.nd] = e[i].sig1;
+      C.sig_stride[C.nd] = e[i].sig_stride;
+      for (int t = 0; t < kMaxTerms; ++t) {
+        C.dep1[C.nd][t] = e[i].dep1[t];
+        C.dep_stride[C.nd][t] = e[i].dep_stride[t];
+      }
+      ++C.nd;
+    }
+  }
+  int* mem = nullptr;
+  SSLAB_TRY(chain_counters(&mem));
+  C.cnt = mem;
+  C.ctl = mem + kChainBlocks;
+  C.spin_limit = 1 << 18;
+  const GemmPolicy saved = gemm_get_policy();
+  if (g_bwd_npolicy > 0) gemm_set_policy(g_bwd_policy[0]);
+  const int rc = gemm_chain_launch<double>(C, st);
+  if (g_bwd_npolicy > 0) gemm_set_policy(saved);
+  SSLAB_TRY(rc);
+  *done = true;
+  return kOk;
+}
+
+int rb_chain_status(int* aborted) {
+  SSLAB_REQUIRE(aborted != nullptr, "rb_chain_status: null output");
+  int dev = 0;
+  SSLAB_HIP_CHECK(hipGetDevice(&dev));
+  SSLAB_REQUIRE(dev >= 0 && dev < kChainDevices, "rb chain: device %d out of range", dev);
+  int* p = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_chain_mu);
+    p = g_chain_mem[dev];
+  }
+  *aborted = 0;
+  if (p == nullptr) return kOk;
+  SSLAB_HIP_CHECK(hipDeviceSynchronize());
+  SSLAB_HIP_CHECK(hipMemcpy(aborted, p + kChainBlocks + 2, sizeof(int), hipMemcpyDeviceToHost));
+  if (*aborted) SSLAB_HIP_CHECK(hipMemset(p + kChainBlocks + 2, 0, sizeof(int)));
   return kOk;
 }
 
@@ -574,6 +688,13 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
     SSLAB_TRY(copy_block_batched<T>(S.m, S.r, S.V + bv, S.ldv, 2 * bv, S.X + bx, S.ldx, 2 * bx,
                                     S.p.lv[0].nO, st));
   }
+  if constexpr (std::is_same<T, double>::value) {
+    if (nsys == 1 && sys[0].chain && sys[0].r > 0 && sys[0].phase != 1) {
+      bool done = false;
+      SSLAB_TRY(rb_bwd_chain(sys[0], st, &done));
+      if (done) return kOk;
+    }
+  }
   const GemmPolicy saved = gemm_get_policy();
   for (int u = 0; u < Lmax; ++u) {
     if (u < g_bwd_npolicy) gemm_set_policy(g_bwd_policy[u]);
@@ -601,14 +722,15 @@ static int rb_solve_sub(int nb, int m, int r, const T* factor, T* V, int ldv, T*
 
 template <typename T>
 static int rb_solve_t(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
-                      hipStream_t st, bool ident0 = false, bool rb0 = false, int phase = 0) {
-  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0, rb0, phase, 0};
+                      hipStream_t st, bool ident0 = false, bool rb0 = false, int phase = 0,
+                      bool chain = false) {
+  RbSolveSys<T> S{nb, m, r, factor, V, ldv, X, ldx, RbPlan{}, ident0, rb0, phase, 0, chain};
   return rb_solve_multi_t<T>(1, &S, st);
 }
 
 int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, double* X,
                 int ldx, int flags, hipStream_t st) {
-  SSLAB_REQUIRE((flags & ~15) == 0, "rb_solve: unknown flags 0x%x", flags);
+  SSLAB_REQUIRE((flags & ~31) == 0, "rb_solve: unknown flags 0x%x", flags);
   SSLAB_REQUIRE(!(flags & 2) || ((flags & 1) && nb >= 2),
                 "rb_solve: the level-0-reduced entry state needs identity diagonal blocks and "
                 "nb >= 2");
@@ -617,7 +739,7 @@ int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, 
                 "rb_solve: backward-only already starts from the forward sweep's result");
   const int phase = (flags & 4) ? 1 : ((flags & 8) ? 2 : 0);
   return rb_solve_t<double>(nb, m, r, factor, V, ldv, X, ldx, st, (flags & 1) != 0,
-                            (flags & 2) != 0, phase);
+                            (flags & 2) != 0, phase, (flags & 16) != 0);
 }
 
 int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,

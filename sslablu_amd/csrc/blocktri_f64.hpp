@@ -27,6 +27,9 @@ int rb_get_backward_policy_count();
 
 int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, double* X,
                 int ldx, int flags, hipStream_t st);
+// Abort flag of the dependency-ordered backward sweeps on the current GPU (synchronises
+// the device; clears the flag).
+int rb_chain_status(int* aborted);
 int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,
                    double* const* V, const int* ldv, double* const* X, const int* ldx,
                    hipStream_t st);

@@ -197,6 +197,8 @@ int sslab_rb_solve_ex(int nb, int m, int nrhs, const double* factor, double* V, 
   return sslab::rb_solve_ex(nb, m, nrhs, factor, V, ldv, X, ldx, flags, as_stream(stream));
 }
 
+int sslab_rb_chain_status(int* aborted) { return sslab::rb_chain_status(aborted); }
+
 int sslab_rb_solve_multi(int nsys, const int* nb, const int* m, const int* nrhs,
                          const double* const* factor, double* const* V, const int* ldv,
                          double* const* X, const int* ldx, void* stream) {

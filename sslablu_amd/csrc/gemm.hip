@@ -556,22 +556,15 @@ __device__ __forceinline__ void fast_desc_pipe(const GemmDescT<T>& D, int b, int
 // takes 268 and runs one wave per SIMD.
 // kAligned: the host has checked that every term takes 16-byte loads for A and B (then
 // only that path is compiled, which keeps the register count of the common case low).
+// One output tile of k_gemm_fast: tile index `tile` of the launch, which belongs to
+// descriptor D (D.tile_begin <= tile).
 template <class Cfg, typename T, int kMode>
-__global__ void __launch_bounds__(Cfg::NT) __attribute__((amdgpu_waves_per_eu(2)))
-k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
+__device__ __forceinline__ void fast_tile(const GemmDescT<T>& D, int tile, T* smem) {
   constexpr bool kAligned = kMode >= 1;
   using G = FastGeom<Cfg, T>;
   using M4 = Mfma<T>;
   using v4 = typename M4::v4;
   constexpr int VEC = G::VEC;
-  __shared__ __attribute__((aligned(16))) T smem[2 * G::STAGE];
-
-  const int tile = xcd_remap(blockIdx.x, total_tiles);
-  int di = 0;
-#pragma unroll
-  for (int d = 1; d < kMaxDesc; ++d)
-    if (d < L.nd && tile >= L.d[d].tile_begin) di = d;
-  const GemmDescT<T>& D = L.d[di];
 
   const int per_part = D.tiles_m * D.tiles_n;
   int local = tile - D.tile_begin;
@@ -659,6 +652,110 @@ k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
     }
   }
 }
+
+template <class Cfg, typename T, int kMode>
+__global__ void __launch_bounds__(Cfg::NT) __attribute__((amdgpu_waves_per_eu(2)))
+k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
+  __shared__ __attribute__((aligned(16))) T smem[2 * FastGeom<Cfg, T>::STAGE];
+  const int tile = xcd_remap(blockIdx.x, total_tiles);
+  int di = 0;
+#pragma unroll
+  for (int d = 1; d < kMaxDesc; ++d)
+    if (d < L.nd && tile >= L.d[d].tile_begin) di = d;
+  fast_tile<Cfg, T, kMode>(L.d[di], tile, smem);
+}
+
+// ---- dependency-ordered launches -------------------------------------------------------
+//
+// k_gemm_chain runs the tiles of several dependent product levels in one launch.  Tiles
+// are taken in launch order from a ticket counter (L.ctl[0]), so when a workgroup holds
+// tile t every tile below t is held by a workgroup that is already running.  A tile
+// first waits until, for each block its terms read (dep1 / dep_stride), the counter of
+// that block has reached the number of this launch's tiles that write it (need[]); the
+// writers all come before it in launch order, so the wait always ends.  After its
+// stores and atomics it adds one to the counter of the block it writes (sig1 /
+// sig_stride).  Waits are bounded: past spin_limit polls the workgroup sets the abort
+// flag L.ctl[2] and goes on (its result is then wrong and the host reports it).  The
+// last workgroup to finish clears the ticket, the finish count and every block counter,
+// so the next launch starts from zero without a fill kernel.
+//
+// Memory order: every wave of the writer issues a device-scope release fence (waits for
+// its own stores and atomics, writes the XCD's L2 back) before the workgroup barrier
+// after which one thread increments the counter; the reader polls with relaxed
+// device-scope loads and every wave issues one device-scope acquire fence (L1 / L2
+// invalidation) after the wait.
+__device__ __forceinline__ int chain_load(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class Cfg, typename T, int kMode>
+__global__ void __launch_bounds__(Cfg::NT) __attribute__((amdgpu_waves_per_eu(2)))
+k_gemm_chain(const GemmChainT<T> L, int total_tiles) {
+  __shared__ __attribute__((aligned(16))) T smem[2 * FastGeom<Cfg, T>::STAGE];
+  __shared__ int s_tile;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_tile = __hip_atomic_fetch_add(&L.ctl[0], 1, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int tile = s_tile;
+  if (tile < 0 || tile >= total_tiles) {  // counters not at zero on entry: give up
+    if (tid == 0) {
+      __hip_atomic_store(&L.ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int done = __hip_atomic_fetch_add(&L.ctl[1], 1, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+      if (done == total_tiles - 1) {
+        for (int i = 0; i < kChainBlocks; ++i)
+          __hip_atomic_store(&L.cnt[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&L.ctl[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&L.ctl[1], 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    return;
+  }
+  int di = 0;
+#pragma unroll
+  for (int d = 1; d < kChainDesc; ++d)
+    if (d < L.nd && tile >= L.d[d].tile_begin) di = d;
+  const GemmDescT<T>& D = L.d[di];
+  const int b = (tile - D.tile_begin) / (D.tiles_m * D.tiles_n * D.ksplit);
+  if (tid == 0) {
+    for (int t = 0; t < D.nterms; ++t) {
+      if (L.dep1[di][t] == 0) continue;
+      const int blk = L.dep1[di][t] - 1 + b * L.dep_stride[di][t];
+      const int need = L.need[blk];
+      int it = 0;
+      while (chain_load(&L.cnt[blk]) < need) {
+        if (++it > L.spin_limit || chain_load(&L.ctl[2]) != 0) {
+          __hip_atomic_store(&L.ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // drop stale L1 / L2 lines
+  fast_tile<Cfg, T, kMode>(D, tile, smem);
+  // every wave waits for its own stores / atomics and writes the XCD's L2 back, then one
+  // thread signals
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (tid == 0) {
+    if (L.sig1[di] != 0) {
+      __hip_atomic_fetch_add(&L.cnt[L.sig1[di] - 1 + b * L.sig_stride[di]], 1,
+                             __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const int done = __hip_atomic_fetch_add(&L.ctl[1], 1, __ATOMIC_ACQ_REL,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+    if (done == total_tiles - 1) {  // every other tile has finished: reset for the next launch
+      for (int i = 0; i < kChainBlocks; ++i)
+        __hip_atomic_store(&L.cnt[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&L.ctl[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&L.ctl[1], 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 
 // ---- narrow N: A streamed from HBM straight into MFMA operand registers ----------------
 //
@@ -851,8 +948,8 @@ k_gemm_adirect(const GemmLaunchT<T> L, int total_tiles) {
 // 1: the vector-load kernel, per-term choice of 16-byte or element loads;
 // 2: the vector-load kernel with 16-byte loads everywhere (aligned operands, leading
 //    dimensions, every K and N multiples of the vector width).
-template <typename T>
-static int fast_kind(const GemmLaunchT<T>& P) {
+template <typename T, class LaunchT>
+static int fast_kind(const LaunchT& P) {
   constexpr int VEC = 16 / (int)sizeof(T);
   static const bool enabled = [] {
     const char* s = getenv("SSLAB_GEMM_FAST");
@@ -941,8 +1038,8 @@ static long long total_k(const GemmDescT<T>& d) {
 // budget.  Descriptors of equal K are then split ksplit ways; in a launch of mixed K the
 // long products get more parts than the short ones, so no workgroup runs much longer than
 // the rest.  Returns 0 when nothing is worth splitting.
-template <class Cfg, typename T>
-static long long split_chunk(const GemmLaunchT<T>& L, int ksplit, int maxparts) {
+template <class Cfg, typename T, class LaunchT>
+static long long split_chunk(const LaunchT& L, int ksplit, int maxparts) {
   long long tiles = 0, kmax = 0;
   for (int i = 0; i < L.nd; ++i) {
     const GemmDescT<T>& d = L.d[i];
@@ -1025,7 +1122,7 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit, int max
   if constexpr (kADirect && ADGeom<Cfg, T>::ok) {
     int nmax = 0;
     for (int i = 0; i < P.nd; ++i) nmax = P.d[i].N > nmax ? P.d[i].N : nmax;
-    if (nmax <= Cfg::BN && fast_kind(P) == 2) {
+    if (nmax <= Cfg::BN && fast_kind<T>(P) == 2) {
       hipLaunchKernelGGL((k_gemm_adirect<Cfg, T>), dim3(total), dim3(Cfg::NT), 0, stream, P,
                          total);
       SSLAB_HIP_CHECK(hipGetLastError());
@@ -1033,7 +1130,7 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit, int max
     }
   }
   if constexpr (FastGeom<Cfg, T>::ok) {
-    const int kind = fast_kind(P);
+    const int kind = fast_kind<T>(P);
     if (kind == 2) {
       static const bool pipe = [] {
         const char* e = getenv("SSLAB_GEMM_PIPE");
@@ -1147,6 +1244,111 @@ int gemm_launch(GemmLaunchT<T>& L, hipStream_t stream) {
   if (count_tiles(L, 64, 32) >= target) return launch_id<T>(4, L, stream, 1);
   return launch_id<T>(5, L, stream, 1);
 }
+
+// ---- dependency-ordered launch (k_gemm_chain) ------------------------------------------
+
+template <class Cfg, typename T>
+static int chain_cfg(GemmChainT<T>& L, hipStream_t stream, int ksplit, int maxparts) {
+  if constexpr (!FastGeom<Cfg, T>::ok) {
+    set_error("gemm chain: tile without the fast kernel");
+    return kBadArgument;
+  } else {
+    const long long chunk = ksplit > 1 ? split_chunk<Cfg, T>(L, ksplit, maxparts) : 0;
+    int total = 0;
+    for (int i = 0; i < kChainBlocks; ++i) L.need[i] = 0;
+    for (int i = 0; i < L.nd; ++i) {
+      GemmDescT<T>& d = L.d[i];
+      SSLAB_REQUIRE(d.batch > 0 && d.M > 0 && d.N > 0, "gemm chain: empty descriptor");
+      const bool inplace = d.Cin == d.C && d.beta == T(1) && d.ldcin == d.ldc &&
+                           d.strideCin == d.strideC;
+      // only in-place products split K (their parts add atomically into C); any other is
+      // written once by plain stores, C = beta Cin + sum of terms, with no init pass
+      const bool single = d.nosplit || !inplace;
+      const long long K = total_k(d);
+      long long parts = chunk > 0 && !single ? (K + chunk - 1) / chunk : 1;
+      parts = parts < 1 ? 1 : (parts > maxparts ? maxparts : parts);
+      d.kchunk = (ceil_div((int)K, (int)parts) + Cfg::BK - 1) / Cfg::BK * Cfg::BK;
+      if (d.kchunk < Cfg::BK) d.kchunk = Cfg::BK;
+      d.ksplit = parts > 1 ? ceil_div((int)K, d.kchunk) : 1;
+      if (d.ksplit == 1) d.kchunk = (int)(K > 0 ? K : 1);
+      SSLAB_REQUIRE(d.ksplit == 1 || inplace, "gemm chain: split-K needs an in-place product");
+      d.tiles_m = ceil_div(d.M, Cfg::BM);
+      d.tiles_n = ceil_div(d.N, Cfg::BN);
+      d.tile_begin = total;
+      const int per = d.tiles_m * d.tiles_n * d.ksplit;
+      total += per * d.batch;
+      for (int t = 0; t < d.nterms; ++t)
+        SSLAB_REQUIRE(L.dep1[i][t] == 0 ||
+                          (L.dep1[i][t] >= 1 &&
+                           L.dep1[i][t] - 1 + (long long)(d.batch - 1) * L.dep_stride[i][t] <
+                               kChainBlocks),
+                      "gemm chain: dependency block out of range");
+      if (L.sig1[i] != 0) {
+        SSLAB_REQUIRE(L.sig1[i] >= 1 &&
+                          L.sig1[i] - 1 + (long long)(d.batch - 1) * L.sig_stride[i] < kChainBlocks,
+                      "gemm chain: signalled block out of range");
+        for (int b = 0; b < d.batch; ++b) L.need[L.sig1[i] - 1 + b * L.sig_stride[i]] += per;
+      }
+    }
+    // a tile may only wait on blocks written by earlier descriptors of the launch
+    for (int i = 0; i < L.nd; ++i)
+      for (int t = 0; t < L.d[i].nterms; ++t) {
+        if (L.dep1[i][t] == 0) continue;
+        for (int j = i; j < L.nd; ++j)
+          for (int b = 0; b < L.d[i].batch && L.sig1[j] != 0; ++b)
+            for (int c = 0; c < L.d[j].batch; ++c)
+              SSLAB_REQUIRE(L.dep1[i][t] - 1 + b * L.dep_stride[i][t] !=
+                                L.sig1[j] - 1 + c * L.sig_stride[j],
+                            "gemm chain: descriptor %d waits on a block written by %d", i, j);
+      }
+    if (total == 0) return kOk;
+    const int kind = fast_kind<T>(L);
+    if (kind == 2)
+      hipLaunchKernelGGL((k_gemm_chain<Cfg, T, 2>), dim3(total), dim3(Cfg::NT), 0, stream, L,
+                         total);
+    else
+      hipLaunchKernelGGL((k_gemm_chain<Cfg, T, 0>), dim3(total), dim3(Cfg::NT), 0, stream, L,
+                         total);
+    SSLAB_HIP_CHECK(hipGetLastError());
+    return kOk;
+  }
+}
+
+template <typename T>
+int gemm_chain_launch(GemmChainT<T>& L, hipStream_t stream) {
+  SSLAB_REQUIRE(L.nd >= 1 && L.nd <= kChainDesc, "gemm chain: bad descriptor count %d", L.nd);
+  SSLAB_REQUIRE(L.cnt && L.ctl && L.spin_limit > 0, "gemm chain: missing counters");
+  for (int i = 0; i < L.nd; ++i) {
+    const GemmDescT<T>& d = L.d[i];
+    SSLAB_REQUIRE(d.nterms >= 1 && d.nterms <= kMaxTerms, "gemm chain: bad term count");
+    for (int t = 0; t < d.nterms; ++t)
+      SSLAB_REQUIRE(d.term[t].A && d.term[t].B && d.term[t].K >= 0, "gemm chain: bad term");
+  }
+  const GemmPolicy& P = g_policy;
+  const int ks = P.max_ksplit < 1 ? 1 : P.max_ksplit;
+  long long tiles = 0;
+  for (int i = 0; i < L.nd; ++i)
+    tiles += (long long)ceil_div(L.d[i].M, kCfgBM[P.cfg_split]) *
+             ceil_div(L.d[i].N, kCfgBN[P.cfg_split]) * L.d[i].batch;
+  const long long want = (P.split_wgs + tiles - 1) / (tiles > 0 ? tiles : 1);
+  const int split = (int)(want < ks ? want : ks);
+  switch (P.cfg_split) {
+    case 2: return chain_cfg<Cfg64, T>(L, stream, split, ks);
+    case 3: return chain_cfg<Cfg64s, T>(L, stream, split, ks);
+    case 4: return chain_cfg<Cfg64x32, T>(L, stream, split, ks);
+    case 5: return chain_cfg<Cfg32, T>(L, stream, split, ks);
+    case 9: return chain_cfg<Cfg32w, T>(L, stream, split, ks);
+    case 12: return chain_cfg<Cfg32k, T>(L, stream, split, ks);
+    case 13: return chain_cfg<Cfg32k64, T>(L, stream, split, ks);
+    case 14: return chain_cfg<Cfg64x32k, T>(L, stream, split, ks);
+    default:
+      set_error("gemm chain: tile id %d is not available for dependency-ordered launches",
+                P.cfg_split);
+      return kBadArgument;
+  }
+}
+
+template int gemm_chain_launch<double>(GemmChainT<double>&, hipStream_t);
 
 template <typename T>
 int gemm_strided(int transA, int transB, int M, int N, int K, typename same_t<T>::type alpha,

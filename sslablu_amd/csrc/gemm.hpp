@@ -62,6 +62,33 @@ struct GemmLaunchT {
   int nd;
 };
 
+// A dependency-ordered launch (k_gemm_chain): up to kChainDesc descriptors whose tiles
+// run in descriptor order inside one launch.  Batch element b of descriptor i writes the
+// output block sig1[i] - 1 + b * sig_stride[i] (sig1 = 0: none) and reads, through term
+// t's B operand, block dep1[i][t] - 1 + b * dep_stride[i][t] (dep1 = 0: none); a tile
+// starts once every block it reads has been written by all of the launch's tiles that
+// write it (need[], filled in by gemm_chain_launch).  Blocks may only be read by later
+// descriptors than the ones that write them.  cnt (kChainBlocks ints) and ctl (3 ints:
+// ticket, finished tiles, abort flag) are device memory that starts at zero; the kernel
+// leaves the counters at zero again (the abort flag stays set until the host clears it).
+// Launches that share cnt / ctl must not overlap.
+constexpr int kChainDesc = 8;
+constexpr int kChainBlocks = 128;
+
+template <typename T>
+struct GemmChainT {
+  GemmDescT<T> d[kChainDesc];
+  int nd;
+  int sig1[kChainDesc];
+  int sig_stride[kChainDesc];
+  int dep1[kChainDesc][kMaxTerms];
+  int dep_stride[kChainDesc][kMaxTerms];
+  int need[kChainBlocks];
+  int* cnt;
+  int* ctl;
+  int spin_limit;  // polls of a counter before a tile gives up and sets the abort flag
+};
+
 using GemmTerm = GemmTermT<double>;
 using GemmDesc = GemmDescT<double>;
 using GemmLaunch = GemmLaunchT<double>;
@@ -107,6 +134,11 @@ GemmPolicy gemm_get_policy();
 // batch == 0, M == 0 or N == 0 are skipped.  Returns a Status.  (double, float)
 template <typename T>
 int gemm_launch(GemmLaunchT<T>& L, hipStream_t stream);
+
+// Launch a dependency-ordered chain (FP64) on `stream` with the current policy's split
+// tile (cfg_split: ids 2-5, 9, 12-14), split budget (split_wgs) and part cap (max_ksplit).
+template <typename T>
+int gemm_chain_launch(GemmChainT<T>& L, hipStream_t stream);
 
 // Convenience: single strided-batched product with one term.
 template <typename T>
