@@ -10,8 +10,14 @@ leaf half-sizes, as in the reference drivers (test/3D/cubeTest.py:114-118).
 solverWrapper.construct(geom, PDE) discretizes one double slab and exposes the
 reference's attributes on its skeleton: XX (node coordinates), Ib (the slab boundary,
 ordered left, right, bottom, top), Ii (the interior skeleton: the centre line), plus
-smap(J) = (A_ii^{-1} A_ib[:, J])[Ic] as device tensors.  solver_ii (the full interior
-solve) belongs to interior reconstruction, which is not provided.
+smap(J) = (A_ii^{-1} A_ib[:, J])[Ic] as device tensors.  The skeleton system is the
+flux balance of the two single slabs' DtN maps T^a, T^b on the centre line c:
+    A_ii = T^a_cc + T^b_cc,   A_ib = [T^a_cb | T^b_cb]   (columns in Ib order),
+the static condensation of the HPS leaves onto it (the reference's hpsalt condenses onto
+all leaf faces; the maps it yields, smap and the rhs, are the same).  solver_ii is
+A_ii^{-1} as a linear operator (its LU, batched-LU kernels; modes N and T), Aib the
+coupling block; in 2-D also Aii, Abi and Abb (the boundary rows of the same condensed
+system).  ('spectral' slabs expose smap only.)
 """
 from __future__ import annotations
 
@@ -152,7 +158,163 @@ class solverWrapper:
             return Y[:, m:2 * m]
         return Y[:, 2 * m:]
 
+    # -- the condensed skeleton system (solver/solver.py:100-144 attributes) ------------
+    def _skeleton2d(self):
+        """Aii, Aib, Abi, Abb of a 2-D HPS double slab from its single slabs' DtN maps."""
+        if getattr(self, "_sk", None) is None:
+            from ..hps import pair_maps
+            g = self.op.g
+            dev = self.op.device
+            Ta, Tb = self.op.Ts[0], self.op.Ts[1]
+            ca, cb, cola, colb = (torch.as_tensor(x, device=dev)
+                                  for x in pair_maps(g.nx, g.ny, g.p))
+            ina, inb = cola >= 0, colb >= 0
+            ja, jb = cola.clamp(min=0), colb.clamp(min=0)
+            Aii = Ta[ca][:, ca] + Tb[cb][:, cb]
+            Aib = Ta[ca][:, ja] * ina + Tb[cb][:, jb] * inb
+            Abi = Ta[ja][:, ca] * ina[:, None] + Tb[jb][:, cb] * inb[:, None]
+            Abb = (Ta[ja][:, ja] * (ina[:, None] & ina[None, :])
+                   + Tb[jb][:, jb] * (inb[:, None] & inb[None, :]))
+            self._sk = (Aii.contiguous(), Aib.contiguous(), Abi.contiguous(), Abb.contiguous())
+        return self._sk
+
+    def _require_hps(self):
+        if not self.constructed:
+            raise ValueError("solverWrapper: call construct first")
+        if self.type == "spectral":
+            raise NotImplementedError("'spectral' slabs expose the S maps (smap) only")
+
+    @property
+    def Aii(self):
+        self._require_hps()
+        if self.ndim == 3:
+            raise NotImplementedError("3-D: A_ii is kept only as its LU (solver_ii)")
+        return self._skeleton2d()[0]
+
+    @property
+    def Aib(self):
+        self._require_hps()
+        if self.ndim == 3:
+            op = self.op
+            return torch.cat([op.Bl[0], op.Br[0], op.G[0]], dim=1)
+        return self._skeleton2d()[1]
+
+    @property
+    def Abi(self):
+        self._require_hps()
+        if self.ndim == 3:
+            raise NotImplementedError("3-D: only A_ii^{-1} and A_ib are kept")
+        return self._skeleton2d()[2]
+
+    @property
+    def Abb(self):
+        self._require_hps()
+        if self.ndim == 3:
+            raise NotImplementedError("3-D: only A_ii^{-1} and A_ib are kept")
+        return self._skeleton2d()[3]
+
     @property
     def solver_ii(self):
-        raise NotImplementedError("interior solves (solution reconstruction inside slabs) are "
-                                  "not provided; use the S maps (smap)")
+        """A_ii^{-1} of the condensed skeleton system as a linear operator
+        (solver/solver.py:117-118, 137-140; solver_Aii of the reference's HPS solvers)."""
+        self._require_hps()
+        if getattr(self, "_sii", None) is None:
+            if self.ndim == 3:
+                op = self.op
+                self._sii = InteriorSolve(op.lu[0], op.piv[0])
+            else:
+                self._sii = InteriorSolve.factor(self._skeleton2d()[0])
+        return self._sii
+
+
+def _solve_dir(self, uu_dir, ff_body=None):
+    """u_i = A_ii^{-1} (ff_body - A_ib u_b) on the skeleton (solver/pde_solver.py:169-201):
+    uu_dir (n_b,) or (n_b, r) in Ib order, ff_body (n_i,) / (n_i, r) or None; numpy in ->
+    numpy out."""
+    was_np = not isinstance(uu_dir, torch.Tensor)
+    Aib = self.Aib
+    u = torch.as_tensor(np.asarray(uu_dir, dtype=np.float64)) if was_np else uu_dir
+    u = u.to(Aib.device, torch.float64)
+    one = u.dim() == 1
+    U = u[:, None] if one else u
+    rhs = -(Aib @ U)
+    if ff_body is not None:
+        f = torch.as_tensor(np.asarray(ff_body, dtype=np.float64)) if not isinstance(
+            ff_body, torch.Tensor) else ff_body
+        f = f.to(Aib.device, torch.float64)
+        rhs = rhs + (f[:, None] if f.dim() == 1 else f)
+    X = self.solver_ii @ rhs
+    X = X[:, 0] if one else X
+    return X.cpu().numpy() if was_np else X
+
+
+solverWrapper.solve_dir = _solve_dir
+
+
+class InteriorSolve:
+    """X -> A^{-1} X (mode N) or A^{-T} X (mode T) from batched-LU factors of one m x m
+    matrix (sslab_dgetrf_batched / sslab_dgetrs_batched / sslab_dgetrs_trans_batched);
+    numpy in -> numpy out, device tensors in -> device tensors out, (m,) or (m, r)."""
+
+    device_op = True
+
+    def __init__(self, lu, piv, mode="N"):
+        self.lu, self.piv = lu, piv
+        m = lu.shape[0]
+        self.shape = (m, m)
+        self.dtype = np.float64
+        self.mode = mode
+
+    @staticmethod
+    def factor(A):
+        from .._lib import call, ptr, stream_handle
+        m = A.shape[0]
+        lu = A.detach().to(torch.float64).clone().contiguous()
+        piv = torch.empty(m, dtype=torch.int32, device=lu.device)
+        info = torch.zeros(1, dtype=torch.int32, device=lu.device)
+        call("sslab_dgetrf_batched", m, ptr(lu), m, m * m, ptr(piv), m, ptr(info), 1,
+             stream_handle())
+        if int(info.item()) != 0:
+            import warnings
+            warnings.warn("solver_ii: singular A_ii", RuntimeWarning)
+        return InteriorSolve(lu, piv)
+
+    @property
+    def T(self):
+        return InteriorSolve(self.lu, self.piv, "T" if self.mode == "N" else "N")
+
+    def _apply(self, v, trans):
+        from .._lib import call, ptr, stream_handle
+        was_np = not isinstance(v, torch.Tensor)
+        x = torch.as_tensor(np.asarray(v, dtype=np.float64)) if was_np else v
+        x = x.to(self.lu.device, torch.float64)
+        one = x.dim() == 1
+        B = (x[:, None] if one else x).contiguous()
+        m, r = B.shape
+        if m != self.shape[0]:
+            raise ValueError(f"solver_ii: {self.shape[0]} rows expected, got {m}")
+        X = torch.empty_like(B)
+        perm = torch.empty(m, dtype=torch.int32, device=B.device)
+        s = stream_handle()
+        if r:
+            if trans:
+                W = torch.empty_like(B)
+                call("sslab_dgetrs_trans_batched", m, r, ptr(self.lu), m, m * m, ptr(self.piv),
+                     m, ptr(B), r, m * r, ptr(X), r, m * r, ptr(W), ptr(perm), 1, s)
+            else:
+                call("sslab_dgetrs_batched", m, r, ptr(self.lu), m, m * m, ptr(self.piv), m,
+                     ptr(B), r, m * r, ptr(X), r, m * r, ptr(perm), 1, s)
+        X = X[:, 0] if one else X
+        return X.cpu().numpy() if was_np else X
+
+    def matmat(self, v):
+        return self._apply(v, self.mode == "T")
+
+    def rmatmat(self, v):
+        return self._apply(v, self.mode == "N")
+
+    matvec = matmat
+    rmatvec = rmatmat
+
+    def __matmul__(self, v):
+        return self.matmat(v)
