@@ -40,7 +40,7 @@ CALIB_MIB = 256
 # the step's 4 GEMM dispatches, each counter scaled by its reading for the calibration
 # copy (summary and commands: profiles/rb_fold_solve_c2_helmholtz_hbm_traffic.md).  A
 # measured constant of the current kernels: re-measure after changing them.
-TRAFFIC = 56811078
+TRAFFIC = 56823397
 # Tile policies of the timed step (backward sweep, boundary-map launch, one per backward
 # level 2, 1, 0), as the autotuner picks them on this workload (bench.py --autotune prints
 # them in timed_policy_flags); pinned so that a profiler run of the same flags traces the
