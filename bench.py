@@ -38,7 +38,7 @@ SEED = 0
 CALIB_MIB = 256
 # HBM bytes per GEMM launch of one step, from rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE over
 # the step's 4 GEMM dispatches, each counter scaled by its reading for the calibration
-# copy (summary and commands: profiles/rb_fold_solve_c2_helmholtz_hbm_traffic.md).  A
+# copy (the HBM-traffic summary of this step under profiles/ gives both commands).  A
 # measured constant of the current kernels: re-measure after changing them.
 TRAFFIC = 56823397
 # Tile policies of the timed step (backward sweep, boundary-map launch, one per backward
