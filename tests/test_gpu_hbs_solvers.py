@@ -147,3 +147,29 @@ def test_multislab_cyclic_functions(T, nb):
                              np.zeros((m, m)) if i == nb - 1 else S[i][-1]) for i in range(nb)],
                            [np.eye(m)] * nb)
     assert rel(xn, np.linalg.solve(dn, rhs)) < 1e-10
+
+
+def test_old_hbs_thomas_api_vs_oracle(T):
+    """multislab/omsdirectsolveHBS.py (the older Thomas API with tree, quad and rank
+    explicit) on config-1 blocks compressed accurately (rank 40, leaves of 40): against the
+    oracle's dense FP64 solve of its own oms S-system and the exact solution."""
+    from sslablu_amd.geometry.geom_2D import square
+    from sslablu_amd.matAssembly.matAssembler import rkHMatAssembler
+    from sslablu_amd.multislab import omsdirectsolveHBS as old_hbs
+    from sslablu_amd.multislab.oms import oms
+    from sslablu_amd.pdo.pdo import PDO2d, const
+    from sslablu_amd.solver.solver import solverOptions
+    N, q, ny = 8, 12, 16
+    dSlabs, conn, H = square.dSlabs(N)
+    opts = solverOptions("hps", [q, q], a=[H / 2, 1 / (2 * ny)])
+    pdo = PDO2d(c11=const(1), c22=const(1))
+    OMS = oms(dSlabs, pdo, square.gb_np, opts, conn)
+    S_rk_list, rhs_list, Ntot, nc = OMS.construct_Stot_helper(exp_sin, rkHMatAssembler(40, 40))
+    _, rhstot = OMS.construct_Stot_and_rhstot_linearOperator(S_rk_list, rhs_list, Ntot, nc)
+    Tf = old_hbs.build_block_tridiagonal_solver(S_rk_list, S_rk_list[0][0].tree,
+                                                S_rk_list[0][0].quad, 40)
+    x = old_hbs.block_tridiagonal_solve(OMS, Tf, rhstot)
+    ref = oh.oms_stot(pdo, N, (H, 1 / ny), q, bc=exp_sin)
+    u = np.linalg.solve(oh.stot_dense(ref["S_rk_list"]), np.concatenate(ref["rhs_list"]))
+    assert rel(x, u) < 1e-8
+    assert np.max(np.abs(x - exp_sin(np.concatenate(ref["Ic_pts"])))) < 1e-7
