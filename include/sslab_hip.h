@@ -87,6 +87,20 @@ int sslab_dgetrs_batched(int n, int nrhs, const double* LU, int lda, long long s
                          long long strideB, double* X, int ldx, long long strideX,
                          int* perm_work, int batch, void* stream);
 
+/* Explicit inverses X_b = A_b^{-1} of `batch` n x n matrices (A read only, X distinct),
+ * by 2 x 2 block elimination on MFMA GEMMs down to 64 x 64 blocks that one wave inverts
+ * with partial pivoting; the residual G - A X G is then sampled on four +-1 columns (one
+ * stream synchronisation) and, when ||.||_F / ||G||_F exceeds tol (tol <= 0: 1e-11) or is
+ * not finite for some matrix, the whole batch is inverted again by pivoted LU against
+ * the identity.  info (device, batch ints, may be NULL): that LU's info, else zeros;
+ * *used_lu (host, may be NULL): 1 when the LU ran.  work / iwork: sslab_inv_sizes.
+ * Replaces the explicit block inverses of the red-black levels, lu_factor(T) followed by
+ * lu_solve(T, I) (direct_solve/omsdirectsolve.py:31-36, 211, 308-333). */
+int sslab_inv_sizes(int n, int batch, long long* work_doubles, long long* work_ints);
+int sslab_dinv_batched(int n, const double* A, int lda, long long strideA, double* X, int ldx,
+                       long long strideX, int batch, double* work, int* iwork, int* info,
+                       double tol, int* used_lu, void* stream);
+
 /* Solve A_b^T X_b = B_b with the factors of sslab_dgetrf_batched (P A = L U, so
  * U^T L^T (P X) = B).  B is read, X (distinct from B) written; work: batch x n x nrhs
  * doubles (stride n*nrhs), perm_work: batch x n ints.  Replaces the transposed sparse

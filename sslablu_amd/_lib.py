@@ -39,6 +39,9 @@ _SIGS = {
     "sslab_dgetrf_batched": [_c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int, _vp],
     "sslab_dgetrs_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp, _c_int,
                              _c_ll, _vp, _c_int, _c_ll, _vp, _c_int, _vp],
+    "sslab_inv_sizes": [_c_int, _c_int, _vp, _vp],
+    "sslab_dinv_batched": [_c_int, _vp, _c_int, _c_ll, _vp, _c_int, _c_ll, _c_int, _vp, _vp,
+                           _vp, _c_d, _vp, _vp],
     "sslab_dgetrs_trans_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_ll, _vp,
                                    _c_int, _c_ll, _vp, _c_int, _c_ll, _vp, _vp, _c_int, _vp],
     "sslab_dgeqrf_batched": [_c_int, _c_int, _vp, _c_int, _c_ll, _vp, _c_int, _c_ll, _vp,

@@ -14,6 +14,10 @@
 //   next level    :  T'_b   = T_i - W^L_b SiP_{i-1} - W^R_b SiM_{i+1}
 //                    SiM'_b = -W^L_b SiM_{i-1},   SiP'_b = -W^R_b SiP_{i+1}
 //
+// Tinv_j (and the coarsest block's inverse) come from inv_batched: block elimination on
+// MFMA GEMMs for all odd blocks of a level at once, with a sampled residual check that
+// falls back to the pivoted LU (the reference's lu_factor / lu_solve(., I)).
+//
 // Solve, per level, forward (one launch):  v_i -= W^L_b v_{i-1} + W^R_b v_{i+1}
 // for even i and y_j = Tinv_j v_j for odd j;  backward (one launch):
 // x_j = y_j - (Tinv SiM)_j x_{j-1} - (Tinv SiP)_j x_{j+1}.  Level-l node k lives
@@ -36,6 +40,7 @@
 #include <vector>
 
 #include "gemm.hpp"
+#include "inv.hpp"
 #include "lu_f64.hpp"
 
 namespace sslab {
@@ -163,12 +168,11 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
   const long long m2 = (long long)m * m;
   const long long setsz = 3 * p.set_blocks * m2;
   E* sets[2] = {work, work + setsz};
+  // the inverses' workspace (4 m^2 per odd node: recursion, residual sample, or the LU
+  // fallback's factors and identity), then the coarsest level's
   E* LUw = work + 2 * setsz;
-  E* Rs = LUw + (long long)p.max_odd * m2;
-  E* Dw = Rs + (long long)p.max_odd * 3 * m2;  // coarsest level: dense matrix, then I
-  int* ipiv = iwork;
-  int* perm = iwork + (long long)p.piv_blocks * m;
-  int* linfo = perm + (long long)p.piv_blocks * m;
+  E* Dw = LUw + (long long)p.max_odd * 4 * m2;
+  int* ipiv = iwork;  // the inverses' ints: pivots, permutations, check flags
   // info layout: one int per odd node per level, then one for the top block.
 
   const E* cT = T;
@@ -202,20 +206,13 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
       info_base += nO;
       SSLAB_TRY(set_identity_batched(m, nT, m, m2, nE, st));
     } else {
-      // 1-2. LU of the odd diagonal blocks.
-      SSLAB_TRY(copy_block_batched(m, m, cT + m2, m, 2 * m2, LUw, m, m2, nO, st));
-      SSLAB_TRY(getrf_batched(m, LUw, m, m2, ipiv, m, linfo, nO, st));
-      if (info) {
-        SSLAB_HIP_CHECK(hipMemcpyAsync(info + info_base, linfo, sizeof(int) * nO,
-                                       hipMemcpyDeviceToDevice, st));
-      }
-      info_base += nO;
-      // 3-4. Tinv_j (first block of TG) from the LU, then
-      //      TG_j[:, m:3m] = Tinv_j [SiM_j | SiP_j] as one MFMA GEMM launch (SiP of a
+      // 1-3. Tinv_j (first block of TG): batched block-elimination inverse of the odd
+      //      diagonal blocks (pivoted LU when its sampled residual fails), then
+      // 4.   TG_j[:, m:3m] = Tinv_j [SiM_j | SiP_j] as one MFMA GEMM launch (SiP of a
       //      last odd node without a right neighbour is zero).
-      SSLAB_TRY(set_identity_batched(m, Rs, m, m2, nO, st));
-      SSLAB_TRY(getrs_batched(m, m, LUw, m, m2, ipiv, m, Rs, m, m2, TG, 3 * m, 3 * m2, perm, nO,
-                              st));
+      SSLAB_TRY(inv_batched<E>(m, cT + m2, m, 2 * m2, TG, 3 * m, 3 * m2, nO, LUw, ipiv,
+                               info ? info + info_base : nullptr, 0.0, nullptr, st));
+      info_base += nO;
       {
         GemmLaunch G{};
         G.nd = 0;
@@ -297,17 +294,9 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
     if (info) zero_int_async(info + info_base, 1, st);
     return set_identity_batched(m, factor + p.top_off, m, m2, 1, st);
   }
-  if (nt == 1) {
-    SSLAB_TRY(copy_block_batched(m, m, cT, m, m2, LUw, m, m2, 1, st));
-    SSLAB_TRY(getrf_batched(m, LUw, m, m2, ipiv, m, linfo, 1, st));
-    if (info) {
-      SSLAB_HIP_CHECK(
-          hipMemcpyAsync(info + info_base, linfo, sizeof(int), hipMemcpyDeviceToDevice, st));
-    }
-    SSLAB_TRY(set_identity_batched(m, Rs, m, m2, 1, st));
-    return getrs_batched(m, m, LUw, m, m2, ipiv, m, Rs, m, m2, factor + p.top_off, m, m2, perm, 1,
-                         st);
-  }
+  if (nt == 1)
+    return inv_batched<E>(m, cT, m, m2, factor + p.top_off, m, m2, 1, LUw, ipiv,
+                          info ? info + info_base : nullptr, 0.0, nullptr, st);
   // nt > 1: red-black factor of the nt-node system (its blocks are the current level set,
   // stride m^2) down to one block -- its info entries fill this plan's remaining slots --
   // then its solve for the identity gives the inverse.

@@ -5,6 +5,7 @@
 #include "blocktri_f64.hpp"
 #include "gemm.hpp"
 #include "hps_f64.hpp"
+#include "inv.hpp"
 #include "spectral_f64.hpp"
 #include "krylov_f64.hpp"
 #include "lu_f64.hpp"
@@ -139,6 +140,22 @@ int sslab_gemm_get_policy(int* p, int n) {
 int sslab_dgetrf_batched(int n, double* A, int lda, long long strideA, int* ipiv,
                          long long strideP, int* info, int batch, void* stream) {
   return sslab::getrf_batched(n, A, lda, strideA, ipiv, strideP, info, batch, as_stream(stream));
+}
+
+int sslab_inv_sizes(int n, int batch, long long* work_doubles, long long* work_ints) {
+  if (n < 1 || batch < 0) {
+    sslab::set_error("sslab_inv_sizes: bad sizes");
+    return sslab::kBadArgument;
+  }
+  sslab::inv_sizes(n, batch, work_doubles, work_ints);
+  return sslab::kOk;
+}
+
+int sslab_dinv_batched(int n, const double* A, int lda, long long strideA, double* X, int ldx,
+                       long long strideX, int batch, double* work, int* iwork, int* info,
+                       double tol, int* used_lu, void* stream) {
+  return sslab::inv_batched<double>(n, A, lda, strideA, X, ldx, strideX, batch, work, iwork, info,
+                                    tol, used_lu, as_stream(stream));
 }
 
 int sslab_dgetrs_batched(int n, int nrhs, const double* LU, int lda, long long strideA,
