@@ -13,9 +13,16 @@
 //   T_new = blkdiag(T^a_ee, T^b_ee) - [T^a_ec; T^b_ec] (T^a_cc + T^b_cc)^{-1} [T^a_ce T^b_ce]
 // The caller supplies index maps that place every node of the merged box in its
 // canonical order, so one gather kernel builds all four operands.
+//
+// Every local solve (A_ii in the leaves, T^a_cc + T^b_cc in the merges and the double-slab
+// pair solves) is an explicit batched inverse followed by an MFMA product: inv_batched
+// works on the whole batch at once (block elimination on GEMMs, 64 x 64 Gauss-Jordan
+// blocks, a sampled backward-error check with a pivoted-LU fallback), where a batched LU
+// would spend most of its time in one-workgroup-per-matrix panels and triangular solves.
 #include "hps_f64.hpp"
 
 #include "gemm.hpp"
+#include "inv.hpp"
 #include "lu_f64.hpp"
 
 namespace sslab {
@@ -157,8 +164,11 @@ int gather2(int batch, int nrows, int ncols, const double* s0, int ld0, long lon
 int hps_leaf_sizes(int nleaves, int q, long long* work_doubles, long long* work_ints) {
   SSLAB_REQUIRE(nleaves >= 1 && q >= 3, "hps_leaf_sizes: bad sizes");
   const long long p = q - 2, p2 = p * p;
-  if (work_doubles) *work_doubles = (long long)nleaves * (p2 * p2 + 2 * p2 * 4 * p);
-  if (work_ints) *work_ints = (long long)nleaves * 2 * p2 + nleaves;
+  long long iw = 0, ii = 0;
+  inv_sizes((int)p2, nleaves, &iw, &ii);
+  // A_ii, A_ii^{-1}, A_ib, X, then the inverse's workspace
+  if (work_doubles) *work_doubles = (long long)nleaves * (2 * p2 * p2 + 2 * p2 * 4 * p) + iw;
+  if (work_ints) *work_ints = ii + nleaves;
   return kOk;
 }
 
@@ -171,23 +181,27 @@ int hps_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx, const 
   SSLAB_REQUIRE(Dx && Dxx && Dy && Dyy && Nb && Ni && T && work && iwork,
                 "hps_leaf_dtn: null pointer");
   const int p = q - 2, p2 = p * p, nb4 = 4 * p;
+  const long long s2 = (long long)p2 * p2, sb = (long long)p2 * nb4;
   double* Aii = work;
-  double* Aib = Aii + (long long)nleaves * p2 * p2;
-  double* Xs = X ? X : Aib + (long long)nleaves * p2 * nb4;
-  int* ipiv = iwork;
-  int* perm = iwork + (long long)nleaves * p2;
-  int* linfo = info ? info : perm + (long long)nleaves * p2;
+  double* Ainv = Aii + nleaves * s2;
+  double* Aib = Ainv + nleaves * s2;
+  double* Xs = X ? X : Aib + nleaves * sb;
+  double* iw = Aib + 2 * nleaves * sb;
+  long long iwn = 0;
+  inv_sizes(p2, nleaves, nullptr, &iwn);
+  int* linfo = info ? info : iwork + iwn;
   hipLaunchKernelGGL(k_leaf_assemble, dim3(ceil_div(p2 + nb4, 256), p2, nleaves), dim3(256), 0,
                      st, q, Dx, Dxx, Dy, Dyy, c11, c22, c1, c2, c0, Aii, Aib);
   SSLAB_HIP_CHECK(hipGetLastError());
-  SSLAB_TRY(getrf_batched(p2, Aii, p2, (long long)p2 * p2, ipiv, p2, linfo, nleaves, st));
-  SSLAB_TRY(getrs_batched(p2, nb4, Aii, p2, (long long)p2 * p2, ipiv, p2, Aib, nb4,
-                          (long long)p2 * nb4, Xs, nb4, (long long)p2 * nb4, perm, nleaves, st));
-  // T = N_b - N_i X   (N shared by all leaves: batch stride 0)
+  // X = A_ii^{-1} A_ib, then T = N_b - N_i X   (N shared by all leaves: batch stride 0)
+  SSLAB_TRY(inv_batched<double>(p2, Aii, p2, s2, Ainv, p2, s2, nleaves, iw, iwork, linfo, 0.0,
+                                nullptr, st));
+  SSLAB_TRY(gemm_strided(0, 0, p2, nb4, p2, 1.0, Ainv, p2, s2, Aib, nb4, sb, 0.0, Xs, nb4, sb,
+                         nleaves, st));
   GemmLaunch G{};
   G.nd = 1;
   G.d[0] = gemm_desc(nb4, nb4, nleaves, T, nb4, (long long)nb4 * nb4, Nb, nb4, 0, 1.0);
-  gemm_add_term(G.d[0], gemm_term(Ni, p2, 0, 0, Xs, nb4, (long long)p2 * nb4, 0, p2, -1.0));
+  gemm_add_term(G.d[0], gemm_term(Ni, p2, 0, 0, Xs, nb4, sb, 0, p2, -1.0));
   SSLAB_TRY(gemm_launch(G, st));
   return kOk;
 }
@@ -195,8 +209,11 @@ int hps_leaf_dtn(int nleaves, int q, const double* Dx, const double* Dxx, const 
 int hps3_leaf_sizes(int nleaves, int q, long long* work_doubles, long long* work_ints) {
   SSLAB_REQUIRE(nleaves >= 1 && q >= 3, "hps3_leaf_sizes: bad sizes");
   const long long p = q - 2, p3 = p * p * p, nb6 = 6 * p * p;
-  if (work_doubles) *work_doubles = (long long)nleaves * (p3 * p3 + 2 * p3 * nb6);
-  if (work_ints) *work_ints = (long long)nleaves * 2 * p3 + nleaves;
+  long long iw = 0, ii = 0;
+  inv_sizes((int)p3, nleaves, &iw, &ii);
+  // A_ii, A_ii^{-1}, A_ib, X, then the inverse's workspace
+  if (work_doubles) *work_doubles = (long long)nleaves * (2 * p3 * p3 + 2 * p3 * nb6) + iw;
+  if (work_ints) *work_ints = ii + nleaves;
   return kOk;
 }
 
@@ -209,33 +226,40 @@ int hps3_leaf_dtn(int nleaves, int q, const double* const* D, const double* Nb, 
   for (int i = 0; i < 6; ++i) SSLAB_REQUIRE(D[i] != nullptr, "hps3_leaf_dtn: null derivative");
   const int p = q - 2, p3 = p * p * p, nb6 = 6 * p * p;
   SSLAB_REQUIRE(p3 <= 65535, "hps3_leaf_dtn: leaf order too large");
+  const long long s3 = (long long)p3 * p3, sb = (long long)p3 * nb6;
   double* Aii = work;
-  double* Aib = Aii + (long long)nleaves * p3 * p3;
-  double* Xs = X ? X : Aib + (long long)nleaves * p3 * nb6;
-  int* ipiv = iwork;
-  int* perm = iwork + (long long)nleaves * p3;
-  int* linfo = info ? info : perm + (long long)nleaves * p3;
+  double* Ainv = Aii + nleaves * s3;
+  double* Aib = Ainv + nleaves * s3;
+  double* Xs = X ? X : Aib + nleaves * sb;
+  double* iw = Aib + 2 * nleaves * sb;
+  long long iwn = 0;
+  inv_sizes(p3, nleaves, nullptr, &iwn);
+  int* linfo = info ? info : iwork + iwn;
   hipLaunchKernelGGL(k_leaf_assemble3, dim3(ceil_div(p3 + nb6, 256), p3, nleaves), dim3(256), 0,
                      st, q, D[0], D[1], D[2], D[3], D[4], D[5], coef[0], coef[1], coef[2], coef[3],
                      coef[4], coef[5], coef[6], Aii, Aib);
   SSLAB_HIP_CHECK(hipGetLastError());
-  SSLAB_TRY(getrf_batched(p3, Aii, p3, (long long)p3 * p3, ipiv, p3, linfo, nleaves, st));
-  SSLAB_TRY(getrs_batched(p3, nb6, Aii, p3, (long long)p3 * p3, ipiv, p3, Aib, nb6,
-                          (long long)p3 * nb6, Xs, nb6, (long long)p3 * nb6, perm, nleaves, st));
-  // T = N_b - N_i X   (N shared by all leaves: batch stride 0)
+  // X = A_ii^{-1} A_ib, then T = N_b - N_i X   (N shared by all leaves: batch stride 0)
+  SSLAB_TRY(inv_batched<double>(p3, Aii, p3, s3, Ainv, p3, s3, nleaves, iw, iwork, linfo, 0.0,
+                                nullptr, st));
+  SSLAB_TRY(gemm_strided(0, 0, p3, nb6, p3, 1.0, Ainv, p3, s3, Aib, nb6, sb, 0.0, Xs, nb6, sb,
+                         nleaves, st));
   GemmLaunch G{};
   G.nd = 1;
   G.d[0] = gemm_desc(nb6, nb6, nleaves, T, nb6, (long long)nb6 * nb6, Nb, nb6, 0, 1.0);
-  gemm_add_term(G.d[0], gemm_term(Ni, p3, 0, 0, Xs, nb6, (long long)p3 * nb6, 0, p3, -1.0));
+  gemm_add_term(G.d[0], gemm_term(Ni, p3, 0, 0, Xs, nb6, sb, 0, p3, -1.0));
   SSLAB_TRY(gemm_launch(G, st));
   return kOk;
 }
 
 int hps_merge_sizes(int nmerge, int n_new, int c, long long* work_doubles, long long* work_ints) {
   SSLAB_REQUIRE(nmerge >= 1 && n_new >= 1 && c >= 1, "hps_merge_sizes: bad sizes");
+  long long iw = 0, ii = 0;
+  inv_sizes(c, nmerge, &iw, &ii);
+  // T_cc, its inverse, R, X, E, then the inverse's workspace
   if (work_doubles)
-    *work_doubles = (long long)nmerge * ((long long)c * c + 3LL * c * n_new);
-  if (work_ints) *work_ints = (long long)nmerge * (2LL * c + 1);
+    *work_doubles = (long long)nmerge * (2LL * c * c + 3LL * c * n_new) + iw;
+  if (work_ints) *work_ints = ii + nmerge;
   return kOk;
 }
 
@@ -249,15 +273,17 @@ int hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta, 
                 "hps_merge: null pointer");
   const long long c2 = (long long)c * c, cn = (long long)c * n_new;
   double* Tcc = work;
-  double* R = Tcc + nmerge * c2;
+  double* Tinv = Tcc + nmerge * c2;
+  double* R = Tinv + nmerge * c2;
   // X = (T^a_cc + T^b_cc)^{-1} [T_{c,new}]: the shared edge's values are -X u_new
   // (kept in Xsol, stride sXsol, when the caller wants the merge's solution operator)
   double* X = Xsol ? Xsol : R + nmerge * cn;
   const long long sX = Xsol ? sXsol : cn;
   double* E = R + 2 * nmerge * cn;
-  int* ipiv = iwork;
-  int* perm = iwork + (long long)nmerge * c;
-  int* linfo = info ? info : perm + (long long)nmerge * c;
+  double* iw = E + nmerge * cn;
+  long long iwn = 0;
+  inv_sizes(c, nmerge, nullptr, &iwn);
+  int* linfo = info ? info : iwork + iwn;
   // na[i] / nbix[i]: index of merged node i in alpha / beta (-1 if it is not there).
   SSLAB_TRY(gather2(nmerge, c, c, Ta, n_a, sTa, Tb, n_b, sTb, ca, ca, cb, cb, Tcc, c, c2, st));
   SSLAB_TRY(gather2(nmerge, c, n_new, Ta, n_a, sTa, Tb, n_b, sTb, ca, na, cb, nbix, R, n_new, cn,
@@ -265,9 +291,10 @@ int hps_merge(int nmerge, int n_a, int n_b, int n_new, int c, const double* Ta, 
   SSLAB_TRY(gather2(nmerge, n_new, c, Ta, n_a, sTa, Tb, n_b, sTb, na, ca, nbix, cb, E, c, cn, st));
   SSLAB_TRY(gather2(nmerge, n_new, n_new, Ta, n_a, sTa, Tb, n_b, sTb, na, na, nbix, nbix, Tnew,
                     n_new, sTnew, st));
-  SSLAB_TRY(getrf_batched(c, Tcc, c, c2, ipiv, c, linfo, nmerge, st));
-  SSLAB_TRY(getrs_batched(c, n_new, Tcc, c, c2, ipiv, c, R, n_new, cn, X, n_new, sX, perm, nmerge,
-                          st));
+  SSLAB_TRY(inv_batched<double>(c, Tcc, c, c2, Tinv, c, c2, nmerge, iw, iwork, linfo, 0.0, nullptr,
+                                st));
+  SSLAB_TRY(gemm_strided(0, 0, c, n_new, c, 1.0, Tinv, c, c2, R, n_new, cn, 0.0, X, n_new, sX,
+                         nmerge, st));
   SSLAB_TRY(gemm_strided(0, 0, n_new, n_new, c, -1.0, E, c, cn, X, n_new, sX, 1.0, Tnew, n_new,
                          sTnew, nmerge, st));
   return kOk;
@@ -280,20 +307,23 @@ int hps_pair_solve(int npair, int n_s, int c, int ncols, const double* Ts, long 
   SSLAB_REQUIRE(Ts && ca && cb && cola && colb && Y && work && iwork,
                 "hps_pair_solve: null pointer");
   const long long c2 = (long long)c * c, cn = (long long)c * ncols;
+  // workspace as laid out by hps_merge_sizes(npair, ncols, c)
   double* Tcc = work;
-  double* R = Tcc + npair * c2;
-  int* ipiv = iwork;
-  int* perm = iwork + (long long)npair * c;
-  int* linfo = info ? info : perm + (long long)npair * c;
+  double* Tinv = Tcc + npair * c2;
+  double* R = Tinv + npair * c2;
+  double* iw = R + 3 * npair * cn;
+  long long iwn = 0;
+  inv_sizes(c, npair, nullptr, &iwn);
+  int* linfo = info ? info : iwork + iwn;
   // alpha = slab i, beta = slab i+1 (same array, offset by one slab).
   SSLAB_TRY(gather2(npair, c, c, Ts, n_s, sTs, Ts + sTs, n_s, sTs, ca, ca, cb, cb, Tcc, c, c2,
                     st));
   SSLAB_TRY(gather2(npair, c, ncols, Ts, n_s, sTs, Ts + sTs, n_s, sTs, ca, cola, cb, colb, R,
                     ncols, cn, st));
-  SSLAB_TRY(getrf_batched(c, Tcc, c, c2, ipiv, c, linfo, npair, st));
-  SSLAB_TRY(getrs_batched(c, ncols, Tcc, c, c2, ipiv, c, R, ncols, cn, Y, ncols, cn, perm, npair,
-                          st));
-  return kOk;
+  SSLAB_TRY(inv_batched<double>(c, Tcc, c, c2, Tinv, c, c2, npair, iw, iwork, linfo, 0.0, nullptr,
+                                st));
+  return gemm_strided(0, 0, c, ncols, c, 1.0, Tinv, c, c2, R, ncols, cn, 0.0, Y, ncols, cn, npair,
+                      st);
 }
 
 }  // namespace sslab

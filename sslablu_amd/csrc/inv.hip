@@ -8,18 +8,21 @@
 //   X12 = -W Y,  X21 = -Y Z               (one grouped launch)
 //   X11 = X11 - X12 Z                     (one launch, in place)
 // so an n x n inverse is 2 n^3 flops of MFMA GEMMs plus the 64 x 64 diagonal blocks,
-// which k_inv_gj inverts with one wave per matrix (Gauss-Jordan with partial pivoting
-// inside the block, no barriers between waves).  The pivoted LU this replaces
+// which k_inv_gj inverts with one workgroup of eight waves per matrix (Gauss-Jordan with
+// partial pivoting inside the block, one barrier per pivot step).  The pivoted LU this replaces
 // (getrf_batched + getrs_batched against I) spends about 2 us per column on a one-
 // workgroup panel; here the sequential part is the 64-column blocks and four launches per
 // halving, and every launch covers the whole batch.
 //
 // The elimination does not pivot between blocks, so its accuracy rests on the diagonal
 // blocks and Schur complements being well conditioned (true of the red-black levels of
-// the slab S-systems, whose blocks are identity-dominated Schur complements).  Every call
-// therefore samples the residual G - A X G on four +-1 columns and falls back to the
+// the slab S-systems, whose blocks are identity-dominated Schur complements, and of the
+// HPS leaf and merge systems).  Every call therefore samples the normwise backward error
+// ||G - A X G||_F / (||A||_F ||X G||_F) on four +-1 columns G and falls back to the
 // pivoted LU when it misses the tolerance or is not finite (an exactly singular block
-// included), so the result is never worse than the LU path's.
+// included), so the result is never worse than the LU path's.  The ratio is scale-free:
+// a matrix of large norm (a leaf's spectral second derivatives scale with p^4 / h^2)
+// passes when its inverse is as good as a backward-stable one.
 #include "inv.hpp"
 
 #include <vector>
@@ -32,7 +35,8 @@ namespace sslab {
 
 namespace {
 
-constexpr int kInvBase = 64;  // diagonal blocks inverted by one wave
+constexpr int kInvBase = 64;  // diagonal blocks inverted by one workgroup (Gauss-Jordan)
+constexpr int kGjWaves = 8;   // waves of that workgroup; each owns kInvBase / kGjWaves columns
 constexpr int kInvCheckCols = 4;
 
 inline int inv_split(int n) {
@@ -50,138 +54,184 @@ long long inv_rec_elems(int n) {
   return here > left ? here : left;
 }
 
-// One wave inverts one n x n block (n <= 64) by in-place Gauss-Jordan with partial
-// pivoting.  Lane i keeps row i in registers as s_i * a_i[.]: the pivot row R of step k
-// (a row not pivoted yet, so s = 1) is broadcast unscaled through LDS, the other rows
-// subtract (a_i[k] / R[k]) R and set a_i[k] = -a_i[k] / R[k] (no scale enters), and the
-// pivot lane only sets its scale to 1 / R[k] and a_r[k] to 1 -- so no lane runs a
-// divergent row update.  Pivot: the largest |a_i[k]| over the rows not pivoted yet
-// (first lane on ties).  After step k the row pivoted at step k holds row k of the
-// inverse with its columns permuted: X[k][p_j] = s_i a_i[j] for i = p_k, where p_j is
-// the lane pivoted at step j; the rows are put in place through an LDS tile so that the
-// stores are coalesced.  Rows / columns n..63 are the identity's and never pivot.
-// The steps are instantiated one by one (GjStep<E, K>) so that every a[K] index is a
-// compile-time constant and the row stays in registers.
-template <typename E>
-struct GjState {
-  E a[kInvBase];
-  E s;       // row scale
-  int step;  // step at which this row was the pivot (-1: not yet)
-};
+// Pivot key of a column value: the high 32 bits of |x| (sign cleared; NaN above +inf)
+// plus one, so every candidate row ranks above the rows that may not pivot (key 0).  For
+// FP64 the key drops the low 32 mantissa bits: the chosen pivot is within a relative
+// 2^-20 of the largest, which partial pivoting's stability does not notice.
+__device__ __forceinline__ unsigned gj_key(double x) {
+  return (unsigned)((unsigned long long)__double_as_longlong(fabs(x)) >> 32) + 1u;
+}
+__device__ __forceinline__ unsigned gj_key(float x) { return __float_as_uint(fabsf(x)) + 1u; }
 
-template <typename E, int K>
-struct GjStep {
-  __device__ __forceinline__ static void run(GjState<E>& g, int lane, bool real, int n,
-                                             E* prow, int* pl) {
-    constexpr int NB = kInvBase;
-    if (K < n) {
-      // candidates: the open rows (|a| >= 0; NaN counts as +inf, so the maximum is never
-      // NaN and always belongs to an open row -- a non-finite column then makes the
-      // result non-finite, which the caller's residual check rejects); the rest -1
-      const bool open = real && g.step < 0;
-      const E x = fabs(g.a[K]);
-      const E v = open ? (x == x ? x : E(INFINITY)) : E(-1);
-      const E vmax = wave::wave_max(v);
-      const int r = wave::wave_min(v == vmax ? lane : NB);
-      if (lane == r) {
-#pragma unroll
-        for (int j = 0; j < NB; ++j) prow[j] = g.a[j];
-        pl[K] = lane;
-        g.step = K;
-      }
-      __syncthreads();
-      // the whole row into registers first, so its LDS reads are in flight together
-      E R[NB];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) R[j] = prow[j];
-      const E ip = E(1) / R[K];
-      const bool me = lane == r;
-      const E c = me ? E(0) : g.a[K] * ip;
-#pragma unroll
-      for (int j = 0; j < NB; ++j)
-        if (j != K) g.a[j] = fma(-c, R[j], g.a[j]);
-      g.a[K] = me ? E(1) : -c;
-      if (me) g.s = ip;
-    }
-    GjStep<E, K + 1>::run(g, lane, real, n, prow, pl);
-  }
-};
-
+// One workgroup of eight waves inverts one n x n block (n <= 64) by in-place Gauss-Jordan
+// with partial pivoting.  Lane i of wave w keeps row i restricted to the wave's eight
+// columns 8w .. 8w + 7, as s_i * (stored values).  Step k belongs to the wave that
+// owns column k: it picks the pivot row p (the largest |a_i[k]| over the rows not pivoted
+// yet, first lane on ties), and publishes p, 1 / R[k] and every row's multiplier
+// c_i = a_i[k] / R[k] (c_p = 0) through LDS -- one barrier per step, the buffers
+// alternating by step parity so the next step's writes never meet this step's reads.
+// Every wave then reads the pivot row R of its own columns from its lane p (readlane),
+// subtracts c_i R from its rows and, in the owning wave, sets a_i[k] = -c_i (a_p[k] = 1);
+// lane p sets its scale to 1 / R[k].  The pivot row is broadcast unscaled and its own
+// values stay as they were, so no lane runs a divergent row update.  After step k the row
+// pivoted at step k holds row k of the inverse with its columns permuted:
+// X[k][p_j] = s_i a_i[j] for i = p_k, where p_j is the row pivoted at step j; the rows are
+// put in place through an LDS tile so that the stores are coalesced.  Rows / columns
+// n..63 are the identity's and never pivot.
 template <typename E>
-struct GjStep<E, kInvBase> {
-  __device__ __forceinline__ static void run(GjState<E>&, int, bool, int, E*, int*) {}
-};
-
-template <typename E>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void k_inv_gj(int n, const E* __restrict__ A, int lda, long long sA, E* __restrict__ X, int ldx,
-              long long sX) {
+__global__ __launch_bounds__(kGjWaves * 64) void k_inv_gj(int n, const E* __restrict__ A, int lda,
+                                                long long sA, E* __restrict__ X, int ldx,
+                                                long long sX) {
   constexpr int NB = kInvBase;
+  constexpr int CW = NB / kGjWaves;  // columns per wave
   constexpr int LD = NB + 1;
   __shared__ E tile[NB * LD];
-  __shared__ E prow[NB];
+  __shared__ E cbuf[2][NB];
+  __shared__ E ibuf[2];
+  __shared__ int pbuf[2];
   __shared__ int pl[NB];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
   const long long b = blockIdx.x;
   A += b * sA;
   X += b * sX;
   // coalesced load of the block through the tile
   if (lane < n)
-    for (int i = 0; i < n; ++i) tile[i * LD + lane] = A[(long long)i * lda + lane];
+    for (int i = w; i < n; i += kGjWaves) tile[i * LD + lane] = A[(long long)i * lda + lane];
   __syncthreads();
   const bool real = lane < n;
-  GjState<E> g;
+  E a[CW];
 #pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    const E v = tile[lane * LD + j];
-    g.a[j] = (real && j < n) ? v : (lane == j ? E(1) : E(0));
+  for (int j = 0; j < CW; ++j) {
+    const int c = w * CW + j;
+    const E v = tile[lane * LD + c];  // read for every lane; replaced outside the block
+    a[j] = (real && c < n) ? v : (lane == c ? E(1) : E(0));
   }
-  g.s = E(1);
-  g.step = -1;
-  GjStep<E, 0>::run(g, lane, real, n, prow, pl);
+  E s = E(1);
+  int step = -1;  // step at which this row was the pivot (-1: not yet)
+  for (int kw = 0; kw * CW < n; ++kw) {
+#pragma unroll
+    for (int kk = 0; kk < CW; ++kk) {
+      const int k = kw * CW + kk;
+      if (k >= n) continue;  // uniform over the workgroup (no barrier is skipped by some)
+      const int par = k & 1;
+      E c_own = E(0), ip_own = E(0);
+      int p_own = 0;
+      if (w == kw) {
+        const unsigned key = (real && step < 0) ? gj_key(a[kk]) : 0u;
+        const unsigned kmax = wave::wave_max_u32(key);
+        // the maximum is attained by some candidate row (n - k of them remain)
+        const unsigned long long bal = __ballot(key == kmax);
+        p_own = __builtin_ctzll(bal);
+        const E piv = wave::readlane_e(a[kk], p_own);
+        ip_own = E(1) / piv;
+        c_own = (lane == p_own) ? E(0) : a[kk] * ip_own;
+        cbuf[par][lane] = c_own;
+        if (lane == 0) {
+          pbuf[par] = p_own;
+          ibuf[par] = ip_own;
+          pl[k] = p_own;
+        }
+      }
+      __syncthreads();
+      int p;
+      E c, ip;
+      if (w == kw) {
+        p = p_own;
+        c = c_own;
+        ip = ip_own;
+      } else {
+        p = __builtin_amdgcn_readfirstlane(pbuf[par]);
+        c = cbuf[par][lane];
+        ip = ibuf[par];
+      }
+#pragma unroll
+      for (int j = 0; j < CW; ++j) {
+        const E r = wave::readlane_e(a[j], p);
+        a[j] = fma(-c, r, a[j]);
+      }
+      if (w == kw) a[kk] = (lane == p) ? E(1) : -c;
+      if (lane == p) {
+        s = ip;
+        step = k;
+      }
+    }
+  }
   __syncthreads();
   // scatter X[step][p_j] = s a[j] into the tile, then store its rows
-  if (real && g.step >= 0) {  // (every real row pivots exactly once)
+  if (real) {  // (every real row pivots exactly once)
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (j < n) tile[g.step * LD + pl[j]] = g.s * g.a[j];
+    for (int j = 0; j < CW; ++j) {
+      const int c = w * CW + j;
+      if (c < n) tile[step * LD + pl[c]] = s * a[j];
+    }
   }
   __syncthreads();
-  if (real)
-    for (int i = 0; i < n; ++i) X[(long long)i * ldx + lane] = tile[i * LD + lane];
+  if (lane < n)
+    for (int i = w; i < n; i += kGjWaves) X[(long long)i * ldx + lane] = tile[i * LD + lane];
 }
 
-// G[i][c] = +-1 (a fixed hash of i and c), n x kInvCheckCols.
+// G[i][c] = +-1 (a fixed hash of i and c), n x kInvCheckCols, written to Gs (shared by the
+// batch) and to columns kInvCheckCols.. of every matrix's n x (2 kInvCheckCols) block B_b.
 template <typename E>
-__global__ void k_inv_fill_g(int n, E* __restrict__ G) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n * kInvCheckCols) return;
-  unsigned h = (unsigned)e * 2654435761u + 0x9e3779b9u;
+__global__ void k_inv_fill_g(int n, int batch, E* __restrict__ Gs, E* __restrict__ B) {
+  constexpr int C = kInvCheckCols;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= (long long)n * C * (batch + 1)) return;
+  const int b = (int)(e / ((long long)n * C)) - 1;  // -1: the shared copy
+  const int ic = (int)(e % ((long long)n * C));
+  unsigned h = (unsigned)ic * 2654435761u + 0x9e3779b9u;
   h ^= h >> 15;
   h *= 2246822519u;
   h ^= h >> 13;
-  G[e] = (h & 1u) ? E(1) : E(-1);
+  const E g = (h & 1u) ? E(1) : E(-1);
+  if (b < 0) Gs[ic] = g;
+  else B[(long long)b * n * 2 * C + (long long)(ic / C) * 2 * C + C + ic % C] = g;
 }
 
-// flag[b] = 1 unless sum(Q_b^2) <= thr2 (NaN / Inf fail); Q_b: n x kInvCheckCols.
+// Per matrix b: B_b = [P | G] (P = X_b G) and Y_b = A_b B_b = [A P | A G], n x 2C each.
+// Q = G - A P is the residual of the sample; E ||A g||^2 = ||A||_F^2 for a column g of
+// independent +-1 entries, so ||A G||_F^2 / C estimates ||A||_F^2 without a pass of its
+// own over A (an underestimate only makes the test stricter).  flag[b] = 0 when
+// ||Q||_F^2 <= tol^2 (||A G||_F^2 / C) ||P||_F^2, else 1 (NaN / Inf fail).
 template <typename E>
-__global__ __launch_bounds__(256) void k_inv_check(int n, const E* __restrict__ Q, double thr2,
+__global__ __launch_bounds__(256) void k_inv_check(int n, const E* __restrict__ B,
+                                                   const E* __restrict__ Y, double tol2,
                                                    int* __restrict__ flag) {
+  constexpr int C = kInvCheckCols;
   const long long b = blockIdx.x;
-  const int cnt = n * kInvCheckCols;
-  const E* q = Q + b * cnt;
-  double ss = 0.0;
-  for (int e = threadIdx.x; e < cnt; e += 256) {
-    const double v = (double)q[e];
-    ss += v * v;
+  const long long cnt = (long long)n * 2 * C;
+  const E* bb = B + b * cnt;
+  const E* yy = Y + b * cnt;
+  double qq = 0.0, pp = 0.0, aa = 0.0;
+  for (long long e = threadIdx.x; e < cnt; e += 256) {
+    const int c = (int)(e % (2 * C));
+    if (c < C) {
+      const double q = (double)bb[e + C] - (double)yy[e];  // g - (A P)
+      const double v = (double)bb[e];
+      qq += q * q;
+      pp += v * v;
+    } else {
+      const double v = (double)yy[e];
+      aa += v * v;
+    }
   }
-  __shared__ double red[4];
-  ss = wave::wave_sum(ss);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+  __shared__ double red[3][4];
+  qq = wave::wave_sum(qq);
+  pp = wave::wave_sum(pp);
+  aa = wave::wave_sum(aa);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = qq;
+    red[1][w] = pp;
+    red[2][w] = aa;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const double t = (red[0] + red[1]) + (red[2] + red[3]);
-    flag[b] = (t <= thr2) ? 0 : 1;
+    const double tq = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    const double tp = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    const double ta = (red[2][0] + red[2][1]) + (red[2][2] + red[2][3]);
+    flag[b] = (tq <= tol2 * (ta / C) * tp) ? 0 : 1;
   }
 }
 
@@ -194,8 +244,8 @@ template <typename E>
 int inv_rec(int n, const E* A, int lda, long long sA, E* X, int ldx, long long sX, E* W,
             long long sW, int batch, hipStream_t st) {
   if (n <= kInvBase) {
-    hipLaunchKernelGGL(k_inv_gj<E>, dim3(batch), dim3(kInvBase), 0, st, n, A, lda, sA, X, ldx,
-                       sX);
+    hipLaunchKernelGGL(k_inv_gj<E>, dim3(batch), dim3(kGjWaves * 64), 0, st, n, A, lda, sA, X,
+                       ldx, sX);
     SSLAB_HIP_CHECK(hipGetLastError());
     return kOk;
   }
@@ -240,7 +290,7 @@ int inv_rec(int n, const E* A, int lda, long long sA, E* X, int ldx, long long s
 void inv_sizes(int n, int batch, long long* work, long long* iwork) {
   const long long nn = (long long)n * n;
   long long w = inv_rec_elems(n) * batch;
-  const long long chk = (long long)kInvCheckCols * n * (1 + 2LL * batch);
+  const long long chk = (long long)kInvCheckCols * n * (1 + 4LL * batch);
   const long long lu = 2 * nn * batch;
   if (chk > w) w = chk;
   if (lu > w) w = lu;
@@ -255,29 +305,30 @@ int inv_batched(int n, const E* A, int lda, long long sA, E* X, int ldx, long lo
   SSLAB_REQUIRE(n >= 1 && batch >= 0 && lda >= n && ldx >= n, "inv_batched: bad sizes");
   if (batch == 0) return kOk;
   SSLAB_REQUIRE(A && X && work && iwork, "inv_batched: null pointer");
-  if (tol <= 0) tol = sizeof(E) == 8 ? 1e-11 : 1e-3;
+  if (tol <= 0) tol = sizeof(E) == 8 ? 1e-11 : 1e-3;  // normwise backward error
   const long long rec = inv_rec_elems(n);
   SSLAB_TRY(inv_rec<E>(n, A, lda, sA, X, ldx, sX, work, rec, batch, st));
 
-  // sampled residual: Q = G - A (X G)
-  E* Gm = work;
-  E* P = Gm + (long long)kInvCheckCols * n;
-  E* Q = P + (long long)kInvCheckCols * n * batch;
-  const long long sPQ = (long long)kInvCheckCols * n;
+  // sampled backward error: B_b = [X_b G | G], Y_b = A_b B_b
+  constexpr int C = kInvCheckCols;
+  E* Gs = work;
+  E* Bm = Gs + (long long)C * n;
+  E* Ym = Bm + 2LL * C * n * batch;
+  const long long sB = 2LL * C * n;
   int* flag = iwork + 2LL * n * batch;
-  hipLaunchKernelGGL(k_inv_fill_g<E>, dim3(ceil_div(n * kInvCheckCols, 256)), dim3(256), 0, st, n,
-                     Gm);
+  const long long nfill = (long long)n * C * (batch + 1);
+  hipLaunchKernelGGL(k_inv_fill_g<E>, dim3((unsigned)((nfill + 255) / 256)), dim3(256), 0, st, n,
+                     batch, Gs, Bm);
   SSLAB_HIP_CHECK(hipGetLastError());
   GemmLaunchT<E> G{};
   G.nd = 1;
-  G.d[0] = gemm_desc<E>(n, kInvCheckCols, batch, P, kInvCheckCols, sPQ, nullptr, 0, 0, 0.0);
-  gemm_add_term<E>(G.d[0], gemm_term<E>(X, ldx, sX, 0, Gm, kInvCheckCols, 0, 0, n, 1.0));
+  G.d[0] = gemm_desc<E>(n, C, batch, Bm, 2 * C, sB, nullptr, 0, 0, 0.0);
+  gemm_add_term<E>(G.d[0], gemm_term<E>(X, ldx, sX, 0, Gs, C, 0, 0, n, 1.0));
   SSLAB_TRY(gemm_launch<E>(G, st));
-  G.d[0] = gemm_desc<E>(n, kInvCheckCols, batch, Q, kInvCheckCols, sPQ, Gm, kInvCheckCols, 0, 1.0);
-  gemm_add_term<E>(G.d[0], gemm_term<E>(A, lda, sA, 0, P, kInvCheckCols, sPQ, 0, n, -1.0));
+  G.d[0] = gemm_desc<E>(n, 2 * C, batch, Ym, 2 * C, sB, nullptr, 0, 0, 0.0);
+  gemm_add_term<E>(G.d[0], gemm_term<E>(A, lda, sA, 0, Bm, 2 * C, sB, 0, n, 1.0));
   SSLAB_TRY(gemm_launch<E>(G, st));
-  const double thr2 = tol * tol * (double)kInvCheckCols * n;
-  hipLaunchKernelGGL(k_inv_check<E>, dim3(batch), dim3(256), 0, st, n, Q, thr2, flag);
+  hipLaunchKernelGGL(k_inv_check<E>, dim3(batch), dim3(256), 0, st, n, Bm, Ym, tol * tol, flag);
   SSLAB_HIP_CHECK(hipGetLastError());
   std::vector<int> hf(batch);
   SSLAB_HIP_CHECK(hipMemcpyAsync(hf.data(), flag, sizeof(int) * batch, hipMemcpyDeviceToHost, st));

@@ -13,9 +13,10 @@ void inv_sizes(int n, int batch, long long* work, long long* iwork);
 // X_b = A_b^{-1} for `batch` n x n matrices (row-major; A_b = A + b sA, X_b = X + b sX;
 // A is read only and may not overlap X).  The inverse is formed by 2 x 2 block
 // elimination down to 64 x 64 diagonal blocks, which one wave inverts by Gauss-Jordan
-// with partial pivoting; every block product is an MFMA GEMM launch.  Its residual is
-// then sampled (||G - A_b (X_b G)||_F / ||G||_F for four +-1 columns G, one host
-// synchronisation); when some matrix misses `tol` (or is not finite) the whole batch is
+// with partial pivoting; every block product is an MFMA GEMM launch.  Its normwise
+// backward error is then sampled (||G - A_b X_b G||_F / (||A_b||_F ||X_b G||_F) for four
+// +-1 columns G, one host synchronisation); when some matrix misses `tol` (or is not
+// finite) the whole batch is
 // inverted again through the pivoted LU (getrf_batched + getrs_batched against I), and
 // info receives getrf's per-matrix info (else zeros).  tol <= 0 selects the default
 // (1e-11 in FP64, 1e-3 in FP32).  *fallback (may be null) is set to 1 when the LU path

@@ -61,6 +61,18 @@ __device__ __forceinline__ int wave_min(int v) {
   const int b = min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48));
   return min(a, b);
 }
+// Wave maximum of unsigned 32-bit keys.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  v = max(v, (unsigned)dpp_i32<kDppXor1>((int)v));
+  v = max(v, (unsigned)dpp_i32<kDppXor2>((int)v));
+  v = max(v, (unsigned)dpp_i32<kDppHalfMirror>((int)v));
+  v = max(v, (unsigned)dpp_i32<kDppMirror>((int)v));
+  const unsigned a = max((unsigned)__builtin_amdgcn_readlane((int)v, 0),
+                         (unsigned)__builtin_amdgcn_readlane((int)v, 16));
+  const unsigned b = max((unsigned)__builtin_amdgcn_readlane((int)v, 32),
+                         (unsigned)__builtin_amdgcn_readlane((int)v, 48));
+  return max(a, b);
+}
 // Sum over each row of 16 lanes (lanes 16 q .. 16 q + 15), the same in the row's lanes.
 template <typename E>
 __device__ __forceinline__ E row16_sum(E v) {

@@ -54,8 +54,12 @@ def test_getrs_trans_batched(T, n, nrhs):
     perm = T.empty(3, n, dtype=T.int32, device="cuda")
     call("sslab_dgetrs_trans_batched", n, nrhs, ptr(dA), n, n * n, ptr(piv), n, ptr(dB), nrhs,
          n * nrhs, ptr(X), nrhs, n * nrhs, ptr(W), ptr(perm), 3, None)
+    # Forward error bound of a backward-stable solve, cond(A) eps with a margin: the
+    # trailing updates add split-K partial sums atomically, so the rounding (not the
+    # accuracy) varies from run to run (random A of size 600 here: cond ~ 1e3).
     for b in range(3):
-        assert rel(X[b], np.linalg.solve(A[b].T, B[b])) < 1e-12
+        tol = max(1e-12, 50 * np.linalg.cond(A[b]) * np.finfo(float).eps)
+        assert rel(X[b], np.linalg.solve(A[b].T, B[b])) < tol
 
 
 @pytest.mark.parametrize("q", [5, 8])

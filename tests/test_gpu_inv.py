@@ -110,3 +110,30 @@ def test_tolerance_forces_the_lu_path(env):
     assert used == 1 and np.all(info == 0)
     for b in range(2):
         assert rel(X[b], np.linalg.inv(A[b])) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [5, 50, 64])
+def test_row_pivoting_inside_a_block(env, n):
+    """A permuted, well-conditioned matrix with a zero diagonal: every Gauss-Jordan step
+    inside the 64 x 64 base block must pivot away from the diagonal row."""
+    rng = np.random.default_rng(n)
+    P = np.eye(n)[rng.permutation(n)]
+    while np.any(np.diag(P) != 0):  # a derangement: no row may pivot on its own diagonal
+        P = np.eye(n)[rng.permutation(n)]
+    A = (P + 0.05 * rng.standard_normal((n, n)) / np.sqrt(n)) * (1 - np.eye(n))
+    X, info, used = ginv(env, A[None])
+    assert used == 0 and np.all(info == 0)
+    assert rel(X[0], np.linalg.inv(A)) <= 1e-12
+
+
+def test_scaled_matrices_pass_the_backward_error_check(env):
+    """The check is scale-free: matrices of norm 1e8 and 1e-6 (a spectral leaf operator
+    scales with p^4 / h^2) are inverted by block elimination, without the LU fallback."""
+    rng = np.random.default_rng(6)
+    n = 200
+    B = np.eye(n) + 0.3 * rng.standard_normal((n, n)) / np.sqrt(n)
+    A = np.stack([1e8 * B, 1e-6 * B])
+    X, info, used = ginv(env, A)
+    assert used == 0 and np.all(info == 0)
+    for b in range(2):
+        assert rel(X[b], np.linalg.inv(A[b])) <= 1e-12
