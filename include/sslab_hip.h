@@ -158,7 +158,7 @@ int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int
 #define SSLAB_RB_IDENTITY_DIAGONAL 1
 /* bit 1 (SSLAB_RB_LEVEL0_REDUCED, requires bit 0 and nb >= 2): V and X hold the entry
  * state written by sslab_oms_rhs_rb0 -- V's even block rows the first level's reduced
- * right-hand sides v_i - W^L_i v_{i-1} - W^R_i v_{i+1}, X's odd block rows v_j, X's even
+ * right-hand sides v_i - SiM_i v_{i-1} - SiP_i v_{i+1}, X's odd block rows v_j, X's even
  * block rows zero -- so the first level's forward sweep and the zeroing of X are
  * skipped and its backward sweep updates X in place. */
 #define SSLAB_RB_LEVEL0_REDUCED 2

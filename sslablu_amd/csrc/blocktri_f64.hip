@@ -10,16 +10,21 @@
 // the solve needs, with explicit inverses so every solve step is a GEMM:
 //
 //   odd  j = 2c+1 :  TG_c = [ Tinv_j | Tinv_j SiM_j | Tinv_j SiP_j ]      (m x 3m)
-//   even i = 2b   :  WW_b = [ SiM_i Tinv_{i-1} | SiP_i Tinv_{i+1} ]        (m x 2m)
-//   next level    :  T'_b   = T_i - W^L_b SiP_{i-1} - W^R_b SiM_{i+1}
-//                    SiM'_b = -W^L_b SiM_{i-1},   SiP'_b = -W^R_b SiP_{i+1}
+//   even i = 2b   :  WW_b = [ SiM_i | SiP_i ]                              (m x 2m)
+//   next level    :  T'_b   = T_i - SiM_i (Tinv SiP)_{i-1} - SiP_i (Tinv SiM)_{i+1}
+//                    SiM'_b = -SiM_i (Tinv SiM)_{i-1},   SiP'_b = -SiP_i (Tinv SiP)_{i+1}
+//
+// Every product of the next level reuses TG, so a level costs the inverses, TG's 4 m^3
+// per odd node and 8 m^3 per even node; forming the reference's W = SiM_i Tinv_{i-1} as
+// well (for a one-launch forward step) would add another 4 m^3 per even node, a fifth
+// of the factor.
 //
 // Tinv_j (and the coarsest block's inverse) come from inv_batched: block elimination on
 // MFMA GEMMs for all odd blocks of a level at once, with a sampled residual check that
 // falls back to the pivoted LU (the reference's lu_factor / lu_solve(., I)).
 //
-// Solve, per level, forward (one launch):  v_i -= W^L_b v_{i-1} + W^R_b v_{i+1}
-// for even i and y_j = Tinv_j v_j for odd j;  backward (one launch):
+// Solve, per level, forward (two launches):  y_j = Tinv_j v_j for odd j, then
+// v_i -= SiM_i y_{i-1} + SiP_i y_{i+1} for even i;  backward (one launch):
 // x_j = y_j - (Tinv SiM)_j x_{j-1} - (Tinv SiP)_j x_{j+1}.  Level-l node k lives
 // in global block slot k << l, so every level works in place on V and X.
 //
@@ -162,7 +167,7 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
   SSLAB_REQUIRE(nb >= 1 && m >= 1, "rb_factor: nb and m must be positive");
   SSLAB_REQUIRE(factor && work && iwork, "rb_factor: null pointer");
   // T == NULL: identity diagonal blocks (the oms S-system u_i + S u = rhs).  Level 0 then
-  // needs no LU: T^{-1} = I, W = [SiM | SiP], T' = I - W^L SiP - W^R SiM.
+  // needs no inverse: T^{-1} = I, TG = [I | SiM | SiP], T' = I - SiM SiP - SiP SiM.
   SSLAB_REQUIRE(nb == 1 || (SiM && SiP), "rb_factor: null off-diagonal blocks");
   const RbPlan p = rb_plan(nb, m, top_max);
   const long long m2 = (long long)m * m;
@@ -198,10 +203,6 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
       if (nOr < nO)
         SSLAB_TRY(zero_block_batched(m, m, TG + (long long)nOr * 3 * m2 + 2 * m, 3 * m, 3 * m2, 1,
                                      st));
-      if (nE > 1)
-        SSLAB_TRY(copy_block_batched(m, m, cM + 2 * m2, m, 2 * m2, WW + 2 * m2, 2 * m, 2 * m2,
-                                     nE - 1, st));
-      SSLAB_TRY(copy_block_batched(m, m, cP, m, 2 * m2, WW + m, 2 * m, 2 * m2, nO, st));
       if (info) zero_int_async(info + info_base, nO, st);
       info_base += nO;
       SSLAB_TRY(set_identity_batched(m, nT, m, m2, nE, st));
@@ -229,18 +230,18 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
         SSLAB_TRY(zero_block_batched(m, m, TG + (long long)nOr * 3 * m2 + 2 * m, 3 * m, 3 * m2, 1,
                                      st));
     }
-    // 5. W^L_b = SiM_{2b} Tinv_{b-1} (b >= 1),  W^R_b = SiP_{2b} Tinv_b (b < nO).
-    if (!ident) {
-      GemmLaunch G{};
-      G.nd = 2;
-      G.d[0] = gemm_desc(m, m, nE - 1, WW + 2 * m2, 2 * m, 2 * m2, nullptr, 0, 0, 0.0);
-      gemm_add_term(G.d[0], gemm_term(cM + 2 * m2, m, 2 * m2, 0, TG, 3 * m, 3 * m2, 0, m, 1.0));
-      G.d[1] = gemm_desc(m, m, nO, WW + m, 2 * m, 2 * m2, nullptr, 0, 0, 0.0);
-      gemm_add_term(G.d[1], gemm_term(cP, m, 2 * m2, 0, TG, 3 * m, 3 * m2, 0, m, 1.0));
-      SSLAB_TRY(gemm_launch(G, st));
-    }
-    // 6. Next level: T', SiM', SiP'.
+    // 5. WW_b = [SiM_{2b} | SiP_{2b}]: the even nodes' own couplings, for the forward
+    //    sweep (SiM_0 and, for odd n, the last even node's SiP do not exist).
+    if (nE > 1)
+      SSLAB_TRY(copy_block_batched(m, m, cM + 2 * m2, m, 2 * m2, WW + 2 * m2, 2 * m, 2 * m2,
+                                   nE - 1, st));
+    SSLAB_TRY(copy_block_batched(m, m, cP, m, 2 * m2, WW + m, 2 * m, 2 * m2, nO, st));
+    // 6. Next level: T', SiM', SiP' from the even nodes' couplings and TG of their odd
+    //    neighbours (left of even node b: odd node b - 1; right: odd node b).
     {
+      const E* TGM = TG + m;      // (Tinv SiM)_c, stride 3 m^2
+      const E* TGP = TG + 2 * m;  // (Tinv SiP)_c
+      const long long s3 = 3 * m2;
       GemmLaunch G{};
       int k = 0;
       const int both = (nE < nO ? nE : nO) - 1;  // b in [1, min(nE,nO))
@@ -249,31 +250,30 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
       const long long scin = ident ? m2 : 2 * m2;
       if (both > 0) {
         GemmDesc d = gemm_desc(m, m, both, nT + m2, m, m2, cin + scin, m, scin, 1.0);
-        gemm_add_term(d, gemm_term(WW + 2 * m2, 2 * m, 2 * m2, 0, cP + m2, m, 2 * m2, 0, m, -1.0));
-        gemm_add_term(d, gemm_term(WW + 2 * m2 + m, 2 * m, 2 * m2, 0, cM + 3 * m2, m, 2 * m2, 0, m,
-                                   -1.0));
+        gemm_add_term(d, gemm_term(cM + 2 * m2, m, 2 * m2, 0, TGP, 3 * m, s3, 0, m, -1.0));
+        gemm_add_term(d, gemm_term(cP + 2 * m2, m, 2 * m2, 0, TGM + s3, 3 * m, s3, 0, m, -1.0));
         G.d[k++] = d;
       }
       {  // b = 0: right neighbour only (n >= 2 so node 1 exists)
         GemmDesc d = gemm_desc(m, m, 1, nT, m, m2, cin, m, m2, 1.0);
-        gemm_add_term(d, gemm_term(WW + m, 2 * m, 2 * m2, 0, cM + m2, m, m2, 0, m, -1.0));
+        gemm_add_term(d, gemm_term(cP, m, m2, 0, TGM, 3 * m, s3, 0, m, -1.0));
         G.d[k++] = d;
       }
       if (nE > nO && nE - 1 >= 1) {  // n odd: last even node has a left neighbour only
         const long long b = nE - 1;
         GemmDesc d = gemm_desc(m, m, 1, nT + b * m2, m, m2, cin + b * scin, m, m2, 1.0);
-        gemm_add_term(d, gemm_term(WW + b * 2 * m2, 2 * m, 2 * m2, 0, cP + (2 * b - 1) * m2, m, m2,
-                                   0, m, -1.0));
+        gemm_add_term(d, gemm_term(cM + 2 * b * m2, m, m2, 0, TGP + (b - 1) * s3, 3 * m, s3, 0, m,
+                                   -1.0));
         G.d[k++] = d;
       }
-      if (nE - 1 > 0) {  // SiM'_b = -W^L_b SiM_{2b-1}, b in [1, nE)
+      if (nE - 1 > 0) {  // SiM'_b = -SiM_{2b} (Tinv SiM)_{2b-1}, b in [1, nE)
         GemmDesc d = gemm_desc(m, m, nE - 1, nM + m2, m, m2, nullptr, 0, 0, 0.0);
-        gemm_add_term(d, gemm_term(WW + 2 * m2, 2 * m, 2 * m2, 0, cM + m2, m, 2 * m2, 0, m, -1.0));
+        gemm_add_term(d, gemm_term(cM + 2 * m2, m, 2 * m2, 0, TGM, 3 * m, s3, 0, m, -1.0));
         G.d[k++] = d;
       }
-      if (nE - 1 > 0) {  // SiP'_b = -W^R_b SiP_{2b+1}, b in [0, nE-1)
+      if (nE - 1 > 0) {  // SiP'_b = -SiP_{2b} (Tinv SiP)_{2b+1}, b in [0, nE-1)
         GemmDesc d = gemm_desc(m, m, nE - 1, nP, m, m2, nullptr, 0, 0, 0.0);
-        gemm_add_term(d, gemm_term(WW + m, 2 * m, 2 * m2, 0, cP + m2, m, 2 * m2, 0, m, -1.0));
+        gemm_add_term(d, gemm_term(cP, m, 2 * m2, 0, TGP, 3 * m, s3, 0, m, -1.0));
         G.d[k++] = d;
       }
       G.nd = k;
@@ -342,7 +342,7 @@ struct RbSolveSys {
   RbPlan p;
   bool ident0;  // the first level's diagonal blocks are identities (T^{-1} = I there)
   bool rb0;     // entry state of oms_rhs_rb0 (requires ident0): V's even slots hold the
-                // level-0 reduced v_i - W^L v_{i-1} - W^R v_{i+1}, X's odd slots y_j = v_j,
+                // level-0 reduced v_i - SiM_i v_{i-1} - SiP_i v_{i+1}, X's odd slots y_j = v_j,
                 // X's even slots zero; level 0's forward sweep and X's zeroing are skipped
   int phase;    // 0: whole solve; 1: forward sweep and coarsest level only, leaving in X
                 // the backward sweep's entry state (y_j at every level's odd nodes, x at
@@ -418,46 +418,59 @@ static int rb_route(GemmLaunchT<T>& G, GemmLaunchT<T>& Gs, hipStream_t st) {
   return kOk;
 }
 
-// Forward sweep of level l:  v_i -= W^L v_{i-1} + W^R v_{i+1} (even i),  y_j = Tinv_j v_j
-// (odd j, accumulated into the zero-initialised X).
+// Forward sweep of level l, first launch: y_j = Tinv_j v_j for odd j, accumulated into the
+// zero-initialised X (with Tinv = I at an identity level 0, y_j = v_j is read from V by the
+// even update and the backward sweep instead).
 template <typename T>
-static int rb_fwd_level(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStream_t st) {
+static int rb_fwd_odd(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStream_t st) {
+  if (l == 0 && S.ident0) return kOk;
   const int m = S.m, r = S.r, ldv = S.ldv, ldx = S.ldx;
-  T* V = S.V;
-  T* X = S.X;
   const long long m2 = (long long)m * m;
   const long long bv = (long long)m * ldv, bx = (long long)m * ldx;
   const RbLevel& v = S.p.lv[l];
-  const int nE = v.nE, nO = v.nO;
   const T* TG = S.factor + v.tg_off;
-  const T* WW = S.factor + v.ww_off;
   const long long sv = (2LL << l) * bv, sx = (2LL << l) * bx;  // same-parity node stride
   const long long ov = (1LL << l) * bv, ox = (1LL << l) * bx;  // first odd node
+  GemmDescT<T> d = gemm_desc<T>(m, r, v.nO, S.X + ox, ldx, sx, S.X + ox, ldx, sx, 1.0);
+  gemm_add_term<T>(d, gemm_term<T>(TG, 3 * m, 3 * m2, 0, S.V + ov, ldv, sv, 0, m, 1.0));
+  return rb_push(G, d, st);
+}
+
+// Forward sweep of level l, second launch: v_i -= SiM_i y_{i-1} + SiP_i y_{i+1} (even i).
+template <typename T>
+static int rb_fwd_even(GemmLaunchT<T>& G, const RbSolveSys<T>& S, int l, hipStream_t st) {
+  if (l == 0 && S.rb0) return kOk;  // the caller applied it with the rhs
+  const int m = S.m, r = S.r, ldv = S.ldv;
+  T* V = S.V;
+  const long long m2 = (long long)m * m;
+  const long long bv = (long long)m * ldv;
+  const RbLevel& v = S.p.lv[l];
+  const int nE = v.nE, nO = v.nO;
+  const T* WW = S.factor + v.ww_off;
+  const long long sv = (2LL << l) * bv, ov = (1LL << l) * bv;
+  const bool from_v = l == 0 && S.ident0;
+  const T* Y = from_v ? (const T*)V : (const T*)S.X;
+  const int ldy = from_v ? ldv : S.ldx;
+  const long long by = (long long)m * ldy;
+  const long long sy = (2LL << l) * by, oy = (1LL << l) * by;
   const int both = (nE < nO ? nE : nO) - 1;
-  const bool even = !(l == 0 && S.rb0);  // else the caller applied it with the rhs
-  if (even && both > 0) {  // even b in [1, min(nE,nO))
+  if (both > 0) {  // even b in [1, min(nE,nO))
     GemmDescT<T> d = gemm_desc<T>(m, r, both, V + sv, ldv, sv, V + sv, ldv, sv, 1.0);
-    gemm_add_term<T>(d, gemm_term<T>(WW + 2 * m2, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
-    gemm_add_term<T>(d, gemm_term<T>(WW + 2 * m2 + m, 2 * m, 2 * m2, 0, V + sv + ov, ldv, sv, 0, m,
+    gemm_add_term<T>(d, gemm_term<T>(WW + 2 * m2, 2 * m, 2 * m2, 0, Y + oy, ldy, sy, 0, m, -1.0));
+    gemm_add_term<T>(d, gemm_term<T>(WW + 2 * m2 + m, 2 * m, 2 * m2, 0, Y + sy + oy, ldy, sy, 0, m,
                                      -1.0));
     SSLAB_TRY(rb_push(G, d, st));
   }
-  if (even) {  // b = 0
+  {  // b = 0
     GemmDescT<T> d = gemm_desc<T>(m, r, 1, V, ldv, sv, V, ldv, sv, 1.0);
-    gemm_add_term<T>(d, gemm_term<T>(WW + m, 2 * m, 2 * m2, 0, V + ov, ldv, sv, 0, m, -1.0));
+    gemm_add_term<T>(d, gemm_term<T>(WW + m, 2 * m, 2 * m2, 0, Y + oy, ldy, sy, 0, m, -1.0));
     SSLAB_TRY(rb_push(G, d, st));
   }
-  if (even && nE > nO && nE - 1 >= 1) {
+  if (nE > nO && nE - 1 >= 1) {
     const long long b = nE - 1;
     GemmDescT<T> d = gemm_desc<T>(m, r, 1, V + b * sv, ldv, sv, V + b * sv, ldv, sv, 1.0);
-    gemm_add_term<T>(d, gemm_term<T>(WW + b * 2 * m2, 2 * m, 2 * m2, 0, V + b * sv - ov, ldv, sv,
+    gemm_add_term<T>(d, gemm_term<T>(WW + b * 2 * m2, 2 * m, 2 * m2, 0, Y + b * sy - oy, ldy, sy,
                                      0, m, -1.0));
-    SSLAB_TRY(rb_push(G, d, st));
-  }
-  if (!(l == 0 && S.ident0)) {  // odd: y_j = Tinv_j v_j (with Tinv = I, y_j = v_j is read
-                                 // from V by the backward sweep instead)
-    GemmDescT<T> d = gemm_desc<T>(m, r, nO, X + ox, ldx, sx, X + ox, ldx, sx, 1.0);
-    gemm_add_term<T>(d, gemm_term<T>(TG, 3 * m, 3 * m2, 0, V + ov, ldv, sv, 0, m, 1.0));
     SSLAB_TRY(rb_push(G, d, st));
   }
   return kOk;
@@ -661,11 +674,20 @@ static int rb_solve_multi_t(int nsys, RbSolveSys<T>* sys, hipStream_t st) {
   G.nd = 0;
   Gs.nd = 0;
   for (int t = 0; t <= Lmax; ++t) {
+    // level t's odd products (and the coarsest blocks of the systems whose levels ended),
+    // then, once they are complete, level t's even updates, which read their y_j
     for (int s = 0; s < nsys; ++s) {
       const RbSolveSys<T>& S = sys[s];
       if (S.r == 0 || S.phase == 2) continue;
-      if (t < S.p.L) SSLAB_TRY(rb_fwd_level(Gs, S, t, st));
+      if (t < S.p.L) SSLAB_TRY(rb_fwd_odd(Gs, S, t, st));
       else if (t == S.p.L) SSLAB_TRY(rb_top(Gs, S, st));
+      SSLAB_TRY(rb_route(G, Gs, st));
+    }
+    SSLAB_TRY(rb_flush(G, st));
+    for (int s = 0; s < nsys; ++s) {
+      const RbSolveSys<T>& S = sys[s];
+      if (S.r == 0 || S.phase == 2 || t >= S.p.L) continue;
+      SSLAB_TRY(rb_fwd_even(Gs, S, t, st));
       SSLAB_TRY(rb_route(G, Gs, st));
     }
     SSLAB_TRY(rb_flush(G, st));

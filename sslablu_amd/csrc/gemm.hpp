@@ -6,7 +6,7 @@
 // One launch executes up to kMaxDesc descriptors; each descriptor is a strided
 // batch of equally-shaped products with up to kMaxTerms summed terms (the terms
 // share M and N but may have different K).  Summed terms let one launch express
-// the block-tridiagonal sweeps  v_i - W^L_i v_{i-1} - W^R_i v_{i+1}  without a
+// the block-tridiagonal sweeps  v_i - SiM_i y_{i-1} - SiP_i y_{i+1}  without a
 // second pass over the output.  Cin may alias C (in-place update).
 #pragma once
 
