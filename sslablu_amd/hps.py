@@ -27,6 +27,7 @@ maps and coefficient samples (user callables evaluated on the host).
 from __future__ import annotations
 
 import ctypes
+import functools
 
 import numpy as np
 import torch
@@ -36,6 +37,9 @@ from ._lib import call, ptr, sizes2, stream_handle
 
 F64 = torch.float64
 I32 = torch.int32
+
+# read-only device constants shared by all operators, keyed by what they depend on
+_DEV_CONST: dict = {}
 
 
 def cheb_unit(q: int):
@@ -51,6 +55,7 @@ def cheb_unit(q: int):
     return (1.0 - x) / 2.0, -2.0 * D
 
 
+@functools.lru_cache(maxsize=32)
 def leaf_constants(q: int, hx: float, hy: float):
     """Derivative matrices of one leaf and its outward-normal flux rows
     (Nb: 4p x 4p on the edge nodes, Ni: 4p x p^2 on the interior nodes)."""
@@ -73,7 +78,10 @@ def leaf_constants(q: int, hx: float, hy: float):
             Nb[e, 3 * p + k] += sgn * Dy[b0, q - 1]
             for b2 in range(1, q - 1):
                 Ni[e, (a - 1) * p + (b2 - 1)] += sgn * Dy[b0, b2]
-    return dict(t=t, Dx=Dx, Dxx=Dxx, Dy=Dy, Dyy=Dyy, Nb=Nb, Ni=Ni)
+    out = dict(t=t, Dx=Dx, Dxx=Dxx, Dy=Dy, Dyy=Dyy, Nb=Nb, Ni=Ni)
+    for a in out.values():
+        a.setflags(write=False)  # cached: shared by every caller
+    return out
 
 
 def _box_layout(nxb, nyb, p):
@@ -84,6 +92,7 @@ def _box_layout(nxb, nyb, p):
     return dict(left=0, right=L, bottom=2 * L, top=2 * L + B, n=2 * (L + B), L=L, B=B)
 
 
+@functools.lru_cache(maxsize=256)
 def merge_maps(kind, nxb, nyb, p):
     """Index maps of merging two nxb x nyb boxes across (kind 'h': alpha left of beta)
     or up (kind 'v': alpha below beta).  Returns (ca, cb, na, nb, n_new, (nxb', nyb'))."""
@@ -110,9 +119,12 @@ def merge_maps(kind, nxb, nyb, p):
                               none(s["L"]), s["right"] + ar(s["L"]),
                               none(s["B"]), s["top"] + ar(s["B"])])
         new = (nxb, 2 * nyb)
+    for a in (ca, cb, na, nbm):
+        a.setflags(write=False)  # cached: shared by every caller
     return ca, cb, na, nbm, len(na), new
 
 
+@functools.lru_cache(maxsize=64)
 def pair_maps(nx, ny, p):
     """Index maps of the double-slab pair solve: shared edge and the column layout
     [alpha.left | beta.right | alpha.bottom | beta.bottom | alpha.top | beta.top]."""
@@ -126,6 +138,8 @@ def pair_maps(nx, ny, p):
                            s["top"] + ar(B), none(B)])
     colb = np.concatenate([none(m), s["right"] + ar(m), none(B), s["bottom"] + ar(B),
                            none(B), s["top"] + ar(B)])
+    for a in (ca, cb, cola, colb):
+        a.setflags(write=False)  # cached: shared by every caller
     return ca, cb, cola, colb
 
 
@@ -257,19 +271,43 @@ class HPSSlabOperator:
 
     # -------------------------------------------------------------------------
     def _dev(self, a, dtype=F64):
-        return torch.as_tensor(np.ascontiguousarray(a)).to(device=self.device, dtype=dtype)
+        a = np.ascontiguousarray(a)
+        if not a.flags.writeable:  # torch wraps only writable arrays
+            a = a.copy()
+        return torch.as_tensor(a).to(device=self.device, dtype=dtype)
+
+    def _const(self, key, make, dtype=F64):
+        """Device copy of a read-only constant (leaf matrices, index maps) that depends on
+        `key` only, shared by every operator on the device: later builds skip the host
+        construction and the synchronous upload."""
+        k = (key, str(self.device), dtype)
+        t = _DEV_CONST.get(k)
+        if t is None:
+            if len(_DEV_CONST) >= 512:
+                _DEV_CONST.clear()
+            t = self._dev(make(), dtype)
+            _DEV_CONST[k] = t
+        return t
 
     def leaf_dtn(self, stream=None):
         g = self.g
         q, p = g.q, g.p
-        c = leaf_constants(q, g.hx, g.hy)
         P = g.interior_points()
         pdo = self.pdo
-        coefs = []
-        for name in ("c11", "c22", "c1", "c2", "c"):
-            v = _coef(getattr(pdo, name, None), P)
-            coefs.append(self._dev(v) if v is not None else None)
-        consts = {k: self._dev(c[k]) for k in ("Dx", "Dxx", "Dy", "Dyy", "Nb", "Ni")}
+        vals = [_coef(getattr(pdo, name, None), P) for name in ("c11", "c22", "c1", "c2", "c")]
+        # the coefficient samples go up in one copy from pinned memory
+        have = [i for i, v in enumerate(vals) if v is not None]
+        coefs = [None] * len(vals)
+        if have:
+            host = torch.empty((len(have), P.shape[0]), dtype=F64, pin_memory=True)
+            for r, i in enumerate(have):
+                host[r].numpy()[:] = vals[i]
+            dev = host.to(self.device, non_blocking=True)
+            for r, i in enumerate(have):
+                coefs[i] = dev[r]
+        ckey = ("leaf2", q, float(g.hx), float(g.hy))
+        consts = {k: self._const(ckey + (k,), lambda k=k: leaf_constants(q, g.hx, g.hy)[k])
+                  for k in ("Dx", "Dxx", "Dy", "Dyy", "Nb", "Ni")}
         nl = g.nleaves
         T = torch.empty(nl, 4 * p, 4 * p, dtype=F64, device=self.device)
         X = torch.empty(nl, p * p, 4 * p, dtype=F64, device=self.device) if self.keep_leaf_ops else None
@@ -298,7 +336,8 @@ class HPSSlabOperator:
             n_old = boxes.shape[1]
             nmerge = boxes.shape[0] // 2
             c = len(ca)
-            maps = [self._dev(x, I32) for x in (ca, cb, na, nbm)]
+            maps = [self._const(("merge2", kind, nxb, nyb, p, j), lambda x=x: x, I32)
+                    for j, x in enumerate((ca, cb, na, nbm))]
             out = torch.empty(nmerge, n_new, n_new, dtype=F64, device=self.device)
             wd, wi = sizes2("sslab_hps_merge_sizes", nmerge, n_new, c)
             work = torch.empty(wd, dtype=F64, device=self.device)
@@ -337,7 +376,8 @@ class HPSSlabOperator:
         ca, cb, cola, colb = pair_maps(g.nx, g.ny, p)
         ncols = len(cola)
         n_s = Ts.shape[1]
-        maps = [self._dev(x, I32) for x in (ca, cb, cola, colb)]
+        maps = [self._const(("pair2", g.nx, g.ny, p, j), lambda x=x: x, I32)
+                for j, x in enumerate((ca, cb, cola, colb))]
         Y = torch.empty(nb, m, ncols, dtype=F64, device=self.device)
         wd, wi = sizes2("sslab_hps_merge_sizes", nb, ncols, m)
         work = torch.empty(wd, dtype=F64, device=self.device)
