@@ -995,7 +995,7 @@ def main():
             step()  # warm-up on a side stream before capture
         stream.wait_stream(side)
         graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph):
+        with torch.cuda.graph(graph):  # (a bench-only capture; nothing is freed meanwhile)
             step()
         replay_err = []
         for _ in range(max(2, args.warmup)):  # every warm-up replay checked against eager

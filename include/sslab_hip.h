@@ -91,7 +91,7 @@ int sslab_dgetrs_batched(int n, int nrhs, const double* LU, int lda, long long s
  * by 2 x 2 block elimination on MFMA GEMMs down to 64 x 64 blocks that one wave inverts
  * with partial pivoting; the normwise backward error ||G - A X G||_F / (||A||_F ||X G||_F)
  * is then sampled on four +-1 columns G (one stream synchronisation) and, when it exceeds
- * tol (tol <= 0: 1e-11) or is not finite for some matrix, the whole batch is inverted
+ * tol (tol <= 0: 1e-12) or is not finite for some matrix, the whole batch is inverted
  * again by pivoted LU against the identity.  info (device, batch ints, may be NULL): that LU's info, else zeros;
  * *used_lu (host, may be NULL): 1 when the LU ran.  work / iwork: sslab_inv_sizes.
  * Replaces the explicit block inverses of the red-black levels, lu_factor(T) followed by
@@ -179,6 +179,18 @@ int sslab_rb_solve(int nb, int m, int nrhs, const double* factor, double* V, int
 #define SSLAB_RB_CHAINED_BACKWARD 16
 int sslab_rb_solve_ex(int nb, int m, int nrhs, const double* factor, double* V, int ldv,
                       double* X, int ldx, int flags, void* stream);
+
+/* The forward sweep and coarsest level of sslab_rb_solve_ex(..., SSLAB_RB_IDENTITY_DIAGONAL
+ * | SSLAB_RB_FORWARD_ONLY) for nb >= 2 and a V (nb*m x ncols, ldv) whose block row i is
+ * zero outside the columns [lo[i], hi[i]) (host int arrays of length nb): every product
+ * touches only the columns where its operands can be nonzero.  X (zeroed here) receives
+ * the same entry state; V is overwritten.  Builds the boundary-map fold's maps from the rhs
+ * maps of a column-permuted identity, whose block rows read one interval of boundary
+ * columns each.  No reference counterpart (the reference forms rhs per solve,
+ * multislab/oms.py:180-186, and sweeps every column, direct_solve/omsdirectsolve.py:
+ * 221-279). */
+int sslab_rb_forward_windowed(int nb, int m, int ncols, const double* factor, double* V, int ldv,
+                              double* X, int ldx, const int* lo, const int* hi, void* stream);
 
 /* Whether a chained backward sweep on the current GPU gave up waiting since the last
  * call (*aborted = 1: a tile's bounded wait ran out and that solve's result is not

@@ -54,6 +54,8 @@ _SIGS = {
     "sslab_rb_factor": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "sslab_rb_solve": [_c_int, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _vp],
     "sslab_rb_solve_ex": [_c_int, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _c_int, _vp],
+    "sslab_rb_forward_windowed": [_c_int, _c_int, _c_int, _vp, _vp, _c_int, _vp, _c_int, _vp, _vp,
+                                  _vp],
     "sslab_rb_chain_status": [_vp],
     "sslab_rb_solve_multi": [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "sslab_cast_f64_to_f32": [_c_ll, _vp, _vp, _vp],

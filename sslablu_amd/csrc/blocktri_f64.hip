@@ -40,6 +40,7 @@
 // sequential variant: A_i = SiM_i B_{i-1}^{-1}, B_i = T_i - A_i SiP_{i-1}.
 #include "blocktri_f64.hpp"
 
+#include <algorithm>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -68,7 +69,8 @@ struct RbPlan {
   // the stacked [I | SiM | SiP] right-hand sides.
   long long set_blocks;    // blocks per array in one ping-pong set
   long long work_doubles;
-  long long work_ints;     // ipiv + perm + info
+  long long work_ints;     // ipiv + perm + info, then the deferred inverse checks' flags
+  long long flag_off;      // int offset of those flags (one per inverted block, as info)
   int max_odd;
   int piv_blocks;          // ipiv / perm capacity in blocks of m: max(max_odd, n_top)
 };
@@ -120,6 +122,8 @@ RbPlan rb_plan(int nb, int m, int top_max = 0) {
     p.work_doubles += sp.factor_doubles + sp.work_doubles + nt2;
     if (sp.work_ints > p.work_ints) p.work_ints = sp.work_ints;
   }
+  p.flag_off = p.work_ints;
+  p.work_ints += rb_info_count(nb);
   return p;
 }
 
@@ -159,9 +163,13 @@ template <typename T>
 static int rb_solve_sub(int nb, int m, int r, const T* factor, T* V, int ldv, T* X, int ldx,
                         hipStream_t st, int top_max = 1);
 
+// inv_mode: kInvDeferred (every inverse's check writes its flags at `flags`, one int per
+// inverted block in the layout of info, for the caller to read) or kInvLU (pivoted LU
+// inverses only).
 template <typename E>
 static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E* factor, E* work,
-                       int* iwork, int* info, hipStream_t st, int top_max = 0) {
+                       int* iwork, int* info, hipStream_t st, int top_max, int inv_mode,
+                       int* flags) {
   using GemmLaunch = GemmLaunchT<E>;
   using GemmDesc = GemmDescT<E>;
   SSLAB_REQUIRE(nb >= 1 && m >= 1, "rb_factor: nb and m must be positive");
@@ -212,7 +220,8 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
       // 4.   TG_j[:, m:3m] = Tinv_j [SiM_j | SiP_j] as one MFMA GEMM launch (SiP of a
       //      last odd node without a right neighbour is zero).
       SSLAB_TRY(inv_batched<E>(m, cT + m2, m, 2 * m2, TG, 3 * m, 3 * m2, nO, LUw, ipiv,
-                               info ? info + info_base : nullptr, 0.0, nullptr, st));
+                               info ? info + info_base : nullptr, 0.0, nullptr, st, inv_mode,
+                               flags ? flags + info_base : nullptr));
       info_base += nO;
       {
         GemmLaunch G{};
@@ -296,7 +305,8 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
   }
   if (nt == 1)
     return inv_batched<E>(m, cT, m, m2, factor + p.top_off, m, m2, 1, LUw, ipiv,
-                          info ? info + info_base : nullptr, 0.0, nullptr, st);
+                          info ? info + info_base : nullptr, 0.0, nullptr, st, inv_mode,
+                          flags ? flags + info_base : nullptr);
   // nt > 1: red-black factor of the nt-node system (its blocks are the current level set,
   // stride m^2) down to one block -- its info entries fill this plan's remaining slots --
   // then its solve for the identity gives the inverse.
@@ -305,19 +315,43 @@ static int rb_factor_t(int nb, int m, const E* T, const E* SiM, const E* SiP, E*
   E* subW = subF + sp.factor_doubles;
   E* Iw = subW + sp.work_doubles;
   SSLAB_TRY(rb_factor_t<E>(nt, m, cT, cM, cP, subF, subW, iwork, info ? info + info_base : nullptr,
-                           st, 1));
+                           st, 1, inv_mode, flags ? flags + info_base : nullptr));
   SSLAB_TRY(set_identity_batched(Nt, Iw, Nt, nt2, 1, st));
   return rb_solve_sub<E>(nt, m, Nt, subF, Iw, Nt, factor + p.top_off, Nt, st);
 }
 
+// The whole factorization with every inverse's backward-error check deferred: one
+// synchronisation reads all the flags at the end, and if any inverse missed its
+// tolerance the factorization is redone with pivoted-LU inverses throughout (every level
+// depends on the ones before it).
+template <typename E>
+static int rb_factor_checked(int nb, int m, const E* T, const E* SiM, const E* SiP, E* factor,
+                             E* work, int* iwork, int* info, hipStream_t st) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1, "rb_factor: nb and m must be positive");
+  SSLAB_REQUIRE(iwork, "rb_factor: null pointer");
+  const RbPlan p = rb_plan(nb, m);
+  const int nflag = rb_info_count(nb);
+  int* flags = iwork + p.flag_off;
+  zero_int_async(flags, nflag, st);
+  SSLAB_TRY(rb_factor_t<E>(nb, m, T, SiM, SiP, factor, work, iwork, info, st, 0, kInvDeferred,
+                           flags));
+  std::vector<int> hf(nflag);
+  SSLAB_HIP_CHECK(hipMemcpyAsync(hf.data(), flags, sizeof(int) * nflag, hipMemcpyDeviceToHost, st));
+  SSLAB_HIP_CHECK(hipStreamSynchronize(st));
+  for (int i = 0; i < nflag; ++i)
+    if (hf[i] != 0)
+      return rb_factor_t<E>(nb, m, T, SiM, SiP, factor, work, iwork, info, st, 0, kInvLU, nullptr);
+  return kOk;
+}
+
 int rb_factor(int nb, int m, const double* T, const double* SiM, const double* SiP,
               double* factor, double* work, int* iwork, int* info, hipStream_t st) {
-  return rb_factor_t<double>(nb, m, T, SiM, SiP, factor, work, iwork, info, st);
+  return rb_factor_checked<double>(nb, m, T, SiM, SiP, factor, work, iwork, info, st);
 }
 
 int rb_factor_f32(int nb, int m, const float* T, const float* SiM, const float* SiP,
                   float* factor, float* work, int* iwork, int* info, hipStream_t st) {
-  return rb_factor_t<float>(nb, m, T, SiM, SiP, factor, work, iwork, info, st);
+  return rb_factor_checked<float>(nb, m, T, SiM, SiP, factor, work, iwork, info, st);
 }
 
 int rb_info_count(int nb) {
@@ -751,6 +785,98 @@ int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, 
   const int phase = (flags & 4) ? 1 : ((flags & 8) ? 2 : 0);
   return rb_solve_t<double>(nb, m, r, factor, V, ldv, X, ldx, st, (flags & 1) != 0,
                             (flags & 2) != 0, phase, (flags & 16) != 0);
+}
+
+// Forward sweep and coarsest level of the identity-diagonal red-black solve for a V whose
+// block row i is zero outside the columns [lo[i], hi[i]) (host arrays): every product reads
+// and writes only the columns where its operands can be nonzero -- an odd node's y_j keeps
+// the support of v_j, an even node's update takes the union of its own and its
+// neighbours' supports, and the coarsest level adds each Inv_kj v_j into x_k over v_j's
+// support.  X (zeroed here) receives the backward sweep's entry state, exactly as
+// rb_solve_ex with the identity-diagonal and forward-only flags leaves it; V is
+// overwritten.  The boundary-map fold runs this on the rhs maps of the identity, whose
+// block rows read a few boundary windows each.
+int rb_forward_windowed(int nb, int m, int ncols, const double* factor, double* V, int ldv,
+                        double* X, int ldx, const int* lo, const int* hi, hipStream_t st) {
+  SSLAB_REQUIRE(nb >= 2 && m >= 1 && ncols >= 1, "rb_forward_windowed: bad sizes");
+  SSLAB_REQUIRE(factor && V && X && (void*)V != (void*)X && lo && hi && ldv >= ncols &&
+                    ldx >= ncols,
+                "rb_forward_windowed: bad pointers / strides");
+  std::vector<int> slo(nb), shi(nb);
+  for (int i = 0; i < nb; ++i) {
+    SSLAB_REQUIRE(lo[i] >= 0 && lo[i] < hi[i] && hi[i] <= ncols,
+                  "rb_forward_windowed: block %d support [%d, %d) outside [0, %d)", i, lo[i],
+                  hi[i], ncols);
+    slo[i] = lo[i];
+    shi[i] = hi[i];
+  }
+  const RbPlan p = rb_plan(nb, m);
+  const long long m2 = (long long)m * m;
+  const long long bv = (long long)m * ldv, bx = (long long)m * ldx;
+  SSLAB_TRY(zero_2d<double>((long long)nb * m, ncols, X, ldx, st));
+  GemmLaunch G{};
+  G.nd = 0;
+  for (int l = 0; l < p.L; ++l) {
+    const RbLevel& v = p.lv[l];
+    const double* TG = factor + v.tg_off;
+    const double* WW = factor + v.ww_off;
+    if (l > 0) {  // y_j = Tinv_j v_j (level 0: T = I, y_j = v_j is read from V)
+      for (int c = 0; c < v.nO; ++c) {
+        const long long j = (long long)(2 * c + 1) << l;
+        const int w = shi[j] - slo[j];
+        GemmDesc d = gemm_desc(m, w, 1, X + j * bx + slo[j], ldx, 0, X + j * bx + slo[j], ldx, 0,
+                               1.0);
+        gemm_add_term(d, gemm_term(TG + (long long)c * 3 * m2, 3 * m, 0, 0, V + j * bv + slo[j],
+                                   ldv, 0, 0, m, 1.0));
+        SSLAB_TRY(rb_push(G, d, st));
+      }
+      SSLAB_TRY(rb_flush(G, st));
+    }
+    const double* Y = l == 0 ? (const double*)V : (const double*)X;
+    const long long by = l == 0 ? bv : bx;
+    const int ldy = l == 0 ? ldv : ldx;
+    for (int b = 0; b < v.nE; ++b) {  // v_i -= SiM_i y_{i-1} + SiP_i y_{i+1}
+      const long long i = (long long)(2 * b) << l;
+      const bool left = b >= 1, right = b < v.nO;
+      const long long jl = i - (1LL << l), jr = i + (1LL << l);
+      int nlo = slo[i], nhi = shi[i];
+      if (left) nlo = std::min(nlo, slo[jl]), nhi = std::max(nhi, shi[jl]);
+      if (right) nlo = std::min(nlo, slo[jr]), nhi = std::max(nhi, shi[jr]);
+      GemmDesc d = gemm_desc(m, nhi - nlo, 1, V + i * bv + nlo, ldv, 0, V + i * bv + nlo, ldv, 0,
+                             1.0);
+      if (left)
+        gemm_add_term(d, gemm_term(WW + (long long)b * 2 * m2, 2 * m, 0, 0, Y + jl * by + nlo, ldy,
+                                   0, 0, m, -1.0));
+      if (right)
+        gemm_add_term(d, gemm_term(WW + (long long)b * 2 * m2 + m, 2 * m, 0, 0, Y + jr * by + nlo,
+                                   ldy, 0, 0, m, -1.0));
+      SSLAB_TRY(rb_push(G, d, st));
+      slo[i] = nlo;
+      shi[i] = nhi;
+    }
+    SSLAB_TRY(rb_flush(G, st));
+  }
+  // level 0's y_j = v_j into X (zero outside the supports, so whole rows are copied)
+  SSLAB_TRY(copy_block_batched<double>(m, ncols, V + bv, ldv, 2 * bv, X + bx, ldx, 2 * bx,
+                                       p.lv[0].nO, st));
+  // coarsest level: x_k += Inv_kj v_j over v_j's support, atomically when several j
+  const int nt = p.n_top, L = p.L;
+  const long long Nt = (long long)nt * m;
+  const double* Inv = factor + p.top_off;
+  for (int k = 0; k < nt; ++k) {
+    const long long sk = (long long)k << L;
+    for (int j = 0; j < nt; ++j) {
+      const long long sj = (long long)j << L;
+      const int w = shi[sj] - slo[sj];
+      GemmDesc d = gemm_desc(m, w, 1, X + sk * bx + slo[sj], ldx, 0, X + sk * bx + slo[sj], ldx, 0,
+                             1.0);
+      d.atomic = nt > 1;
+      gemm_add_term(d, gemm_term(Inv + (long long)k * m * Nt + (long long)j * m, (int)Nt, 0, 0,
+                                 V + sj * bv + slo[sj], ldv, 0, 0, m, 1.0));
+      SSLAB_TRY(rb_push(G, d, st));
+    }
+  }
+  return rb_flush(G, st);
 }
 
 int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,

@@ -30,6 +30,11 @@ int rb_solve_ex(int nb, int m, int r, const double* factor, double* V, int ldv, 
 // Abort flag of the dependency-ordered backward sweeps on the current GPU (synchronises
 // the device; clears the flag).
 int rb_chain_status(int* aborted);
+// Forward sweep + coarsest level of the identity-diagonal red-black solve for a V whose
+// block rows are zero outside host-given column intervals [lo[i], hi[i]); X receives the
+// backward sweep's entry state.  See blocktri_f64.hip.
+int rb_forward_windowed(int nb, int m, int ncols, const double* factor, double* V, int ldv,
+                        double* X, int ldx, const int* lo, const int* hi, hipStream_t st);
 int rb_solve_multi(int nsys, const int* nb, const int* m, const int* r, const double* const* factor,
                    double* const* V, const int* ldv, double* const* X, const int* ldx,
                    hipStream_t st);

@@ -214,6 +214,12 @@ int sslab_rb_solve_ex(int nb, int m, int nrhs, const double* factor, double* V, 
   return sslab::rb_solve_ex(nb, m, nrhs, factor, V, ldv, X, ldx, flags, as_stream(stream));
 }
 
+int sslab_rb_forward_windowed(int nb, int m, int ncols, const double* factor, double* V, int ldv,
+                              double* X, int ldx, const int* lo, const int* hi, void* stream) {
+  return sslab::rb_forward_windowed(nb, m, ncols, factor, V, ldv, X, ldx, lo, hi,
+                                    as_stream(stream));
+}
+
 int sslab_rb_chain_status(int* aborted) { return sslab::rb_chain_status(aborted); }
 
 int sslab_rb_solve_multi(int nsys, const int* nb, const int* m, const int* nrhs,

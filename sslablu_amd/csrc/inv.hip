@@ -285,6 +285,34 @@ int inv_rec(int n, const E* A, int lda, long long sA, E* X, int ldx, long long s
   return gemm_launch<E>(G, st);
 }
 
+template <typename E>
+int inv_elim_check(int n, const E* A, int lda, long long sA, E* X, int ldx, long long sX,
+                   int batch, E* work, double tol, int* flag, hipStream_t st) {
+  const long long rec = inv_rec_elems(n);
+  SSLAB_TRY(inv_rec<E>(n, A, lda, sA, X, ldx, sX, work, rec, batch, st));
+  // B_b = [X_b G | G], Y_b = A_b B_b
+  constexpr int C = kInvCheckCols;
+  E* Gs = work;
+  E* Bm = Gs + (long long)C * n;
+  E* Ym = Bm + 2LL * C * n * batch;
+  const long long sB = 2LL * C * n;
+  const long long nfill = (long long)n * C * (batch + 1);
+  hipLaunchKernelGGL(k_inv_fill_g<E>, dim3((unsigned)((nfill + 255) / 256)), dim3(256), 0, st, n,
+                     batch, Gs, Bm);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  GemmLaunchT<E> G{};
+  G.nd = 1;
+  G.d[0] = gemm_desc<E>(n, C, batch, Bm, 2 * C, sB, nullptr, 0, 0, 0.0);
+  gemm_add_term<E>(G.d[0], gemm_term<E>(X, ldx, sX, 0, Gs, C, 0, 0, n, 1.0));
+  SSLAB_TRY(gemm_launch<E>(G, st));
+  G.d[0] = gemm_desc<E>(n, 2 * C, batch, Ym, 2 * C, sB, nullptr, 0, 0, 0.0);
+  gemm_add_term<E>(G.d[0], gemm_term<E>(A, lda, sA, 0, Bm, 2 * C, sB, 0, n, 1.0));
+  SSLAB_TRY(gemm_launch<E>(G, st));
+  hipLaunchKernelGGL(k_inv_check<E>, dim3(batch), dim3(256), 0, st, n, Bm, Ym, tol * tol, flag);
+  SSLAB_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
 }  // namespace
 
 void inv_sizes(int n, int batch, long long* work, long long* iwork) {
@@ -300,47 +328,34 @@ void inv_sizes(int n, int batch, long long* work, long long* iwork) {
 
 template <typename E>
 int inv_batched(int n, const E* A, int lda, long long sA, E* X, int ldx, long long sX, int batch,
-                E* work, int* iwork, int* info, double tol, int* fallback, hipStream_t st) {
+                E* work, int* iwork, int* info, double tol, int* fallback, hipStream_t st,
+                int mode, int* flags) {
   if (fallback) *fallback = 0;
   SSLAB_REQUIRE(n >= 1 && batch >= 0 && lda >= n && ldx >= n, "inv_batched: bad sizes");
+  SSLAB_REQUIRE(mode >= kInvChecked && mode <= kInvLU && (mode != kInvDeferred || flags),
+                "inv_batched: bad mode or missing flags");
   if (batch == 0) return kOk;
   SSLAB_REQUIRE(A && X && work && iwork, "inv_batched: null pointer");
-  if (tol <= 0) tol = sizeof(E) == 8 ? 1e-11 : 1e-3;  // normwise backward error
-  const long long rec = inv_rec_elems(n);
-  SSLAB_TRY(inv_rec<E>(n, A, lda, sA, X, ldx, sX, work, rec, batch, st));
-
-  // sampled backward error: B_b = [X_b G | G], Y_b = A_b B_b
-  constexpr int C = kInvCheckCols;
-  E* Gs = work;
-  E* Bm = Gs + (long long)C * n;
-  E* Ym = Bm + 2LL * C * n * batch;
-  const long long sB = 2LL * C * n;
-  int* flag = iwork + 2LL * n * batch;
-  const long long nfill = (long long)n * C * (batch + 1);
-  hipLaunchKernelGGL(k_inv_fill_g<E>, dim3((unsigned)((nfill + 255) / 256)), dim3(256), 0, st, n,
-                     batch, Gs, Bm);
-  SSLAB_HIP_CHECK(hipGetLastError());
-  GemmLaunchT<E> G{};
-  G.nd = 1;
-  G.d[0] = gemm_desc<E>(n, C, batch, Bm, 2 * C, sB, nullptr, 0, 0, 0.0);
-  gemm_add_term<E>(G.d[0], gemm_term<E>(X, ldx, sX, 0, Gs, C, 0, 0, n, 1.0));
-  SSLAB_TRY(gemm_launch<E>(G, st));
-  G.d[0] = gemm_desc<E>(n, 2 * C, batch, Ym, 2 * C, sB, nullptr, 0, 0, 0.0);
-  gemm_add_term<E>(G.d[0], gemm_term<E>(A, lda, sA, 0, Bm, 2 * C, sB, 0, n, 1.0));
-  SSLAB_TRY(gemm_launch<E>(G, st));
-  hipLaunchKernelGGL(k_inv_check<E>, dim3(batch), dim3(256), 0, st, n, Bm, Ym, tol * tol, flag);
-  SSLAB_HIP_CHECK(hipGetLastError());
-  std::vector<int> hf(batch);
-  SSLAB_HIP_CHECK(hipMemcpyAsync(hf.data(), flag, sizeof(int) * batch, hipMemcpyDeviceToHost, st));
-  SSLAB_HIP_CHECK(hipStreamSynchronize(st));
-  bool bad = false;
-  for (int b = 0; b < batch; ++b) bad = bad || hf[b] != 0;
-  if (!bad) {
-    if (info) {
-      hipLaunchKernelGGL(k_inv_zero_int, dim3(ceil_div(batch, 256)), dim3(256), 0, st, info, batch);
-      SSLAB_HIP_CHECK(hipGetLastError());
+  if (tol <= 0) tol = sizeof(E) == 8 ? 1e-12 : 1e-3;  // normwise backward error
+  if (mode != kInvLU) {
+    int* flag = mode == kInvDeferred ? flags : iwork + 2LL * n * batch;
+    SSLAB_TRY(inv_elim_check<E>(n, A, lda, sA, X, ldx, sX, batch, work, tol, flag, st));
+    bool bad = false;
+    if (mode == kInvChecked) {
+      std::vector<int> hf(batch);
+      SSLAB_HIP_CHECK(
+          hipMemcpyAsync(hf.data(), flag, sizeof(int) * batch, hipMemcpyDeviceToHost, st));
+      SSLAB_HIP_CHECK(hipStreamSynchronize(st));
+      for (int b = 0; b < batch; ++b) bad = bad || hf[b] != 0;
     }
-    return kOk;
+    if (!bad) {
+      if (info) {
+        hipLaunchKernelGGL(k_inv_zero_int, dim3(ceil_div(batch, 256)), dim3(256), 0, st, info,
+                           batch);
+        SSLAB_HIP_CHECK(hipGetLastError());
+      }
+      return kOk;
+    }
   }
   // pivoted LU against the identity for the whole batch
   if (fallback) *fallback = 1;
@@ -356,8 +371,8 @@ int inv_batched(int n, const E* A, int lda, long long sA, E* X, int ldx, long lo
 }
 
 template int inv_batched<double>(int, const double*, int, long long, double*, int, long long, int,
-                                 double*, int*, int*, double, int*, hipStream_t);
+                                 double*, int*, int*, double, int*, hipStream_t, int, int*);
 template int inv_batched<float>(int, const float*, int, long long, float*, int, long long, int,
-                                float*, int*, int*, double, int*, hipStream_t);
+                                float*, int*, int*, double, int*, hipStream_t, int, int*);
 
 }  // namespace sslab

@@ -19,10 +19,17 @@ void inv_sizes(int n, int batch, long long* work, long long* iwork);
 // finite) the whole batch is
 // inverted again through the pivoted LU (getrf_batched + getrs_batched against I), and
 // info receives getrf's per-matrix info (else zeros).  tol <= 0 selects the default
-// (1e-11 in FP64, 1e-3 in FP32).  *fallback (may be null) is set to 1 when the LU path
+// (1e-12 in FP64, 1e-3 in FP32).  *fallback (may be null) is set to 1 when the LU path
 // ran.  work / iwork: at least inv_sizes(n, batch).
+//
+// mode kInvDeferred: no synchronisation and no fallback -- the check writes flags[b] = 1
+// for every matrix that misses tol (else 0, device ints) and info (if any) is zeroed; the
+// caller reads the flags once after a chain of such calls and redoes the chain with
+// kInvLU (the pivoted LU alone) when one is set.
+enum InvMode { kInvChecked = 0, kInvDeferred = 1, kInvLU = 2 };
 template <typename E>
 int inv_batched(int n, const E* A, int lda, long long sA, E* X, int ldx, long long sX, int batch,
-                E* work, int* iwork, int* info, double tol, int* fallback, hipStream_t st);
+                E* work, int* iwork, int* info, double tol, int* fallback, hipStream_t st,
+                int mode = kInvChecked, int* flags = nullptr);
 
 }  // namespace sslab

@@ -10,6 +10,8 @@ and returns the captured output panel (overwritten by the next call).
 """
 from __future__ import annotations
 
+import gc
+
 import torch
 
 
@@ -26,8 +28,16 @@ class CapturedSolve:
             solve(self.B)  # warm-up on the capture stream (code objects, allocator pools)
         torch.cuda.current_stream(dev).wait_stream(side)
         self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            self.X = torch.as_tensor(solve(self.B), device=dev)
+        # no garbage collection while capturing: cyclic garbage freed then (another graph,
+        # pinned memory) would touch the device or other streams inside the capture
+        gc_was = gc.isenabled()
+        gc.disable()
+        try:
+            with torch.cuda.graph(self.graph):
+                self.X = torch.as_tensor(solve(self.B), device=dev)
+        finally:
+            if gc_was:
+                gc.enable()
 
     def __call__(self, B: torch.Tensor) -> torch.Tensor:
         if B.shape != self.B.shape:

@@ -645,16 +645,62 @@ class BoundaryFold:
         self.handle = None
         s = stream if stream is not None else torch.cuda.current_stream()
         with torch.cuda.device(op.device), torch.cuda.stream(s):
-            R = op.rhs_from_boundary(torch.eye(self.nF, dtype=F64, device=op.device), stream=s)
-            M = torch.empty_like(R)
-            fac.solve_(R, M, stream=s, phase="forward")
-            del R
+            if nb >= 2:
+                # Boundary columns reordered [left | (bottom_s, top_s) per slab s | right], so
+                # block row i of the rhs maps reads the one interval of slabs i and i+1 (plus
+                # the left / right column block at the ends); the windowed forward sweep
+                # keeps every product on the columns its operands can touch.
+                u = 2 * nxp
+                nsl = g.N
+                bot = np.arange(nsl * nxp).reshape(nsl, nxp)
+                top = nsl * nxp + bot
+                left = 2 * nsl * nxp + np.arange(m)
+                perm = np.concatenate([left, np.concatenate([bot, top], axis=1).ravel(), m + left])
+                lo = (ctypes.c_int * nb)(*[0 if i == 0 else m + u * i for i in range(nb)])
+                hi = (ctypes.c_int * nb)(*[self.nF if i == nb - 1 else m + u * (i + 2)
+                                           for i in range(nb)])
+                pd = torch.as_tensor(perm, device=op.device)
+                R = self._permuted_rhs_maps(op)
+                Mp = torch.empty_like(R)
+                call("sslab_rb_forward_windowed", nb, m, self.nF, ptr(fac.factor), ptr(R),
+                     R.stride(0), ptr(Mp), Mp.stride(0), ctypes.addressof(lo),
+                     ctypes.addressof(hi), stream_handle(s))
+                del R
+                inv = torch.empty_like(pd)
+                inv[pd] = torch.arange(self.nF, device=op.device)
+                M = Mp.index_select(1, inv)
+                del Mp
+            else:
+                R = op.rhs_from_boundary(torch.eye(self.nF, dtype=F64, device=op.device),
+                                         stream=s)
+                M = torch.empty_like(R)
+                fac.solve_(R, M, stream=s, phase="forward")
+                del R
             h = ctypes.c_void_p()
             call("sslab_oms_fold_create", nb, m, self.nF, ptr(M), M.stride(0),
                  ctypes.addressof(h), stream_handle(s))
             s.synchronize()  # the compact copies have read M
             del M
         self.handle = h.value
+
+    def _permuted_rhs_maps(self, op):
+        """The rhs maps of the column-permuted identity, rhs_from_boundary(I[:, perm]):
+        block row i holds -Y_i's boundary-data columns at its window [m + u i, m + u (i+2))
+        (slab i's bottom and top, then slab i+1's), block 0 also -Y_0[:, :m] at the left
+        columns [0, m) and block nb-1 -Y_{nb-1}[:, m:2m] at the right ones -- the values the
+        rhs GEMM computes for this identity, placed by one gather instead."""
+        g = op.g
+        m, nb, nxp = g.m, g.nb, g.nx * g.p
+        u, nF = 2 * nxp, self.nF
+        R = torch.zeros(nb * m, nF, dtype=F64, device=op.device)
+        o = np.arange(nxp)
+        ycols = 2 * m + np.concatenate([o, 2 * nxp + o, nxp + o, 3 * nxp + o])
+        Yb = op.Y.index_select(2, torch.as_tensor(ycols, device=op.device))
+        W = R.as_strided((nb, m, 2 * u), (m * nF + u, nF, 1), m)
+        W.copy_(Yb).neg_()
+        R[:m, :m] = -op.Y[0, :, :m]
+        R[(nb - 1) * m:, nF - m:] = -op.Y[nb - 1, :, m:2 * m]
+        return R
 
     @property
     def nbytes(self) -> int:

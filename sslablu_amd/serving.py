@@ -23,6 +23,7 @@ times from a request's arrival to the completion of the batch that carried it.
 """
 from __future__ import annotations
 
+import gc
 import math
 import time
 from collections import OrderedDict, deque
@@ -164,7 +165,10 @@ class FactorCache:
             victim = next((s for s, e in self.ops.items() if e.pins == 0), None)
             if victim is None:
                 return False
-            del self.ops[victim]
+            e = self.ops.pop(victim)
+            # its captured programs go now, outside any capture (a graph destroyed while
+            # another stream is capturing aborts the process)
+            e.graphs.clear()
             self.evictions += 1
         return self.nbytes + need <= self.budget and len(self.ops) < self.max_ops
 
@@ -244,7 +248,11 @@ class _BatchProgram:
     def __init__(self, e: CachedOperator, W: int, precision: str, iters: int, use_graph: bool,
                  stream):
         dev = e.op.device
-        self.e, self.W, self.precision, self.iters = e, W, precision, iters
+        # the operator's parts, not the CachedOperator itself: the operator keeps its
+        # programs (e.graphs), and a reference back would make a cycle that only the
+        # garbage collector frees -- possibly in the middle of another program's capture
+        self.op, self.fac, self.fold = e.op, e.fac, e.fold
+        self.W, self.precision, self.iters = W, precision, iters
         self.Fb = torch.zeros(e.ngb, W, dtype=F64, device=dev)
         folded = precision != "mixed" and e.fold is not None
         self.V = None if folded else torch.empty(e.N, W, dtype=F64, device=dev)
@@ -261,8 +269,16 @@ class _BatchProgram:
         stream.synchronize()
         if use_graph:
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g, stream=stream):
-                self._run()
+            # no garbage collection while capturing: freeing unrelated cyclic garbage (a
+            # graph, pinned memory) on another stream during the capture is not allowed
+            gc_was = gc.isenabled()
+            gc.disable()
+            try:
+                with torch.cuda.graph(g, stream=stream):
+                    self._run()
+            finally:
+                if gc_was:
+                    gc.enable()
             stream.synchronize()
             with torch.cuda.stream(stream):
                 g.replay()  # the first replay uploads the graph: pay it here, not in a batch
@@ -270,15 +286,14 @@ class _BatchProgram:
             self.graph = g
 
     def _run(self):
-        e = self.e
         if self.V is None:  # FP64: folded forward sweep, then the backward sweep
-            e.fold.solve_(self.Fb, self.Xb)
+            self.fold.solve_(self.Fb, self.Xb)
             return
-        e.op.rhs_from_boundary(self.Fb, out=self.V)
+        self.op.rhs_from_boundary(self.Fb, out=self.V)
         if self.precision == "mixed":
-            e.fac.solve_mixed_(self.V, self.Xb, iters=self.iters, work=self.work)
+            self.fac.solve_mixed_(self.V, self.Xb, iters=self.iters, work=self.work)
         else:
-            e.fac.solve_(self.V, self.Xb)
+            self.fac.solve_(self.V, self.Xb)
 
     def launch(self):
         if self.graph is not None:

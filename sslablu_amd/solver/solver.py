@@ -166,7 +166,7 @@ class solverWrapper:
             g = self.op.g
             dev = self.op.device
             Ta, Tb = self.op.Ts[0], self.op.Ts[1]
-            ca, cb, cola, colb = (torch.as_tensor(x, device=dev)
+            ca, cb, cola, colb = (torch.tensor(x, device=dev)  # copies the cached maps
                                   for x in pair_maps(g.nx, g.ny, g.p))
             ina, inb = cola >= 0, colb >= 0
             ja, jb = cola.clamp(min=0), colb.clamp(min=0)

@@ -336,3 +336,70 @@ def test_rb_split_phases(torch_dev, nb, m, ident):
     assert rel(X1.cpu().numpy(), X0.cpu().numpy()) < 1e-13
     with pytest.raises(ValueError):
         BlockTriFactor("thomas").factorize(T, SiM, SiP).solve_(B.clone(), X1, phase="forward")
+
+
+@pytest.mark.parametrize("bad_level", [0, 1])
+def test_rb_factor_recovers_from_a_failed_inverse(torch_dev, bad_level):
+    """The red-black factor defers every inverse's backward-error check to one read at the
+    end; a diagonal block whose leading 64 x 64 block is zero breaks the block elimination,
+    so the whole factorization must be redone with pivoted-LU inverses and still solve the
+    system (against the oracle's Thomas solve and the assembled operator).  bad_level 0
+    plants the block in an odd node (inverted at level 0); 1 plants it in node 2, which
+    reaches level 1's inverses through its Schur complement (node 2's couplings vanish in
+    those 64 rows, so the block stays exactly zero there)."""
+    torch, dev = torch_dev
+    from sslablu_amd.blocktri import BlockTriFactor
+    rng = np.random.default_rng(11 + bad_level)
+    nb, m, r, h = 7, 128, 3, 64
+    T = [np.eye(m) + 0.3 * rng.standard_normal((m, m)) / np.sqrt(m) for _ in range(nb)]
+    B = np.eye(h) + 0.3 * rng.standard_normal((h, h)) / np.sqrt(h)
+    C = np.eye(h) + 0.3 * rng.standard_normal((h, h)) / np.sqrt(h)
+    T[1 if bad_level == 0 else 2] = np.block([[np.zeros((h, h)), B], [C, np.zeros((h, h))]])
+    S = [(0.02 * rng.standard_normal((m, m)) / np.sqrt(m) if i > 0 else np.zeros((m, m)),
+          0.02 * rng.standard_normal((m, m)) / np.sqrt(m) if i < nb - 1 else np.zeros((m, m)))
+         for i in range(nb)]
+    if bad_level == 1:
+        S[2][0][:h] = 0.0
+        S[2][1][:h] = 0.0
+    rhs = rng.standard_normal((nb * m, r))
+    fac = BlockTriFactor("rb").factorize(np.stack(T), np.stack([s[0] for s in S]),
+                                         np.stack([s[1] for s in S]))
+    x = fac.solve(torch.tensor(rhs, device=dev)).cpu().numpy()
+    x_ref = ob.thomas_direct(S, T, rhs)
+    assert rel(x, x_ref) < 1e-11
+    A = ob.assemble_dense(S, T)
+    assert np.linalg.norm(A @ x - rhs) / np.linalg.norm(rhs) < 1e-12
+
+
+@pytest.mark.parametrize("nb,m,ncols", [(2, 24, 90), (5, 24, 200), (8, 40, 300), (13, 16, 257)])
+def test_rb_forward_windowed_matches_forward_only(torch_dev, nb, m, ncols):
+    """sslab_rb_forward_windowed on a V whose block rows vanish outside random column
+    intervals gives the entry state of the dense forward-only sweep (identity diagonal
+    blocks) on the same V."""
+    import ctypes
+    torch, dev = torch_dev
+    from sslablu_amd.blocktri import BlockTriFactor
+    from sslablu_amd._lib import call, ptr, stream_handle
+    rng = np.random.default_rng(nb * 7 + m)
+    S = [(0.5 * rng.standard_normal((m, m)) / np.sqrt(m), 0.5 * rng.standard_normal((m, m)) / np.sqrt(m))
+         for _ in range(nb)]
+    fac = BlockTriFactor("rb").factorize(None, np.stack([s[0] for s in S]),
+                                         np.stack([s[1] for s in S]))
+    lo = rng.integers(0, ncols - 1, nb)
+    hi = np.minimum(ncols, lo + 1 + rng.integers(0, ncols // 3, nb))
+    V = np.zeros((nb * m, ncols))
+    for i in range(nb):
+        V[i * m:(i + 1) * m, lo[i]:hi[i]] = rng.standard_normal((m, hi[i] - lo[i]))
+    V0 = torch.tensor(V, device=dev)
+    X0 = torch.empty_like(V0)
+    call("sslab_rb_solve_ex", nb, m, ncols, ptr(fac.factor), ptr(V0), ncols, ptr(X0), ncols,
+         1 | 4, stream_handle())
+    V1 = torch.tensor(V, device=dev)
+    X1 = torch.full_like(V1, float("nan"))
+    lo_a = (ctypes.c_int * nb)(*lo.tolist())
+    hi_a = (ctypes.c_int * nb)(*hi.tolist())
+    call("sslab_rb_forward_windowed", nb, m, ncols, ptr(fac.factor), ptr(V1), ncols, ptr(X1),
+         ncols, ctypes.addressof(lo_a), ctypes.addressof(hi_a), stream_handle())
+    torch.cuda.synchronize()
+    assert torch.isfinite(X1).all()
+    assert (torch.linalg.norm(X1 - X0) / torch.linalg.norm(X0)).item() < 1e-13

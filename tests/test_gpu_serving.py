@@ -114,3 +114,31 @@ def test_cache_eviction_and_budget(S):
     with pytest.raises(RuntimeError):
         serving.BatchingServer(tiny).serve(
             serving.make_requests(specs[:1], 2, math.inf, rmax=2, seed=5), realtime=False)
+
+
+def test_evicted_operator_is_freed_without_the_garbage_collector(S):
+    """An evicted operator and its captured programs are released when the cache drops
+    them, not later by the cyclic garbage collector (which may run inside another
+    program's graph capture, where destroying a graph is not allowed): with the collector
+    off, the evicted entry is gone right after the eviction."""
+    import gc
+    import weakref
+    torch, serving, specs = S
+    cache = serving.FactorCache(max_operators=1)
+    srv = serving.BatchingServer(cache, slots=1, max_cols=8)
+    e0 = cache.acquire(specs[0])
+    srv._program(e0, 8, srv.streams[0])  # a captured program, kept in e0.graphs
+    assert e0.graphs
+    cache.release(e0)
+    ref = weakref.ref(e0)
+    del e0
+    gc_was = gc.isenabled()
+    gc.disable()
+    try:
+        e1 = cache.acquire(specs[1])  # evicts specs[0]
+        assert cache.evictions == 1
+        assert ref() is None
+        cache.release(e1)
+    finally:
+        if gc_was:
+            gc.enable()
