@@ -18,7 +18,9 @@ Prints one JSON line (rank 0, N = 1 only).
 from __future__ import annotations
 
 import argparse
+import contextlib
 import dataclasses
+import gc
 import itertools
 import json
 import os
@@ -28,6 +30,25 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+@contextlib.contextmanager
+def no_gc_pauses():
+    """Timed regions run without Python's cyclic garbage collector, as timeit runs them:
+    the heap is collected and frozen first, then the collector stays off until the region
+    ends.  A full collection of this process's heap takes tens of milliseconds of host time
+    whenever allocation counts happen to trigger one, which stalls the launch loop (a
+    fresh-box run measured one 36 ms pause inside 200 steps of 0.09 ms)."""
+    gc.collect()
+    gc.freeze()
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
+        gc.unfreeze()
 sys.path.insert(0, ROOT)
 
 KAPPA = 20.0
@@ -204,13 +225,13 @@ def serving_bench(dev, n_req=2000, seed=0):
            "timing": "steady state: one untimed pass of the request set per path first",
            "factor_s_total": round(cache.factor_s, 2)}
     runs = {}
-    plain = fresh(base)
-    runs["plain_fp64"] = sv.serve_plain(cache, plain, realtime=False)
-    b64 = fresh(base)
-    runs["batched_fp64"] = srv64.serve(b64, realtime=False)
-    runs["batched_fp64_eager"] = srv64g.serve(fresh(base), realtime=False)
-    bmx = fresh(base)
-    runs["batched_mixed"] = srvmx.serve(bmx, realtime=False)
+    plain, b64, bmx = fresh(base), fresh(base), fresh(base)
+    eager = fresh(base)
+    with no_gc_pauses():  # every path timed without collector pauses
+        runs["plain_fp64"] = sv.serve_plain(cache, plain, realtime=False)
+        runs["batched_fp64"] = srv64.serve(b64, realtime=False)
+        runs["batched_fp64_eager"] = srv64g.serve(eager, realtime=False)
+        runs["batched_mixed"] = srvmx.serve(bmx, realtime=False)
     err64 = max(((a.X - b.X).norm() / b.X.norm()).item() for a, b in zip(b64, plain))
     errmx = max(((a.X - b.X).norm() / b.X.norm()).item() for a, b in zip(bmx, plain))
     sat = {k: {"solves_per_s": round(v.solves_per_s, 1), "requests_per_s": round(v.requests_per_s, 1),
@@ -230,7 +251,9 @@ def serving_bench(dev, n_req=2000, seed=0):
     rt = {}
     for name, fn in (("plain_fp64", lambda q: sv.serve_plain(cache, q)),
                      ("batched_fp64", lambda q: srv64.serve(q))):
-        st = fn(fresh(base, arr))
+        reqs = fresh(base, arr)
+        with no_gc_pauses():
+            st = fn(reqs)
         rt[name] = {"solves_per_s": round(st.solves_per_s, 1), "p50_ms": round(st.p50_ms, 3),
                     "p99_ms": round(st.p99_ms, 3), "batches": st.batches}
     out["poisson_arrivals"] = {"rate_per_s": round(rate, 1), **rt}
@@ -858,7 +881,8 @@ def main():
 
     factor()  # warm-up (library load, allocator, code objects)
     torch.cuda.synchronize()
-    op, fac, fold, tf = factor()
+    with no_gc_pauses():
+        op, fac, fold, tf = factor()
     t_factor = tf[2] - tf[0]
     t_assembly_rb = tf[1] - tf[0]
     t_fold = tf[2] - tf[1]
@@ -958,9 +982,6 @@ def main():
     tuning.set_policy(tuning.lookup(fac.tune_key(NRHS)) or pol)
     tuning.set_backward_policies(getattr(fac, "_bwd_policies", {}).get(NRHS, ()))
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
     def timed(run):
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(args.steps)]
@@ -973,15 +994,34 @@ def main():
         wall_ = time.perf_counter() - t0
         return wall_, np.array([e0.elapsed_time(e1) for e0, e1 in ev])
 
-    # throughput: K steps back to back, bracketed by synchronisations (no per-step events);
-    # latency percentiles: the same K steps again, each between two events on the stream
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    wall_ev, lat_ms = timed(step)
+    # latency percentiles: K steps, each between two events on the stream; throughput: the
+    # same K steps again, back to back, bracketed by synchronisations (no per-step events).
+    # The event-timed pass goes first so that the back-to-back pass, short when K is small,
+    # does not start on a device still ramping up from the setup's gaps; the collector is
+    # stopped before the warm-up, so that its collection pass does not leave the device
+    # idle between the warm-up and the timed steps.
+    # Device warm-up (setup, untimed, reported): the step runs back to back for at least
+    # 0.1 s and 200 steps first, so the timed steps see the clocks of a busy device (a fresh
+    # run measured 0.0995 ms per step after 5 warm-up steps, 0.0924 after 200 and 0.0922
+    # after 3000; the autotuner, when it runs, has the same effect).
+    with no_gc_pauses():
+        n_dw, t_dw = 0, time.perf_counter()
+        while n_dw < 200 or time.perf_counter() - t_dw < 0.1:
+            for _ in range(50):
+                step()
+            n_dw += 50
+            torch.cuda.synchronize()
+        t_dw = time.perf_counter() - t_dw
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        wall_ev, lat_ms = timed(step)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
 
     # ---- the same step replayed from a HIP graph (captured once, one host launch per
     # step).  Reported beside the eager launches, which stay the headline: on this stack
@@ -1035,10 +1075,11 @@ def main():
         def timed_factor(fn):
             fn()
             torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            r_ = fn()
-            torch.cuda.synchronize()
-            return r_, time.perf_counter() - t0
+            with no_gc_pauses():
+                t0 = time.perf_counter()
+                r_ = fn()
+                torch.cuda.synchronize()
+                return r_, time.perf_counter() - t0
 
         _, t_rb64 = timed_factor(lambda: BlockTriFactor("rb", dev).factorize(None, op.SiM, op.SiP))
         fac32, t_rb32 = timed_factor(
@@ -1066,13 +1107,14 @@ def main():
 
             for _ in range(args.warmup):
                 step_mixed()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()  # throughput as for the FP64 headline: no per-step events
-            for _ in range(args.steps):
-                step_mixed()
-            torch.cuda.synchronize()
-            wall_m = time.perf_counter() - t0
-            _, lat_m = timed(step_mixed)
+            with no_gc_pauses():
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()  # throughput as for the FP64 headline: no per-step events
+                for _ in range(args.steps):
+                    step_mixed()
+                torch.cuda.synchronize()
+                wall_m = time.perf_counter() - t0
+                _, lat_m = timed(step_mixed)
             step_mixed()  # the error of the timed configuration (after the autotuned policy)
             err_timed = (torch.linalg.norm(Xm - X) / torch.linalg.norm(X)).item()
             mixed = {"value": round(NRHS * args.steps / wall_m, 1), "unit": "RHS solves/s",
@@ -1134,9 +1176,12 @@ def main():
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
         "launch": "eager (five kernel launches per step)",
-        "timing": "value / ms_per_step: K steps back to back between synchronisations; "
-                  "p50 / p99: the K steps again, each between two HIP events",
+        "timing": "after the W warm-up steps, p50 / p99: K steps, each between two HIP "
+                  "events; value / ms_per_step: the K steps again, back to back between "
+                  "synchronisations",
         "ms_per_step_with_events": round(wall_ev * 1e3 / args.steps, 4),
+        "device_warmup": {"steps": n_dw, "s": round(t_dw, 3),
+                          "note": "untimed setup before the W warm-up steps"},
         "graph_replay": graph_info,
         "factor_time_s": round(t_factor, 4),
         "factor_breakdown_s": {"hps_assembly_and_rb_factor": round(t_assembly_rb, 4),
