@@ -55,12 +55,61 @@ struct GemmCfg {
   static_assert((BM * BK) % NT == 0 && (BK * BN) % NT == 0, "load split");
 };
 
-// Map the hardware block id so that contiguous ranges of logical tiles land on the
-// same XCD (the dispatcher assigns workgroups round-robin to the 8 XCDs, each with
-// its own L2).  Bijective on [0, n).
-__device__ __forceinline__ int xcd_remap(int bid, int n) {
-  const int q = n >> 3;
-  if (bid < (q << 3)) return (bid & 7) * q + (bid >> 3);
+// Write one wave's accumulator fragments: added atomically (split K, atomic descriptors:
+// C already holds beta Cin) or stored as C = acc + beta Cin.  Every Cin value of the
+// wave is loaded before the first store: Cin may be C itself (in-place products), so the
+// compiler cannot move a load above an earlier store, and value by value the epilogue
+// would wait for one memory round trip per element.  Fragment (i, j) covers rows
+// row_base + 16 i + Mfma::row(lane, v), columns col_base + 16 j + lane % 16.
+template <typename T, int FM, int FN>
+__device__ __forceinline__ void store_acc(const typename Mfma<T>::v4 (&acc)[FM][FN], T* C,
+                                          int ldc, const T* Cin, int ldcin, T beta, bool split,
+                                          int row_base, int col_base, int M, int N, int lane) {
+  using M4 = Mfma<T>;
+  const bool use_cin = !split && Cin != nullptr && beta != T(0);
+  T cv[FM][FN][4];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = col_base + j * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row_base + i * 16 + M4::row(lane, v);
+        cv[i][j][v] = (use_cin && row < M && col < N) ? Cin[(long long)row * ldcin + col] : T(0);
+      }
+    }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = col_base + j * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row_base + i * 16 + M4::row(lane, v);
+        if (row < M && col < N) {
+          if (split) atomicAdd(&C[(long long)row * ldc + col], acc[i][j][v]);
+          else C[(long long)row * ldc + col] = acc[i][j][v] + beta * cv[i][j][v];
+        }
+      }
+    }
+}
+
+// Map the hardware block id to a logical tile (the dispatcher assigns workgroups
+// round-robin to the 8 XCDs, each with its own L2).  Chunks of cz consecutive logical
+// tiles -- the tiles of one product, which share its A and B -- are dealt round-robin to
+// the XCDs, so every XCD works on whole products and a launch whose products differ in
+// cost (summed K, split parts) is spread evenly over the chip; the tail of fewer than
+// 8 cz tiles is cut into 8 contiguous ranges.  Bijective on [0, n).
+__device__ __forceinline__ int xcd_remap(int bid, int n, int cz) {
+  const long long span = 8LL * cz;
+  const int full = (int)(n / span * span);
+  if (bid < full) {
+    const int j = bid >> 3;  // position among the XCD's workgroups
+    return ((bid & 7) + 8 * (j / cz)) * cz + j % cz;
+  }
+  const int b = bid - full, q = (n - full) >> 3;
+  if (b < (q << 3)) return full + (b & 7) * q + (b >> 3);
   return bid;
 }
 
@@ -151,7 +200,7 @@ __global__ void __launch_bounds__(Cfg::NT) k_gemm(const GemmLaunchT<T> L, int to
   __shared__ T As[Cfg::BK * Cfg::LDA_S];
   __shared__ T Bs[Cfg::BK * Cfg::LDB_S];
 
-  const int tile = xcd_remap(blockIdx.x, total_tiles);
+  const int tile = xcd_remap(blockIdx.x, total_tiles, L.xcd_chunk);
 
   // Select the descriptor with a fully unrolled scan (constant kernarg offsets).
   GemmDescT<T> D = L.d[0];
@@ -226,30 +275,8 @@ __global__ void __launch_bounds__(Cfg::NT) k_gemm(const GemmLaunchT<T> L, int to
 
   T* C = D.C + (long long)b * D.strideC;
   const T* Cin = D.Cin ? D.Cin + (long long)b * D.strideCin : nullptr;
-  const T beta = D.beta;
-  const bool use_cin = (Cin != nullptr) && (beta != T(0));
-  // C already holds beta*Cin: accumulate atomically
-  const bool split = D.ksplit > 1 || D.atomic;
-#pragma unroll
-  for (int i = 0; i < Cfg::FM; ++i) {
-#pragma unroll
-    for (int j = 0; j < Cfg::FN; ++j) {
-      const int col = col0 + wcol + j * 16 + (lane & 15);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = row0 + wrow + i * 16 + M4::row(lane, v);
-        if (row < M && col < N) {
-          T val = acc[i][j][v];
-          if (split) {
-            atomicAdd(&C[(long long)row * D.ldc + col], val);
-          } else {
-            if (use_cin) val += beta * Cin[(long long)row * D.ldcin + col];
-            C[(long long)row * D.ldc + col] = val;
-          }
-        }
-      }
-    }
-  }
+  store_acc<T, Cfg::FM, Cfg::FN>(acc, C, D.ldc, Cin, D.ldcin, D.beta, D.ksplit > 1 || D.atomic,
+                                 row0 + wrow, col0 + wcol, M, N, lane);
 }
 
 // ---- fast path: every term as stored (row-major), 16-byte aligned operands ------------
@@ -627,37 +654,15 @@ __device__ __forceinline__ void fast_tile(const GemmDescT<T>& D, int tile, T* sm
 
   T* C = D.C + (long long)b * D.strideC;
   const T* Cin = D.Cin ? D.Cin + (long long)b * D.strideCin : nullptr;
-  const T beta = D.beta;
-  const bool use_cin = (Cin != nullptr) && (beta != T(0));
-  const bool split = D.ksplit > 1 || D.atomic;
-  const int ldc = D.ldc, ldcin = D.ldcin;
-#pragma unroll
-  for (int i = 0; i < Cfg::FM; ++i) {
-#pragma unroll
-    for (int j = 0; j < Cfg::FN; ++j) {
-      const int col = col0 + wcol + j * 16 + li;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = row0 + wrow + i * 16 + M4::row(lane, v);
-        if (row < M && col < N) {
-          T val = acc[i][j][v];
-          if (split) {
-            atomicAdd(&C[(long long)row * ldc + col], val);
-          } else {
-            if (use_cin) val += beta * Cin[(long long)row * ldcin + col];
-            C[(long long)row * ldc + col] = val;
-          }
-        }
-      }
-    }
-  }
+  store_acc<T, Cfg::FM, Cfg::FN>(acc, C, D.ldc, Cin, D.ldcin, D.beta, D.ksplit > 1 || D.atomic,
+                           row0 + wrow, col0 + wcol, M, N, lane);
 }
 
 template <class Cfg, typename T, int kMode>
 __global__ void __launch_bounds__(Cfg::NT) __attribute__((amdgpu_waves_per_eu(2)))
 k_gemm_fast(const GemmLaunchT<T> L, int total_tiles) {
   __shared__ __attribute__((aligned(16))) T smem[2 * FastGeom<Cfg, T>::STAGE];
-  const int tile = xcd_remap(blockIdx.x, total_tiles);
+  const int tile = xcd_remap(blockIdx.x, total_tiles, L.xcd_chunk);
   int di = 0;
 #pragma unroll
   for (int d = 1; d < kMaxDesc; ++d)
@@ -791,7 +796,7 @@ k_gemm_adirect(const GemmLaunchT<T> L, int total_tiles) {
   constexpr int FM = Cfg::FM, FN = Cfg::FN;
   __shared__ __attribute__((aligned(16))) T smem[2 * G::STAGE];
 
-  const int tile = xcd_remap(blockIdx.x, total_tiles);
+  const int tile = xcd_remap(blockIdx.x, total_tiles, L.xcd_chunk);
   int di = 0;
 #pragma unroll
   for (int d = 1; d < kMaxDesc; ++d)
@@ -918,30 +923,8 @@ k_gemm_adirect(const GemmLaunchT<T> L, int total_tiles) {
 
   T* C = D.C + (long long)b * D.strideC;
   const T* Cin = D.Cin ? D.Cin + (long long)b * D.strideCin : nullptr;
-  const T beta = D.beta;
-  const bool use_cin = (Cin != nullptr) && (beta != T(0));
-  const bool split = D.ksplit > 1 || D.atomic;
-  const int ldc = D.ldc, ldcin = D.ldcin;
-#pragma unroll
-  for (int i = 0; i < FM; ++i) {
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int col = col0 + j * 16 + li;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = row0 + wrow + i * 16 + M4::row(lane, v);
-        if (row < M && col < N) {
-          T val = acc[i][j][v];
-          if (split) {
-            atomicAdd(&C[(long long)row * ldc + col], val);
-          } else {
-            if (use_cin) val += beta * Cin[(long long)row * ldcin + col];
-            C[(long long)row * ldc + col] = val;
-          }
-        }
-      }
-    }
-  }
+  store_acc<T, FM, FN>(acc, C, D.ldc, Cin, D.ldcin, D.beta, D.ksplit > 1 || D.atomic,
+                           row0 + wrow, col0, M, N, lane);
 }
 
 // 0: some term uses a transposed operand (the LDS-transposing kernel k_gemm);
@@ -1070,6 +1053,18 @@ static long long split_chunk(const LaunchT& L, int ksplit, int maxparts) {
   return lo * Cfg::BK;
 }
 
+// Tiles per XCD chunk (xcd_remap): the fewest tiles of one product (output tiles x K
+// parts) over the launch's descriptors, so no XCD holds a run of one descriptor's products.
+template <typename T>
+static int xcd_chunk(const GemmLaunchT<T>& P, int total) {
+  int cz = total;
+  for (int i = 0; i < P.nd; ++i) {
+    const int per = P.d[i].tiles_m * P.d[i].tiles_n * P.d[i].ksplit;
+    cz = per < cz ? per : cz;
+  }
+  return cz < 1 ? 1 : cz;
+}
+
 // ksplit > 1 splits the output tiles' summed K ranges over about ksplit x (output tiles)
 // workgroups in all (split_chunk), at most maxparts per tile.
 template <class Cfg, typename T, bool kADirect = false>
@@ -1119,6 +1114,7 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit, int max
     P.d[P.nd++] = d;
   }
   if (total == 0) return kOk;
+  P.xcd_chunk = xcd_chunk(P, total);
   if constexpr (kADirect && ADGeom<Cfg, T>::ok) {
     int nmax = 0;
     for (int i = 0; i < P.nd; ++i) nmax = P.d[i].N > nmax ? P.d[i].N : nmax;

@@ -60,6 +60,7 @@ template <typename T>
 struct GemmLaunchT {
   GemmDescT<T> d[kMaxDesc];
   int nd;
+  int xcd_chunk;  // logical tiles per XCD chunk (set by gemm_launch)
 };
 
 // A dependency-ordered launch (k_gemm_chain): up to kChainDesc descriptors whose tiles

@@ -8,11 +8,11 @@
 //   X12 = -W Y,  X21 = -Y Z               (one grouped launch)
 //   X11 = X11 - X12 Z                     (one launch, in place)
 // so an n x n inverse is 2 n^3 flops of MFMA GEMMs plus the 64 x 64 diagonal blocks,
-// which k_inv_gj inverts with one workgroup of eight waves per matrix (Gauss-Jordan with
-// partial pivoting inside the block, one barrier per pivot step).  The pivoted LU this replaces
-// (getrf_batched + getrs_batched against I) spends about 2 us per column on a one-
-// workgroup panel; here the sequential part is the 64-column blocks and four launches per
-// halving, and every launch covers the whole batch.
+// which k_inv_gj inverts with one workgroup of sixteen waves per matrix (Gauss-Jordan
+// with partial pivoting inside the block, one barrier per panel of four pivot steps).  The
+// pivoted LU this replaces (getrf_batched + getrs_batched against I) spends about 2 us per
+// column on a one-workgroup panel; here the sequential part is the 64-column blocks and
+// four launches per halving, and every launch covers the whole batch.
 //
 // The elimination does not pivot between blocks, so its accuracy rests on the diagonal
 // blocks and Schur complements being well conditioned (true of the red-black levels of
@@ -36,7 +36,7 @@ namespace sslab {
 namespace {
 
 constexpr int kInvBase = 64;  // diagonal blocks inverted by one workgroup (Gauss-Jordan)
-constexpr int kGjWaves = 8;   // waves of that workgroup; each owns kInvBase / kGjWaves columns
+constexpr int kGjWaves = 16;  // waves of that workgroup; each owns kInvBase / kGjWaves columns
 constexpr int kInvCheckCols = 4;
 
 inline int inv_split(int n) {
@@ -63,18 +63,23 @@ __device__ __forceinline__ unsigned gj_key(double x) {
 }
 __device__ __forceinline__ unsigned gj_key(float x) { return __float_as_uint(fabsf(x)) + 1u; }
 
-// One workgroup of eight waves inverts one n x n block (n <= 64) by in-place Gauss-Jordan
-// with partial pivoting.  Lane i of wave w keeps row i restricted to the wave's eight
-// columns 8w .. 8w + 7, as s_i * (stored values).  Step k belongs to the wave that
-// owns column k: it picks the pivot row p (the largest |a_i[k]| over the rows not pivoted
-// yet, first lane on ties), and publishes p, 1 / R[k] and every row's multiplier
-// c_i = a_i[k] / R[k] (c_p = 0) through LDS -- one barrier per step, the buffers
-// alternating by step parity so the next step's writes never meet this step's reads.
-// Every wave then reads the pivot row R of its own columns from its lane p (readlane),
-// subtracts c_i R from its rows and, in the owning wave, sets a_i[k] = -c_i (a_p[k] = 1);
-// lane p sets its scale to 1 / R[k].  The pivot row is broadcast unscaled and its own
-// values stay as they were, so no lane runs a divergent row update.  After step k the row
-// pivoted at step k holds row k of the inverse with its columns permuted:
+// One workgroup of kGjWaves waves inverts one n x n block (n <= 64) by in-place
+// Gauss-Jordan with partial pivoting.  Lane i of wave w keeps row i restricted to the
+// wave's CW = 64 / kGjWaves columns CW w .. CW w + CW - 1, as s_i * (stored values).  The
+// steps go by panels: the CW steps k of panel g pivot on the columns of wave g.  Step k
+// picks the pivot row p (the largest |a_i[k]| over the rows not pivoted yet, first lane on
+// ties), sets every row's multiplier c_i = a_i[k] / R[k] (c_p = 0) and, in every column j,
+// subtracts c_i R[j] from row i (R: the pivot row, read from lane p with readlane);
+// column k then becomes a_i[k] = -c_i (a_p[k] = 1) and lane p's scale 1 / R[k].  The
+// pivot row is used unscaled and keeps its own values, so no lane runs a divergent row
+// update.  Wave g runs its panel's steps on its own columns alone and then publishes the
+// panel's pivots, 1 / R[k] and multipliers through LDS (one barrier per panel, the buffers
+// alternating by panel parity so that the next panel's writes never meet this panel's
+// reads); every other wave then applies the same steps, in the same order, to its columns.
+// A column sees exactly the operations of step-by-step Gauss-Jordan, so the result does
+// not depend on the panel width; only the barriers (one per CW steps) and the
+// serialisation change: panel g + 1 starts once wave g + 1 has applied panel g.  After
+// step k the row pivoted at step k holds row k of the inverse with its columns permuted:
 // X[k][p_j] = s_i a_i[j] for i = p_k, where p_j is the row pivoted at step j; the rows are
 // put in place through an LDS tile so that the stores are coalesced.  Rows / columns
 // n..63 are the identity's and never pivot.
@@ -83,12 +88,12 @@ __global__ __launch_bounds__(kGjWaves * 64) void k_inv_gj(int n, const E* __rest
                                                 long long sA, E* __restrict__ X, int ldx,
                                                 long long sX) {
   constexpr int NB = kInvBase;
-  constexpr int CW = NB / kGjWaves;  // columns per wave
+  constexpr int CW = NB / kGjWaves;  // columns per wave = steps per panel
   constexpr int LD = NB + 1;
   __shared__ E tile[NB * LD];
-  __shared__ E cbuf[2][NB];
-  __shared__ E ibuf[2];
-  __shared__ int pbuf[2];
+  __shared__ E cbuf[2][CW][NB];
+  __shared__ E ibuf[2][CW];
+  __shared__ int pbuf[2][CW];
   __shared__ int pl[NB];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
@@ -109,51 +114,60 @@ __global__ __launch_bounds__(kGjWaves * 64) void k_inv_gj(int n, const E* __rest
   }
   E s = E(1);
   int step = -1;  // step at which this row was the pivot (-1: not yet)
-  for (int kw = 0; kw * CW < n; ++kw) {
+  for (int g = 0; g * CW < n; ++g) {
+    const int par = g & 1;
+    if (w == g) {
 #pragma unroll
-    for (int kk = 0; kk < CW; ++kk) {
-      const int k = kw * CW + kk;
-      if (k >= n) continue;  // uniform over the workgroup (no barrier is skipped by some)
-      const int par = k & 1;
-      E c_own = E(0), ip_own = E(0);
-      int p_own = 0;
-      if (w == kw) {
-        const unsigned key = (real && step < 0) ? gj_key(a[kk]) : 0u;
-        const unsigned kmax = wave::wave_max_u32(key);
-        // the maximum is attained by some candidate row (n - k of them remain)
-        const unsigned long long bal = __ballot(key == kmax);
-        p_own = __builtin_ctzll(bal);
-        const E piv = wave::readlane_e(a[kk], p_own);
-        ip_own = E(1) / piv;
-        c_own = (lane == p_own) ? E(0) : a[kk] * ip_own;
-        cbuf[par][lane] = c_own;
-        if (lane == 0) {
-          pbuf[par] = p_own;
-          ibuf[par] = ip_own;
-          pl[k] = p_own;
+      for (int kk = 0; kk < CW; ++kk) {
+        const int k = g * CW + kk;
+        if (k < n) {  // uniform over the wave
+          const unsigned key = (real && step < 0) ? gj_key(a[kk]) : 0u;
+          const unsigned kmax = wave::wave_max_u32(key);
+          // the maximum is attained by some candidate row (n - k of them remain)
+          const unsigned long long bal = __ballot(key == kmax);
+          const int p = __builtin_ctzll(bal);
+          const E piv = wave::readlane_e(a[kk], p);
+          const E ip = E(1) / piv;
+          const E c = (lane == p) ? E(0) : a[kk] * ip;
+          cbuf[par][kk][lane] = c;
+          if (lane == 0) {
+            pbuf[par][kk] = p;
+            ibuf[par][kk] = ip;
+            pl[k] = p;
+          }
+#pragma unroll
+          for (int j = 0; j < CW; ++j) {
+            const E r = wave::readlane_e(a[j], p);
+            a[j] = fma(-c, r, a[j]);
+          }
+          a[kk] = (lane == p) ? E(1) : -c;
+          if (lane == p) {
+            s = ip;
+            step = k;
+          }
         }
       }
-      __syncthreads();
-      int p;
-      E c, ip;
-      if (w == kw) {
-        p = p_own;
-        c = c_own;
-        ip = ip_own;
-      } else {
-        p = __builtin_amdgcn_readfirstlane(pbuf[par]);
-        c = cbuf[par][lane];
-        ip = ibuf[par];
-      }
+    }
+    __syncthreads();
+    if (w != g) {
+      E c[CW];
 #pragma unroll
-      for (int j = 0; j < CW; ++j) {
-        const E r = wave::readlane_e(a[j], p);
-        a[j] = fma(-c, r, a[j]);
-      }
-      if (w == kw) a[kk] = (lane == p) ? E(1) : -c;
-      if (lane == p) {
-        s = ip;
-        step = k;
+      for (int kk = 0; kk < CW; ++kk) c[kk] = cbuf[par][kk][lane];
+#pragma unroll
+      for (int kk = 0; kk < CW; ++kk) {
+        const int k = g * CW + kk;
+        if (k < n) {  // uniform over the workgroup
+          const int p = __builtin_amdgcn_readfirstlane(pbuf[par][kk]);
+#pragma unroll
+          for (int j = 0; j < CW; ++j) {
+            const E r = wave::readlane_e(a[j], p);
+            a[j] = fma(-c[kk], r, a[j]);
+          }
+          if (lane == p) {
+            s = ibuf[par][kk];
+            step = k;
+          }
+        }
       }
     }
   }
