@@ -61,7 +61,7 @@ CALIB_MIB = 256
 # the step's 4 GEMM dispatches, each counter scaled by its reading for the calibration
 # copy (the HBM-traffic summary of this step under profiles/ gives both commands).  A
 # measured constant of the current kernels: re-measure after changing them.
-TRAFFIC = 56823397
+TRAFFIC = 57258911
 # Tile policies of the timed step (backward sweep, boundary-map launch, one per backward
 # level 2, 1, 0), as the autotuner picks them on this workload (bench.py --autotune prints
 # them in timed_policy_flags); pinned so that a profiler run of the same flags traces the

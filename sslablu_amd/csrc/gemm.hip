@@ -993,18 +993,39 @@ static long long count_tiles(const GemmLaunchT<T>& L, int BM, int BN) {
   return t;
 }
 
-// C_b = beta * Cin_b (Cin may be null or alias C): the initial value of a split-K
-// output, which the K parts then accumulate into atomically.
+// The initial values of a launch's split-K outputs that do not accumulate in place,
+// C_b = beta * Cin_b (Cin may be null or alias C), which the K parts then add into
+// atomically: one table of outputs per GEMM launch and one init launch for all of them.
+// Rows of all outputs are numbered in turn (output q owns rows row_begin.. of its batch x
+// M); a workgroup writes one row at a time, 256 columns per pass.
 template <typename T>
-__global__ void k_scale_init(int M, int N, const T* __restrict__ Cin, int ldcin, long long sCin,
-                             T beta, T* __restrict__ C, int ldc, long long sC) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const long long b = blockIdx.z;
-  if (j >= N) return;
-  for (int i = blockIdx.y; i < M; i += gridDim.y) {
-    const T v = (Cin != nullptr && beta != T(0)) ? beta * Cin[b * sCin + (long long)i * ldcin + j]
-                                                  : T(0);
-    C[b * sC + (long long)i * ldc + j] = v;
+struct InitJob {
+  T* C;
+  const T* Cin;
+  long long sC, sCin;
+  int ldc, ldcin, M, N;
+  T beta;
+  int row_begin;
+};
+template <typename T>
+struct InitTable {
+  InitJob<T> j[kMaxDesc];
+  int n, rows;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale_init(const InitTable<T> t) {
+  for (int r = blockIdx.x; r < t.rows; r += gridDim.x) {
+    InitJob<T> J = t.j[0];  // selected by an unrolled scan (constant kernarg offsets)
+#pragma unroll
+    for (int q = 1; q < kMaxDesc; ++q)
+      if (q < t.n && r >= t.j[q].row_begin) J = t.j[q];
+    const int lr = r - J.row_begin;
+    const int b = lr / J.M, i = lr - b * J.M;
+    T* c = J.C + b * J.sC + (long long)i * J.ldc;
+    const bool use = J.Cin != nullptr && J.beta != T(0);
+    const T* ci = J.Cin + (use ? b * J.sCin + (long long)i * J.ldcin : 0);
+    for (int col = threadIdx.x; col < J.N; col += 256) c[col] = use ? J.beta * ci[col] : T(0);
   }
 }
 
@@ -1072,6 +1093,8 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit, int max
   int total = 0;
   GemmLaunchT<T> P{};
   P.nd = 0;
+  InitTable<T> init{};
+  long long init_rows = 0;
   const long long chunk = split_chunk<Cfg, T>(L, ksplit, maxparts);
   for (int i = 0; i < L.nd; ++i) {
     GemmDescT<T> d = L.d[i];
@@ -1095,12 +1118,19 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit, int max
     d.ksplit = ks > 1 ? ceil_div((int)K, d.kchunk) : 1;
     if (d.ksplit == 1) d.kchunk = (int)(K > 0 ? K : 1);
     if (d.ksplit > 1 && !inplace) {
-      SSLAB_REQUIRE(d.batch <= 65535, "gemm: split-K init batch too large");
-      const int gy = d.M < 4096 ? d.M : 4096;
-      hipLaunchKernelGGL(k_scale_init<T>, dim3(ceil_div(d.N, 256), gy, d.batch), dim3(256), 0,
-                         stream, d.M, d.N, d.Cin, d.ldcin, d.strideCin, d.beta, d.C, d.ldc,
-                         d.strideC);
-      SSLAB_HIP_CHECK(hipGetLastError());
+      InitJob<T>& J = init.j[init.n++];
+      J.C = d.C;
+      J.Cin = d.Cin;
+      J.sC = d.strideC;
+      J.sCin = d.strideCin;
+      J.ldc = d.ldc;
+      J.ldcin = d.ldcin;
+      J.M = d.M;
+      J.N = d.N;
+      J.beta = d.beta;
+      J.row_begin = (int)init_rows;
+      init_rows += (long long)d.batch * d.M;
+      SSLAB_REQUIRE(init_rows <= 0x7fffffffLL, "gemm: split-K init table too large");
     }
     d.tiles_m = ceil_div(d.M, Cfg::BM);
     d.tiles_n = ceil_div(d.N, Cfg::BN);
@@ -1114,6 +1144,12 @@ static int launch_cfg(GemmLaunchT<T>& L, hipStream_t stream, int ksplit, int max
     P.d[P.nd++] = d;
   }
   if (total == 0) return kOk;
+  if (init.n > 0) {
+    init.rows = (int)init_rows;
+    const int grid = init.rows < 65536 ? init.rows : 65536;
+    hipLaunchKernelGGL(k_scale_init<T>, dim3(grid), dim3(256), 0, stream, init);
+    SSLAB_HIP_CHECK(hipGetLastError());
+  }
   P.xcd_chunk = xcd_chunk(P, total);
   if constexpr (kADirect && ADGeom<Cfg, T>::ok) {
     int nmax = 0;

@@ -83,11 +83,12 @@ __device__ __forceinline__ unsigned gj_key(float x) { return __float_as_uint(fab
 // X[k][p_j] = s_i a_i[j] for i = p_k, where p_j is the row pivoted at step j; the rows are
 // put in place through an LDS tile so that the stores are coalesced.  Rows / columns
 // n..63 are the identity's and never pivot.
-template <typename E>
-__global__ __launch_bounds__(kGjWaves * 64) void k_inv_gj(int n, const E* __restrict__ A, int lda,
-                                                long long sA, E* __restrict__ X, int ldx,
-                                                long long sX) {
+template <typename E, bool kFull>
+__global__ __launch_bounds__(kGjWaves * 64) void k_inv_gj(int n_arg, const E* __restrict__ A,
+                                                int lda, long long sA, E* __restrict__ X,
+                                                int ldx, long long sX) {
   constexpr int NB = kInvBase;
+  const int n = kFull ? NB : n_arg;  // full blocks: every step guard folds away
   constexpr int CW = NB / kGjWaves;  // columns per wave = steps per panel
   constexpr int LD = NB + 1;
   __shared__ E tile[NB * LD];
@@ -258,8 +259,12 @@ template <typename E>
 int inv_rec(int n, const E* A, int lda, long long sA, E* X, int ldx, long long sX, E* W,
             long long sW, int batch, hipStream_t st) {
   if (n <= kInvBase) {
-    hipLaunchKernelGGL(k_inv_gj<E>, dim3(batch), dim3(kGjWaves * 64), 0, st, n, A, lda, sA, X,
-                       ldx, sX);
+    if (n == kInvBase)
+      hipLaunchKernelGGL((k_inv_gj<E, true>), dim3(batch), dim3(kGjWaves * 64), 0, st, n, A, lda,
+                         sA, X, ldx, sX);
+    else
+      hipLaunchKernelGGL((k_inv_gj<E, false>), dim3(batch), dim3(kGjWaves * 64), 0, st, n, A,
+                         lda, sA, X, ldx, sX);
     SSLAB_HIP_CHECK(hipGetLastError());
     return kOk;
   }
