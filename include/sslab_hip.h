@@ -403,7 +403,21 @@ int sslab_oms_fold_create(int nb, int m, int nF, const double* maps, int ldm, vo
                           void* stream);
 int sslab_oms_fold_apply(const void* plan, const double* F, int ldF, int nrhs, double* X,
                          int ldx, void* stream);
-/* device bytes held by a plan; number of descriptor groups of its launch */
+/* Mixed-precision solve from boundary data (no reference counterpart: the reference is
+ * FP64 throughout).  sslab_oms_fold_enable_f32 keeps an FP32 copy of the plan's maps
+ * (once; freed with the plan).  sslab_oms_fold_solve_mixed then solves the identity-
+ * diagonal S-system for boundary data F (nF x nrhs, contiguous FP64): X0 from the FP32
+ * maps and the FP32 backward sweep (factor32: the FP64 red-black factor rounded to float,
+ * as for sslab_rb_solve_mixed), then `iters` refinement steps x += M (B - S x) as there.
+ * B = rhs(F) in FP64 (sslab_oms_rhs), X: nb*m x nrhs contiguous doubles; work64:
+ * nb*m*nrhs doubles; work32: (2*nb*m + nF)*nrhs floats. */
+int sslab_oms_fold_enable_f32(void* plan, void* stream);
+int sslab_oms_fold_solve_mixed(const void* plan, const float* factor32, const double* SiM,
+                               const double* SiP, const double* F, int ldF, int nrhs,
+                               const double* B, double* X, double* work64, float* work32,
+                               int iters, void* stream);
+/* device bytes held by a plan (its FP32 maps included); number of descriptor groups of
+ * its launch */
 long long sslab_oms_fold_bytes(const void* plan);
 int sslab_oms_fold_groups(const void* plan);
 void sslab_oms_fold_destroy(void* plan);

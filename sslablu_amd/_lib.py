@@ -89,6 +89,9 @@ _SIGS = {
                           _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp],
     "sslab_oms_fold_create": [_c_int, _c_int, _c_int, _vp, _c_int, _vp, _vp],
     "sslab_oms_fold_apply": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp],
+    "sslab_oms_fold_enable_f32": [_vp, _vp],
+    "sslab_oms_fold_solve_mixed": [_vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                   _c_int, _vp],
     "sslab_oms_fold_bytes": [_vp],
     "sslab_oms_fold_groups": [_vp],
     "sslab_oms_fold_destroy": [_vp],

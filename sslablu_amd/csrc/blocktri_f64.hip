@@ -955,6 +955,23 @@ int rb_residual(int nb, int m, int r, const double* T, const double* SiM, const 
 // an iterative refinement whose residual r = b - S x is FP64:
 //   x = M b;  repeat `iters` times:  r = b - S x (FP64),  x += M r.
 // B, X: N x r (contiguous, N = nb*m); R64: N x r doubles; V32, X32: N x r floats each.
+int rb_refine_mixed(int nb, int m, int r, const float* factor32, const double* T,
+                    const double* SiM, const double* SiP, const double* B, double* X,
+                    double* R64, float* V32, float* X32, int iters, hipStream_t st) {
+  SSLAB_REQUIRE(nb >= 1 && m >= 1 && r >= 0 && iters >= 0, "rb_refine_mixed: bad sizes");
+  if (r == 0 || iters == 0) return kOk;
+  SSLAB_REQUIRE(factor32 && B && X && V32 && X32 && R64 && SiM && SiP,
+                "rb_refine_mixed: null pointer");
+  const long long n = (long long)nb * m * r;
+  for (int it = 0; it < iters; ++it) {
+    SSLAB_TRY(rb_residual(nb, m, r, T, SiM, SiP, B, r, X, r, R64, r, st));
+    SSLAB_TRY(cast_f64_to_f32(n, R64, V32, st));
+    SSLAB_TRY(rb_solve_t<float>(nb, m, r, factor32, V32, r, X32, r, st));
+    SSLAB_TRY(acc_f32_into_f64(n, X32, X, st));
+  }
+  return kOk;
+}
+
 int rb_solve_mixed(int nb, int m, int r, const float* factor32, const double* T,
                    const double* SiM, const double* SiP, const double* B, double* X, double* R64,
                    float* V32, float* X32, int iters, hipStream_t st) {
@@ -966,13 +983,22 @@ int rb_solve_mixed(int nb, int m, int r, const float* factor32, const double* T,
   SSLAB_TRY(cast_f64_to_f32(n, B, V32, st));
   SSLAB_TRY(rb_solve_t<float>(nb, m, r, factor32, V32, r, X32, r, st));
   SSLAB_TRY(cast_f32_to_f64(n, X32, X, st));
-  for (int it = 0; it < iters; ++it) {
-    SSLAB_TRY(rb_residual(nb, m, r, T, SiM, SiP, B, r, X, r, R64, r, st));
-    SSLAB_TRY(cast_f64_to_f32(n, R64, V32, st));
-    SSLAB_TRY(rb_solve_t<float>(nb, m, r, factor32, V32, r, X32, r, st));
-    SSLAB_TRY(acc_f32_into_f64(n, X32, X, st));
-  }
-  return kOk;
+  return rb_refine_mixed(nb, m, r, factor32, T, SiM, SiP, B, X, R64, V32, X32, iters, st);
+}
+
+int rb_solve_f32_ex(int nb, int m, int r, const float* factor, float* V, int ldv, float* X,
+                    int ldx, int flags, hipStream_t st) {
+  SSLAB_REQUIRE((flags & ~15) == 0, "rb_solve_f32: unknown flags 0x%x", flags);
+  SSLAB_REQUIRE(!(flags & 2) || ((flags & 1) && nb >= 2),
+                "rb_solve_f32: the level-0-reduced entry state needs identity diagonal blocks "
+                "and nb >= 2");
+  SSLAB_REQUIRE((flags & 12) != 12,
+                "rb_solve_f32: forward-only and backward-only exclude each other");
+  SSLAB_REQUIRE(!((flags & 2) && (flags & 8)),
+                "rb_solve_f32: backward-only already starts from the forward sweep's result");
+  const int phase = (flags & 4) ? 1 : ((flags & 8) ? 2 : 0);
+  return rb_solve_t<float>(nb, m, r, factor, V, ldv, X, ldx, st, (flags & 1) != 0,
+                           (flags & 2) != 0, phase);
 }
 
 // ---------------------------------------------------------------------------

@@ -46,6 +46,14 @@ int rb_residual(int nb, int m, int r, const double* T, const double* SiM, const 
 int rb_solve_mixed(int nb, int m, int r, const float* factor32, const double* T,
                    const double* SiM, const double* SiP, const double* B, double* X, double* R64,
                    float* V32, float* X32, int iters, hipStream_t st);
+// The refinement steps of rb_solve_mixed from a given FP64 X: `iters` times r = B - S X
+// (FP64), its FP32 solve and X += (FP64) correction.
+int rb_refine_mixed(int nb, int m, int r, const float* factor32, const double* T,
+                    const double* SiM, const double* SiP, const double* B, double* X,
+                    double* R64, float* V32, float* X32, int iters, hipStream_t st);
+// rb_solve_ex with an FP32 factor (flags 1, 2, 4, 8 as there; no chained sweep).
+int rb_solve_f32_ex(int nb, int m, int r, const float* factor, float* V, int ldv, float* X,
+                    int ldx, int flags, hipStream_t st);
 
 // rb_solve_mixed with its workspace in a static per-GPU device array (64 MiB: 16 bytes
 // per element of the nb*m x r panel, each of its three parts rounded up to 256 bytes).

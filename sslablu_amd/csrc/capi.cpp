@@ -375,6 +375,19 @@ int sslab_oms_fold_apply(const void* plan, const double* F, int ldF, int nrhs, d
                                as_stream(stream));
 }
 
+int sslab_oms_fold_enable_f32(void* plan, void* stream) {
+  return sslab::oms_fold_enable_f32(static_cast<sslab::FoldPlan*>(plan), as_stream(stream));
+}
+
+int sslab_oms_fold_solve_mixed(const void* plan, const float* factor32, const double* SiM,
+                               const double* SiP, const double* F, int ldF, int nrhs,
+                               const double* B, double* X, double* work64, float* work32,
+                               int iters, void* stream) {
+  return sslab::oms_fold_solve_mixed(static_cast<const sslab::FoldPlan*>(plan), factor32, SiM,
+                                     SiP, F, ldF, nrhs, B, X, work64, work32, iters,
+                                     as_stream(stream));
+}
+
 long long sslab_oms_fold_bytes(const void* plan) {
   return sslab::oms_fold_bytes(static_cast<const sslab::FoldPlan*>(plan));
 }

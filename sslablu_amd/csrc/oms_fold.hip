@@ -55,6 +55,7 @@ struct FoldPlan {
   int* zero_rows;    // device: first X row of every block of the split groups
   int nzero;         // number of such blocks
   int nosplit_k;     // groups whose summed K is below this are never split
+  float* maps32;     // the maps rounded to FP32 (oms_fold_enable_f32), or null
 };
 
 // Groups whose summed K is below this are never split: their blocks are written once
@@ -74,15 +75,16 @@ int fold_nosplit_k() {
 namespace {
 
 // Zero rows [row_b, row_b + m) x [0, r) of X for every listed block b (one launch).
+template <typename T>
 __global__ void k_zero_blocks(int nblk, const int* __restrict__ rows, int m, int r,
-                              double* __restrict__ X, int ldx) {
+                              T* __restrict__ X, int ldx) {
   const long long n = (long long)nblk * m * r;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x) {
     const long long q = e / r;
     const int c = (int)(e - q * r);
     const int b = (int)(q / m), i = (int)(q - (long long)b * m);
-    X[(long long)(rows[b] + i) * ldx + c] = 0.0;
+    X[(long long)(rows[b] + i) * ldx + c] = T(0);
   }
 }
 
@@ -226,7 +228,7 @@ int oms_fold_create(int nb, int m, int nF, const double* maps, int ldm, FoldPlan
   }
 
   // 3. compact maps: group g holds count blocks of m x Ktot (its runs side by side)
-  FoldPlan* P = new FoldPlan{nb, m, nF, nullptr, 0, {}, nullptr, 0, kFoldNoSplitK};
+  FoldPlan* P = new FoldPlan{nb, m, nF, nullptr, 0, {}, nullptr, 0, kFoldNoSplitK, nullptr};
   long long total = 0;
   for (const Proto& p : protos) {
     int kt = 0;
@@ -292,39 +294,38 @@ int oms_fold_create(int nb, int m, int nF, const double* maps, int ldm, FoldPlan
   return kOk;
 }
 
-int oms_fold_apply(const FoldPlan* P, const double* F, int ldF, int r, double* X, int ldx,
-                   hipStream_t st) {
-  SSLAB_REQUIRE(P != nullptr, "oms_fold_apply: null plan");
-  SSLAB_REQUIRE(r >= 0, "oms_fold_apply: negative column count");
-  if (r == 0) return kOk;
-  SSLAB_REQUIRE(F && X && ldF >= r && ldx >= r, "oms_fold_apply: bad pointers / strides");
+// X = M F for r columns (the backward sweep's entry state), with the maps in T.
+template <typename T>
+static int fold_apply_t(const FoldPlan* P, const T* maps, const T* F, int ldF, int r, T* X,
+                        int ldx, hipStream_t st) {
   // long groups accumulate (split-K) into zeroed blocks; short ones store once, unsplit
   if (P->nzero > 0) {
     const long long n = (long long)P->nzero * P->m * r;
     const long long gsz = (n + 255) / 256;
-    hipLaunchKernelGGL(k_zero_blocks, dim3((unsigned)(gsz < 8192 ? gsz : 8192)), dim3(256), 0,
+    hipLaunchKernelGGL(k_zero_blocks<T>, dim3((unsigned)(gsz < 8192 ? gsz : 8192)), dim3(256), 0,
                        st, P->nzero, P->zero_rows, P->m, r, X, ldx);
     SSLAB_HIP_CHECK(hipGetLastError());
   }
-  GemmLaunch G{};
+  GemmLaunchT<T> G{};
   G.nd = 0;
   for (const FoldGroup& g : P->groups) {
     if (G.nd == kMaxDesc) {  // groups write disjoint blocks: any split into launches works
       SSLAB_TRY(gemm_launch(G, st));
       G.nd = 0;
     }
-    double* C = X + (long long)g.x_row * ldx;
+    T* C = X + (long long)g.x_row * ldx;
     const long long sC = (long long)g.x_rstride * ldx;
     int kt = 0;
     for (int t = 0; t < g.nterms; ++t) kt += g.t[t].K;
     const bool split_ok = kt >= P->nosplit_k;
-    GemmDesc d = split_ok ? gemm_desc(P->m, r, g.batch, C, ldx, sC, C, ldx, sC, 1.0)
-                          : gemm_desc(P->m, r, g.batch, C, ldx, sC, nullptr, 0, 0, 0.0);
+    GemmDescT<T> d = split_ok ? gemm_desc<T>(P->m, r, g.batch, C, ldx, sC, C, ldx, sC, 1.0)
+                              : gemm_desc<T>(P->m, r, g.batch, C, ldx, sC, nullptr, 0, 0, 0.0);
     d.nosplit = split_ok ? 0 : 1;
     for (int t = 0; t < g.nterms; ++t) {
-      const FoldTerm& T = g.t[t];
-      gemm_add_term(d, gemm_term(P->maps + T.a_off, T.lda, T.sA, 0, F + (long long)T.f_row * ldF,
-                                 ldF, (long long)T.f_rstride * ldF, 0, T.K, 1.0));
+      const FoldTerm& Tm = g.t[t];
+      gemm_add_term<T>(d, gemm_term<T>(maps + Tm.a_off, Tm.lda, Tm.sA, 0,
+                                       F + (long long)Tm.f_row * ldF, ldF,
+                                       (long long)Tm.f_rstride * ldF, 0, Tm.K, 1.0));
     }
     G.d[G.nd++] = d;
   }
@@ -332,8 +333,64 @@ int oms_fold_apply(const FoldPlan* P, const double* F, int ldF, int r, double* X
   return kOk;
 }
 
+int oms_fold_apply(const FoldPlan* P, const double* F, int ldF, int r, double* X, int ldx,
+                   hipStream_t st) {
+  SSLAB_REQUIRE(P != nullptr, "oms_fold_apply: null plan");
+  SSLAB_REQUIRE(r >= 0, "oms_fold_apply: negative column count");
+  if (r == 0) return kOk;
+  SSLAB_REQUIRE(F && X && ldF >= r && ldx >= r, "oms_fold_apply: bad pointers / strides");
+  return fold_apply_t<double>(P, P->maps, F, ldF, r, X, ldx, st);
+}
+
+int oms_fold_enable_f32(FoldPlan* P, hipStream_t st) {
+  SSLAB_REQUIRE(P != nullptr, "oms_fold_enable_f32: null plan");
+  if (P->maps32 != nullptr) return kOk;
+  if (P->maps_doubles > 0) {
+    float* p = nullptr;
+    SSLAB_HIP_CHECK(hipMalloc(&p, sizeof(float) * P->maps_doubles));
+    const int rc = cast_f64_to_f32(P->maps_doubles, P->maps, p, st);
+    if (rc != kOk) {
+      (void)hipFree(p);
+      return rc;
+    }
+    P->maps32 = p;
+  }
+  return kOk;
+}
+
+// Mixed-precision solve from boundary data: the FP32 maps and FP32 backward sweep give
+// X0 (the folded solve in FP32), then `iters` refinement steps, each an FP64 residual
+// r = B - S X, its FP32 red-black solve and an FP64 update.  B = rhs(F) in FP64 (the
+// caller's), the system's diagonal blocks are identities (the oms S-system).
+int oms_fold_solve_mixed(const FoldPlan* P, const float* factor32, const double* SiM,
+                         const double* SiP, const double* F, int ldF, int r, const double* B,
+                         double* X, double* work64, float* work32, int iters, hipStream_t st) {
+  SSLAB_REQUIRE(P != nullptr, "oms_fold_solve_mixed: null plan");
+  SSLAB_REQUIRE(r >= 0 && iters >= 0, "oms_fold_solve_mixed: bad sizes");
+  if (r == 0) return kOk;
+  SSLAB_REQUIRE(P->maps32 != nullptr || P->maps_doubles == 0,
+                "oms_fold_solve_mixed: the plan has no FP32 maps (oms_fold_enable_f32)");
+  SSLAB_REQUIRE(factor32 && F && B && X && work32 &&
+                    (iters == 0 || (SiM && SiP && work64)) && (P->nb == 1 || (SiM && SiP)),
+                "oms_fold_solve_mixed: null pointer");
+  const long long n = (long long)P->nb * P->m * r;
+  float* X32 = work32;
+  float* V32 = X32 + n;
+  float* F32 = V32 + n;
+  SSLAB_REQUIRE(ldF == r, "oms_fold_solve_mixed: F must be contiguous (ldF == nrhs)");
+  // F32 = F, then X32 = maps32 F32 and the FP32 backward sweep
+  SSLAB_TRY(cast_f64_to_f32((long long)P->nF * r, F, F32, st));
+  SSLAB_TRY(fold_apply_t<float>(P, P->maps32, F32, r, r, X32, r, st));
+  SSLAB_TRY(rb_solve_f32_ex(P->nb, P->m, r, factor32, nullptr, r, X32, r, 1 | 8, st));
+  SSLAB_TRY(cast_f32_to_f64(n, X32, X, st));
+  return rb_refine_mixed(P->nb, P->m, r, factor32, nullptr, SiM, SiP, B, X, work64, V32, X32,
+                         iters, st);
+}
+
 long long oms_fold_bytes(const FoldPlan* P) {
-  return P ? (long long)sizeof(double) * P->maps_doubles : 0;
+  if (!P) return 0;
+  return (long long)sizeof(double) * P->maps_doubles +
+         (P->maps32 ? (long long)sizeof(float) * P->maps_doubles : 0);
 }
 
 int oms_fold_groups(const FoldPlan* P) { return P ? (int)P->groups.size() : 0; }
@@ -341,6 +398,7 @@ int oms_fold_groups(const FoldPlan* P) { return P ? (int)P->groups.size() : 0; }
 void oms_fold_destroy(FoldPlan* P) {
   if (!P) return;
   if (P->maps) (void)hipFree(P->maps);
+  if (P->maps32) (void)hipFree(P->maps32);
   if (P->zero_rows) (void)hipFree(P->zero_rows);
   delete P;
 }

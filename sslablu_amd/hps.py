@@ -760,6 +760,43 @@ class BoundaryFold:
         X = torch.empty(self.fac.nb * self.fac.m, F.shape[1], dtype=F64, device=self.op.device)
         return self.solve_(F, X, stream)
 
+    # -- mixed precision ------------------------------------------------------------------
+    def enable_mixed(self, stream=None):
+        """Keep an FP32 copy of the maps for solve_mixed_ (the factor needs its FP32 copy
+        first: BlockTriFactor.enable_mixed with the S-system's SiM / SiP)."""
+        if getattr(self.fac, "factor32", None) is None or getattr(self.fac, "_M64", None) is None:
+            raise ValueError("BoundaryFold.enable_mixed: call the factor's enable_mixed first")
+        call("sslab_oms_fold_enable_f32", self.handle, stream_handle(stream))
+        return self
+
+    def mixed_work(self, r: int):
+        """Workspace of solve_mixed_ for r columns: (FP64 residual, FP32 panels, FP64 rhs)."""
+        N = self.fac.nb * self.fac.m
+        dev = self.op.device
+        return (torch.empty(N * r, dtype=F64, device=dev),
+                torch.empty((2 * N + self.nF) * r, dtype=torch.float32, device=dev),
+                torch.empty(N, r, dtype=F64, device=dev))
+
+    def solve_mixed_(self, F: torch.Tensor, X: torch.Tensor, iters: int = 1, work=None,
+                     stream=None):
+        """X = S^{-1} rhs(F) in mixed precision: the folded solve in FP32 (FP32 maps and
+        backward sweep), then `iters` refinement steps with FP64 residuals and FP32
+        red-black corrections (sslab_oms_fold_solve_mixed).  F: contiguous float64 (nF, r)."""
+        r = F.shape[1]
+        nbm = self.fac.nb * self.fac.m
+        if (F.shape[0] != self.nF or F.dtype != F64 or not F.is_contiguous()
+                or X.shape != (nbm, r) or X.dtype != F64 or not X.is_contiguous()):
+            raise ValueError(f"solve_mixed_: contiguous float64 F ({self.nF}, r) and X ({nbm}, r)")
+        if work is None:
+            work = self.mixed_work(r)
+        w64, w32, B = work
+        self.op.rhs_from_boundary(F, out=B, stream=stream)
+        fac = self.fac
+        call("sslab_oms_fold_solve_mixed", self.handle, ptr(fac.factor32), ptr(fac._M64),
+             ptr(fac._P64), ptr(F), r, r, ptr(B), ptr(X), ptr(w64), ptr(w32), int(iters),
+             stream_handle(stream))
+        return X
+
     def __del__(self):
         h, self.handle = getattr(self, "handle", None), None
         if h and _lib is not None:

@@ -87,7 +87,8 @@ def make_pdo(spec: ProblemSpec):
 class CachedOperator:
     """A built operator: rhs maps (op.Y), red-black factor, the boundary-folded forward
     sweep (FP64 batches: one launch + the backward sweep; built in every cache, so FP64
-    batches on a mixed cache are folded too), optional FP32 factor."""
+    batches on a mixed cache are folded too), and in mixed caches the FP32 factor and
+    FP32 fold maps (mixed batches: the folded solve in FP32, then FP64 refinement)."""
 
     def __init__(self, spec: ProblemSpec, device, mixed: bool):
         t0 = time.perf_counter()
@@ -99,6 +100,8 @@ class CachedOperator:
         if mixed:
             fac.enable_mixed(None, op.SiM, op.SiP)
         self.fold = op.fold(fac)
+        if mixed:
+            self.fold.enable_mixed()
         torch.cuda.synchronize(device)
         self.spec, self.op, self.fac = spec, op, fac
         self.N = nb * m
@@ -112,7 +115,8 @@ class CachedOperator:
     @property
     def nbytes(self):
         # fac.nbytes covers the FP64 factor and, in mixed caches, its FP32 copy; the FP64
-        # blocks the refinement residual reads are op.SiM / op.SiP themselves (no copy)
+        # blocks the refinement residual reads are op.SiM / op.SiP themselves (no copy);
+        # fold.nbytes counts its FP32 maps in mixed caches
         n = self.fac.nbytes + (self.op.Y.numel() + self.op.SiM.numel() + self.op.SiP.numel()) * 8
         if self.fold is not None:
             n += self.fold.nbytes
@@ -144,7 +148,8 @@ class FactorCache:
         mixed caches, the maps Y and coupling blocks, and an upper bound of the folded
         boundary maps (an odd node of level l reads at most 2^(l+2) strip windows of
         bottom and top data plus both boundary segments, a coarsest-level node all nF
-        boundary values)."""
+        boundary values; 12 bytes per element in mixed caches, which keep them in FP32
+        too)."""
         nb, m, p = spec.nb, spec.m, spec.p
         ncols = 2 * m + 4 * p
         rb, _, _ = _lib.sizes("rb", nb, m)
@@ -157,7 +162,7 @@ class FactorCache:
         for lvl in range(L):
             cols += (k // 2) * min(nF, (4 << lvl) * 2 * p + 2 * m)
             k = (k + 1) // 2
-        n += 8 * m * cols
+        n += (12 if self.mixed else 8) * m * cols
         return n
 
     def _evict_for(self, need: int) -> bool:
@@ -254,15 +259,17 @@ class _BatchProgram:
         self.op, self.fac, self.fold = e.op, e.fac, e.fold
         self.W, self.precision, self.iters = W, precision, iters
         self.Fb = torch.zeros(e.ngb, W, dtype=F64, device=dev)
-        folded = precision != "mixed" and e.fold is not None
+        folded = e.fold is not None
         self.V = None if folded else torch.empty(e.N, W, dtype=F64, device=dev)
         self.Xb = torch.empty(e.N, W, dtype=F64, device=dev)
         self.work = None
         if precision == "mixed":
-            self.work = (torch.empty(e.N * W, dtype=F64, device=dev),
-                         torch.empty(2 * e.N * W, dtype=torch.float32, device=dev))
+            # folded: FP64 residual, FP32 panels (+ boundary data) and the FP64 rhs
+            self.work = (e.fold.mixed_work(W) if folded else
+                         (torch.empty(e.N * W, dtype=F64, device=dev),
+                          torch.empty(2 * e.N * W, dtype=torch.float32, device=dev)))
         self.nbytes = (self.Fb.numel() + (1 if folded else 2) * e.N * W) * 8 + (
-            e.N * W * 16 if self.work is not None else 0)
+            sum(t.numel() * t.element_size() for t in self.work) if self.work is not None else 0)
         self.graph = None
         with torch.cuda.stream(stream):
             self._run()  # eager once: code objects, tuned policy
@@ -286,8 +293,11 @@ class _BatchProgram:
             self.graph = g
 
     def _run(self):
-        if self.V is None:  # FP64: folded forward sweep, then the backward sweep
-            self.fold.solve_(self.Fb, self.Xb)
+        if self.V is None:  # folded forward sweep, then the backward sweep
+            if self.precision == "mixed":
+                self.fold.solve_mixed_(self.Fb, self.Xb, iters=self.iters, work=self.work)
+            else:
+                self.fold.solve_(self.Fb, self.Xb)
             return
         self.op.rhs_from_boundary(self.Fb, out=self.V)
         if self.precision == "mixed":

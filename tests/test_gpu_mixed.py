@@ -128,3 +128,40 @@ def test_c2_size_mixed_vs_oracle(T):
     nrm = np.linalg.norm(x_ref)
     assert np.linalg.norm(Xm.cpu().numpy() - x_ref) / nrm < 1e-8
     assert np.linalg.norm(X64.cpu().numpy() - x_ref) / nrm < 1e-10
+    # the folded mixed solve (FP32 maps + FP32 backward sweep, one FP64 refinement step)
+    f64.enable_mixed(None, op.SiM, op.SiP)
+    fold = op.fold(f64).enable_mixed()
+    Xf = T.empty_like(B)
+    fold.solve_mixed_(F.contiguous(), Xf, iters=1)
+    assert np.linalg.norm(Xf.cpu().numpy() - x_ref) / nrm < 1e-8
+
+
+@pytest.mark.parametrize("pde,slabs,p", [("poisson", 8, 8), ("helmholtz", 16, 12),
+                                         ("helmholtz", 5, 8)])
+def test_folded_mixed_solve(T, pde, slabs, p):
+    """sslab_oms_fold_solve_mixed against the FP64 folded solve (itself pinned to the
+    oracle) of the same boundary data: without refinement the FP32 result is off by
+    about the FP32 solve's error (between 1e-9 and 1e-3: FP32 arithmetic really ran),
+    one step reaches <= 1e-8, two <= 1e-10; the fold's FP32 maps count in its bytes."""
+    from sslablu_amd.blocktri import BlockTriFactor
+    from sslablu_amd.hps import HPSSlabOperator, SlabGrid2D
+    from sslablu_amd.serving import ProblemSpec, make_pdo
+    spec = ProblemSpec(pde, slabs, p, 32)
+    g = SlabGrid2D(spec.slabs, 1, spec.leaves, spec.p + 2)
+    op = HPSSlabOperator(make_pdo(spec), g).build()
+    fac = BlockTriFactor("rb").factorize(None, op.SiM, op.SiP)
+    fold = op.fold(fac)
+    b64 = fold.nbytes
+    fac.enable_mixed(None, op.SiM, op.SiP)
+    fold.enable_mixed()
+    assert fold.nbytes == b64 + b64 // 2
+    rng = np.random.default_rng(slabs + p)
+    F = T.tensor(rng.standard_normal((g.gb_points().shape[0], 7)), device="cuda")
+    X64 = fold.solve(F).cpu().numpy()
+    nrm = np.linalg.norm(X64)
+    work = fold.mixed_work(7)
+    for iters, lo, hi in ((0, 1e-9, 1e-3), (1, 0.0, 1e-8), (2, 0.0, 1e-10)):
+        X = T.full((g.nb * g.m, 7), float("nan"), dtype=T.float64, device="cuda")
+        fold.solve_mixed_(F, X, iters=iters, work=work)
+        err = np.linalg.norm(X.cpu().numpy() - X64) / nrm
+        assert lo < err < hi, (iters, err)

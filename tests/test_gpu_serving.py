@@ -79,8 +79,9 @@ def test_mixed_precision_serving(S):
     srv.serve(reqs, realtime=False)
     for q, xa in zip(reqs, alone):
         assert _rel(q.X, xa) < 1e-8
-    # byte accounting of a mixed cache: the FP32 factor copy counted once, and inside
-    # the admission estimate; FP64 batches on this cache run the folded path
+    # byte accounting of a mixed cache: the FP32 factor copy counted once, the fold's
+    # FP32 maps in its bytes, both inside the admission estimate; FP64 and mixed batches
+    # on this cache run the folded path
     for e in cache.ops.values():
         f = e.fac
         held = (f.factor.numel() * 8 + f.factor32.numel() * 4
@@ -89,7 +90,7 @@ def test_mixed_precision_serving(S):
         assert held <= cache.estimate_bytes(e.spec)
         assert f._M64.data_ptr() == e.op.SiM.data_ptr()  # residual blocks are not copies
         prog = srv._program(e, 16, torch.cuda.current_stream())
-        assert prog.V is not None  # mixed batches: rhs maps + refined FP32 solve
+        assert prog.V is None and prog.work is not None  # mixed: folded FP32 + refinement
         prog64 = serving._BatchProgram(e, 8, "fp64", 1, False, torch.cuda.current_stream())
         assert prog64.V is None      # FP64 batches on a mixed cache: the folded solve
 
