@@ -88,14 +88,15 @@ int sslab_dgetrs_batched(int n, int nrhs, const double* LU, int lda, long long s
                          int* perm_work, int batch, void* stream);
 
 /* Explicit inverses X_b = A_b^{-1} of `batch` n x n matrices (A read only, X distinct),
- * by 2 x 2 block elimination on MFMA GEMMs down to 64 x 64 blocks that one wave inverts
- * with partial pivoting; the normwise backward error ||G - A X G||_F / (||A||_F ||X G||_F)
+ * by 2 x 2 block elimination on MFMA GEMMs down to 64 x 64 blocks that one workgroup
+ * inverts by Gauss-Jordan with partial pivoting; the normwise backward error ||G - A X G||_F / (||A||_F ||X G||_F)
  * is then sampled on four +-1 columns G (one stream synchronisation) and, when it exceeds
  * tol (tol <= 0: 1e-12) or is not finite for some matrix, the whole batch is inverted
  * again by pivoted LU against the identity.  info (device, batch ints, may be NULL): that LU's info, else zeros;
  * *used_lu (host, may be NULL): 1 when the LU ran.  work / iwork: sslab_inv_sizes.
  * Replaces the explicit block inverses of the red-black levels, lu_factor(T) followed by
- * lu_solve(T, I) (direct_solve/omsdirectsolve.py:31-36, 211, 308-333). */
+ * lu_solve(T, I) (direct_solve/omsdirectsolve.py:31-36, 211, 308-333), and the local
+ * solves of the HPS assembly (2-D leaves, merges, pair solves; the 3-D pair factors). */
 int sslab_inv_sizes(int n, int batch, long long* work_doubles, long long* work_ints);
 int sslab_dinv_batched(int n, const double* A, int lda, long long strideA, double* X, int ldx,
                        long long strideX, int batch, double* work, int* iwork, int* info,

@@ -18,12 +18,13 @@ slab is discretized once, on the GPU:
 2. binary merge trees over each slab's leaves (sslab_hps_merge, batched over all
    slabs), alternating the axes so that boxes stay near cubic -> one DtN map T_s per
    single slab, boundary ordered [x-lo | x-hi | y-lo | y-hi | z-lo | z-hi];
-3. per interface i the pair factor: LU of T^i_RR + T^{i+1}_LL (batched getrf), and the
+3. per interface i the pair factor: the explicit inverse of T^i_RR + T^{i+1}_LL (batched
+   block-elimination inverse on MFMA GEMMs, checked, pivoted LU as its fallback), and the
    coupling blocks T^i_RL, T^{i+1}_LR and the side-face blocks, gathered once.
 
 The S maps are then operators, as in the reference (never formed densely unless asked):
     S_l(i) v = (T^i_RR + T^{i+1}_LL)^{-1} T^i_RL v,    S_r(i) v = (...)^{-1} T^{i+1}_LR v,
-with transposes through the transposed LU solve (sslab_dgetrs_trans_batched), and the
+applied as GEMMs with those inverses (transposes through the transposed operand), and the
 S-system right-hand side for boundary data F is -(...)^{-1} (sum of the side-face and
 end-face terms).  Global boundary data are ordered [side faces of slab 0 | ... | side
 faces of slab N-1 | left end face | right end face], each slab's side faces as
@@ -376,10 +377,9 @@ class HPSSlabOperator3D:
             Tw = torch.stack([Ts[g.N - 1], Ts[0]])
             wrap = self._pair_factor(Tw, 1, stream)
             del Tw
-            self.lu, self.Bl, self.Br, self.G, self.piv = (
-                torch.cat([w, r]) for w, r in zip(wrap, reg))
+            self.tinv, self.Bl, self.Br, self.G = (torch.cat([w, r]) for w, r in zip(wrap, reg))
             return
-        self.lu, self.Bl, self.Br, self.G, self.piv = self._pair_factor(Ts, g.nb, stream)
+        self.tinv, self.Bl, self.Br, self.G = self._pair_factor(Ts, g.nb, stream)
 
     def _pair_factor(self, Ts, nb, stream=None):
         g = self.g
@@ -408,12 +408,18 @@ class HPSSlabOperator3D:
              None, None, ptr(G), 2 * ns, m * 2 * ns, s)
         call("sslab_gather2", nb, m, ns, ptr(Ts) + 8 * sT, n_s, sT, None, 0, 0, ptr(dL),
              ptr(dside), None, None, ptr(G) + 8 * ns, 2 * ns, m * 2 * ns, s)
-        piv = torch.empty(nb, m, dtype=I32, device=dev)
+        # (T^i_RR + T^{i+1}_LL)^{-1}: every later solve with it is then one GEMM (the LU's
+        # one-workgroup panels over m rows were most of the 3-D assembly time)
+        tinv = torch.empty(nb, m, m, dtype=F64, device=dev)
+        wd, wi = sizes2("sslab_inv_sizes", m, nb)
+        work = torch.empty(max(wd, 1), dtype=F64, device=dev)
+        iwork = torch.empty(max(wi, 1), dtype=I32, device=dev)
         info = torch.zeros(nb, dtype=I32, device=dev)
-        call("sslab_dgetrf_batched", m, ptr(lu), m, m * m, ptr(piv), m, ptr(info), nb, s)
+        call("sslab_dinv_batched", m, ptr(lu), m, m * m, ptr(tinv), m, m * m, nb, ptr(work),
+             ptr(iwork), ptr(info), 0.0, None, s)
         self._infos.append(info)
-        del neg_m
-        return lu, Bl, Br, G, piv
+        del neg_m, work, iwork, lu
+        return tinv, Bl, Br, G
 
     def build(self, stream=None):
         """Run the whole assembly; returns self."""
@@ -430,20 +436,10 @@ class HPSSlabOperator3D:
 
     # -- applies ---------------------------------------------------------------------
     def _solve(self, B, trans=False, stream=None):
-        """Tcc_i^{-1} B_i (or Tcc_i^{-T} B_i) for B (nb, m, r) contiguous; new tensor."""
-        nb, m = self.g.nb, self.g.m
-        r = B.shape[2]
-        X = torch.empty_like(B)
-        perm = torch.empty(nb, m, dtype=I32, device=self.device)
-        s = stream_handle(stream)
-        if trans:
-            W = torch.empty_like(B)
-            call("sslab_dgetrs_trans_batched", m, r, ptr(self.lu), m, m * m, ptr(self.piv), m,
-                 ptr(B), r, m * r, ptr(X), r, m * r, ptr(W), ptr(perm), nb, s)
-        else:
-            call("sslab_dgetrs_batched", m, r, ptr(self.lu), m, m * m, ptr(self.piv), m, ptr(B),
-                 r, m * r, ptr(X), r, m * r, ptr(perm), nb, s)
-        return X
+        """Tcc_i^{-1} B_i (or Tcc_i^{-T} B_i) for B (nb, m, r) contiguous; new tensor
+        (one batched GEMM with the inverses)."""
+        from .hbs import bgemm
+        return bgemm(self.tinv, B, ta=trans)
 
     def smaps_apply(self, side, X, transpose=False, stream=None, fp32=False):
         """S_l(i) X_i (side 'l') or S_r(i) X_i ('r') for every interface at once; X
@@ -568,7 +564,7 @@ class HPSSlabOperator3D:
 
     @property
     def nbytes(self):
-        return 8 * sum(t.numel() for t in (self.lu, self.Bl, self.Br, self.G)) + 4 * self.piv.numel()
+        return 8 * sum(t.numel() for t in (self.tinv, self.Bl, self.Br, self.G))
 
 
 class PairSMap:
@@ -674,26 +670,14 @@ class SideMap:
 
 
 class _SubFactor:
-    """The LU factors of one interface (a batch-of-one view)."""
+    """The pair factor of one interface (a batch-of-one view of its inverse)."""
 
     def __init__(self, op, i):
         self.op, self.i = op, i
 
     def solve(self, B, trans):
-        op, i = self.op, self.i
-        m = op.g.m
-        r = B.shape[2]
-        X = torch.empty_like(B)
-        perm = torch.empty(1, m, dtype=I32, device=op.device)
-        lu, piv = op.lu[i], op.piv[i]
-        if trans:
-            W = torch.empty_like(B)
-            call("sslab_dgetrs_trans_batched", m, r, ptr(lu), m, m * m, ptr(piv), m, ptr(B), r,
-                 m * r, ptr(X), r, m * r, ptr(W), ptr(perm), 1, stream_handle())
-        else:
-            call("sslab_dgetrs_batched", m, r, ptr(lu), m, m * m, ptr(piv), m, ptr(B), r, m * r,
-                 ptr(X), r, m * r, ptr(perm), 1, stream_handle())
-        return X
+        from .hbs import bgemm
+        return bgemm(self.op.tinv[self.i:self.i + 1], B.contiguous(), ta=trans)
 
 
 def compress_s_blocks(op: HPSSlabOperator3D, tree, rk: int, samples: int, seed: int = 0,

@@ -221,7 +221,7 @@ class solverWrapper:
         if getattr(self, "_sii", None) is None:
             if self.ndim == 3:
                 op = self.op
-                self._sii = InteriorSolve(op.lu[0], op.piv[0])
+                self._sii = InteriorSolve(None, None, inv=op.tinv[0])
             else:
                 self._sii = InteriorSolve.factor(self._skeleton2d()[0])
         return self._sii
@@ -253,14 +253,15 @@ solverWrapper.solve_dir = _solve_dir
 
 class InteriorSolve:
     """X -> A^{-1} X (mode N) or A^{-T} X (mode T) from batched-LU factors of one m x m
-    matrix (sslab_dgetrf_batched / sslab_dgetrs_batched / sslab_dgetrs_trans_batched);
-    numpy in -> numpy out, device tensors in -> device tensors out, (m,) or (m, r)."""
+    matrix (sslab_dgetrf_batched / sslab_dgetrs_batched / sslab_dgetrs_trans_batched), or
+    from its explicit inverse (inv: one GEMM per apply); numpy in -> numpy out, device
+    tensors in -> device tensors out, (m,) or (m, r)."""
 
     device_op = True
 
-    def __init__(self, lu, piv, mode="N"):
-        self.lu, self.piv = lu, piv
-        m = lu.shape[0]
+    def __init__(self, lu, piv, mode="N", inv=None):
+        self.lu, self.piv, self.inv = lu, piv, inv
+        m = (inv if inv is not None else lu).shape[0]
         self.shape = (m, m)
         self.dtype = np.float64
         self.mode = mode
@@ -281,18 +282,23 @@ class InteriorSolve:
 
     @property
     def T(self):
-        return InteriorSolve(self.lu, self.piv, "T" if self.mode == "N" else "N")
+        return InteriorSolve(self.lu, self.piv, "T" if self.mode == "N" else "N", self.inv)
 
     def _apply(self, v, trans):
         from .._lib import call, ptr, stream_handle
         was_np = not isinstance(v, torch.Tensor)
         x = torch.as_tensor(np.asarray(v, dtype=np.float64)) if was_np else v
-        x = x.to(self.lu.device, torch.float64)
+        x = x.to((self.inv if self.inv is not None else self.lu).device, torch.float64)
         one = x.dim() == 1
         B = (x[:, None] if one else x).contiguous()
         m, r = B.shape
         if m != self.shape[0]:
             raise ValueError(f"solver_ii: {self.shape[0]} rows expected, got {m}")
+        if self.inv is not None:
+            from ..hbs import bgemm
+            X = bgemm(self.inv[None], B[None], ta=trans)[0] if r else torch.empty_like(B)
+            X = X[:, 0] if one else X
+            return X.cpu().numpy() if was_np else X
         X = torch.empty_like(B)
         perm = torch.empty(m, dtype=torch.int32, device=B.device)
         s = stream_handle()
