@@ -147,6 +147,19 @@ def _is_pow2(k):
     return k >= 1 and (k & (k - 1)) == 0
 
 
+@functools.lru_cache(maxsize=16)
+def _interior_points(N, nx, ny, q, x0, width, y0, height):
+    g = SlabGrid2D(N, nx, ny, q, x0, width, y0, height)
+    lx, ly = g.leaf_origins()
+    ti = g.t[1:-1]
+    ga, gb = np.meshgrid(ti, ti, indexing="ij")
+    P = np.empty((lx.shape[0] * ti.shape[0] ** 2, 2))
+    P[:, 0].reshape(lx.shape[0], -1)[:] = lx[:, None] + ga.ravel()[None, :] * g.hx
+    P[:, 1].reshape(lx.shape[0], -1)[:] = ly[:, None] + gb.ravel()[None, :] * g.hy
+    P.setflags(write=False)
+    return P
+
+
 class SlabGrid2D:
     """Leaf tiling of the rectangle [x0, x0+width] x [y0, y0+height] (default the unit
     square) by N single slabs of nx x ny leaves with q points."""
@@ -185,13 +198,10 @@ class SlabGrid2D:
         return x0, y0
 
     def interior_points(self):
-        """(nleaves*p^2, 2) interior collocation nodes, row r = (a-1)*p + (b-1)."""
-        x0, y0 = self.leaf_origins()
-        ti = self.t[1:-1]
-        ga, gb = np.meshgrid(ti, ti, indexing="ij")
-        X = x0[:, None] + ga.ravel()[None, :] * self.hx
-        Y = y0[:, None] + gb.ravel()[None, :] * self.hy
-        return np.stack([X.ravel(), Y.ravel()], axis=1)
+        """(nleaves*p^2, 2) interior collocation nodes, row r = (a-1)*p + (b-1); a
+        read-only array shared by every grid of the same tiling (built once)."""
+        return _interior_points(self.N, self.nx, self.ny, self.q, self.x0, self.width,
+                                self.y0, self.height)
 
     def edge_coords(self, n_leaves, h):
         ti = self.t[1:-1]
