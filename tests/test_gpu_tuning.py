@@ -106,3 +106,41 @@ def test_backward_level_policies(T):
     assert len(best) == levels and us > 0 and fac._bwd_policies[r] == best
     x = fac.solve(B)
     assert (T.linalg.norm(x - ref) / T.linalg.norm(ref)).item() < 1e-12
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("cfg", [3, 4, 5, 12])
+def test_split_k_outputs_initialised_together(T, cfg):
+    """A grouped launch forced to split K: its non-in-place outputs start as beta Cin (or
+    zero without Cin) through one initialisation launch for all descriptors (different
+    shapes, batches, leading dimensions, betas), then the K parts add atomically; an
+    in-place descriptor (Cin == C) accumulates onto its own values.  Against numpy."""
+    from sslablu_amd import tuning
+    from sslablu_amd.hbs import gdesc, ggemm
+    rng = np.random.default_rng(cfg)
+    dev = "cuda"
+    cases = []
+    for nb, M, N, K, beta, kind in ((3, 70, 40, 900, 0.5, "cin"), (1, 33, 64, 1200, 0.0, "none"),
+                                    (5, 128, 24, 700, -2.0, "cin"), (2, 50, 16, 1500, 1.0, "inplace")):
+        A = rng.standard_normal((nb, M, K))
+        B = rng.standard_normal((nb, K, N))
+        C0 = rng.standard_normal((nb, M, N))
+        ref = A @ B + (beta * C0 if kind != "none" else 0.0)
+        tA, tB = T.tensor(A, device=dev), T.tensor(B, device=dev)
+        if kind == "inplace":
+            out = T.tensor(C0, device=dev)
+            d = gdesc(out, [(tA, tB, 1.0)], cin=out, beta=beta)
+        else:
+            out = T.full((nb, M, N), float("nan"), dtype=T.float64, device=dev)
+            cin = T.tensor(C0, device=dev) if kind == "cin" else None
+            d = gdesc(out, [(tA, tB, 1.0)], cin=cin, beta=beta)
+        cases.append((d, out, ref))
+    # a target no tile reaches and a small split threshold: every descriptor splits K
+    with tuning.policy(tuning.Policy(1 << 30, 4096, 16, 64, cfg, cfg, 0)):
+        ggemm(*[d for d, _, _ in cases])
+    T.cuda.synchronize()
+    for _, out, ref in cases:
+        assert rel(out.cpu().numpy(), ref) < 1e-13
